@@ -1,0 +1,192 @@
+"""Throughput of the TRiSK shallow-water RK4 step on one MI355X.
+
+Workload: create_hexagon_mesh(9; centroidal=true) (2,621,442 cells), TRiSK PV flux,
+cell kinetic energy, Earth parameters on the unit sphere, seeded synthetic state,
+RK4 dt = 45 s.  A "step" is one RK4 step of the whole mesh (four tendencies and the
+stage updates, 12 kernel launches) with the state resident in HBM.  The FP64 path and
+the FP32-bulk path (FP32 stencils, FP64 state/heights/accumulation) are timed in the
+same run; ``value`` is the FP32-bulk throughput in cell-steps/s.
+
+Prints one JSON line (rank 0).  The CPU baseline is the oracle (FP64 restatement of the
+reference kernels, OpenMP over the host cores) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12                      # B/s, MI355X HBM3E
+MODEL_BYTES = {"fp64": 4064.0, "fp32_bulk": 2704.0}   # algorithmic bytes per cell-step
+METRIC = "cell-steps/s at SCVT lvl9, FP32-bulk vs FP64 (same GPU) + % of HBM peak"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--level", type=int, default=9)
+    p.add_argument("--dt", type=float, default=45.0)
+    p.add_argument("--precision", choices=["both", "fp64", "fp32_bulk"], default="both")
+    p.add_argument("--cpu-steps", type=int, default=3, help="RK4 steps of the CPU baseline sample (0: skip)")
+    p.add_argument("--no-kernel-events", action="store_true")
+    return p.parse_args()
+
+
+def kernel_stats(m):
+    import ctypes as C
+
+    from dmplex_amd import _device as D
+
+    names = (C.c_char_p * 8)()
+    avg = np.zeros(8)
+    nl = np.zeros(8, dtype=np.int64)
+    cnt = C.c_int32()
+    D.check(D.lib().trisk_kernel_times(m.handle, 8, names, avg.ctypes.data_as(C.POINTER(C.c_double)),
+                                       nl.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(cnt)))
+    by = np.zeros(8)
+    cnt2 = C.c_int32()
+    D.check(D.lib().trisk_kernel_bytes(m.handle, 8, by.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cnt2)))
+    return [dict(name=names[i].decode(), avg_ms=float(avg[i]), launches=int(nl[i]), bytes=float(by[i]))
+            for i in range(cnt.value)]
+
+
+def load_traffic(kernel: str, precision: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(precision, {}).get(kernel)
+    except (ValueError, OSError):
+        return None
+
+
+def run_gpu(grid, y0, precision, args, params):
+    import torch
+
+    from dmplex_amd import HIPBackend, TRiSKShallowWater, diagnostics, rk4_steps_
+    from dmplex_amd import _device as D
+
+    log(f"[bench] building {precision} model")
+    m = TRiSKShallowWater(grid, backend=HIPBackend(0, precision), **params)
+    y = m.upload_state(y0)
+    mass0, energy0, _ = diagnostics(m, y)
+    rk4_steps_(y, args.dt, args.warmup, m)
+    torch.cuda.synchronize()
+    D.check(D.lib().trisk_set_timing(m.handle, 0 if args.no_kernel_events else 1))
+    stream = torch.cuda.current_stream()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter()
+    t0.record(stream)
+    rk4_steps_(y, args.dt, args.steps, m)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall
+    ms = t0.elapsed_time(t1)
+    D.check(D.lib().trisk_set_timing(m.handle, 0))
+    stats = kernel_stats(m) if not args.no_kernel_events else []
+    mass, energy, _ = diagnostics(m, y)
+    nc = grid.num_cells()
+    value = nc * args.steps / (ms / 1e3)
+    out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall,
+               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK,
+               mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0, kernels=stats)
+    if stats:
+        dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
+        achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
+        out["roofline"] = dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
+                               frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], precision),
+                               avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"])
+    log(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/step, {value:.3e} cell-steps/s, "
+        f"model HBM fraction {out['hbm_frac_model']:.3f}")
+    del m
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_cpu(grid, y0, args, params):
+    from dmplex_amd import ModelTables
+    from oracle.oracle import Oracle
+
+    t = ModelTables(grid, **params)
+    o = Oracle(t)
+    threads = o.threads()
+    y = y0.copy()
+    o.tendency(y)  # warm-up tendency, as the reference's timing driver does
+    wall = time.perf_counter()
+    o.rk4_steps(y, args.dt, args.cpu_steps)
+    wall = time.perf_counter() - wall
+    nc = grid.num_cells()
+    return dict(value=nc * args.cpu_steps / wall, unit="cell-steps/s", cores=threads, kind="port",
+                sample=f"{args.cpu_steps} RK4 steps of the level-{args.level} workload (FP64 oracle: the "
+                       f"reference's kernel decomposition, int64 column-major tables, OpenMP), {wall:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU halo exchange is "
+                                   "design-only)"}))
+        sys.exit(2)
+    import torch  # noqa: F401
+
+    from dmplex_amd import TRiSKGrid, create_hexagon_mesh
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+
+    t = time.perf_counter()
+    log(f"[bench] generating SCVT hexagon mesh level {args.level}")
+    dm = create_hexagon_mesh(args.level, centroidal=True)
+    grid = TRiSKGrid(dm, centers=dm.generators)
+    params = earth_parameters()
+    y0 = seeded_state(grid)
+    log(f"[bench] mesh + tables + state in {time.perf_counter() - t:.1f} s: "
+        f"{grid.num_cells()} cells, {grid.num_edges()} edges, {grid.num_vertices()} vertices")
+    precisions = ["fp64", "fp32_bulk"] if args.precision == "both" else [args.precision]
+    res = {p: run_gpu(grid, y0, p, args, params) for p in precisions}
+    head = res.get("fp32_bulk", res.get("fp64"))
+    head_p = "fp32_bulk" if "fp32_bulk" in res else "fp64"
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"[bench] CPU baseline: {args.cpu_steps} RK4 steps on the host")
+        cpu = run_cpu(grid, y0, args, params)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
+    line = {
+        "metric": METRIC, "value": head["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if head_p == "fp32_bulk" else "f64",
+        "data": "synthetic: seeded streamfunction state (l<=8 harmonics, max|u|=30 m/s, linear balance, "
+                "N(0,5 m) noise), Earth parameters on the unit sphere",
+        "config": {"workload": f"SCVT hexagon level {args.level}, TRiSK PV flux, cell KE, RK4 dt={args.dt} s",
+                   "cells": grid.num_cells(), "edges": grid.num_edges(), "vertices": grid.num_vertices(),
+                   "precision": head_p, "model_bytes_per_cell_step": MODEL_BYTES[head_p]},
+        "hbm_frac_model": head["hbm_frac_model"],
+        "roofline": head.get("roofline"),
+        "cpu_baseline": cpu,
+    }
+    for p, r in res.items():
+        line[p] = {k: v for k, v in r.items() if k != "kernels"}
+        line[p]["kernels"] = r["kernels"]
+    if "fp64" in res and "fp32_bulk" in res:
+        line["speedup_fp32_bulk_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

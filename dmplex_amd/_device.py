@@ -1,0 +1,84 @@
+"""ctypes binding of ``libtrisk_hip.so`` (include/trisk_hip.h).
+
+The library is the only compute path of the package: if it is missing, or no HIP
+device is visible, these calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from ._native import LIB_DIR, NativeError, f64p, i64, i64p
+
+vp = C.c_void_p
+
+
+class TriskModelDesc(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("kc", i64), ("kv", i64), ("kt", i64), ("kg", i64),
+                ("cell_edges", i64p), ("divergence", f64p), ("kinetic", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_height", f64p), ("inverse_dual", f64p),
+                ("vertex_edges", i64p), ("curl", f64p), ("vertex_cells", i64p), ("vertex_height", f64p),
+                ("coriolis", f64p), ("trisk_edges", i64p), ("trisk", f64p), ("cell_areas", f64p),
+                ("vertex_areas", f64p), ("gravity", C.c_double),
+                ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p), ("vk_height", f64p),
+                ("g_edges", i64p), ("g_pv_edges", i64p), ("g_weights", f64p), ("g_rhombus", f64p),
+                ("g_coriolis", f64p), ("pv_flux", C.c_int32), ("reserved", C.c_int32),
+                ("cell_points", f64p)]
+
+
+class TriskOptions(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_size", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+TRISK_FP64, TRISK_FP32_BULK = 0, 1
+PV_FLUX = {"trisk": 0, "gassmann": 1, "gassmann_mean": 2}
+
+# Every entry point declared in include/trisk_hip.h, with its argument types.
+SIGNATURES = {
+    "trisk_create": [C.POINTER(TriskModelDesc), C.POINTER(TriskOptions), C.POINTER(vp)],
+    "trisk_destroy": [vp],
+    "trisk_state_length": [vp, i64p],
+    "trisk_permutation": [vp, i64p, i64p, i64p],
+    "trisk_upload_state": [vp, f64p, vp, vp],
+    "trisk_download_state": [vp, vp, f64p, vp],
+    "trisk_download_field": [vp, C.c_int32, vp, f64p, vp],
+    "trisk_upload_field": [vp, C.c_int32, f64p, vp, vp],
+    "trisk_tendency": [vp, vp, vp, vp],
+    "trisk_rk4_steps": [vp, vp, C.c_double, i64, vp],
+    "trisk_diagnostics": [vp, vp, f64p, vp],
+    "trisk_kinetic_energy": [vp, vp, vp, vp, vp],
+    "trisk_vertex_pv": [vp, vp, vp, vp, vp, vp],
+    "trisk_relative_vorticity": [vp, vp, vp, vp],
+    "trisk_tangential_velocity": [vp, vp, vp, vp],
+    "trisk_set_timing": [vp, C.c_int32],
+    "trisk_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
+    "trisk_kernel_bytes": [vp, C.c_int32, f64p, C.POINTER(C.c_int32)],
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The device library (loaded once); raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        path = LIB_DIR / "libtrisk_hip.so"
+        if not path.exists():
+            raise NativeError(f"{path} is missing: run `python -m dmplex_amd.build`")
+        dll = C.CDLL(str(path))
+        dll.trisk_last_error.restype = C.c_char_p
+        dll.trisk_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            fn = getattr(dll, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = dll
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().trisk_last_error().decode()
+        if rc == -1:
+            raise ValueError(msg)
+        raise NativeError(f"libtrisk_hip call failed ({rc}): {msg}")

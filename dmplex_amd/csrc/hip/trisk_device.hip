@@ -1,0 +1,973 @@
+// gfx950 kernels and C ABI of the TRiSK shallow-water step (include/trisk_hip.h).
+//
+// One tendency (TRiSKShallowWater.jl:379-394) is three launches, each covering two
+// entity ranges (cells | vertices, edges | cells, edges | cells):
+//   P1 cell_vertex : K_i, B_i = K_i + g h_i  (kinetic_energy_kernel!, :202-210)
+//                    q_v = (zeta_v + f_v)/h_v, and zeta_v, h_v, K_v when needed
+//                    (vertex_pv_kernel! :246-259, vertex_kinetic_energy_kernel! :212-219)
+//   P2 edge        : F_e = h_e u_e and the edge PV q_e, stored as one (F, q) pair
+//                    (mass_flux_kernel! :232-244, edge_pv_kernel! :261-264,
+//                    rhombus_pv_kernel! :266-273); with vertex kinetic energy also the
+//                    cells' vertex part of K and B (cell_from_vertex_kernel! :222-230)
+//   P3 update      : du_e (momentum_kernel! :275-284 or gassmann_momentum_kernel!
+//                    :286-296) and dh_i (continuity_kernel! :298-305), each followed by
+//                    the RK4 stage update of rk4_step! (RungeKutta.jl:18-28) or a plain
+//                    store of the tendency.
+// Every gather sums its slots in the reference's table order; FP contraction is off, so
+// the FP64 path rounds exactly as the reference formulas (and the oracle) do.  Each
+// thread owns one entity (gathers only, no atomics); block -> tile mapping gives each
+// XCD contiguous ranges of the Hilbert-ordered entities so neighbour data is reused in
+// its L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "trisk_layout.hpp"
+
+namespace trisk {
+
+static thread_local std::string g_error;
+
+static int set_err(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kXcd = 8;
+
+// ---------------------------------------------------------------------------------
+// precision policies
+
+struct Fp64 {
+    using C = double;  // coefficients of the stencils
+    using I = double;  // intermediates F, q_e, q_v, zeta_v, h_v, K_v
+    using U = double;  // stage velocity
+    static constexpr int id = TRISK_FP64;
+};
+struct Fp32Bulk {
+    using C = float;
+    using I = float;
+    using U = float;
+    static constexpr int id = TRISK_FP32_BULK;
+};
+
+template <class I>
+struct Pair {
+    I F, q;
+};
+template <>
+struct alignas(16) Pair<double> {
+    double F, q;
+};
+template <>
+struct alignas(8) Pair<float> {
+    float F, q;
+};
+
+// ---------------------------------------------------------------------------------
+// kernel parameters (device pointers, entity-major tables)
+
+template <class P>
+struct Params {
+    using C = typename P::C;
+    using I = typename P::I;
+    int64_t nc, ne, nv;
+    int kc, kv, kt, kg;
+    double g;
+    // cells
+    const int32_t* ce;
+    const double* div;
+    const C* kin;
+    const int32_t* cv;
+    const C* vkk;
+    const double* area_c;
+    // edges
+    const int32_t* ec;
+    const int32_t* ev;
+    const C* eh;
+    const C* invd;
+    const int32_t* te;
+    const C* tw;
+    const C* vkh;
+    const int32_t* ge;
+    const int32_t* gp;
+    const C* gw;
+    const C* grh;
+    const C* gf;
+    // vertices
+    const int32_t* ve;
+    const C* curl;
+    const int32_t* vc;
+    const C* vh;
+    const C* fv;
+    const C* vkw;
+    const double* area_v;
+    // work
+    double* B;          // nc
+    double* K;          // nc (vertex KE or diagnostics)
+    Pair<I>* Fq;        // ne
+    I* qv;              // nv
+    I* zv;              // nv (Gassmann, diagnostics)
+    I* hv;              // nv (Gassmann, vertex KE, diagnostics)
+    I* Kv;              // nv (vertex KE)
+    // optional FP64 outputs of the diagnostics (NULL: not written)
+    double* K_out;
+    double* qv_out;
+    double* zv_out;
+    double* hv_out;
+};
+
+// Block b of a grid of kXcd * (t0 + t1) blocks -> (range, tile).  The hardware deals
+// blocks round-robin to the XCDs; XCD x receives tiles [x*t0, (x+1)*t0) of range 0
+// and [x*t1, (x+1)*t1) of range 1, contiguous in the Hilbert order.
+struct Tile {
+    int range;
+    int64_t first;  // first entity of the tile (may be past the range's end)
+};
+__device__ __forceinline__ Tile tile_of(int64_t b, int64_t t0, int64_t t1) {
+    int64_t x = b % kXcd, j = b / kXcd;
+    if (j < t0) return {0, (x * t0 + j) * kBlock};
+    return {1, (x * t1 + (j - t0)) * kBlock};
+}
+
+inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
+
+template <class T>
+__device__ __forceinline__ T ld(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// ---------------------------------------------------------------------------------
+// P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
+
+template <class P, class UIn, class HIn, bool VK, bool ZH>
+__global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const UIn* __restrict__ u,
+                                                             const HIn* __restrict__ h, int64_t t0, int64_t t1) {
+    using C = typename P::C;
+    using I = typename P::I;
+    Tile t = tile_of(blockIdx.x, t0, t1);
+    if (t.range == 0) {
+        int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        C s = 0;
+        for (int k = 0; k < p.kc; ++k) {
+            C ue = C(u[ld(p.ce + k * p.nc + i)]);
+            s += ld(p.kin + k * p.nc + i) * (ue * ue);
+        }
+        double hi = double(h[i]);
+        double K = double(s);
+        if (VK) {
+            p.K[i] = K;  // completed with the vertex part in P2
+            p.B[i] = K + p.g * hi;
+        } else {
+            p.B[i] = K + p.g * hi;
+            if (p.K_out) p.K_out[i] = K;
+        }
+    } else {
+        int64_t v = t.first + threadIdx.x;
+        if (v >= p.nv) return;
+        C z = 0, eta = 0;
+        for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
+        for (int k = 0; k < p.kv; ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
+        C q = (z + ld(p.fv + v)) / eta;
+        p.qv[v] = I(q);
+        if (ZH) {
+            p.zv[v] = I(z);
+            p.hv[v] = I(eta);
+        }
+        if (p.qv_out) {
+            p.qv_out[v] = double(q);
+            p.zv_out[v] = double(z);
+            p.hv_out[v] = double(eta);
+        }
+        if (VK) {
+            C s = 0;
+            for (int k = 0; k < p.kv; ++k) {
+                C ue = C(u[ld(p.ve + k * p.nv + v)]);
+                s += ld(p.vkw + k * p.nv + v) * (ue * ue);
+            }
+            p.Kv[v] = I(s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// P2: edges (F, q_e) | cells (vertex part of K, B; only with vertex kinetic energy)
+
+template <class P, class UIn, class HIn, bool VK, bool GM>
+__global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __restrict__ u,
+                                                      const HIn* __restrict__ h, int64_t t0, int64_t t1) {
+    using C = typename P::C;
+    using I = typename P::I;
+    Tile t = tile_of(blockIdx.x, t0, t1);
+    if (t.range == 0) {
+        int64_t e = t.first + threadIdx.x;
+        if (e >= p.ne) return;
+        int32_t c0 = ld(p.ec + e), c1 = ld(p.ec + p.ne + e);
+        int32_t v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
+        C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
+        if (VK) he = he + ld(p.vkh + e) * C(p.hv[v0]) + ld(p.vkh + p.ne + e) * C(p.hv[v1]);
+        C F = he * C(u[e]);
+        C q;
+        if (GM) {
+            C r0 = ld(p.grh + e), r1 = ld(p.grh + p.ne + e);
+            C z = r0 * C(p.zv[v0]) + r1 * C(p.zv[v1]);
+            C eta = r0 * C(p.hv[v0]) + r1 * C(p.hv[v1]);
+            q = (z + ld(p.gf + e)) / eta;
+        } else {
+            q = (C(p.qv[v0]) + C(p.qv[v1])) / 2;
+        }
+        p.Fq[e] = Pair<I>{I(F), I(q)};
+    } else {
+        if (!VK) return;
+        int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        C s = 0;
+        for (int k = 0; k < p.kc; ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[ld(p.cv + k * p.nc + i)]);
+        double K = p.K[i] + double(s);
+        p.B[i] = p.B[i] + double(s);
+        p.K[i] = K;
+        if (p.K_out) p.K_out[i] = K;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// P3: edges (du) | cells (dh), then the RK4 stage update
+
+struct Rk {
+    int mode;            // 0: store the tendency in dy; 1..4: RK4 stage
+    double* dy;          // mode 0
+    double* y;           // state
+    double* acc;         // k1 + 2k2 + 2k3 accumulator
+    void* su;            // stage velocity (ne, type U)
+    double* sh;          // stage height (nc)
+    double dt, dt2, dt6;
+};
+
+// y + c dt k and the accumulator, in the order of rk4_step!'s broadcasts
+template <class U>
+__device__ __forceinline__ void rk_update(const Rk& r, int64_t j, double k, U* stage) {
+    switch (r.mode) {
+        case 0:
+            r.dy[j] = k;
+            break;
+        case 1:
+            r.acc[j] = k;
+            stage[0] = U(r.y[j] + r.dt2 * k);
+            break;
+        case 2:
+            r.acc[j] = r.acc[j] + 2 * k;
+            stage[0] = U(r.y[j] + r.dt2 * k);
+            break;
+        case 3:
+            r.acc[j] = r.acc[j] + 2 * k;
+            stage[0] = U(r.y[j] + r.dt * k);
+            break;
+        default:
+            r.y[j] = r.y[j] + r.dt6 * (r.acc[j] + k);
+            break;
+    }
+}
+
+template <class P, bool GM, bool GMEAN>
+__global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64_t t0, int64_t t1) {
+    using C = typename P::C;
+    using I = typename P::I;
+    using U = typename P::U;
+    Tile t = tile_of(blockIdx.x, t0, t1);
+    if (t.range == 0) {
+        int64_t e = t.first + threadIdx.x;
+        if (e >= p.ne) return;
+        double du;
+        double grad = (p.B[ld(p.ec + p.ne + e)] - p.B[ld(p.ec + e)]) * double(ld(p.invd + e));
+        if (!GM) {
+            C qe = C(p.Fq[e].q);
+            C s = 0;
+            for (int k = 0; k < p.kt; ++k) {
+                Pair<I> f = p.Fq[ld(p.te + k * p.ne + e)];
+                s += ld(p.tw + k * p.ne + e) * C(f.F) * (qe + C(f.q));
+            }
+            du = double(s / 2) - grad;
+        } else {
+            C qe = C(p.Fq[e].q);
+            C s = 0;
+            for (int k = 0; k < p.kg; ++k) {
+                Pair<I> f = p.Fq[ld(p.ge + k * p.ne + e)];
+                C q = GMEAN ? (qe + C(f.q)) / 2 : C(p.Fq[ld(p.gp + k * p.ne + e)].q);
+                s += ld(p.gw + k * p.ne + e) * C(f.F) * q;
+            }
+            du = double(s) - grad;
+        }
+        rk_update<U>(r, e, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    } else {
+        int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        double s = 0.0;
+        for (int k = 0; k < p.kc; ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[ld(p.ce + k * p.nc + i)].F);
+        rk_update<double>(r, p.ne + i, -s, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// generic gather y_j = sum_k w[k, j] x[idx[k, j]] (edge_sum_kernel!, :307-314)
+
+__global__ __launch_bounds__(kBlock) void edge_sum_kernel(int64_t n, int K, const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ w, const double* __restrict__ x,
+                                                          double* __restrict__ out) {
+    int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s += w[k * n + j] * x[idx[k * n + j]];
+    out[j] = s;
+}
+
+// ---------------------------------------------------------------------------------
+// compensated sums: Neumaier per thread over a fixed chunk, then in thread order
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+// term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
+__global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
+                                                        const double* __restrict__ a, const double* __restrict__ b,
+                                                        double g, double* __restrict__ partial) {
+    __shared__ double ss[kBlock], cc[kBlock];
+    int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    int64_t lo = int64_t(blockIdx.x) * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0, c = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        double x;
+        if (term == 0)
+            x = area[i] * a[i];
+        else if (term == 1)
+            x = area[i] * a[i] * (b[i] + g * a[i] / 2);
+        else
+            x = area[i] * a[i] * (b[i] * b[i]);
+        neumaier(s, c, x);
+    }
+    ss[threadIdx.x] = s;
+    cc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = 0.0, Cc = 0.0;
+        for (int k = 0; k < kBlock; ++k) {
+            neumaier(S, Cc, ss[k]);
+            Cc += cc[k];
+        }
+        partial[2 * blockIdx.x] = S;
+        partial[2 * blockIdx.x + 1] = Cc;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// handle
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+template <class T>
+static int upload(std::vector<DevBuf>& owned, const std::vector<T>& v, const T** out) {
+    if (v.empty()) {
+        *out = nullptr;
+        return 0;
+    }
+    DevBuf b;
+    b.bytes = v.size() * sizeof(T);
+    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a table failed");
+    owned.push_back(b);
+    HIP_TRY(hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(b.p);
+    return 0;
+}
+
+static int alloc(std::vector<DevBuf>& owned, size_t bytes, void** out) {
+    DevBuf b;
+    b.bytes = bytes ? bytes : 16;
+    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a work vector failed");
+    owned.push_back(b);
+    *out = b.p;
+    return 0;
+}
+
+static std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+enum KernelId { K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_COUNT = 3 };
+static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel", "update_kernel"};
+
+}  // namespace trisk
+
+using namespace trisk;
+
+struct trisk_handle {
+    Layout L;
+    int precision = TRISK_FP64;
+    int device = 0;
+    std::vector<DevBuf> owned;
+    Params<Fp64> p64{};
+    Params<Fp32Bulk> p32{};
+    // RK4 work (acc, stage) and reduction scratch
+    double* acc = nullptr;
+    void* su = nullptr;    // stage velocity (ne, FP64 or FP32)
+    double* sh = nullptr;  // stage height (nc)
+    double* partial = nullptr;
+    int reduce_blocks = 1024;
+    double* diag_K = nullptr;
+    double* diag_qv = nullptr;
+    double* diag_zv = nullptr;
+    double* diag_hv = nullptr;
+    std::vector<double> host_stage;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;     // pairs (start, stop)
+    std::vector<int> event_kernel;      // kernel id per pair
+    size_t events_used = 0;
+    double kernel_ms[K_COUNT] = {0, 0, 0};
+    int64_t kernel_launches[K_COUNT] = {0, 0, 0};
+};
+
+namespace trisk {
+
+template <class P>
+static Params<P>& params(trisk_handle* h);
+template <>
+Params<Fp64>& params<Fp64>(trisk_handle* h) {
+    return h->p64;
+}
+template <>
+Params<Fp32Bulk>& params<Fp32Bulk>(trisk_handle* h) {
+    return h->p32;
+}
+
+template <class P>
+static int setup_params(trisk_handle* H) {
+    using C = typename P::C;
+    using I = typename P::I;
+    Layout& L = H->L;
+    Params<P>& p = params<P>(H);
+    std::vector<DevBuf>& o = H->owned;
+    p.nc = L.nc;
+    p.ne = L.ne;
+    p.nv = L.nv;
+    p.kc = L.kc;
+    p.kv = L.kv;
+    p.kt = L.kt;
+    p.kg = L.kg;
+    p.g = L.gravity;
+    auto coef = [&](const std::vector<double>& v, const C** out) -> int {
+        if constexpr (sizeof(C) == sizeof(double)) {
+            return upload(o, v, reinterpret_cast<const double**>(out));
+        } else {
+            return upload(o, to_float(v), reinterpret_cast<const float**>(out));
+        }
+    };
+    int rc = 0;
+    rc |= upload(o, L.ce, &p.ce);
+    rc |= upload(o, L.div, &p.div);
+    rc |= coef(L.kin, &p.kin);
+    rc |= upload(o, L.cv, &p.cv);
+    rc |= coef(L.vkk, &p.vkk);
+    rc |= upload(o, L.area_c, &p.area_c);
+    rc |= upload(o, L.ec, &p.ec);
+    rc |= upload(o, L.ev, &p.ev);
+    rc |= coef(L.eh, &p.eh);
+    rc |= coef(L.invd, &p.invd);
+    rc |= upload(o, L.te, &p.te);
+    rc |= coef(L.tw, &p.tw);
+    rc |= coef(L.vkh, &p.vkh);
+    rc |= upload(o, L.ge, &p.ge);
+    rc |= upload(o, L.gp, &p.gp);
+    rc |= coef(L.gw, &p.gw);
+    rc |= coef(L.grh, &p.grh);
+    rc |= coef(L.gf, &p.gf);
+    rc |= upload(o, L.ve, &p.ve);
+    rc |= coef(L.curl, &p.curl);
+    rc |= upload(o, L.vc, &p.vc);
+    rc |= coef(L.vh, &p.vh);
+    rc |= coef(L.fv, &p.fv);
+    rc |= coef(L.vkw, &p.vkw);
+    rc |= upload(o, L.area_v, &p.area_v);
+    if (rc) return rc;
+    void* q;
+    if ((rc = alloc(o, L.nc * sizeof(double), &q))) return rc;
+    p.B = static_cast<double*>(q);
+    if ((rc = alloc(o, L.nc * sizeof(double), &q))) return rc;
+    p.K = static_cast<double*>(q);
+    if ((rc = alloc(o, L.ne * sizeof(Pair<I>), &q))) return rc;
+    p.Fq = static_cast<Pair<I>*>(q);
+    for (I** w : {&p.qv, &p.zv, &p.hv, &p.Kv}) {
+        if ((rc = alloc(o, L.nv * sizeof(I), &q))) return rc;
+        *w = static_cast<I*>(q);
+    }
+    p.K_out = p.qv_out = p.zv_out = p.hv_out = nullptr;
+    // RK4 work
+    int64_t n = L.ne + L.nc;
+    if ((rc = alloc(o, n * sizeof(double), &q))) return rc;
+    H->acc = static_cast<double*>(q);
+    if ((rc = alloc(o, L.ne * sizeof(typename P::U), &q))) return rc;
+    H->su = q;
+    if ((rc = alloc(o, L.nc * sizeof(double), &q))) return rc;
+    H->sh = static_cast<double*>(q);
+    if ((rc = alloc(o, 2 * H->reduce_blocks * sizeof(double), &q))) return rc;
+    H->partial = static_cast<double*>(q);
+    for (double** w : {&H->diag_K, &H->diag_qv, &H->diag_zv, &H->diag_hv}) {
+        size_t cnt = w == &H->diag_K ? L.nc : L.nv;
+        if ((rc = alloc(o, cnt * sizeof(double), &q))) return rc;
+        *w = static_cast<double*>(q);
+    }
+    return 0;
+}
+
+static int timed_begin(trisk_handle* H, int kernel, hipStream_t s, hipEvent_t* stop) {
+    *stop = nullptr;
+    if (!H->timing) return 0;
+    if (H->events_used + 2 > H->events.size()) {
+        for (int k = 0; k < 256; ++k) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            H->events.push_back(e);
+        }
+        H->event_kernel.resize(H->events.size() / 2);
+    }
+    size_t i = H->events_used;
+    H->events_used += 2;
+    H->event_kernel[i / 2] = kernel;
+    HIP_TRY(hipEventRecord(H->events[i], s));
+    *stop = H->events[i + 1];
+    return 0;
+}
+
+static int timed_end(hipEvent_t stop, hipStream_t s) {
+    if (stop) HIP_TRY(hipEventRecord(stop, s));
+    return 0;
+}
+
+// One tendency: P1, P2, P3 with the given stage input and update mode.
+template <class P, class UIn>
+static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
+    const Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    const bool vk = L.vertex_ke, gm = L.gassmann;
+    const bool zh = gm || vk || p.qv_out != nullptr;
+    hipEvent_t stop;
+    int rc;
+    // P1
+    {
+        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
+        dim3 grid(unsigned(kXcd * (t0 + t1)));
+        if ((rc = timed_begin(H, K_CELL_VERTEX, s, &stop))) return rc;
+        if (vk)
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+                               t1);
+        else if (zh)
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true>), grid, dim3(kBlock), 0, s, p, u, h,
+                               t0, t1);
+        else
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, false>), grid, dim3(kBlock), 0, s, p, u, h,
+                               t0, t1);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    // P2
+    {
+        int64_t t0 = tiles_per_xcd(L.ne), t1 = vk ? tiles_per_xcd(L.nc) : 0;
+        dim3 grid(unsigned(kXcd * (t0 + t1)));
+        if ((rc = timed_begin(H, K_EDGE, s, &stop))) return rc;
+        if (vk && gm)
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+        else if (vk)
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+        else if (gm)
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, true>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+        else
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, false>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+                               t1);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    // P3
+    {
+        int64_t t0 = tiles_per_xcd(L.ne), t1 = tiles_per_xcd(L.nc);
+        dim3 grid(unsigned(kXcd * (t0 + t1)));
+        if ((rc = timed_begin(H, K_UPDATE, s, &stop))) return rc;
+        if (!gm)
+            hipLaunchKernelGGL((update_kernel<P, false, false>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        else if (L.gassmann_mean)
+            hipLaunchKernelGGL((update_kernel<P, true, true>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        else
+            hipLaunchKernelGGL((update_kernel<P, true, false>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    return 0;
+}
+
+template <class P>
+static int rk4_steps(trisk_handle* H, double* y, double dt, int64_t nsteps, hipStream_t s) {
+    using U = typename P::U;
+    const Layout& L = H->L;
+    Rk r{};
+    r.y = y;
+    r.acc = H->acc;
+    r.su = H->su;
+    r.sh = H->sh;
+    r.dt = dt;
+    r.dt2 = dt / 2;
+    r.dt6 = dt / 6;
+    int rc;
+    for (int64_t step = 0; step < nsteps; ++step) {
+        r.mode = 1;
+        if ((rc = run_tendency<P, double>(H, y, y + L.ne, r, s))) return rc;
+        for (int mode = 2; mode <= 4; ++mode) {
+            r.mode = mode;
+            if ((rc = run_tendency<P, U>(H, static_cast<const U*>(H->su), H->sh, r, s))) return rc;
+        }
+    }
+    return 0;
+}
+
+template <class P>
+static int tendency(trisk_handle* H, const double* y, double* dy, hipStream_t s) {
+    Rk r{};
+    r.mode = 0;
+    r.dy = dy;
+    return run_tendency<P, double>(H, y, y + H->L.ne, r, s);
+}
+
+// P1 (and with vertex kinetic energy the cell part of P2) with the FP64 diagnostic outputs
+template <class P>
+static int diagnostic_pass(trisk_handle* H, const double* y, double* K_out, double* qv_out, double* zv_out,
+                           double* hv_out, hipStream_t s) {
+    Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    p.K_out = K_out;
+    p.qv_out = qv_out;
+    p.zv_out = zv_out;
+    p.hv_out = hv_out;
+    const double* u = y;
+    const double* h = y + L.ne;
+    int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
+    if (L.vertex_ke)
+        hipLaunchKernelGGL((cell_vertex_kernel<P, double, double, true, true>), dim3(unsigned(kXcd * (t0 + t1))),
+                           dim3(kBlock), 0, s, p, u, h, t0, t1);
+    else
+        hipLaunchKernelGGL((cell_vertex_kernel<P, double, double, false, true>), dim3(unsigned(kXcd * (t0 + t1))),
+                           dim3(kBlock), 0, s, p, u, h, t0, t1);
+    hipError_t e1 = hipGetLastError();
+    if (e1 == hipSuccess && L.vertex_ke) {
+        int64_t e0 = tiles_per_xcd(L.ne), c1 = tiles_per_xcd(L.nc);
+        if (L.gassmann)
+            hipLaunchKernelGGL((edge_kernel<P, double, double, true, true>), dim3(unsigned(kXcd * (e0 + c1))),
+                               dim3(kBlock), 0, s, p, u, h, e0, c1);
+        else
+            hipLaunchKernelGGL((edge_kernel<P, double, double, true, false>), dim3(unsigned(kXcd * (e0 + c1))),
+                               dim3(kBlock), 0, s, p, u, h, e0, c1);
+        e1 = hipGetLastError();
+    }
+    p.K_out = p.qv_out = p.zv_out = p.hv_out = nullptr;
+    if (e1 != hipSuccess) return set_err(TRISK_EHIP, std::string("diagnostic pass: ") + hipGetErrorString(e1));
+    return 0;
+}
+
+static int reduce(trisk_handle* H, int term, int64_t n, const double* area, const double* a, const double* b,
+                  double* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(unsigned(H->reduce_blocks)), dim3(kBlock), 0, s, term, n, area, a, b,
+                       H->L.gravity, H->partial);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(2 * H->reduce_blocks);
+    HIP_TRY(hipMemcpyAsync(part.data(), H->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double S = 0.0, Cc = 0.0;
+    for (int b2 = 0; b2 < H->reduce_blocks; ++b2) {
+        double x = part[2 * b2], t = S + x;
+        if (std::fabs(S) >= std::fabs(x))
+            Cc += (S - t) + x;
+        else
+            Cc += (x - t) + S;
+        S = t;
+        Cc += part[2 * b2 + 1];
+    }
+    *out = S + Cc;
+    return 0;
+}
+
+}  // namespace trisk
+
+#define CHECK_HANDLE(h) \
+    if (!(h)) return set_err(TRISK_EINVAL, "null handle")
+
+extern "C" {
+
+const char* trisk_last_error(void) { return g_error.c_str(); }
+
+int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_handle** out) {
+    if (!out) return set_err(TRISK_EINVAL, "null output pointer");
+    *out = nullptr;
+    trisk_options o{};
+    if (opt) o = *opt;
+    if (o.precision != TRISK_FP64 && o.precision != TRISK_FP32_BULK)
+        return set_err(TRISK_EINVAL, "precision must be TRISK_FP64 or TRISK_FP32_BULK");
+    if (o.block_size != 0 && o.block_size != kBlock)
+        return set_err(TRISK_EINVAL, "block_size must be 0 or 256");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (o.device < 0 || o.device >= ndev) return set_err(TRISK_EINVAL, "no such HIP device");
+    HIP_TRY(hipSetDevice(o.device));
+    auto* H = new trisk_handle;
+    H->precision = o.precision;
+    H->device = o.device;
+    std::string err;
+    if (!build_layout(desc, H->L, err)) {
+        delete H;
+        return set_err(TRISK_EINVAL, err);
+    }
+    int rc = o.precision == TRISK_FP64 ? setup_params<Fp64>(H) : setup_params<Fp32Bulk>(H);
+    if (rc) {
+        for (auto& b : H->owned) (void)hipFree(b.p);
+        delete H;
+        return rc;
+    }
+    *out = H;
+    return 0;
+}
+
+int trisk_destroy(trisk_handle* h) {
+    if (!h) return 0;
+    (void)hipSetDevice(h->device);
+    for (auto& e : h->events) (void)hipEventDestroy(e);
+    for (auto& b : h->owned) (void)hipFree(b.p);
+    delete h;
+    return 0;
+}
+
+int trisk_state_length(const trisk_handle* h, int64_t* n) {
+    CHECK_HANDLE(h);
+    *n = h->L.ne + h->L.nc;
+    return 0;
+}
+
+int trisk_permutation(const trisk_handle* h, int64_t* cells, int64_t* edges, int64_t* vertices) {
+    CHECK_HANDLE(h);
+    if (cells) std::memcpy(cells, h->L.cperm.data(), h->L.nc * sizeof(int64_t));
+    if (edges) std::memcpy(edges, h->L.eperm.data(), h->L.ne * sizeof(int64_t));
+    if (vertices) std::memcpy(vertices, h->L.vperm.data(), h->L.nv * sizeof(int64_t));
+    return 0;
+}
+
+static const std::vector<int64_t>* perm_of(const trisk_handle* h, int kind) {
+    return kind == 0 ? &h->L.cperm : kind == 1 ? &h->L.eperm : kind == 2 ? &h->L.vperm : nullptr;
+}
+
+int trisk_upload_field(trisk_handle* h, int32_t kind, const double* x_host, double* x_dev, void* stream) {
+    CHECK_HANDLE(h);
+    const std::vector<int64_t>* perm = perm_of(h, kind);
+    if (!perm || !x_host || !x_dev) return set_err(TRISK_EINVAL, "bad field kind or pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<double> tmp(perm->size());
+    for (size_t j = 0; j < tmp.size(); ++j) tmp[j] = x_host[(*perm)[j]];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(x_dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int trisk_download_field(trisk_handle* h, int32_t kind, const double* x_dev, double* x_host, void* stream) {
+    CHECK_HANDLE(h);
+    const std::vector<int64_t>* perm = perm_of(h, kind);
+    if (!perm || !x_host || !x_dev) return set_err(TRISK_EINVAL, "bad field kind or pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    std::vector<double> tmp(perm->size());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), x_dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t j = 0; j < tmp.size(); ++j) x_host[(*perm)[j]] = tmp[j];
+    return 0;
+}
+
+int trisk_upload_state(trisk_handle* h, const double* y_host, double* y_dev, void* stream) {
+    CHECK_HANDLE(h);
+    int rc = trisk_upload_field(h, 1, y_host, y_dev, stream);
+    if (rc) return rc;
+    return trisk_upload_field(h, 0, y_host + h->L.ne, y_dev + h->L.ne, stream);
+}
+
+int trisk_download_state(trisk_handle* h, const double* y_dev, double* y_host, void* stream) {
+    CHECK_HANDLE(h);
+    int rc = trisk_download_field(h, 1, y_dev, y_host, stream);
+    if (rc) return rc;
+    return trisk_download_field(h, 0, y_dev + h->L.ne, y_host + h->L.ne, stream);
+}
+
+int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* stream) {
+    CHECK_HANDLE(h);
+    if (!y_dev || !dy_dev) return set_err(TRISK_EINVAL, "null state pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return h->precision == TRISK_FP64 ? tendency<Fp64>(h, y_dev, dy_dev, s) : tendency<Fp32Bulk>(h, y_dev, dy_dev, s);
+}
+
+int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, void* stream) {
+    CHECK_HANDLE(h);
+    if (!y_dev) return set_err(TRISK_EINVAL, "null state pointer");
+    if (nsteps < 0) return set_err(TRISK_EINVAL, "nsteps must be non-negative");
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->timing) h->events_used = 0;
+    int rc = h->precision == TRISK_FP64 ? rk4_steps<Fp64>(h, y_dev, dt, nsteps, s)
+                                        : rk4_steps<Fp32Bulk>(h, y_dev, dt, nsteps, s);
+    return rc;
+}
+
+int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = h->precision == TRISK_FP64
+                 ? diagnostic_pass<Fp64>(h, y_dev, K_dev ? K_dev : h->diag_K, h->diag_qv, h->diag_zv, h->diag_hv, s)
+                 : diagnostic_pass<Fp32Bulk>(h, y_dev, K_dev ? K_dev : h->diag_K, h->diag_qv, h->diag_zv,
+                                             h->diag_hv, s);
+    if (rc) return rc;
+    if (B_dev) {
+        const double* B = h->precision == TRISK_FP64 ? h->p64.B : h->p32.B;
+        HIP_TRY(hipMemcpyAsync(B_dev, B, h->L.nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    return 0;
+}
+
+int trisk_vertex_pv(trisk_handle* h, const double* y_dev, double* qv_dev, double* zv_dev, double* hv_dev,
+                    void* stream) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* q = qv_dev ? qv_dev : h->diag_qv;
+    double* z = zv_dev ? zv_dev : h->diag_zv;
+    double* e = hv_dev ? hv_dev : h->diag_hv;
+    return h->precision == TRISK_FP64 ? diagnostic_pass<Fp64>(h, y_dev, h->diag_K, q, z, e, s)
+                                      : diagnostic_pass<Fp32Bulk>(h, y_dev, h->diag_K, q, z, e, s);
+}
+
+int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void* stream) {
+    CHECK_HANDLE(h);
+    if (!y_dev || !out) return set_err(TRISK_EINVAL, "null pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Layout& L = h->L;
+    int rc = trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);  // also fills diag_K
+    if (rc) return rc;
+    const double* area_c = h->precision == TRISK_FP64 ? h->p64.area_c : h->p32.area_c;
+    const double* area_v = h->precision == TRISK_FP64 ? h->p64.area_v : h->p32.area_v;
+    if ((rc = reduce(h, 0, L.nc, area_c, y_dev + L.ne, nullptr, &out[0], s))) return rc;
+    if ((rc = reduce(h, 1, L.nc, area_c, y_dev + L.ne, h->diag_K, &out[1], s))) return rc;
+    if ((rc = reduce(h, 2, L.nv, area_v, h->diag_hv, h->diag_qv, &out[2], s))) return rc;
+    out[2] /= 2;
+    return 0;
+}
+
+int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_dev, void* stream) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Layout& L = h->L;
+    // FP64 curl table: the FP64 parameters exist only in FP64 mode, so gather from host copies
+    const int32_t* idx = h->precision == TRISK_FP64 ? h->p64.ve : h->p32.ve;
+    static_assert(sizeof(double) == 8, "");
+    if (h->precision == TRISK_FP64) {
+        hipLaunchKernelGGL(edge_sum_kernel, dim3(unsigned((L.nv + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, L.nv,
+                           L.kv, idx, h->p64.curl, y_dev, zeta_dev);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    return trisk_vertex_pv(h, y_dev, nullptr, zeta_dev, nullptr, stream);
+}
+
+int trisk_tangential_velocity(trisk_handle* h, const double* u_dev, double* vt_dev, void* stream) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Layout& L = h->L;
+    if (h->precision != TRISK_FP64)
+        return set_err(TRISK_EINVAL, "tangential_velocity is available on FP64 handles");
+    hipLaunchKernelGGL(edge_sum_kernel, dim3(unsigned((L.ne + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, L.ne, L.kt,
+                       h->p64.te, h->p64.tw, u_dev, vt_dev);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int trisk_set_timing(trisk_handle* h, int32_t on) {
+    CHECK_HANDLE(h);
+    h->timing = on != 0;
+    h->events_used = 0;
+    return 0;
+}
+
+int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double* avg_ms, int64_t* launches,
+                       int32_t* count) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    double ms[K_COUNT] = {0, 0, 0};
+    int64_t n[K_COUNT] = {0, 0, 0};
+    for (size_t i = 0; i + 1 < h->events_used; i += 2) {
+        HIP_TRY(hipEventSynchronize(h->events[i + 1]));
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, h->events[i], h->events[i + 1]));
+        int k = h->event_kernel[i / 2];
+        ms[k] += t;
+        n[k] += 1;
+    }
+    int c = 0;
+    for (int k = 0; k < K_COUNT && c < max; ++k, ++c) {
+        if (names) names[c] = kKernelNames[k];
+        if (avg_ms) avg_ms[c] = n[k] ? ms[k] / double(n[k]) : 0.0;
+        if (launches) launches[c] = n[k];
+    }
+    if (count) *count = c;
+    return 0;
+}
+
+int trisk_kernel_bytes(trisk_handle* h, int32_t max, double* bytes, int32_t* count) {
+    CHECK_HANDLE(h);
+    const Layout& L = h->L;
+    const double c = h->precision == TRISK_FP64 ? 8.0 : 4.0;  // coefficient / intermediate bytes
+    const double iw = 4.0;                                     // index bytes
+    const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv);
+    // each element of every array the kernel must touch, counted once (stage 2-3 launches)
+    double u_in = h->precision == TRISK_FP64 ? 8.0 : 4.0;
+    double p1 = nc * L.kc * (iw + c) + nc * 8.0 /*h*/ + ne * u_in /*u*/ + nc * 8.0 /*B*/ +
+                nv * L.kv * 2 * (iw + c) + nv * c /*f*/ + nv * c /*q_v*/;
+    double p2 = ne * 2 * (iw + c) + ne * 2 * iw /*edge cells, vertices*/ + ne * u_in + nc * 8.0 + nv * c +
+                ne * 2 * c /*(F, q)*/;
+    double p3 = ne * L.kt * (iw + c) + ne * 2 * iw + ne * c /*1/d*/ + ne * 2 * c /*(F, q) read*/ + nc * 8.0 /*B*/ +
+                nc * L.kc * (iw + 8.0) +
+                /* RK4 stage 2/3: y read, acc read+write, stage write */
+                ne * (8.0 + 16.0 + u_in) + nc * (8.0 + 16.0 + 8.0);
+    if (L.vertex_ke || L.gassmann) {
+        // not modelled beyond the TRiSK/cell-KE tables
+    }
+    double v[K_COUNT] = {p1, p2, p3};
+    int k = 0;
+    for (; k < K_COUNT && k < max; ++k) bytes[k] = v[k];
+    if (count) *count = k;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+// Host-side re-layout of the reference's TRiSK tables for the device.
+//
+// Input: the fields of TRiSKShallowWater (src/FiniteVolumes/TRiSKShallowWater.jl:67-92)
+// as k x n column-major tables.  Output: cells renumbered along the 3-D Hilbert curve of
+// `cell_points` (hilbert_permutation, src/Topology/Ordering.jl:85-92), edges by first
+// appearance in the new cells' edge lists and vertices by first appearance in the new
+// edges' vertex pairs (the rule of hilbert_order, Ordering.jl:54-83), every table
+// entity-major with int32 indices.  Per-entity slot order is unchanged, so each
+// gather sums its terms in the reference's order.
+#include "trisk_layout.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace trisk {
+namespace {
+
+// 3-D Hilbert index of an integer point, Skilling's (2004) transpose form.
+uint64_t hilbert_key(uint64_t X0, uint64_t X1, uint64_t X2, int bits) {
+    uint64_t X[3] = {X0, X1, X2};
+    const uint64_t M = uint64_t(1) << (bits - 1);
+    for (uint64_t Q = M; Q > 1; Q >>= 1) {
+        uint64_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                uint64_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint64_t t = 0;
+    for (uint64_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (auto& x : X) x ^= t;
+    uint64_t key = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1);
+    return key;
+}
+
+void hilbert_cells(int64_t n, const double* pts, std::vector<int64_t>& perm) {
+    const int bits = 16;
+    double lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) lo[i] = hi[i] = pts[i];
+    for (int64_t p = 0; p < n; ++p)
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = std::min(lo[i], pts[3 * p + i]);
+            hi[i] = std::max(hi[i], pts[3 * p + i]);
+        }
+    double scale[3];
+    for (int i = 0; i < 3; ++i) scale[i] = (std::ldexp(1.0, bits) - 1) / std::max(hi[i] - lo[i], 2.220446049250313e-16);
+    std::vector<uint64_t> keys(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        uint64_t q[3];
+        for (int i = 0; i < 3; ++i) q[i] = uint64_t(std::nearbyint((pts[3 * p + i] - lo[i]) * scale[i]));
+        keys[p] = hilbert_key(q[0], q[1], q[2], bits);
+    }
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), int64_t(0));
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+}
+
+bool in_range(const int64_t* t, int64_t count, int64_t n) {
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < count; ++i) bad |= (t[i] < 0 || t[i] >= n);
+    return !bad;
+}
+
+// new table[k * n + jnew] = map[old[k + K * perm[jnew]]]
+void remap_index(const int64_t* old, int K, int64_t n, const std::vector<int64_t>& perm,
+                 const std::vector<int64_t>& map, std::vector<int32_t>& out) {
+    out.resize(size_t(K) * n);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j)
+        for (int k = 0; k < K; ++k) out[size_t(k) * n + j] = int32_t(map[old[k + K * perm[j]]]);
+}
+
+void remap_value(const double* old, int K, int64_t n, const std::vector<int64_t>& perm, std::vector<double>& out) {
+    out.resize(size_t(K) * n);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j)
+        for (int k = 0; k < K; ++k) out[size_t(k) * n + j] = old[k + K * perm[j]];
+}
+
+void invert(const std::vector<int64_t>& perm, std::vector<int64_t>& inv) {
+    inv.assign(perm.size(), -1);
+    for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = int64_t(i);
+}
+
+}  // namespace
+
+bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
+    if (!d) {
+        err = "null model descriptor";
+        return false;
+    }
+    if (d->nc <= 0 || d->ne <= 0 || d->nv <= 0 || d->kc <= 0 || d->kv <= 0 || d->kt <= 0 ||
+        d->nc >= (int64_t(1) << 31) || d->ne >= (int64_t(1) << 31) || d->nv >= (int64_t(1) << 31) ||
+        d->kc > 16 || d->kv > 16 || d->kt > 32) {
+        err = "bad sizes or table widths";
+        return false;
+    }
+    const bool vk = d->cell_vertices != nullptr;
+    const bool gm = d->pv_flux != TRISK_PV_TRISK;
+    if (d->pv_flux < 0 || d->pv_flux > 2) {
+        err = "pv_flux must be TRISK_PV_TRISK, TRISK_PV_GASSMANN or TRISK_PV_GASSMANN_MEAN";
+        return false;
+    }
+    if (!d->cell_edges || !d->divergence || !d->kinetic || !d->edge_cells || !d->edge_vertices ||
+        !d->edge_height || !d->inverse_dual || !d->vertex_edges || !d->curl || !d->vertex_cells ||
+        !d->vertex_height || !d->coriolis || !d->trisk_edges || !d->trisk || !d->cell_areas ||
+        !d->vertex_areas) {
+        err = "missing table";
+        return false;
+    }
+    if (vk && (!d->vk_kites || !d->vk_weights || !d->vk_height)) {
+        err = "vertex kinetic energy needs cell_vertices, vk_kites, vk_weights and vk_height";
+        return false;
+    }
+    if (gm && (d->kg <= 0 || d->kg > 16 || !d->g_edges || !d->g_pv_edges || !d->g_weights || !d->g_rhombus ||
+               !d->g_coriolis)) {
+        err = "the Gassmann PV flux needs g_edges, g_pv_edges, g_weights, g_rhombus and g_coriolis";
+        return false;
+    }
+    const int64_t nc = d->nc, ne = d->ne, nv = d->nv;
+    const int KC = int(d->kc), KV = int(d->kv), KT = int(d->kt), KG = gm ? int(d->kg) : 0;
+    struct Check {
+        const int64_t* t;
+        int64_t count, n;
+        const char* name;
+    } checks[] = {{d->cell_edges, KC * nc, ne, "cell_edges"},       {d->edge_cells, 2 * ne, nc, "edge_cells"},
+                  {d->edge_vertices, 2 * ne, nv, "edge_vertices"}, {d->vertex_edges, KV * nv, ne, "vertex_edges"},
+                  {d->vertex_cells, KV * nv, nc, "vertex_cells"},  {d->trisk_edges, KT * ne, ne, "trisk_edges"},
+                  {vk ? d->cell_vertices : nullptr, KC * nc, nv, "cell_vertices"},
+                  {gm ? d->g_edges : nullptr, KG * ne, ne, "g_edges"},
+                  {gm ? d->g_pv_edges : nullptr, KG * ne, ne, "g_pv_edges"}};
+    for (auto& c : checks)
+        if (c.t && !in_range(c.t, c.count, c.n)) {
+            err = std::string("index out of range in ") + c.name;
+            return false;
+        }
+
+    L.nc = nc;
+    L.ne = ne;
+    L.nv = nv;
+    L.kc = KC;
+    L.kv = KV;
+    L.kt = KT;
+    L.kg = KG;
+    L.vertex_ke = vk;
+    L.gassmann = gm;
+    L.gassmann_mean = d->pv_flux == TRISK_PV_GASSMANN_MEAN;
+    L.gravity = d->gravity;
+
+    // cells along the Hilbert curve; edges and vertices by first appearance
+    if (d->cell_points) {
+        hilbert_cells(nc, d->cell_points, L.cperm);
+    } else {
+        L.cperm.resize(nc);
+        std::iota(L.cperm.begin(), L.cperm.end(), int64_t(0));
+    }
+    std::vector<char> seen(ne, 0);
+    L.eperm.clear();
+    L.eperm.reserve(ne);
+    for (int64_t j = 0; j < nc; ++j) {
+        int64_t i = L.cperm[j];
+        for (int k = 0; k < KC; ++k) {
+            if (d->divergence[k + KC * i] == 0.0) continue;  // padded slot
+            int64_t e = d->cell_edges[k + KC * i];
+            if (!seen[e]) {
+                seen[e] = 1;
+                L.eperm.push_back(e);
+            }
+        }
+    }
+    for (int64_t e = 0; e < ne; ++e)
+        if (!seen[e]) L.eperm.push_back(e);
+    std::vector<char> vseen(nv, 0);
+    L.vperm.clear();
+    L.vperm.reserve(nv);
+    for (int64_t j = 0; j < ne; ++j)
+        for (int s = 0; s < 2; ++s) {
+            int64_t v = d->edge_vertices[s + 2 * L.eperm[j]];
+            if (!vseen[v]) {
+                vseen[v] = 1;
+                L.vperm.push_back(v);
+            }
+        }
+    for (int64_t v = 0; v < nv; ++v)
+        if (!vseen[v]) L.vperm.push_back(v);
+    invert(L.cperm, L.cinv);
+    invert(L.eperm, L.einv);
+    invert(L.vperm, L.vinv);
+
+    remap_index(d->cell_edges, KC, nc, L.cperm, L.einv, L.ce);
+    remap_value(d->divergence, KC, nc, L.cperm, L.div);
+    remap_value(d->kinetic, KC, nc, L.cperm, L.kin);
+    remap_value(d->cell_areas, 1, nc, L.cperm, L.area_c);
+    remap_index(d->edge_cells, 2, ne, L.eperm, L.cinv, L.ec);
+    remap_index(d->edge_vertices, 2, ne, L.eperm, L.vinv, L.ev);
+    remap_value(d->edge_height, 2, ne, L.eperm, L.eh);
+    remap_value(d->inverse_dual, 1, ne, L.eperm, L.invd);
+    remap_index(d->trisk_edges, KT, ne, L.eperm, L.einv, L.te);
+    remap_value(d->trisk, KT, ne, L.eperm, L.tw);
+    remap_index(d->vertex_edges, KV, nv, L.vperm, L.einv, L.ve);
+    remap_value(d->curl, KV, nv, L.vperm, L.curl);
+    remap_index(d->vertex_cells, KV, nv, L.vperm, L.cinv, L.vc);
+    remap_value(d->vertex_height, KV, nv, L.vperm, L.vh);
+    remap_value(d->coriolis, 1, nv, L.vperm, L.fv);
+    remap_value(d->vertex_areas, 1, nv, L.vperm, L.area_v);
+    if (vk) {
+        remap_index(d->cell_vertices, KC, nc, L.cperm, L.vinv, L.cv);
+        remap_value(d->vk_kites, KC, nc, L.cperm, L.vkk);
+        remap_value(d->vk_weights, KV, nv, L.vperm, L.vkw);
+        remap_value(d->vk_height, 2, ne, L.eperm, L.vkh);
+    }
+    if (gm) {
+        remap_index(d->g_edges, KG, ne, L.eperm, L.einv, L.ge);
+        remap_index(d->g_pv_edges, KG, ne, L.eperm, L.einv, L.gp);
+        remap_value(d->g_weights, KG, ne, L.eperm, L.gw);
+        remap_value(d->g_rhombus, 2, ne, L.eperm, L.grh);
+        remap_value(d->g_coriolis, 1, ne, L.eperm, L.gf);
+    }
+    return true;
+}
+
+}  // namespace trisk

@@ -1,0 +1,158 @@
+"""Mesh generators of the TRiSK path: icosahedral, SCVT (Lloyd) and hexagon meshes.
+
+Mirrors ``src/Grids/Icosahedral.jl``, ``src/Grids/Hexagonal.jl`` and
+``src/Grids/CellCenters.jl``; the loops run in ``libdmplex_host.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .topology import Plex, plex_from_polygons
+
+
+def _icosahedral(subdivisions: int, centroidal: bool):
+    if subdivisions < 0:
+        raise ValueError("subdivisions must be non-negative")
+    nv, nt = 10 * 4 ** subdivisions + 2, 20 * 4 ** subdivisions
+    x = np.empty((nv, 3))
+    tri = np.empty((nt, 3), dtype=np.int64)
+    its = np.zeros(max(subdivisions, 1), dtype=np.int64)
+    N.check(N.host().dmx_icosahedral(subdivisions, int(centroidal), N.ptr(x, N.f64p), N.ptr(tri, N.i64p),
+                                     N.ptr(its, N.i64p)))
+    return x, tri, its[:subdivisions]
+
+
+def create_spherical_mesh(subdivisions: int, spring_beta=None, centroidal: bool = False) -> Plex:
+    """Icosahedral triangle mesh on the unit sphere (Icosahedral.jl:22-61).
+
+    ``centroidal=True`` runs Lloyd's algorithm after every bisection (SCVT).  The spring
+    dynamics option of the reference (``spring_beta``) is not part of this path.
+    """
+    if spring_beta is not None:
+        raise NotImplementedError("spring dynamics is outside the TRiSK SCVT path")
+    x, tri, _ = _icosahedral(subdivisions, centroidal)
+    return plex_from_polygons((np.arange(0, 3 * len(tri) + 1, 3, dtype=np.int64), tri.reshape(-1)), x)
+
+
+def create_hexagon_mesh(subdivisions: int, spring_beta=None, centroidal: bool = False,
+                        primal: Plex | None = None) -> Plex:
+    """Voronoi dual of the icosahedral mesh (Hexagonal.jl:11-17).
+
+    Triangle circumcenters become vertices and every triangle vertex a polygon; the
+    result is an exact spherical Voronoi diagram whose generators are the icosahedral
+    vertices (kept as ``dm.generators``).
+    """
+    if primal is None:
+        primal = create_spherical_mesh(subdivisions, spring_beta, centroidal)
+    pv = primal.view()
+    off = np.empty(primal.nv + 1, dtype=np.int64)
+    verts = np.empty(3 * primal.nc, dtype=np.int64)
+    centers = np.empty((primal.nc, 3))
+    N.check(N.host().dmx_dual_polygons(C.byref(pv), N.ptr(off, N.i64p), N.ptr(verts, N.i64p),
+                                       N.ptr(centers, N.f64p)))
+    return plex_from_polygons((off, verts), centers, generators=primal.coordinates.copy())
+
+
+def lloyd_(x: np.ndarray, triangles: np.ndarray, fixed: int = 0, tolerance: float = 1e-8,
+           history: int = 16, max_iterations: int = 10_000) -> int:
+    """Lloyd's algorithm with Anderson mixing, in place (Icosahedral.jl:126-184).
+
+    ``fixed`` is the number of leading vertices that do not move (the reference's
+    ``fixed=1:12`` is ``fixed=12``).  Returns the number of iterations; raises
+    ``ValueError`` if the triangulation does not stay Delaunay.
+    """
+    if x.dtype != np.float64 or not x.flags.c_contiguous:
+        raise ValueError("x must be a C-contiguous float64 (n, 3) array")
+    tri = np.ascontiguousarray(triangles, dtype=np.int64)
+    its = np.zeros(1, dtype=np.int64)
+    N.check(N.host().dmx_lloyd(N.ptr(x, N.f64p), len(x), N.ptr(tri, N.i64p), len(tri), fixed, tolerance,
+                               history, max_iterations, N.ptr(its, N.i64p)), warn_ok=True)
+    return int(its[0])
+
+
+def lloyd_residual(x: np.ndarray, triangles: np.ndarray, fixed: int = 0) -> float:
+    """Largest distance of a free vertex from its Voronoi centroid (lloyd_step!)."""
+    tri = np.ascontiguousarray(triangles, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    r = C.c_double()
+    N.check(N.host().dmx_lloyd_residual(N.ptr(x, N.f64p), len(x), N.ptr(tri, N.i64p), len(tri), fixed,
+                                        C.byref(r)))
+    return r.value
+
+
+def orthogonality_defect(dm: Plex, centers) -> float:
+    """Largest |cos| between a primal edge and the dual great circle (CellCenters.jl:90-101)."""
+    c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+    out = C.c_double()
+    N.check(N.host().dmx_orthogonality_defect(C.byref(dm.view()), N.ptr(c, N.f64p), C.byref(out)))
+    return out.value
+
+
+def _normalize(a: np.ndarray) -> np.ndarray:
+    return a / np.linalg.norm(a, axis=-1, keepdims=True)
+
+
+def voronoi_generators(dm: Plex, iterations: int = 4) -> np.ndarray:
+    """Generators of the best-fitting Voronoi diagram by inverse iteration on the
+    homogeneous bisector conditions (CellCenters.jl:49-81), with a sparse LU solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    rows, cols, vals = [], [], []
+    nrow = 0
+    for e in dm.edge_range():
+        left, right = dm.get_edge_cells(e)
+        if left < 0 or right < 0:
+            continue
+        a, b = (dm.get_coordinates(v) for v in dm.get_cone(e))
+        n = _normalize(np.cross(a, b))
+        t1 = _normalize(np.cross(n, np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])))
+        t2 = np.cross(n, t1)
+        for w, sj in ((t1, -1), (t2, -1), (n, 1)):
+            for k in range(3):
+                rows += [nrow, nrow]
+                cols += [3 * left + k, 3 * right + k]
+                vals += [w[k], sj * w[k]]
+            nrow += 1
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(nrow, 3 * dm.nc))
+    B = (A.T @ A).tocsc()
+    B = B + 1e-10 * abs(B.diagonal()).max() * sp.identity(B.shape[0], format="csc")
+    lu = spl.splu(B)
+    centroids = cell_centers(dm, method="centroid")
+    x = centroids.reshape(-1).copy()
+    for _ in range(iterations):
+        x = lu.solve(x)
+        x /= np.linalg.norm(x)
+    gens = _normalize(x.reshape(-1, 3))
+    if np.sum(np.einsum("ij,ij->i", gens, centroids)) < 0:
+        gens = -gens
+    return gens
+
+
+def cell_centers(dm: Plex, method: str = "auto", tolerance: float = 1e-8) -> np.ndarray:
+    """Cell centers that coincide with the orthogonal dual's vertices (CellCenters.jl:25-47)."""
+    if method == "auto":
+        sizes = np.diff(dm.cone_offsets[:dm.nc + 1])
+        if np.all(sizes == 3):
+            return cell_centers(dm, method="circumcenter")
+        gens = dm.generators if dm.generators is not None else cell_centers(dm, method="voronoi")
+        if orthogonality_defect(dm, gens) < tolerance:
+            return np.array(gens, dtype=np.float64)
+        return cell_centers(dm, method="centroid")
+    if method == "centroid":
+        out = np.empty((dm.nc, 3))
+        for c in dm.cell_range():
+            out[c] = _normalize(dm.cell_coordinates(c).sum(axis=0))
+        return out
+    if method == "circumcenter":
+        tri = dm.cell_vertex_table()
+        if tri.shape[1] != 3:
+            raise ValueError("circumcenters need a triangle mesh")
+        a, b, d = (dm.coordinates[tri[:, k]] for k in range(3))
+        return _normalize(np.cross(b - a, d - a))
+    if method == "voronoi":
+        return voronoi_generators(dm)
+    raise ValueError(f"unknown center method {method}")

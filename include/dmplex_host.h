@@ -1,0 +1,161 @@
+/*
+ * dmplex_host.h -- C ABI of libdmplex_host.so, the host-side (CPU, OpenMP) setup
+ * code of the TRiSK shallow-water path: mesh generation, the Plex topology, the
+ * TRiSK geometry tables, Hilbert keys and the seeded synthetic initial state.
+ *
+ * In the Julia drop-in these steps stay in the reference package (they run once,
+ * before the hot loop); this library restates them natively so that the Python
+ * host mirror and the tests can build level-9/10 meshes in seconds.
+ *
+ * Conventions (all 0-based):
+ *   - Plex points are numbered cells [0, nc), edges [nc, nc+ne), vertices
+ *     [nc+ne, nc+ne+nv) (src/Topology/Plex.jl:5-10, shifted to 0-based).
+ *   - Tables are "k x n column-major": entry (k, j) of an n-entity table of
+ *     width K lives at k + j*K, which is the memory layout of a Julia k x n
+ *     Matrix (src/FiniteVolumes/TRiSKGrid.jl:28-31).  Padded slots hold index 0
+ *     and weight 0 (the reference's index 1 / weight 0).
+ *   - Points (coordinates, centers, edge points, normals) are n x 3, one
+ *     contiguous xyz triple per point.
+ * Every function returns 0 on success, a negative code on error; the message is
+ * available from dmx_last_error().
+ */
+#ifndef DMPLEX_HOST_H
+#define DMPLEX_HOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread-local text of the last error. */
+const char* dmx_last_error(void);
+
+/* --- icosahedral mesh, Lloyd SCVT ------------------------------------------------ */
+
+/* Replaces create_spherical_mesh (src/Grids/Icosahedral.jl:22-61) up to the final
+ * plex_from_polygons call: bisection of the icosahedron `subdivisions` times, with
+ * Lloyd's algorithm after every level when centroidal != 0.  Outputs
+ * nv = 10*4^n + 2 vertices (coords, nv x 3) and nt = 20*4^n triangles (nt x 3).
+ * iterations (may be NULL) receives the Lloyd iteration count of each level
+ * (array of length subdivisions). */
+int dmx_icosahedral(int subdivisions, int centroidal, double* coords, int64_t* triangles,
+                    int64_t* iterations);
+
+/* Replaces lloyd! (src/Grids/Icosahedral.jl:126-184), in place on x (nv x 3) for the
+ * counter-clockwise triangulation tri (nt x 3).  Vertices [0, nfixed) stay fixed.
+ * history = 0 is plain Lloyd.  Returns the iteration count in *iterations;
+ * error -3 if the result is not Delaunay (the reference's ArgumentError). */
+int dmx_lloyd(double* x, int64_t nv, const int64_t* tri, int64_t nt, int64_t nfixed,
+              double tolerance, int history, int64_t max_iterations, int64_t* iterations);
+
+/* Lloyd step residual only: largest |centroid - x| over free vertices (lloyd_step!,
+ * src/Grids/Icosahedral.jl:186-207), without moving x. */
+int dmx_lloyd_residual(const double* x, int64_t nv, const int64_t* tri, int64_t nt,
+                       int64_t nfixed, double* residual);
+
+/* --- Plex -------------------------------------------------------------------------- */
+
+typedef struct dmx_plex_build dmx_plex_build;
+
+/* Replaces plex_from_polygons (src/Topology/Plex.jl:49-97) on the unit sphere:
+ * polygon c has vertices poly_vertices[poly_offsets[c] .. poly_offsets[c+1]).
+ * Cells given clockwise are reversed; edges are numbered by first appearance.
+ * dims[0..2] receive nc, ne, nv and dims[3] the cone array length. */
+int dmx_plex_from_polygons(int64_t npoly, const int64_t* poly_offsets, const int64_t* poly_vertices,
+                           int64_t nv, const double* coords, dmx_plex_build** out, int64_t dims[4]);
+/* Copies the CSR arrays (npoints+1 offsets, dims[3] cones/orientations/supports). */
+int dmx_plex_export(const dmx_plex_build* b, int64_t* cone_offsets, int64_t* cones,
+                    int8_t* orientations, int64_t* support_offsets, int64_t* supports);
+void dmx_plex_free(dmx_plex_build* b);
+
+/* Read-only view of a Plex held by the caller. */
+typedef struct {
+    int64_t nc, ne, nv;
+    const int64_t* cone_offsets;     /* npoints + 1 */
+    const int64_t* cones;
+    const int8_t* orientations;      /* 0 or -1 per cone entry */
+    const int64_t* support_offsets;  /* npoints + 1 */
+    const int64_t* supports;
+    const double* coordinates;       /* nv x 3 */
+} dmx_plex;
+
+/* Polygons of the Voronoi dual of a triangle Plex (create_hexagon_mesh,
+ * src/Grids/Hexagonal.jl:11-17 with dual_polygons/sort_ring,
+ * src/Grids/FileGrids.jl:28-42): poly_offsets (nv+1), poly_vertices (3*nt) and
+ * centers (nt x 3, the circumcenters, cell_centers(...; method=:circumcenter),
+ * src/Grids/CellCenters.jl:37-42). */
+int dmx_dual_polygons(const dmx_plex* primal, int64_t* poly_offsets, int64_t* poly_vertices,
+                      double* centers);
+
+/* Largest |cos| between primal edges and the great circles of their cell centers
+ * (orthogonality_defect, src/Grids/CellCenters.jl:90-101). */
+int dmx_orthogonality_defect(const dmx_plex* dm, const double* centers, double* defect);
+
+/* Checks the mesh invariants of check_plex (src/Topology/Plex.jl:329-360);
+ * returns 0 if they hold, -4 with a message otherwise. */
+int dmx_check_plex(const dmx_plex* dm);
+
+/* --- TRiSK geometry -------------------------------------------------------------- */
+
+/* Output arrays of TRiSKGrid (src/FiniteVolumes/TRiSKGrid.jl:37-60), caller-allocated.
+ * kc = max cell size, kv = max vertex degree, kt = max TRiSK stencil. */
+typedef struct {
+    int64_t kc, kv, kt;
+    double* edge_points;      /* ne x 3 */
+    double* edge_normals;     /* ne x 3 */
+    double* primal_lengths;   /* ne */
+    double* dual_lengths;     /* ne */
+    int64_t* edge_cells;      /* 2 x ne (left, right) */
+    int64_t* edge_vertices;   /* 2 x ne */
+    double* cell_distances;   /* 2 x ne */
+    double* cell_areas;       /* nc */
+    double* vertex_areas;     /* nv */
+    int64_t* cell_edges;      /* kc x nc */
+    double* cell_signs;       /* kc x nc */
+    int64_t* cell_vertices;   /* kc x nc */
+    double* cell_kites;       /* kc x nc */
+    int64_t* vertex_edges;    /* kv x nv */
+    double* vertex_signs;     /* kv x nv */
+    int64_t* vertex_cells;    /* kv x nv */
+    double* vertex_kites;     /* kv x nv */
+    int64_t* trisk_edges;     /* kt x ne */
+    double* trisk_weights;    /* kt x ne */
+} dmx_grid_tables;
+
+/* Table widths of TRiSKGrid for this mesh (kc, kv, kt). */
+int dmx_trisk_grid_widths(const dmx_plex* dm, int64_t widths[3]);
+/* TRiSKGrid(dm; centers) (src/FiniteVolumes/TRiSKGrid.jl:76-162). Error -2 for an
+ * open mesh (the reference's ArgumentError at :92). */
+int dmx_trisk_grid(const dmx_plex* dm, const double* centers, dmx_grid_tables* g);
+
+/* vertex_edge_areas (src/FiniteVolumes/TRiSKGrid.jl:179-191): by_vertex (kv x nv),
+ * by_edge (2 x ne). */
+int dmx_vertex_edge_areas(int64_t nc, int64_t ne, int64_t nv, const dmx_grid_tables* g,
+                          const double* centers, const double* vertices, double* by_vertex,
+                          double* by_edge);
+
+/* gassmann_coefficients (src/FiniteVolumes/TRiSKGrid.jl:214-251): edges, pv_edges,
+ * weights (12 x ne).  Error -2 when a vertex does not have three edges. */
+int dmx_gassmann_coefficients(int64_t nc, int64_t ne, int64_t nv, const dmx_grid_tables* g,
+                              const double* vertices, int64_t* edges, int64_t* pv_edges,
+                              double* weights);
+
+/* --- ordering -------------------------------------------------------------------- */
+
+/* Hilbert key of integer points (hilbert_key, src/Topology/Ordering.jl:10-41):
+ * n points x (n x 3), each coordinate in [0, 2^bits). */
+int dmx_hilbert_keys(int64_t n, const uint64_t* x, int bits, uint64_t* keys);
+/* hilbert_permutation (src/Topology/Ordering.jl:85-92) of n points (n x 3):
+ * perm[c] = old index of new entity c (stable sort of the keys). */
+int dmx_hilbert_permutation(int64_t n, const double* points, int bits, int64_t* perm);
+
+/* --- synthetic initial state ---------------------------------------------------- */
+
+/* psi(x) = sum_{l=1..lmax} sum_{m=-l..l} coef[l*l - 1 + (m + l)] Y_lm(x) with real
+ * orthonormal spherical harmonics, at n points (n x 3) on the unit sphere. */
+int dmx_streamfunction(int lmax, const double* coef, int64_t n, const double* points, double* psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

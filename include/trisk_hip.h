@@ -1,0 +1,166 @@
+/*
+ * trisk_hip.h -- C ABI of libtrisk_hip.so, the MI355X (gfx950) implementation of the
+ * nonlinear TRiSK C-grid shallow-water step of dmplex.jl.
+ *
+ * The library replaces the device side of
+ *   TRiSKShallowWater(dm; ...)   src/FiniteVolumes/TRiSKShallowWater.jl:94-170  -> trisk_create
+ *   tendency!(dy, y, m)          src/FiniteVolumes/TRiSKShallowWater.jl:379-394 -> trisk_tendency
+ *   rk4_step!(y, dt, tendency!, RK4Cache(y), m)
+ *                                src/TimeSteppers/RungeKutta.jl:2-28            -> trisk_rk4_steps
+ *   total_mass / total_energy / potential_enstrophy
+ *                                src/FiniteVolumes/TRiSKShallowWater.jl:397-419 -> trisk_diagnostics
+ *   relative_vorticity / tangential_velocity
+ *                                src/FiniteVolumes/TRiSKShallowWater.jl:422-433 -> trisk_relative_vorticity,
+ *                                                                                  trisk_tangential_velocity
+ *   kinetic_energy! / vertex_pv! src/FiniteVolumes/TRiSKShallowWater.jl:332-360 -> trisk_kinetic_energy,
+ *                                                                                  trisk_vertex_pv
+ *   to_backend(backend, x)       src/FEM/DeviceSolvers.jl:7-11                  -> trisk_upload_state
+ *   gather_state(m, y)           src/FiniteVolumes/TRiSKShallowWater.jl:191-195 -> trisk_download_state
+ *
+ * The caller hands over the model's tables exactly as the reference constructor builds
+ * them (host pointers, borrowed for the duration of trisk_create): k x n column-major
+ * tables (entry (k, j) at k + j*K, the memory of a Julia k x n Matrix) with 0-based
+ * int64 indices (the reference's Int tables minus one); padded slots have weight 0.
+ * The library validates every index, renumbers cells along the Hilbert curve of
+ * `cell_points` (edges and vertices by first appearance), stores every table
+ * entity-major (struct of arrays, int32 indices) in device memory, and owns that memory.
+ *
+ * Device state vectors `y` hold [u (ne); h (nc)] in the library's internal (Hilbert)
+ * order; trisk_upload_state / trisk_download_state convert from and to the reference's
+ * global order.  Device vectors are allocated by the caller (any hipMalloc'ed memory)
+ * and must hold trisk_state_length() doubles.  Every call is stream-ordered on the
+ * given hipStream_t (passed as void*, NULL = the null stream) and returns without a
+ * device synchronisation unless stated.  A handle is not re-entrant: calls on one
+ * handle must not overlap.  All calls return 0 or a negative TRISK_E* code; the
+ * message is in trisk_last_error() (thread-local).
+ */
+#ifndef TRISK_HIP_H
+#define TRISK_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TRISK_OK = 0,
+    TRISK_EINVAL = -1,  /* bad argument or table (the reference's ArgumentError) */
+    TRISK_EHIP = -2,    /* a HIP runtime error */
+    TRISK_ENOMEM = -3,  /* device allocation failed */
+};
+
+/* pv_flux keyword (TRiSKShallowWater.jl:94-98) */
+enum { TRISK_PV_TRISK = 0, TRISK_PV_GASSMANN = 1, TRISK_PV_GASSMANN_MEAN = 2 };
+
+/* arithmetic of the stencils */
+enum {
+    TRISK_FP64 = 0,      /* everything in FP64: matches the FP64 oracle */
+    TRISK_FP32_BULK = 1  /* FP32 coefficients, fluxes and stencil arithmetic; FP64 state,
+                            heights, Bernoulli function, RK4 accumulation and sums */
+};
+
+/* The fields of the reference's TRiSKShallowWater struct (TRiSKShallowWater.jl:67-92). */
+typedef struct {
+    int64_t nc, ne, nv;          /* cells, edges, vertices */
+    int64_t kc, kv, kt, kg;      /* widths: cell edges, vertex edges, TRiSK stencil, Gassmann (12 or 0) */
+    const int64_t* cell_edges;   /* kc x nc */
+    const double* divergence;    /* kc x nc   n_{e,i} l_e / A_i */
+    const double* kinetic;       /* kc x nc   alpha l_e d_{e,i} / (2 A_i) */
+    const int64_t* edge_cells;   /* 2 x ne    (left, right) */
+    const int64_t* edge_vertices;/* 2 x ne */
+    const double* edge_height;   /* 2 x ne    alpha d_{e,i} / d_e */
+    const double* inverse_dual;  /* ne        1 / d_e */
+    const int64_t* vertex_edges; /* kv x nv */
+    const double* curl;          /* kv x nv   t_{e,v} d_e / A_v */
+    const int64_t* vertex_cells; /* kv x nv */
+    const double* vertex_height; /* kv x nv   K_{i,v} / A_v */
+    const double* coriolis;      /* nv        f at the vertices */
+    const int64_t* trisk_edges;  /* kt x ne */
+    const double* trisk;         /* kt x ne   w_{e,e'} l_{e'} / d_e */
+    const double* cell_areas;    /* nc */
+    const double* vertex_areas;  /* nv */
+    double gravity;
+    /* vertex_kinetic (alpha != 1), all NULL otherwise */
+    const int64_t* cell_vertices;/* kc x nc */
+    const double* vk_kites;      /* kc x nc   (1 - alpha) K_{i,v} / A_i */
+    const double* vk_weights;    /* kv x nv   A_{e,v} / A_v */
+    const double* vk_height;     /* 2 x ne    2 (1 - alpha) A_{e,v} / (l_e d_e) */
+    /* gassmann (pv_flux != TRISK_PV_TRISK), all NULL otherwise */
+    const int64_t* g_edges;      /* kg x ne */
+    const int64_t* g_pv_edges;   /* kg x ne */
+    const double* g_weights;     /* kg x ne */
+    const double* g_rhombus;     /* 2 x ne    A_v / (A_v1 + A_v2) */
+    const double* g_coriolis;    /* ne        f at the edge points */
+    int32_t pv_flux;             /* TRISK_PV_* */
+    int32_t reserved;
+    const double* cell_points;   /* nc x 3 (NULL: keep the given order) for the Hilbert layout */
+} trisk_model_desc;
+
+typedef struct {
+    int32_t precision;           /* TRISK_FP64 or TRISK_FP32_BULK */
+    int32_t device;              /* HIP device ordinal */
+    int32_t block_size;          /* threads per workgroup (0: library default) */
+    int32_t reserved[5];
+} trisk_options;
+
+typedef struct trisk_handle trisk_handle;
+
+/* Thread-local text of the last error. */
+const char* trisk_last_error(void);
+
+/* TRiSKShallowWater(grid; ...; backend) (TRiSKShallowWater.jl:94-170): validates and
+ * re-lays out the tables and uploads them.  Synchronous. */
+int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_handle** out);
+int trisk_destroy(trisk_handle* h);
+
+/* ne + nc (velocity_dofs, height_dofs; TRiSKShallowWater.jl:173-174). */
+int trisk_state_length(const trisk_handle* h, int64_t* n);
+
+/* new -> reference index maps of cells (nc), edges (ne) and vertices (nv); any may be NULL. */
+int trisk_permutation(const trisk_handle* h, int64_t* cells, int64_t* edges, int64_t* vertices);
+
+/* initial_state / to_backend: host [u; h] in reference order -> device state.  Synchronous. */
+int trisk_upload_state(trisk_handle* h, const double* y_host, double* y_dev, void* stream);
+/* gather_state: device state -> host [u; h] in reference order.  Synchronous. */
+int trisk_download_state(trisk_handle* h, const double* y_dev, double* y_host, void* stream);
+/* A device field on cells (kind 0), edges (1) or vertices (2) -> host, reference order.  Synchronous. */
+int trisk_download_field(trisk_handle* h, int32_t kind, const double* x_dev, double* x_host, void* stream);
+/* Host field in reference order -> device, internal order.  Synchronous. */
+int trisk_upload_field(trisk_handle* h, int32_t kind, const double* x_host, double* x_dev, void* stream);
+
+/* tendency!(dy, y, m): dy = [du; dh] of the state y (both device, internal order). */
+int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* stream);
+
+/* nsteps times rk4_step!(y, dt, tendency!, RK4Cache(y), m) in place, with the RK4 stage
+ * update fused into the tendency kernels (same arithmetic as the reference broadcasts). */
+int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, void* stream);
+
+/* out[0] = total_mass, out[1] = total_energy, out[2] = potential_enstrophy, summed with
+ * Neumaier compensation in a fixed order (bitwise reproducible).  Synchronous. */
+int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void* stream);
+
+/* kinetic_energy!(m, u, h): K and B at the cells (device, internal order). */
+int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream);
+/* vertex_pv!(m, u, h): q_v, zeta_v, h_v at the vertices (device, internal order). */
+int trisk_vertex_pv(trisk_handle* h, const double* y_dev, double* qv_dev, double* zv_dev, double* hv_dev,
+                    void* stream);
+/* relative_vorticity(m, y): zeta at the vertices. */
+int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_dev, void* stream);
+/* tangential_velocity(m, u): (1/d_e) sum_e' w l_e' u_e' at the edges; u_dev has ne entries. */
+int trisk_tangential_velocity(trisk_handle* h, const double* u_dev, double* vt_dev, void* stream);
+
+/* Average duration in ms of each hot-path kernel over the last trisk_rk4_steps call made
+ * with timing enabled (HIP events on the call's stream); names[i] receive static strings.
+ * trisk_set_timing(h, 1) enables the events (they add no synchronisation inside the call). */
+int trisk_set_timing(trisk_handle* h, int32_t on);
+int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double* avg_ms, int64_t* launches,
+                       int32_t* count);
+
+/* Algorithmic HBM bytes of one launch of each hot-path kernel (each array element the
+ * kernel must touch counted once), in the order of trisk_kernel_times. */
+int trisk_kernel_bytes(trisk_handle* h, int32_t max, double* bytes, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

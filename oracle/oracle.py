@@ -1,0 +1,162 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``liboracle.so``, the FP64 CPU
+restatement of the reference TRiSK path (trisk_oracle.c).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it,
+as the checker; the product (``dmplex_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+i64, f64 = C.c_int64, C.c_double
+i64p, f64p = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+
+
+class OracleModel(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("kc", i64), ("kv", i64), ("kt", i64), ("kg", i64),
+                ("cell_edges", i64p), ("divergence", f64p), ("kinetic", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_height", f64p), ("inverse_dual", f64p),
+                ("vertex_edges", i64p), ("curl", f64p), ("vertex_cells", i64p), ("vertex_height", f64p),
+                ("coriolis", f64p), ("trisk_edges", i64p), ("trisk", f64p), ("cell_areas", f64p),
+                ("vertex_areas", f64p), ("gravity", f64),
+                ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p), ("vk_height", f64p),
+                ("g_edges", i64p), ("g_pv_edges", i64p), ("g_weights", f64p), ("g_rhombus", f64p),
+                ("g_coriolis", f64p), ("g_mean", C.c_int),
+                ("K", f64p), ("B", f64p), ("F", f64p), ("qe", f64p), ("zv", f64p), ("hv", f64p), ("qv", f64p),
+                ("Kv", f64p)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    """Compiles liboracle.so (make)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return HERE / "liboracle.so"
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = HERE / "liboracle.so"
+        if not path.exists():
+            build()
+        d = C.CDLL(str(path))
+        vp = C.c_void_p
+        M = C.POINTER(OracleModel)
+        for name, args, res in [
+            ("oracle_tendency", [M, vp, vp], None),
+            ("oracle_rk4_step", [M, vp, f64, vp, vp, vp, vp, vp], None),
+            ("oracle_rk4_steps", [M, vp, f64, i64, vp, vp, vp, vp, vp], None),
+            ("oracle_total_mass", [M, vp], f64),
+            ("oracle_total_energy", [M, vp], f64),
+            ("oracle_potential_enstrophy", [M, vp], f64),
+            ("oracle_kinetic_energy", [M, vp, vp], None),
+            ("oracle_vertex_pv", [M, vp, vp], None),
+            ("oracle_mass_flux", [M, vp, vp], None),
+            ("oracle_edge_sum", [i64, i64, vp, vp, vp, vp], None),
+            ("oracle_threads", [], C.c_int),
+            ("oracle_set_threads", [C.c_int], None),
+        ]:
+            fn = getattr(d, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = d
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Oracle:
+    """The reference algorithm on the host tables of a model (any object with the
+    TRiSKShallowWater table attributes, in the reference's layout)."""
+
+    def __init__(self, m):
+        self.m = m
+        g = m.grid
+        nc, ne, nv = g.num_cells(), g.num_edges(), g.num_vertices()
+        self.nc, self.ne, self.nv = nc, ne, nv
+        self.work = {k: np.zeros(n) for k, n in
+                     dict(K=nc, B=nc, F=ne, qe=ne, zv=nv, hv=nv, qv=nv, Kv=nv).items()}
+        vk, gm = m.vertex_kinetic, m.gassmann
+        w = self.work
+        self.s = OracleModel(
+            nc, ne, nv, m.cell_edges.shape[0], m.vertex_edges.shape[0], m.trisk_edges.shape[0], 12 if gm else 0,
+            _p(m.cell_edges, i64p), _p(m.divergence, f64p), _p(m.kinetic, f64p), _p(m.edge_cells, i64p),
+            _p(m.edge_vertices, i64p), _p(m.edge_height, f64p), _p(m.inverse_dual, f64p),
+            _p(m.vertex_edges, i64p), _p(m.curl, f64p), _p(m.vertex_cells, i64p), _p(m.vertex_height, f64p),
+            _p(m.coriolis, f64p), _p(m.trisk_edges, i64p), _p(m.trisk, f64p), _p(m.cell_areas, f64p),
+            _p(m.vertex_areas, f64p), m.gravity,
+            _p(vk["cell_vertices"], i64p) if vk else None, _p(vk["kites"], f64p) if vk else None,
+            _p(vk["weights"], f64p) if vk else None, _p(vk["height"], f64p) if vk else None,
+            _p(gm["edges"], i64p) if gm else None, _p(gm["pv_edges"], i64p) if gm else None,
+            _p(gm["weights"], f64p) if gm else None, _p(gm["rhombus"], f64p) if gm else None,
+            _p(gm["coriolis"], f64p) if gm else None, int(gm["mean"]) if gm else 0,
+            *(_p(w[k], f64p) for k in ("K", "B", "F", "qe", "zv", "hv", "qv", "Kv")))
+        self._cache = None
+
+    def _ptr(self, a):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data_as(C.c_void_p)
+
+    def tendency(self, y: np.ndarray) -> np.ndarray:
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        dy = np.empty_like(y)
+        lib().oracle_tendency(C.byref(self.s), self._ptr(y), self._ptr(dy))
+        return dy
+
+    def rk4_steps(self, y: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
+        """nsteps RK4 steps in place on the host vector y."""
+        if self._cache is None or len(self._cache[0]) != len(y):
+            self._cache = [np.empty_like(y) for _ in range(5)]
+        k = self._cache
+        lib().oracle_rk4_steps(C.byref(self.s), self._ptr(y), dt, nsteps, *(self._ptr(a) for a in k))
+        return y
+
+    def total_mass(self, y):
+        return lib().oracle_total_mass(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def total_energy(self, y):
+        return lib().oracle_total_energy(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def potential_enstrophy(self, y):
+        return lib().oracle_potential_enstrophy(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def kinetic_energy(self, y):
+        y = np.ascontiguousarray(y)
+        lib().oracle_kinetic_energy(C.byref(self.s), self._ptr(y), self._ptr(y[self.ne:]))
+        return self.work["K"].copy(), self.work["B"].copy()
+
+    def vertex_pv(self, y):
+        y = np.ascontiguousarray(y)
+        lib().oracle_vertex_pv(C.byref(self.s), self._ptr(y), self._ptr(y[self.ne:]))
+        return self.work["qv"].copy(), self.work["zv"].copy(), self.work["hv"].copy()
+
+    def edge_sum(self, edges, weights, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        k, n = edges.shape
+        out = np.empty(n)
+        lib().oracle_edge_sum(n, k, edges.ctypes.data_as(C.c_void_p), weights.ctypes.data_as(C.c_void_p),
+                              self._ptr(x), self._ptr(out))
+        return out
+
+    def relative_vorticity(self, y):
+        return self.edge_sum(self.m.vertex_edges, self.m.curl, y[:self.ne])
+
+    def tangential_velocity(self, u):
+        return self.edge_sum(self.m.trisk_edges, self.m.trisk, u)
+
+    @staticmethod
+    def threads() -> int:
+        return lib().oracle_threads()
+
+    @staticmethod
+    def set_threads(n: int) -> None:
+        lib().oracle_set_threads(n)

@@ -1,0 +1,230 @@
+/*
+ * trisk_oracle.c -- TEST INFRASTRUCTURE ONLY (see trisk_oracle.h).
+ *
+ * Restates, kernel for kernel, src/FiniteVolumes/TRiSKShallowWater.jl and
+ * src/TimeSteppers/RungeKutta.jl of the reference in C:
+ *   kinetic_energy_kernel!         :202-210     vertex_kinetic_energy_kernel!  :212-219
+ *   cell_from_vertex_kernel!       :222-230     mass_flux_kernel!              :232-235
+ *   mixed_mass_flux_kernel!        :238-244     vertex_pv_kernel!              :246-259
+ *   edge_pv_kernel!                :261-264     rhombus_pv_kernel!             :266-273
+ *   momentum_kernel!               :275-284     gassmann_momentum_kernel!      :286-296
+ *   continuity_kernel!             :298-305     edge_sum_kernel!               :307-314
+ *   tendency!                      :379-394     total_mass / total_energy /
+ *                                               potential_enstrophy            :397-419
+ *   rk4_step!                      RungeKutta.jl:18-28 (one pass per broadcast)
+ * Each kernel is a parallel loop with an implicit barrier after it, as run! does on the
+ * CPU backend (:322-327).  The diagnostics sum with Neumaier compensation (the reference
+ * uses BLAS dot / pairwise sum, whose order is not reproducible).
+ */
+#include "trisk_oracle.h"
+
+#include <omp.h>
+#include <stdlib.h>
+
+#define AT(tab, K, k, j) (tab)[(k) + (int64_t)(j) * (K)]
+
+int oracle_threads(void) { return omp_get_max_threads(); }
+void oracle_set_threads(int n) { omp_set_num_threads(n); }
+
+void oracle_kinetic_energy(const oracle_model* m, const double* u, const double* h) {
+    const int64_t KC = m->kc, KV = m->kv;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m->nc; ++i) {
+        double s = 0.0;
+        for (int64_t k = 0; k < KC; ++k) {
+            double ue = u[AT(m->cell_edges, KC, k, i)];
+            s += AT(m->kinetic, KC, k, i) * (ue * ue);
+        }
+        m->K[i] = s;
+        m->B[i] = s + m->gravity * h[i];
+    }
+    if (m->cell_vertices) {
+#pragma omp parallel for schedule(static)
+        for (int64_t v = 0; v < m->nv; ++v) {
+            double s = 0.0;
+            for (int64_t k = 0; k < KV; ++k) {
+                double ue = u[AT(m->vertex_edges, KV, k, v)];
+                s += AT(m->vk_weights, KV, k, v) * (ue * ue);
+            }
+            m->Kv[v] = s;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < m->nc; ++i) {
+            double s = 0.0;
+            for (int64_t k = 0; k < KC; ++k) s += AT(m->vk_kites, KC, k, i) * m->Kv[AT(m->cell_vertices, KC, k, i)];
+            m->K[i] += s;
+            m->B[i] += s;
+        }
+    }
+}
+
+void oracle_vertex_pv(const oracle_model* m, const double* u, const double* h) {
+    const int64_t KV = m->kv;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < m->nv; ++v) {
+        double z = 0.0, eta = 0.0;
+        for (int64_t k = 0; k < KV; ++k) z += AT(m->curl, KV, k, v) * u[AT(m->vertex_edges, KV, k, v)];
+        for (int64_t k = 0; k < KV; ++k) eta += AT(m->vertex_height, KV, k, v) * h[AT(m->vertex_cells, KV, k, v)];
+        m->zv[v] = z;
+        m->hv[v] = eta;
+        m->qv[v] = (z + m->coriolis[v]) / eta;
+    }
+}
+
+void oracle_mass_flux(const oracle_model* m, const double* u, const double* h) {
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < m->ne; ++e) {
+        double he = AT(m->edge_height, 2, 0, e) * h[AT(m->edge_cells, 2, 0, e)] +
+                    AT(m->edge_height, 2, 1, e) * h[AT(m->edge_cells, 2, 1, e)];
+        if (m->vk_height)
+            he = he + AT(m->vk_height, 2, 0, e) * m->hv[AT(m->edge_vertices, 2, 0, e)] +
+                 AT(m->vk_height, 2, 1, e) * m->hv[AT(m->edge_vertices, 2, 1, e)];
+        m->F[e] = he * u[e];
+    }
+}
+
+static void pv_flux(const oracle_model* m, double* du) {
+    if (!m->g_edges) {
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < m->ne; ++e)
+            m->qe[e] = (m->qv[AT(m->edge_vertices, 2, 0, e)] + m->qv[AT(m->edge_vertices, 2, 1, e)]) / 2;
+        const int64_t KT = m->kt;
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < m->ne; ++e) {
+            double s = 0.0;
+            for (int64_t k = 0; k < KT; ++k) {
+                int64_t f = AT(m->trisk_edges, KT, k, e);
+                s += AT(m->trisk, KT, k, e) * m->F[f] * (m->qe[e] + m->qe[f]);
+            }
+            du[e] = s / 2 - (m->B[AT(m->edge_cells, 2, 1, e)] - m->B[AT(m->edge_cells, 2, 0, e)]) * m->inverse_dual[e];
+        }
+    } else {
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < m->ne; ++e) {
+            int64_t v1 = AT(m->edge_vertices, 2, 0, e), v2 = AT(m->edge_vertices, 2, 1, e);
+            double r1 = AT(m->g_rhombus, 2, 0, e), r2 = AT(m->g_rhombus, 2, 1, e);
+            double z = r1 * m->zv[v1] + r2 * m->zv[v2];
+            double eta = r1 * m->hv[v1] + r2 * m->hv[v2];
+            m->qe[e] = (z + m->g_coriolis[e]) / eta;
+        }
+        const int64_t KG = m->kg;
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < m->ne; ++e) {
+            double s = 0.0;
+            for (int64_t k = 0; k < KG; ++k) {
+                int64_t f = AT(m->g_edges, KG, k, e);
+                double q = m->g_mean ? (m->qe[e] + m->qe[f]) / 2 : m->qe[AT(m->g_pv_edges, KG, k, e)];
+                s += AT(m->g_weights, KG, k, e) * m->F[f] * q;
+            }
+            du[e] = s - (m->B[AT(m->edge_cells, 2, 1, e)] - m->B[AT(m->edge_cells, 2, 0, e)]) * m->inverse_dual[e];
+        }
+    }
+}
+
+static void continuity(const oracle_model* m, double* dh) {
+    const int64_t KC = m->kc;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m->nc; ++i) {
+        double s = 0.0;
+        for (int64_t k = 0; k < KC; ++k) s += AT(m->divergence, KC, k, i) * m->F[AT(m->cell_edges, KC, k, i)];
+        dh[i] = -s;
+    }
+}
+
+void oracle_tendency(const oracle_model* m, const double* y, double* dy) {
+    const double* u = y;
+    const double* h = y + m->ne;
+    oracle_kinetic_energy(m, u, h);
+    oracle_vertex_pv(m, u, h);
+    oracle_mass_flux(m, u, h);
+    pv_flux(m, dy);
+    continuity(m, dy + m->ne);
+}
+
+void oracle_rk4_step(const oracle_model* m, double* y, double dt, double* k1, double* k2, double* k3, double* k4,
+                     double* stage) {
+    const int64_t n = m->ne + m->nc;
+    oracle_tendency(m, y, k1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) stage[i] = y[i] + dt / 2 * k1[i];
+    oracle_tendency(m, stage, k2);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) stage[i] = y[i] + dt / 2 * k2[i];
+    oracle_tendency(m, stage, k3);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) stage[i] = y[i] + dt * k3[i];
+    oracle_tendency(m, stage, k4);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) y[i] = y[i] + dt / 6 * (k1[i] + 2 * k2[i] + 2 * k3[i] + k4[i]);
+}
+
+void oracle_rk4_steps(const oracle_model* m, double* y, double dt, int64_t nsteps, double* k1, double* k2,
+                      double* k3, double* k4, double* stage) {
+    for (int64_t s = 0; s < nsteps; ++s) oracle_rk4_step(m, y, dt, k1, k2, k3, k4, stage);
+}
+
+/* Neumaier-compensated sum of n terms, split in fixed blocks so that the result does
+ * not depend on the thread count. */
+typedef double (*term_fn)(const oracle_model*, const double*, int64_t);
+
+static void neumaier(double* s, double* c, double x) {
+    double t = *s + x;
+    if (__builtin_fabs(*s) >= __builtin_fabs(x))
+        *c += (*s - t) + x;
+    else
+        *c += (x - t) + *s;
+    *s = t;
+}
+
+static double comp_sum(const oracle_model* m, const double* y, int64_t n, term_fn f) {
+    enum { B = 8192 };
+    int64_t nb = (n + B - 1) / B;
+    double* ps = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1) * 2);
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nb; ++b) {
+        double s = 0.0, c = 0.0;
+        int64_t hi = (b + 1) * B < n ? (b + 1) * B : n;
+        for (int64_t i = b * B; i < hi; ++i) neumaier(&s, &c, f(m, y, i));
+        ps[2 * b] = s;
+        ps[2 * b + 1] = c;
+    }
+    double s = 0.0, c = 0.0;
+    for (int64_t b = 0; b < nb; ++b) {
+        neumaier(&s, &c, ps[2 * b]);
+        c += ps[2 * b + 1];
+    }
+    free(ps);
+    return s + c;
+}
+
+static double mass_term(const oracle_model* m, const double* y, int64_t i) { return m->cell_areas[i] * y[m->ne + i]; }
+static double energy_term(const oracle_model* m, const double* y, int64_t i) {
+    double h = y[m->ne + i];
+    return m->cell_areas[i] * h * (m->K[i] + m->gravity * h / 2);
+}
+static double enstrophy_term(const oracle_model* m, const double* y, int64_t v) {
+    (void)y;
+    return m->vertex_areas[v] * m->hv[v] * (m->qv[v] * m->qv[v]);
+}
+
+double oracle_total_mass(const oracle_model* m, const double* y) { return comp_sum(m, y, m->nc, mass_term); }
+
+double oracle_total_energy(const oracle_model* m, const double* y) {
+    oracle_kinetic_energy(m, y, y + m->ne);
+    return comp_sum(m, y, m->nc, energy_term);
+}
+
+double oracle_potential_enstrophy(const oracle_model* m, const double* y) {
+    oracle_vertex_pv(m, y, y + m->ne);
+    return comp_sum(m, y, m->nv, enstrophy_term) / 2;
+}
+
+void oracle_edge_sum(int64_t n, int64_t k, const int64_t* edges, const double* weights, const double* x,
+                     double* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int64_t q = 0; q < k; ++q) s += AT(weights, k, q, j) * x[AT(edges, k, q, j)];
+        out[j] = s;
+    }
+}

@@ -1,0 +1,169 @@
+"""Parity of the HIP path (through the C ABI of libtrisk_hip.so) with the FP64 oracle.
+
+The FP64 kernels evaluate every formula in the reference's order without FMA
+contraction, so tendencies, RK4 states and the gathered diagnostics must agree with
+the oracle bit for bit; the compensated sums agree to a few ulps.  Sizes are the small
+meshes the oracle finishes in seconds; the level-9 workload is covered by
+size-independent properties (conservation, determinism).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from dmplex_amd import (HIPBackend, ModelTables, TRiSKShallowWater, create_hexagon_mesh, diagnostics,
+                        gather_state, kinetic_energy_, relative_vorticity, rk4_steps_, tangential_velocity,
+                        tendency_, vertex_pv_)
+from dmplex_amd.synthetic import earth_parameters, seeded_state, williamson2
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+_MESHES = {}
+
+
+def mesh(n):
+    if n not in _MESHES:
+        _MESHES[n] = create_hexagon_mesh(n, centroidal=True)
+    return _MESHES[n]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("pv_flux", ["trisk", "gassmann", "gassmann_mean"])
+@pytest.mark.parametrize("kinetic_energy", ["cell", "mixed", "vertex"])
+def test_tendency_bitwise(pv_flux, kinetic_energy):
+    say(f"tendency parity level 4 {pv_flux} {kinetic_energy}")
+    dm = mesh(4)
+    m = TRiSKShallowWater(dm, centers=dm.generators, pv_flux=pv_flux, kinetic_energy=kinetic_energy,
+                          **earth_parameters())
+    y = seeded_state(m.grid)
+    ref = Oracle(m).tendency(y)
+    yd = m.upload_state(y)
+    dy = m.zeros()
+    tendency_(dy, yd, m)
+    got = gather_state(m, dy)
+    assert np.array_equal(got, ref), rel(got, ref)
+
+
+@pytest.mark.parametrize("pv_flux", ["trisk", "gassmann_mean"])
+def test_rk4_bitwise(pv_flux):
+    say(f"rk4 parity level 5 {pv_flux}")
+    dm = mesh(5)
+    m = TRiSKShallowWater(dm, centers=dm.generators, pv_flux=pv_flux, **earth_parameters())
+    y = seeded_state(m.grid)
+    o = Oracle(m)
+    ref = o.rk4_steps(y.copy(), 600.0, 12)
+    yd = m.upload_state(y)
+    rk4_steps_(yd, 600.0, 12, m)
+    got = gather_state(m, yd)
+    assert np.array_equal(got, ref), rel(got, ref)
+
+
+def test_config0_level5_144_steps():
+    """Configuration 0: level 5, TRiSK, cell KE, dt = 600 s, 144 steps (one day)."""
+    say("config 0: level 5, 144 steps")
+    dm = mesh(5)
+    m = TRiSKShallowWater(dm, centers=dm.generators, **earth_parameters())
+    y0 = seeded_state(m.grid)
+    o = Oracle(m)
+    ref = o.rk4_steps(y0.copy(), 600.0, 144)
+    yd = m.upload_state(y0)
+    mass0, energy0, _ = diagnostics(m, yd)
+    rk4_steps_(yd, 600.0, 144, m)
+    got = gather_state(m, yd)
+    ne = dm.ne
+    for sl in (slice(0, ne), slice(ne, None)):
+        assert rel(got[sl], ref[sl]) < 1e-12
+    assert np.array_equal(got, ref)
+    mass, energy, _ = diagnostics(m, yd)
+    assert abs(mass - mass0) / mass0 < 1e-14
+    assert abs(mass - o.total_mass(ref)) / mass0 < 1e-15
+    assert abs(energy - o.total_energy(ref)) / energy0 < 1e-14
+    # the energy drift equals the oracle's
+    drift_gpu = (energy - energy0) / energy0
+    drift_ref = (o.total_energy(ref) - o.total_energy(y0)) / o.total_energy(y0)
+    assert abs(drift_gpu - drift_ref) < 1e-14
+
+
+def test_diagnostics_match_oracle():
+    say("diagnostics level 4")
+    dm = mesh(4)
+    m = TRiSKShallowWater(dm, centers=dm.generators, kinetic_energy="mixed", **earth_parameters())
+    y = seeded_state(m.grid)
+    o = Oracle(m)
+    mass, energy, ens = diagnostics(m, y)
+    assert abs(mass - o.total_mass(y)) <= 4e-16 * abs(mass)
+    assert abs(energy - o.total_energy(y)) <= 4e-16 * abs(energy)
+    assert abs(ens - o.potential_enstrophy(y)) <= 4e-16 * abs(ens)
+    K, B = kinetic_energy_(m, y).K, kinetic_energy_(m, y).B
+    Kr, Br = o.kinetic_energy(y)
+    assert np.array_equal(K, Kr) and np.array_equal(B, Br)
+    q, z, h = vertex_pv_(m, y)
+    qr, zr, hr = o.vertex_pv(y)
+    assert np.array_equal(q, qr) and np.array_equal(z, zr) and np.array_equal(h, hr)
+
+
+def test_vorticity_and_tangential_velocity():
+    say("relative vorticity / tangential velocity level 3")
+    dm = mesh(3)
+    m = TRiSKShallowWater(dm, centers=dm.generators)
+    y = williamson2(m.grid)
+    o = Oracle(m)
+    assert np.array_equal(relative_vorticity(m, y), o.relative_vorticity(y))
+    u = y[:dm.ne]
+    assert np.array_equal(tangential_velocity(m, u), o.tangential_velocity(u))
+
+
+def test_hilbert_layout_is_a_permutation_and_invisible():
+    say("layout permutation level 4")
+    dm = mesh(4)
+    m = TRiSKShallowWater(dm, centers=dm.generators)
+    c, e, v = m.permutation()
+    for p, n in ((c, dm.nc), (e, dm.ne), (v, dm.nv)):
+        assert np.array_equal(np.sort(p), np.arange(n))
+    plain = TRiSKShallowWater(dm, centers=dm.generators, hilbert=False)
+    y = williamson2(m.grid)
+    a, b = m.zeros(), plain.zeros()
+    tendency_(a, m.upload_state(y), m)
+    tendency_(b, plain.upload_state(y), plain)
+    assert np.array_equal(gather_state(m, a), gather_state(plain, b))
+
+
+def test_fp32_bulk_close_to_fp64():
+    say("fp32-bulk tendency level 5")
+    dm = mesh(5)
+    kw = dict(centers=dm.generators, **earth_parameters())
+    m32 = TRiSKShallowWater(dm, backend=HIPBackend(precision="fp32_bulk"), **kw)
+    y = seeded_state(m32.grid)
+    ref = Oracle(m32).tendency(y)
+    dy = m32.zeros()
+    tendency_(dy, m32.upload_state(y), m32)
+    got = gather_state(m32, dy)
+    ne = dm.ne
+    assert rel(got[:ne], ref[:ne]) < 1e-3
+    assert rel(got[ne:], ref[ne:]) < 1e-5
+    # mass: the continuity stays FP64, so sum A dh telescopes
+    A = m32.cell_areas
+    assert abs(np.dot(A, got[ne:])) < 1e-12 * np.dot(A, np.abs(got[ne:]))
+
+
+def test_invalid_tables_are_rejected():
+    say("invalid tables")
+    dm = mesh(2)
+    t = ModelTables(dm, centers=dm.generators)
+    bad = t.cell_edges.copy(order="F")
+    bad[0, 0] = dm.ne + 5
+    t.cell_edges = bad
+    m = TRiSKShallowWater.__new__(TRiSKShallowWater)
+    m.__dict__.update(t.__dict__)
+    m.backend, m._hilbert, m._handle = HIPBackend(), True, None
+    with pytest.raises(ValueError, match="out of range"):
+        m._create()
