@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--level", type=int, default=9)
     p.add_argument("--dt", type=float, default=45.0)
     p.add_argument("--precision", choices=["both", "fp64", "fp32_bulk"], default="both")
-    p.add_argument("--cpu-steps", type=int, default=3, help="RK4 steps of the CPU baseline sample (0: skip)")
+    p.add_argument("--cpu-steps", type=int, default=100, help="RK4 steps of the CPU baseline sample (0: skip)")
     p.add_argument("--no-kernel-events", action="store_true")
     return p.parse_args()
 
@@ -152,7 +152,7 @@ def main():
 
     t = time.perf_counter()
     log(f"[bench] generating SCVT hexagon mesh level {args.level}")
-    dm = create_hexagon_mesh(args.level, centroidal=True)
+    dm = create_hexagon_mesh(args.level, centroidal=True, device=0, log=lambda m: log(f"[bench] {m}"))
     grid = TRiSKGrid(dm, centers=dm.generators)
     params = earth_parameters()
     y0 = seeded_state(grid)

@@ -53,6 +53,7 @@ SIGNATURES = {
     "trisk_set_timing": [vp, C.c_int32],
     "trisk_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
     "trisk_kernel_bytes": [vp, C.c_int32, f64p, C.POINTER(C.c_int32)],
+    "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 
 _lib = None

@@ -67,6 +67,11 @@ def host() -> C.CDLL:
             "dmx_icosahedral": [C.c_int, C.c_int, f64p, i64p, i64p],
             "dmx_lloyd": [f64p, i64, i64p, i64, i64, C.c_double, C.c_int, i64, i64p],
             "dmx_lloyd_residual": [f64p, i64, i64p, i64, i64, f64p],
+            "dmx_icosahedron": [f64p, i64p],
+            "dmx_bisect": [i64, f64p, i64, i64p, f64p, i64p],
+            "dmx_triangle_stars": [i64, i64p, i64, i64p, i64p],
+            "dmx_delaunay": [f64p, i64, i64p, i64],
+            "dmx_mean_edge_length": [f64p, i64p, i64, f64p],
             "dmx_plex_from_polygons": [i64, i64p, i64p, i64, f64p, C.POINTER(C.c_void_p), i64p],
             "dmx_plex_export": [C.c_void_p, i64p, i64p, i8p, i64p, i64p],
             "dmx_dual_polygons": [C.POINTER(DmxPlex), i64p, i64p, f64p],

@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "lib"
 HOST_SOURCES = ["csrc/host/mesh.cpp", "csrc/host/trisk_grid.cpp"]
-HIP_SOURCES = ["csrc/hip/trisk_device.hip"]
+HIP_SOURCES = ["csrc/hip/trisk_device.hip", "csrc/hip/lloyd_device.hip"]
 HIP_HOST_SOURCES = ["csrc/hip/trisk_layout.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
+    "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
 
 

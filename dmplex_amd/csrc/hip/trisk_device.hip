@@ -38,6 +38,8 @@ static int set_err(int code, const std::string& msg) {
     return code;
 }
 
+int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg); }
+
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
@@ -915,8 +917,7 @@ int trisk_tangential_velocity(trisk_handle* h, const double* u_dev, double* vt_d
 
 int trisk_set_timing(trisk_handle* h, int32_t on) {
     CHECK_HANDLE(h);
-    h->timing = on != 0;
-    h->events_used = 0;
+    h->timing = on != 0;  // recorded events stay readable until the next timed call
     return 0;
 }
 

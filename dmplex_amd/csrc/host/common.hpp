@@ -1,5 +1,6 @@
 // Small shared helpers of the host library: 3-vectors, error text, deterministic sums.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -35,6 +36,38 @@ inline double arc_length(V3 a, V3 b) { return std::atan2(norm(cross(a, b)), dot(
 // Signed spherical triangle area, negative when clockwise (TRiSKGrid.jl:5).
 inline double signed_triangle_area(V3 a, V3 b, V3 c) {
     return 2 * std::atan2(dot(a, cross(b, c)), 1 + dot(a, b) + dot(b, c) + dot(c, a));
+}
+
+// Solves (N + 1e-12 max(diag N) I) g = r for the m x m (m <= 16, row stride 16)
+// normal equations of Anderson mixing (Icosahedral.jl:174), by Gaussian elimination
+// with partial pivoting.
+inline void small_solve(int m, const double* N, const double* r, double* g) {
+    double A[16][17];
+    double dmax = 0;
+    for (int i = 0; i < m; ++i) dmax = std::max(dmax, N[i * 16 + i]);
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) A[i][j] = N[i * 16 + j];
+        A[i][i] += 1e-12 * dmax;
+        A[i][m] = r[i];
+    }
+    for (int c = 0; c < m; ++c) {
+        int p = c;
+        for (int i = c + 1; i < m; ++i)
+            if (std::fabs(A[i][c]) > std::fabs(A[p][c])) p = i;
+        if (p != c)
+            for (int j = 0; j <= m; ++j) std::swap(A[p][j], A[c][j]);
+        double piv = A[c][c];
+        if (piv == 0) continue;
+        for (int i = c + 1; i < m; ++i) {
+            double fct = A[i][c] / piv;
+            for (int j = c; j <= m; ++j) A[i][j] -= fct * A[c][j];
+        }
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        double s = A[i][m];
+        for (int j = i + 1; j < m; ++j) s -= A[i][j] * g[j];
+        g[i] = A[i][i] != 0 ? s / A[i][i] : 0.0;
+    }
 }
 
 void set_error(const std::string& msg);

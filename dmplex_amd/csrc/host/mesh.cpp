@@ -146,37 +146,6 @@ static bool all_delaunay(const double* x, const int64_t* T, int64_t nt, const St
     return !bad;
 }
 
-// Solves (N + lambda I) g = r for a small symmetric system by Gaussian elimination
-// with partial pivoting.
-static void small_solve(int m, const double* N, const double* r, double* g) {
-    double A[16][17];
-    double dmax = 0;
-    for (int i = 0; i < m; ++i) dmax = std::max(dmax, N[i * 16 + i]);
-    for (int i = 0; i < m; ++i) {
-        for (int j = 0; j < m; ++j) A[i][j] = N[i * 16 + j];
-        A[i][i] += 1e-12 * dmax;
-        A[i][m] = r[i];
-    }
-    for (int c = 0; c < m; ++c) {
-        int p = c;
-        for (int i = c + 1; i < m; ++i)
-            if (std::fabs(A[i][c]) > std::fabs(A[p][c])) p = i;
-        if (p != c)
-            for (int j = 0; j <= m; ++j) std::swap(A[p][j], A[c][j]);
-        double piv = A[c][c];
-        if (piv == 0) continue;
-        for (int i = c + 1; i < m; ++i) {
-            double fct = A[i][c] / piv;
-            for (int j = c; j <= m; ++j) A[i][j] -= fct * A[c][j];
-        }
-    }
-    for (int i = m - 1; i >= 0; --i) {
-        double s = A[i][m];
-        for (int j = i + 1; j < m; ++j) s -= A[i][j] * g[j];
-        g[i] = A[i][i] != 0 ? s / A[i][i] : 0.0;
-    }
-}
-
 // lloyd! (Icosahedral.jl:126-184) with Anderson mixing (type II) over `history` steps.
 static int lloyd(double* x, int64_t nv, const int64_t* T, int64_t nt, int64_t nfixed,
                  double tolerance, int history, int64_t max_iterations, int64_t* iterations) {
@@ -267,6 +236,44 @@ static int lloyd(double* x, int64_t nv, const int64_t* T, int64_t nt, int64_t nf
 // ---------------------------------------------------------------------------------
 // Icosahedral bisection (Icosahedral.jl:22-61)
 
+// One bisection of every triangle into four, new vertices at the normalized edge
+// midpoints in order of first request (Icosahedral.jl:38-48).
+static void bisect(std::vector<double>& x, std::vector<int64_t>& T) {
+    int64_t nv = int64_t(x.size() / 3), nt = int64_t(T.size() / 3);
+    std::vector<int64_t> cnt(nv + 1, 0);
+    for (int64_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            int64_t a = T[3 * t + k], b = T[3 * t + (k + 1) % 3];
+            cnt[std::min(a, b) + 1]++;
+        }
+    for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    std::vector<std::pair<int64_t, int64_t>> slot(cnt[nv]);
+    x.reserve(x.size() + 3 * (nv + nt));
+    auto midpoint = [&](int64_t a, int64_t b) -> int64_t {
+        int64_t lo = std::min(a, b), hi = std::max(a, b);
+        for (int64_t q = cnt[lo]; q < fill[lo]; ++q)
+            if (slot[q].first == hi) return slot[q].second;
+        V3 m = normalize(load3(x.data(), a) + load3(x.data(), b));
+        int64_t id = int64_t(x.size() / 3);
+        x.push_back(m.x);
+        x.push_back(m.y);
+        x.push_back(m.z);
+        slot[fill[lo]++] = {hi, id};
+        return id;
+    };
+    std::vector<int64_t> R;
+    R.reserve(12 * nt);
+    for (int64_t t = 0; t < nt; ++t) {
+        int64_t a = T[3 * t], b = T[3 * t + 1], c = T[3 * t + 2];
+        int64_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+        int64_t q[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+        R.insert(R.end(), q, q + 12);
+    }
+    T.swap(R);
+}
+
+
 static int icosahedral(int subdivisions, int centroidal, std::vector<double>& x,
                        std::vector<int64_t>& T, int64_t* iterations) {
     if (subdivisions < 0) return fail(-1, "subdivisions must be non-negative");
@@ -289,39 +296,7 @@ static int icosahedral(int subdivisions, int centroidal, std::vector<double>& x,
     for (auto& t : tri0)
         for (int k = 0; k < 3; ++k) T.push_back(t[k] - 1);
     for (int level = 1; level <= subdivisions; ++level) {
-        int64_t nv = int64_t(x.size() / 3), nt = int64_t(T.size() / 3);
-        // midpoint table: per lower vertex, (upper vertex, midpoint) pairs
-        std::vector<int64_t> cnt(nv + 1, 0);
-        for (int64_t t = 0; t < nt; ++t)
-            for (int k = 0; k < 3; ++k) {
-                int64_t a = T[3 * t + k], b = T[3 * t + (k + 1) % 3];
-                cnt[std::min(a, b) + 1]++;
-            }
-        for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
-        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-        std::vector<std::pair<int64_t, int64_t>> slot(cnt[nv]);
-        x.reserve(x.size() + 3 * (nv + nt));
-        auto midpoint = [&](int64_t a, int64_t b) -> int64_t {
-            int64_t lo = std::min(a, b), hi = std::max(a, b);
-            for (int64_t q = cnt[lo]; q < fill[lo]; ++q)
-                if (slot[q].first == hi) return slot[q].second;
-            V3 m = normalize(load3(x.data(), a) + load3(x.data(), b));
-            int64_t id = int64_t(x.size() / 3);
-            x.push_back(m.x);
-            x.push_back(m.y);
-            x.push_back(m.z);
-            slot[fill[lo]++] = {hi, id};
-            return id;
-        };
-        std::vector<int64_t> R;
-        R.reserve(12 * nt);
-        for (int64_t t = 0; t < nt; ++t) {
-            int64_t a = T[3 * t], b = T[3 * t + 1], c = T[3 * t + 2];
-            int64_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
-            int64_t q[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
-            R.insert(R.end(), q, q + 12);
-        }
-        T.swap(R);
+        bisect(x, T);
         if (centroidal) {
             int64_t its = 0;
             int rc = lloyd(x.data(), int64_t(x.size() / 3), T.data(), int64_t(T.size() / 3), 12, 1e-8,
@@ -486,6 +461,47 @@ int dmx_icosahedral(int subdivisions, int centroidal, double* coords, int64_t* t
 int dmx_lloyd(double* x, int64_t nv, const int64_t* tri, int64_t nt, int64_t nfixed, double tolerance,
               int history, int64_t max_iterations, int64_t* iterations) {
     return lloyd(x, nv, tri, nt, nfixed, tolerance, history, max_iterations, iterations);
+}
+
+int dmx_icosahedron(double* coords, int64_t* triangles) {
+    std::vector<double> x;
+    std::vector<int64_t> T;
+    int rc = icosahedral(0, 0, x, T, nullptr);
+    if (rc < 0) return rc;
+    std::memcpy(coords, x.data(), x.size() * sizeof(double));
+    std::memcpy(triangles, T.data(), T.size() * sizeof(int64_t));
+    return 0;
+}
+
+int dmx_bisect(int64_t nv, const double* x, int64_t nt, const int64_t* tri, double* x_out, int64_t* tri_out) {
+    std::vector<double> X(x, x + 3 * nv);
+    std::vector<int64_t> T(tri, tri + 3 * nt);
+    bisect(X, T);
+    std::memcpy(x_out, X.data(), X.size() * sizeof(double));
+    std::memcpy(tri_out, T.data(), T.size() * sizeof(int64_t));
+    return 0;
+}
+
+int dmx_triangle_stars(int64_t nv, const int64_t* tri, int64_t nt, int64_t* star_offsets, int64_t* stars) {
+    Stars s;
+    int rc = build_stars(nv, tri, nt, s);
+    if (rc) return rc;
+    std::memcpy(star_offsets, s.off.data(), (nv + 1) * sizeof(int64_t));
+    std::memcpy(stars, s.tri.data(), 3 * nt * sizeof(int64_t));
+    return 0;
+}
+
+int dmx_delaunay(const double* x, int64_t nv, const int64_t* tri, int64_t nt) {
+    Stars s;
+    int rc = build_stars(nv, tri, nt, s);
+    if (rc) return rc;
+    if (!all_delaunay(x, tri, nt, s)) return fail(-3, "Lloyd's algorithm left the triangulation non-Delaunay");
+    return 0;
+}
+
+int dmx_mean_edge_length(const double* x, const int64_t* tri, int64_t nt, double* h) {
+    *h = det_sum(nt, [&](int64_t t) { return norm(load3(x, tri[3 * t + 1]) - load3(x, tri[3 * t])); }) / double(nt);
+    return 0;
 }
 
 int dmx_lloyd_residual(const double* x, int64_t nv, const int64_t* tri, int64_t nt, int64_t nfixed,

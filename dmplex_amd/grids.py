@@ -13,32 +13,77 @@ from . import _native as N
 from .topology import Plex, plex_from_polygons
 
 
-def _icosahedral(subdivisions: int, centroidal: bool):
+def _lloyd_device(x: np.ndarray, tri: np.ndarray, device: int, fixed: int = 12, tolerance: float = 1e-8,
+                  history: int = 16, max_iterations: int = 10_000, info: dict | None = None) -> int:
+    """lloyd! on a HIP device (trisk_lloyd of libtrisk_hip.so), then the host Delaunay check."""
+    from . import _device as D
+
+    nv, nt = len(x), len(tri)
+    off = np.empty(nv + 1, dtype=np.int64)
+    star = np.empty(3 * nt, dtype=np.int64)
+    N.check(N.host().dmx_triangle_stars(nv, N.ptr(tri, N.i64p), nt, N.ptr(off, N.i64p), N.ptr(star, N.i64p)))
+    its = np.zeros(1, dtype=np.int64)
+    res = np.zeros(1)
+    rc = D.lib().trisk_lloyd(device, N.ptr(x, N.f64p), nv, N.ptr(tri, N.i64p), nt, N.ptr(off, N.i64p),
+                             N.ptr(star, N.i64p), fixed, tolerance, history, max_iterations, N.ptr(its, N.i64p),
+                             N.ptr(res, N.f64p))
+    if rc < 0:
+        D.check(rc)
+    if info is not None:
+        info.update(iterations=int(its[0]), residual=float(res[0]), converged=rc == 0)
+    N.check(N.host().dmx_delaunay(N.ptr(x, N.f64p), nv, N.ptr(tri, N.i64p), nt))
+    return int(its[0])
+
+
+def _icosahedral(subdivisions: int, centroidal: bool, device: int | None = None, log=None):
     if subdivisions < 0:
         raise ValueError("subdivisions must be non-negative")
-    nv, nt = 10 * 4 ** subdivisions + 2, 20 * 4 ** subdivisions
-    x = np.empty((nv, 3))
-    tri = np.empty((nt, 3), dtype=np.int64)
-    its = np.zeros(max(subdivisions, 1), dtype=np.int64)
-    N.check(N.host().dmx_icosahedral(subdivisions, int(centroidal), N.ptr(x, N.f64p), N.ptr(tri, N.i64p),
-                                     N.ptr(its, N.i64p)))
-    return x, tri, its[:subdivisions]
+    if device is None:
+        nv, nt = 10 * 4 ** subdivisions + 2, 20 * 4 ** subdivisions
+        x = np.empty((nv, 3))
+        tri = np.empty((nt, 3), dtype=np.int64)
+        its = np.zeros(max(subdivisions, 1), dtype=np.int64)
+        N.check(N.host().dmx_icosahedral(subdivisions, int(centroidal), N.ptr(x, N.f64p), N.ptr(tri, N.i64p),
+                                         N.ptr(its, N.i64p)))
+        return x, tri, its[:subdivisions]
+    x = np.empty((12, 3))
+    tri = np.empty((20, 3), dtype=np.int64)
+    N.check(N.host().dmx_icosahedron(N.ptr(x, N.f64p), N.ptr(tri, N.i64p)))
+    its = []
+    for level in range(1, subdivisions + 1):
+        nv, nt = len(x), len(tri)
+        x2 = np.empty((nv + 3 * nt // 2, 3))
+        tri2 = np.empty((4 * nt, 3), dtype=np.int64)
+        N.check(N.host().dmx_bisect(nv, N.ptr(x, N.f64p), nt, N.ptr(tri, N.i64p), N.ptr(x2, N.f64p),
+                                    N.ptr(tri2, N.i64p)))
+        x, tri = x2, tri2
+        if centroidal:
+            info = {}
+            its.append(_lloyd_device(x, tri, device, info=info))
+            if log:
+                state = "converged" if info["converged"] else "stopped (not converged)"
+                log(f"level {level}: {len(x)} vertices, Lloyd {state} after {its[-1]} iterations, "
+                    f"residual {info['residual']:.3e} mean edge lengths")
+    return x, tri, np.array(its, dtype=np.int64)
 
 
-def create_spherical_mesh(subdivisions: int, spring_beta=None, centroidal: bool = False) -> Plex:
+def create_spherical_mesh(subdivisions: int, spring_beta=None, centroidal: bool = False,
+                          device: int | None = None, log=None) -> Plex:
     """Icosahedral triangle mesh on the unit sphere (Icosahedral.jl:22-61).
 
-    ``centroidal=True`` runs Lloyd's algorithm after every bisection (SCVT).  The spring
-    dynamics option of the reference (``spring_beta``) is not part of this path.
+    ``centroidal=True`` runs Lloyd's algorithm after every bisection (SCVT), on the host
+    or, with ``device=<HIP ordinal>``, on the GPU (the way to build levels 8-10 in
+    seconds).  The spring dynamics option of the reference (``spring_beta``) is not part
+    of this path.
     """
     if spring_beta is not None:
         raise NotImplementedError("spring dynamics is outside the TRiSK SCVT path")
-    x, tri, _ = _icosahedral(subdivisions, centroidal)
+    x, tri, _ = _icosahedral(subdivisions, centroidal, device, log)
     return plex_from_polygons((np.arange(0, 3 * len(tri) + 1, 3, dtype=np.int64), tri.reshape(-1)), x)
 
 
 def create_hexagon_mesh(subdivisions: int, spring_beta=None, centroidal: bool = False,
-                        primal: Plex | None = None) -> Plex:
+                        primal: Plex | None = None, device: int | None = None, log=None) -> Plex:
     """Voronoi dual of the icosahedral mesh (Hexagonal.jl:11-17).
 
     Triangle circumcenters become vertices and every triangle vertex a polygon; the
@@ -46,7 +91,7 @@ def create_hexagon_mesh(subdivisions: int, spring_beta=None, centroidal: bool = 
     vertices (kept as ``dm.generators``).
     """
     if primal is None:
-        primal = create_spherical_mesh(subdivisions, spring_beta, centroidal)
+        primal = create_spherical_mesh(subdivisions, spring_beta, centroidal, device, log)
     pv = primal.view()
     off = np.empty(primal.nv + 1, dtype=np.int64)
     verts = np.empty(3 * primal.nc, dtype=np.int64)
@@ -57,7 +102,7 @@ def create_hexagon_mesh(subdivisions: int, spring_beta=None, centroidal: bool = 
 
 
 def lloyd_(x: np.ndarray, triangles: np.ndarray, fixed: int = 0, tolerance: float = 1e-8,
-           history: int = 16, max_iterations: int = 10_000) -> int:
+           history: int = 16, max_iterations: int = 10_000, device: int | None = None) -> int:
     """Lloyd's algorithm with Anderson mixing, in place (Icosahedral.jl:126-184).
 
     ``fixed`` is the number of leading vertices that do not move (the reference's
@@ -67,6 +112,8 @@ def lloyd_(x: np.ndarray, triangles: np.ndarray, fixed: int = 0, tolerance: floa
     if x.dtype != np.float64 or not x.flags.c_contiguous:
         raise ValueError("x must be a C-contiguous float64 (n, 3) array")
     tri = np.ascontiguousarray(triangles, dtype=np.int64)
+    if device is not None:
+        return _lloyd_device(x, tri, device, fixed, tolerance, history, max_iterations)
     its = np.zeros(1, dtype=np.int64)
     N.check(N.host().dmx_lloyd(N.ptr(x, N.f64p), len(x), N.ptr(tri, N.i64p), len(tri), fixed, tolerance,
                                history, max_iterations, N.ptr(its, N.i64p)), warn_ok=True)

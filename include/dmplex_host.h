@@ -48,6 +48,24 @@ int dmx_icosahedral(int subdivisions, int centroidal, double* coords, int64_t* t
 int dmx_lloyd(double* x, int64_t nv, const int64_t* tri, int64_t nt, int64_t nfixed,
               double tolerance, int history, int64_t max_iterations, int64_t* iterations);
 
+/* The 12 vertices and 20 triangles of the icosahedron (Icosahedral.jl:24-35). */
+int dmx_icosahedron(double* coords, int64_t* triangles);
+
+/* One bisection level (Icosahedral.jl:38-48): nv + ne new vertices in x_out
+ * ((nv + 3*nt/2) x 3) and 4*nt triangles in tri_out. */
+int dmx_bisect(int64_t nv, const double* x, int64_t nt, const int64_t* tri, double* x_out, int64_t* tri_out);
+
+/* Counter-clockwise triangle star of every vertex (Icosahedral.jl:130-147), CSR:
+ * star_offsets (nv + 1), stars (3 * nt). */
+int dmx_triangle_stars(int64_t nv, const int64_t* tri, int64_t nt, int64_t* star_offsets, int64_t* stars);
+
+/* is_delaunay for every triangle (Icosahedral.jl:210-216): 0, or -3 with a message. */
+int dmx_delaunay(const double* x, int64_t nv, const int64_t* tri, int64_t nt);
+
+/* Mean length of the first edge of every triangle, the scale of Lloyd's tolerance
+ * (Icosahedral.jl:148), summed in a thread-count independent order. */
+int dmx_mean_edge_length(const double* x, const int64_t* tri, int64_t nt, double* h);
+
 /* Lloyd step residual only: largest |centroid - x| over free vertices (lloyd_step!,
  * src/Grids/Icosahedral.jl:186-207), without moving x. */
 int dmx_lloyd_residual(const double* x, int64_t nv, const int64_t* tri, int64_t nt,

@@ -160,6 +160,19 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
  * kernel must touch counted once), in the order of trisk_kernel_times. */
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, double* bytes, int32_t* count);
 
+/* --- mesh setup on the device ------------------------------------------------------
+ * lloyd!(x, triangles; fixed=1:nfixed, tolerance, history, max_iterations)
+ * (src/Grids/Icosahedral.jl:126-184) on HIP device `device`: x (host, nv x 3) is moved in
+ * place to the centroids of the spherical Voronoi cells of the counter-clockwise
+ * triangulation tri (host, nt x 3); star_offsets/stars are the triangle stars of the
+ * vertices (dmx_triangle_stars of libdmplex_host.so).  Returns 0 when converged, 1 when
+ * max_iterations was reached (the reference's warning); *residual (may be NULL) receives
+ * the last residual in units of the mean edge length.  The Delaunay check of the
+ * result is left to the caller (dmx_delaunay).  Synchronous. */
+int trisk_lloyd(int32_t device, double* x, int64_t nv, const int64_t* tri, int64_t nt, const int64_t* star_offsets,
+                const int64_t* stars, int64_t nfixed, double tolerance, int32_t history, int64_t max_iterations,
+                int64_t* iterations, double* residual);
+
 #ifdef __cplusplus
 }
 #endif
