@@ -26,7 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12                      # B/s, MI355X HBM3E
-MODEL_BYTES = {"fp64": 4064.0, "fp32_bulk": 2704.0}   # algorithmic bytes per cell-step
+MODEL_BYTES = {"fp64": 4064.0, "fp64_pair": 4064.0, "fp32_bulk": 2704.0}  # model bytes per cell-step
 METRIC = "cell-steps/s at SCVT lvl9, FP32-bulk vs FP64 (same GPU) + % of HBM peak"
 
 
@@ -41,9 +41,16 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--level", type=int, default=9)
     p.add_argument("--dt", type=float, default=45.0)
-    p.add_argument("--precision", choices=["both", "fp64", "fp32_bulk"], default="both")
+    p.add_argument("--precision", choices=["both", "fp64", "fp32_bulk"], default="both",
+                   help="both: FP64 (bit-exact tables), FP64 on the cell-pair path, and FP32-bulk")
     p.add_argument("--cpu-steps", type=int, default=100, help="RK4 steps of the CPU baseline sample (0: skip)")
     p.add_argument("--no-kernel-events", action="store_true")
+    p.add_argument("--path", choices=["auto", "tables", "pair"], default="auto",
+                   help="kernel path (auto: bit-exact tables for FP64, cell pairs for FP32-bulk)")
+    p.add_argument("--no-scvt", action="store_true",
+                   help="skip Lloyd (same topology and kernels; for short profiling runs only)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="run a plain device copy of MIB MiB first (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -59,10 +66,13 @@ def kernel_stats(m):
     D.check(D.lib().trisk_kernel_times(m.handle, 8, names, avg.ctypes.data_as(C.POINTER(C.c_double)),
                                        nl.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(cnt)))
     by = np.zeros(8)
+    bnames = (C.c_char_p * 8)()
     cnt2 = C.c_int32()
-    D.check(D.lib().trisk_kernel_bytes(m.handle, 8, by.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cnt2)))
-    return [dict(name=names[i].decode(), avg_ms=float(avg[i]), launches=int(nl[i]), bytes=float(by[i]))
-            for i in range(cnt.value)]
+    D.check(D.lib().trisk_kernel_bytes(m.handle, 8, bnames, by.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(cnt2)))
+    bytes_of = {bnames[i].decode(): float(by[i]) for i in range(cnt2.value)}
+    return [dict(name=names[i].decode(), avg_ms=float(avg[i]), launches=int(nl[i]),
+                 bytes=bytes_of.get(names[i].decode(), 0.0)) for i in range(cnt.value)]
 
 
 def load_traffic(kernel: str, precision: str):
@@ -77,14 +87,16 @@ def load_traffic(kernel: str, precision: str):
         return None
 
 
-def run_gpu(grid, y0, precision, args, params):
+def run_gpu(grid, y0, precision, args, params, path=None):
     import torch
 
     from dmplex_amd import HIPBackend, TRiSKShallowWater, diagnostics, rk4_steps_
     from dmplex_amd import _device as D
 
-    log(f"[bench] building {precision} model")
-    m = TRiSKShallowWater(grid, backend=HIPBackend(0, precision), **params)
+    path = path or args.path
+    log(f"[bench] building {precision} model ({path} path)")
+    m = TRiSKShallowWater(grid, backend=HIPBackend(0, precision), path=path, **params)
+    log(f"[bench] {precision} path: {m.path_info()[1]}")
     y = m.upload_state(y0)
     mass0, energy0, _ = diagnostics(m, y)
     rk4_steps_(y, args.dt, args.warmup, m)
@@ -105,8 +117,8 @@ def run_gpu(grid, y0, precision, args, params):
     mass, energy, _ = diagnostics(m, y)
     nc = grid.num_cells()
     value = nc * args.steps / (ms / 1e3)
-    out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall,
-               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK,
+    out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall, path=m.path_info()[1],
+               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK, path_name=path,
                mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0, kernels=stats)
     if stats:
         dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
@@ -145,21 +157,32 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU halo exchange is "
                                    "design-only)"}))
         sys.exit(2)
-    import torch  # noqa: F401
+    import torch
 
     from dmplex_amd import TRiSKGrid, create_hexagon_mesh
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
     t = time.perf_counter()
     log(f"[bench] generating SCVT hexagon mesh level {args.level}")
-    dm = create_hexagon_mesh(args.level, centroidal=True, device=0, log=lambda m: log(f"[bench] {m}"))
+    if args.calibrate_copy:
+        n = args.calibrate_copy * 2 ** 20 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy} MiB done")
+        del a, b
+    dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0, log=lambda m: log(f"[bench] {m}"))
     grid = TRiSKGrid(dm, centers=dm.generators)
     params = earth_parameters()
     y0 = seeded_state(grid)
     log(f"[bench] mesh + tables + state in {time.perf_counter() - t:.1f} s: "
         f"{grid.num_cells()} cells, {grid.num_edges()} edges, {grid.num_vertices()} vertices")
-    precisions = ["fp64", "fp32_bulk"] if args.precision == "both" else [args.precision]
-    res = {p: run_gpu(grid, y0, p, args, params) for p in precisions}
+    if args.precision == "both":
+        runs = [("fp64", "fp64", None), ("fp64_pair", "fp64", "pair"), ("fp32_bulk", "fp32_bulk", None)]
+    else:
+        runs = [(args.precision, args.precision, None)]
+    res = {key: run_gpu(grid, y0, prec, args, params, path) for key, prec, path in runs}
     head = res.get("fp32_bulk", res.get("fp64"))
     head_p = "fp32_bulk" if "fp32_bulk" in res else "fp64"
     cpu = None
@@ -185,6 +208,8 @@ def main():
         line[p]["kernels"] = r["kernels"]
     if "fp64" in res and "fp32_bulk" in res:
         line["speedup_fp32_bulk_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
+    if "fp64_pair" in res and "fp32_bulk" in res:
+        line["speedup_fp32_bulk_vs_fp64_pair"] = res["fp32_bulk"]["value"] / res["fp64_pair"]["value"]
     print(json.dumps(line), flush=True)
 
 

@@ -22,19 +22,22 @@ class TriskModelDesc(C.Structure):
                 ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p), ("vk_height", f64p),
                 ("g_edges", i64p), ("g_pv_edges", i64p), ("g_weights", f64p), ("g_rhombus", f64p),
                 ("g_coriolis", f64p), ("pv_flux", C.c_int32), ("reserved", C.c_int32),
-                ("cell_points", f64p)]
+                ("cell_points", f64p), ("primal_lengths", f64p), ("dual_lengths", f64p),
+                ("cell_kites", f64p)]
 
 
 class TriskOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_size", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("path", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 TRISK_FP64, TRISK_FP32_BULK = 0, 1
 PV_FLUX = {"trisk": 0, "gassmann": 1, "gassmann_mean": 2}
+PATHS = {"auto": 0, "tables": 1, "pair": 2}
 
 # Every entry point declared in include/trisk_hip.h, with its argument types.
 SIGNATURES = {
+    "trisk_check_tables": [C.POINTER(TriskModelDesc), C.POINTER(C.c_int32)],
     "trisk_create": [C.POINTER(TriskModelDesc), C.POINTER(TriskOptions), C.POINTER(vp)],
     "trisk_destroy": [vp],
     "trisk_state_length": [vp, i64p],
@@ -52,7 +55,8 @@ SIGNATURES = {
     "trisk_tangential_velocity": [vp, vp, vp, vp],
     "trisk_set_timing": [vp, C.c_int32],
     "trisk_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
-    "trisk_kernel_bytes": [vp, C.c_int32, f64p, C.POINTER(C.c_int32)],
+    "trisk_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
+    "trisk_info": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_char_p)],
     "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 

@@ -125,6 +125,14 @@ struct Params {
     I* zv;              // nv (Gassmann, diagnostics)
     I* hv;              // nv (Gassmann, vertex KE, diagnostics)
     I* Kv;              // nv (vertex KE)
+    // cell-pair path
+    const int32_t* cmeta;  // nc: size | negative-sign bits << 4
+    const int32_t* ccv;    // 6 x nc corners
+    const C* pw;           // 15 x nc pair weights
+    const C* len;          // ne: l_e
+    const C* dual;         // ne: d_e
+    I* G;                  // ne: l_e F_e
+    I* T;                  // 2 x ne: PV-flux half sums of the left / right cell
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;
@@ -143,6 +151,13 @@ __device__ __forceinline__ Tile tile_of(int64_t b, int64_t t0, int64_t t1) {
     int64_t x = b % kXcd, j = b / kXcd;
     if (j < t0) return {0, (x * t0 + j) * kBlock};
     return {1, (x * t1 + (j - t0)) * kBlock};
+}
+
+__device__ __forceinline__ Tile tile_of3(int64_t b, int64_t t0, int64_t t1, int64_t t2) {
+    int64_t x = b % kXcd, j = b / kXcd;
+    if (j < t0) return {0, (x * t0 + j) * kBlock};
+    if (j < t0 + t1) return {1, (x * t1 + (j - t0)) * kBlock};
+    return {2, (x * t2 + (j - t0 - t1)) * kBlock};
 }
 
 inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
@@ -324,6 +339,113 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 }
 
 // ---------------------------------------------------------------------------------
+// Cell-pair path (TRiSK PV flux, cell kinetic energy): three launches per tendency.
+// Inside a cell the TRiSK weights are antisymmetric, w_{k',k} = -w_{k,k'} (the kite
+// fractions of a cell add up to one), so a cell stores the 15 weights of its edge
+// pairs k < k' (TRiSKGrid.jl:140-155) instead of every edge storing ten coefficients and
+// indices.  With G_e = l_e F_e the PV flux of edge e is
+//   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),
+// and the continuity is dh_i = -(sum_k s_k G_k)/A_i (TRiSKShallowWater.jl:275-305).
+
+// W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e)
+template <class P, class UIn>
+__global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const UIn* __restrict__ u,
+                                                            const double* __restrict__ h, int64_t t0, int64_t t1,
+                                                            int64_t t2) {
+    using C = typename P::C;
+    using I = typename P::I;
+    Tile t = tile_of3(blockIdx.x, t0, t1, t2);
+    if (t.range == 0) {
+        int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        C s = 0;
+        for (int k = 0; k < p.kc; ++k) {
+            C ue = C(u[ld(p.ce + k * p.nc + i)]);
+            s += ld(p.kin + k * p.nc + i) * (ue * ue);
+        }
+        double K = double(s);
+        p.B[i] = K + p.g * h[i];
+    } else if (t.range == 1) {
+        int64_t v = t.first + threadIdx.x;
+        if (v >= p.nv) return;
+        C z = 0, eta = 0;
+        for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
+        for (int k = 0; k < p.kv; ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
+        p.qv[v] = I((z + ld(p.fv + v)) / eta);
+    } else {
+        int64_t e = t.first + threadIdx.x;
+        if (e >= p.ne) return;
+        int32_t c0 = ld(p.ec + e), c1 = ld(p.ec + p.ne + e);
+        C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
+        p.G[e] = I(ld(p.len + e) * (he * C(u[e])));
+    }
+}
+
+// W2: cells: PV-flux half sums of their edges and the continuity with its RK4 update
+template <class P>
+__global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
+    using C = typename P::C;
+    const int64_t nc = p.nc;
+    int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    const int32_t meta = ld(p.cmeta + i);
+    const int n = meta & 15;
+    int32_t E[6], V[6];
+    C W[15];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        E[k] = ld(p.ce + k * nc + i);
+        V[k] = ld(p.ccv + k * nc + i);
+    }
+#pragma unroll
+    for (int q = 0; q < 15; ++q) W[q] = ld(p.pw + q * nc + i);
+    C G[6], Q[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        G[k] = C(p.G[E[k]]);
+        Q[k] = C(p.qv[V[k]]);
+    }
+    C qe[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) qe[k] = (Q[k] + (k + 1 < n ? Q[k + 1 < 6 ? k + 1 : 0] : Q[0])) / 2;
+    C H[6] = {0, 0, 0, 0, 0, 0};
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
+            if (k2 < n) {
+                const C tq = W[q] * (qe[k] + qe[k2]);
+                H[k] += tq * G[k2];
+                H[k2] -= tq * G[k];
+            }
+        }
+    double div = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (k < n) {
+            const int side = (meta >> (4 + k)) & 1;
+            p.T[side * p.ne + E[k]] = typename P::I(H[k]);
+            div += side ? -double(G[k]) : double(G[k]);
+        }
+    }
+    rk_update<double>(r, p.ne + i, -div / ld(p.area_c + i), r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+}
+
+// W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update
+template <class P>
+__global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r) {
+    using C = typename P::C;
+    using U = typename P::U;
+    int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (e >= p.ne) return;
+    const C d = ld(p.dual + e);
+    const C flux = (C(ld(p.T + e)) + C(ld(p.T + p.ne + e))) / (2 * d);
+    const double grad = (p.B[ld(p.ec + p.ne + e)] - p.B[ld(p.ec + e)]) / double(d);
+    rk_update<U>(r, e, double(flux) - grad, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+}
+
+// ---------------------------------------------------------------------------------
 // generic gather y_j = sum_k w[k, j] x[idx[k, j]] (edge_sum_kernel!, :307-314)
 
 __global__ __launch_bounds__(kBlock) void edge_sum_kernel(int64_t n, int K, const int32_t* __restrict__ idx,
@@ -414,8 +536,10 @@ static int alloc(std::vector<DevBuf>& owned, size_t bytes, void** out) {
 
 static std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
 
-enum KernelId { K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_COUNT = 3 };
-static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel", "update_kernel"};
+enum KernelId { K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_PAIR_STATE = 3, K_PAIR_CELL = 4, K_PAIR_EDGE = 5,
+                K_COUNT = 6 };
+static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel",      "update_kernel",
+                                            "pair_state_kernel",  "pair_cell_kernel", "pair_edge_kernel"};
 
 }  // namespace trisk
 
@@ -444,8 +568,7 @@ struct trisk_handle {
     std::vector<hipEvent_t> events;     // pairs (start, stop)
     std::vector<int> event_kernel;      // kernel id per pair
     size_t events_used = 0;
-    double kernel_ms[K_COUNT] = {0, 0, 0};
-    int64_t kernel_launches[K_COUNT] = {0, 0, 0};
+
 };
 
 namespace trisk {
@@ -509,6 +632,13 @@ static int setup_params(trisk_handle* H) {
     rc |= coef(L.fv, &p.fv);
     rc |= coef(L.vkw, &p.vkw);
     rc |= upload(o, L.area_v, &p.area_v);
+    if (L.pair) {
+        rc |= upload(o, L.cmeta, &p.cmeta);
+        rc |= upload(o, L.ccv, &p.ccv);
+        rc |= coef(L.pw, &p.pw);
+        rc |= coef(L.len, &p.len);
+        rc |= coef(L.dual, &p.dual);
+    }
     if (rc) return rc;
     void* q;
     if ((rc = alloc(o, L.nc * sizeof(double), &q))) return rc;
@@ -517,6 +647,10 @@ static int setup_params(trisk_handle* H) {
     p.K = static_cast<double*>(q);
     if ((rc = alloc(o, L.ne * sizeof(Pair<I>), &q))) return rc;
     p.Fq = static_cast<Pair<I>*>(q);
+    if ((rc = alloc(o, L.ne * sizeof(I), &q))) return rc;
+    p.G = static_cast<I*>(q);
+    if ((rc = alloc(o, 2 * L.ne * sizeof(I), &q))) return rc;
+    p.T = static_cast<I*>(q);
     for (I** w : {&p.qv, &p.zv, &p.hv, &p.Kv}) {
         if ((rc = alloc(o, L.nv * sizeof(I), &q))) return rc;
         *w = static_cast<I*>(q);
@@ -566,9 +700,41 @@ static int timed_end(hipEvent_t stop, hipStream_t s) {
 
 // One tendency: P1, P2, P3 with the given stage input and update mode.
 template <class P, class UIn>
+static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
+    const Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    hipEvent_t stop;
+    int rc;
+    {
+        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = tiles_per_xcd(L.ne);
+        if ((rc = timed_begin(H, K_PAIR_STATE, s, &stop))) return rc;
+        hipLaunchKernelGGL((pair_state_kernel<P, UIn>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0, s, p,
+                           u, h, t0, t1, t2);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    {
+        if ((rc = timed_begin(H, K_PAIR_CELL, s, &stop))) return rc;
+        hipLaunchKernelGGL((pair_cell_kernel<P>), dim3(unsigned((L.nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p,
+                           r);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    {
+        if ((rc = timed_begin(H, K_PAIR_EDGE, s, &stop))) return rc;
+        hipLaunchKernelGGL((pair_edge_kernel<P>), dim3(unsigned((L.ne + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p,
+                           r);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+    }
+    return 0;
+}
+
+template <class P, class UIn>
 static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
     const Params<P>& p = params<P>(H);
     const Layout& L = H->L;
+    if (L.pair && p.qv_out == nullptr) return run_pair<P, UIn>(H, u, h, r, s);
     const bool vk = L.vertex_ke, gm = L.gassmann;
     const bool zh = gm || vk || p.qv_out != nullptr;
     hipEvent_t stop;
@@ -722,6 +888,16 @@ extern "C" {
 
 const char* trisk_last_error(void) { return g_error.c_str(); }
 
+int trisk_check_tables(const trisk_model_desc* desc, int32_t* recomputed) {
+    Layout L;
+    std::string err;
+    if (!build_layout(desc, L, err)) return set_err(TRISK_EINVAL, err);
+    build_pairs(desc, L);
+    if (recomputed) *recomputed = L.pair ? 1 : 0;
+    g_error = L.pair_reason;
+    return 0;
+}
+
 int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_handle** out) {
     if (!out) return set_err(TRISK_EINVAL, "null output pointer");
     *out = nullptr;
@@ -731,10 +907,6 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
         return set_err(TRISK_EINVAL, "precision must be TRISK_FP64 or TRISK_FP32_BULK");
     if (o.block_size != 0 && o.block_size != kBlock)
         return set_err(TRISK_EINVAL, "block_size must be 0 or 256");
-    int ndev = 0;
-    HIP_TRY(hipGetDeviceCount(&ndev));
-    if (o.device < 0 || o.device >= ndev) return set_err(TRISK_EINVAL, "no such HIP device");
-    HIP_TRY(hipSetDevice(o.device));
     auto* H = new trisk_handle;
     H->precision = o.precision;
     H->device = o.device;
@@ -742,6 +914,28 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
     if (!build_layout(desc, H->L, err)) {
         delete H;
         return set_err(TRISK_EINVAL, err);
+    }
+    if (o.path < 0 || o.path > 2) {
+        delete H;
+        return set_err(TRISK_EINVAL, "path must be TRISK_PATH_AUTO, TRISK_PATH_TABLES or TRISK_PATH_PAIR");
+    }
+    const bool want_pair = o.path == TRISK_PATH_PAIR || (o.path == TRISK_PATH_AUTO && o.precision == TRISK_FP32_BULK);
+    if (want_pair) {
+        build_pairs(desc, H->L);
+        if (!H->L.pair && o.path == TRISK_PATH_PAIR) {
+            std::string why = H->L.pair_reason;
+            delete H;
+            return set_err(TRISK_EINVAL, "cell-pair path unavailable: " + why);
+        }
+    } else {
+        H->L.pair_reason = o.precision == TRISK_FP64 ? "FP64: coefficient tables, bit-exact with the reference"
+                                                     : "coefficient tables requested";
+    }
+    int ndev = 0;
+    hipError_t he = hipGetDeviceCount(&ndev);
+    if (he != hipSuccess || o.device < 0 || o.device >= ndev || hipSetDevice(o.device) != hipSuccess) {
+        delete H;
+        return set_err(TRISK_EHIP, "no such HIP device");
     }
     int rc = o.precision == TRISK_FP64 ? setup_params<Fp64>(H) : setup_params<Fp32Bulk>(H);
     if (rc) {
@@ -925,8 +1119,8 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
                        int32_t* count) {
     CHECK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
-    double ms[K_COUNT] = {0, 0, 0};
-    int64_t n[K_COUNT] = {0, 0, 0};
+    double ms[K_COUNT] = {0};
+    int64_t n[K_COUNT] = {0};
     for (size_t i = 0; i + 1 < h->events_used; i += 2) {
         HIP_TRY(hipEventSynchronize(h->events[i + 1]));
         float t = 0;
@@ -936,38 +1130,59 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
         n[k] += 1;
     }
     int c = 0;
-    for (int k = 0; k < K_COUNT && c < max; ++k, ++c) {
+    for (int k = 0; k < K_COUNT && c < max; ++k) {
+        if (n[k] == 0) continue;  // kernels this handle's path does not launch
         if (names) names[c] = kKernelNames[k];
-        if (avg_ms) avg_ms[c] = n[k] ? ms[k] / double(n[k]) : 0.0;
+        if (avg_ms) avg_ms[c] = ms[k] / double(n[k]);
         if (launches) launches[c] = n[k];
+        ++c;
     }
     if (count) *count = c;
     return 0;
 }
 
-int trisk_kernel_bytes(trisk_handle* h, int32_t max, double* bytes, int32_t* count) {
+int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count) {
     CHECK_HANDLE(h);
     const Layout& L = h->L;
-    const double c = h->precision == TRISK_FP64 ? 8.0 : 4.0;  // coefficient / intermediate bytes
-    const double iw = 4.0;                                     // index bytes
+    // Algorithmic HBM bytes of one launch: every element of every array the launch must
+    // read or write counted once (gathered arrays by their unique size), in a stage 2/3
+    // launch (state read as the stage vector, RK4 accumulator read and written).
+    const bool f64 = h->precision == TRISK_FP64;
+    const double c = f64 ? 8.0 : 4.0;   // coefficient and intermediate bytes
+    const double uw = f64 ? 8.0 : 4.0;  // stage velocity
+    const double iw = 4.0;              // int32 index
     const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv);
-    // each element of every array the kernel must touch, counted once (stage 2-3 launches)
-    double u_in = h->precision == TRISK_FP64 ? 8.0 : 4.0;
-    double p1 = nc * L.kc * (iw + c) + nc * 8.0 /*h*/ + ne * u_in /*u*/ + nc * 8.0 /*B*/ +
-                nv * L.kv * 2 * (iw + c) + nv * c /*f*/ + nv * c /*q_v*/;
-    double p2 = ne * 2 * (iw + c) + ne * 2 * iw /*edge cells, vertices*/ + ne * u_in + nc * 8.0 + nv * c +
-                ne * 2 * c /*(F, q)*/;
-    double p3 = ne * L.kt * (iw + c) + ne * 2 * iw + ne * c /*1/d*/ + ne * 2 * c /*(F, q) read*/ + nc * 8.0 /*B*/ +
-                nc * L.kc * (iw + 8.0) +
-                /* RK4 stage 2/3: y read, acc read+write, stage write */
-                ne * (8.0 + 16.0 + u_in) + nc * (8.0 + 16.0 + 8.0);
-    if (L.vertex_ke || L.gassmann) {
-        // not modelled beyond the TRiSK/cell-KE tables
+    const double KC = L.kc, KV = L.kv, KT = L.kt;
+    const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;  // y, acc r/w, stage
+    std::vector<std::pair<const char*, double>> v;
+    if (L.pair) {
+        double w1 = nc * (KC * (iw + c) + 8.0 + 8.0)             /* ce, kin, h, B */
+                    + nv * (KV * 2 * (iw + c) + c + c)             /* ve, curl, vc, vh, f, q_v */
+                    + ne * (2 * iw + 2 * c + uw + c + c);          /* ec, eh, u, l, G */
+        double w2 = nc * (6 * 2 * iw + 15 * c + iw + 8.0 + rk_c)  /* ce, corners, weights, meta, A, RK4 */
+                    + ne * c + nv * c                              /* G, q_v */
+                    + 2 * ne * c;                                  /* half sums */
+        double w3 = ne * (2 * c + 2 * iw + c + rk_e) + nc * 8.0;  /* half sums, ec, d, RK4, B */
+        v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
+    } else {
+        double p1 = nc * (KC * (iw + c) + 8.0 + 8.0) + ne * uw + nv * (KV * 2 * (iw + c) + c + c);
+        double p2 = ne * (2 * iw + 2 * c + 2 * iw + uw + 2 * c) + nc * 8.0 + nv * c;
+        double p3 = ne * (KT * (iw + c) + 2 * iw + c + 2 * c + rk_e) + nc * (KC * (iw + 8.0) + 8.0 + rk_c);
+        v = {{kKernelNames[K_CELL_VERTEX], p1}, {kKernelNames[K_EDGE], p2}, {kKernelNames[K_UPDATE], p3}};
     }
-    double v[K_COUNT] = {p1, p2, p3};
     int k = 0;
-    for (; k < K_COUNT && k < max; ++k) bytes[k] = v[k];
+    for (; k < int(v.size()) && k < max; ++k) {
+        if (names) names[k] = v[k].first;
+        bytes[k] = v[k].second;
+    }
     if (count) *count = k;
+    return 0;
+}
+
+int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note) {
+    CHECK_HANDLE(h);
+    if (recomputed) *recomputed = h->L.pair ? 1 : 0;
+    if (note) *note = h->L.pair_reason.c_str();
     return 0;
 }
 

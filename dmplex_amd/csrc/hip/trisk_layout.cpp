@@ -232,4 +232,141 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     return true;
 }
 
+void build_pairs(const trisk_model_desc* d, Layout& L) {
+    L.pair = false;
+    if (!d->primal_lengths || !d->dual_lengths || !d->cell_kites) {
+        L.pair_reason = "grid fields (primal_lengths, dual_lengths, cell_kites) not given";
+        return;
+    }
+    if (L.vertex_ke || L.gassmann) {
+        L.pair_reason = "vertex kinetic energy and the Gassmann PV flux stream their tables";
+        return;
+    }
+    if (L.kc > kPairSlots) {
+        L.pair_reason = "cells with more than six edges stream their tables";
+        return;
+    }
+    const int64_t nc = L.nc, ne = L.ne;
+    const int KC = L.kc, KT = L.kt;
+    const double* l = d->primal_lengths;
+    const double* dl = d->dual_lengths;
+    std::vector<int32_t> size(nc), neg(nc);
+    std::vector<double> W(size_t(kPairs) * nc, 0.0);
+    int bad = 0;
+    // sizes, signs, divergence (s l)/A and the pair weights of every cell
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < nc; ++i) {
+        int n = 0;
+        while (n < KC && d->divergence[n + KC * i] != 0.0) ++n;
+        for (int k = n; k < KC; ++k) bad |= d->divergence[k + KC * i] != 0.0;
+        if (n < 3) bad |= 1;
+        size[i] = n;
+        int32_t nb = 0;
+        double R[kPairSlots];
+        for (int k = 0; k < n; ++k) {
+            double div = d->divergence[k + KC * i];
+            double sg = div > 0 ? 1.0 : -1.0;
+            if (sg < 0) nb |= 1 << k;
+            int64_t e = d->cell_edges[k + KC * i];
+            double ref = (sg * l[e]) / d->cell_areas[i];
+            bad |= std::fabs(ref - div) > 1e-14 * std::fabs(div);
+            R[k] = d->cell_kites[k + KC * i] / d->cell_areas[i];
+        }
+        neg[i] = nb;
+        for (int k = 0; k < n; ++k) {
+            double S = 0.0, sk = (nb >> k) & 1 ? -1.0 : 1.0;
+            for (int k2 = k + 1; k2 < n; ++k2) {
+                S += R[k2];
+                double s2 = (nb >> k2) & 1 ? -1.0 : 1.0;
+                W[pair_index(k, k2) + size_t(kPairs) * i] = sk * s2 * (0.5 - S);
+            }
+        }
+    }
+    if (bad) {
+        L.pair_reason = "divergence differs from (s l_e)/A_i";
+        return;
+    }
+    // every TRiSK entry of edge e, in the reference's row order, against w l_e'/d_e
+    auto weight = [&](int64_t c, int k, int k2) {  // w of the walk from slot k to slot k2 in cell c
+        return k < k2 ? W[pair_index(k, k2) + size_t(kPairs) * c] : -W[pair_index(k2, k) + size_t(kPairs) * c];
+    };
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t e = 0; e < ne; ++e) {
+        bad |= std::fabs(d->inverse_dual[e] - 1.0 / dl[e]) > 1e-14 * std::fabs(d->inverse_dual[e]);
+        int64_t c0 = d->edge_cells[2 * e], c1 = d->edge_cells[2 * e + 1];
+        int64_t cells[2] = {std::min(c0, c1), std::max(c0, c1)};
+        double rowmax = 0;
+        for (int q = 0; q < KT; ++q) rowmax = std::max(rowmax, std::fabs(d->trisk[q + KT * e]));
+        int pos = 0;
+        for (int side = 0; side < 2; ++side) {
+            int64_t c = cells[side];
+            int n = size[c];
+            int k = 0;
+            while (k < n && d->cell_edges[k + KC * c] != e) ++k;
+            if (k == n) {
+                bad |= 1;
+                break;
+            }
+            for (int j = 1; j < n; ++j) {
+                int k2 = (k + j) % n;
+                int64_t e2 = d->cell_edges[k2 + KC * c];
+                double coef = weight(c, k, k2) * l[e2] / dl[e];
+                if (pos >= KT || d->trisk_edges[pos + KT * e] != e2 ||
+                    std::fabs(coef - d->trisk[pos + KT * e]) > 1e-13 * rowmax) {
+                    bad |= 1;
+                    break;
+                }
+                ++pos;
+            }
+        }
+        for (int q = pos; q < KT; ++q) bad |= d->trisk[q + KT * e] != 0.0;
+    }
+    if (bad) {
+        L.pair_reason = "trisk or inverse_dual differ from their recomputation from the grid fields";
+        return;
+    }
+    // corners: corner k starts edge k (get_cell_vertices, Plex.jl:226-230) and the loop closes
+    std::vector<int64_t> corner(size_t(KC) * nc, 0);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < nc; ++i) {
+        int n = size[i];
+        for (int k = 0; k < n; ++k) {
+            int64_t e = d->cell_edges[k + KC * i];
+            corner[k + size_t(KC) * i] = d->edge_vertices[((neg[i] >> k) & 1) + 2 * e];
+        }
+        for (int k = 0; k < n; ++k) {
+            int64_t e = d->cell_edges[k + KC * i];
+            int64_t stop = d->edge_vertices[1 - ((neg[i] >> k) & 1) + 2 * e];
+            bad |= stop != corner[(k + 1) % n + size_t(KC) * i];
+        }
+    }
+    if (bad) {
+        L.pair_reason = "cell edge lists are not closed loops";
+        return;
+    }
+    // device numbering
+    L.cmeta.resize(nc);
+    L.ccv.assign(size_t(kPairSlots) * nc, 0);
+    L.pw.assign(size_t(kPairs) * nc, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < nc; ++j) {
+        int64_t i = L.cperm[j];
+        L.cmeta[j] = size[i] | (neg[i] << 4);
+        for (int k = 0; k < kPairSlots; ++k) {
+            int64_t v = k < size[i] ? corner[k + size_t(KC) * i] : corner[size_t(KC) * i];
+            L.ccv[size_t(k) * nc + j] = int32_t(L.vinv[v]);
+        }
+        for (int q = 0; q < kPairs; ++q) L.pw[size_t(q) * nc + j] = W[q + size_t(kPairs) * i];
+    }
+    L.len.resize(ne);
+    L.dual.resize(ne);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < ne; ++j) {
+        L.len[j] = l[L.eperm[j]];
+        L.dual[j] = dl[L.eperm[j]];
+    }
+    L.pair = true;
+    L.pair_reason = "cell-pair TRiSK weights (no per-edge TRiSK tables)";
+}
+
 }  // namespace trisk

@@ -27,10 +27,31 @@ struct Layout {
     // vertices
     std::vector<int32_t> ve, vc;
     std::vector<double> curl, vh, fv, vkw, area_v;
+    // cell-pair path (TRiSK PV flux and cell kinetic energy only): per cell the 15
+    // antisymmetric TRiSK weights w_{k,k'} (k < k') of its edge pairs
+    bool pair = false;
+    std::string pair_reason;             // why the table path is used
+    std::vector<int32_t> cmeta;          // per cell: size | negative-sign bits << 4
+    std::vector<int32_t> ccv;            // corner k of cell (start of edge k), kc x nc
+    std::vector<double> pw;              // 15 x nc pair weights (kc <= 6)
+    std::vector<double> len, dual;       // l_e, d_e
 };
+
+constexpr int kPairSlots = 6;   // cells of up to six edges
+constexpr int kPairs = 15;      // pairs k < k' of six slots, in the order of pair_index()
+inline int pair_index(int k, int k2) {  // k < k2 < 6
+    return k * (11 - k) / 2 + (k2 - k - 1);
+}
 
 // Validates `d` (every index in range, widths positive) and fills `L`.  Returns false
 // with a message on a bad table.
 bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
+
+// Builds the cell-pair tables from the grid fields: w_{k,k'} = s_k s_k' (1/2 - sum of the
+// kite fractions of the corners k+1..k') (TRiSKGrid.jl:140-155), and checks that
+// w l_e'/d_e reproduces every entry of the caller's trisk table (to 1e-13 of the row's
+// largest entry), the divergence (s l_e)/A_i and 1/d_e; otherwise L.pair stays false and
+// L.pair_reason says why.
+void build_pairs(const trisk_model_desc* d, Layout& L);
 
 }  // namespace trisk

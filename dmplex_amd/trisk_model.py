@@ -116,26 +116,7 @@ class ModelTables:
                 height=_F((1 - alpha) * 2 * by_edge / (l * d)[None, :]))
 
 
-class TRiSKShallowWater(ModelTables):
-    """Nonlinear rotating shallow water in vector-invariant form on the Voronoi C grid
-    with the TRiSK operators (TRiSKShallowWater.jl:1-66), on a HIP device."""
-
-    def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, pv_flux: str = "trisk",
-                 kinetic_energy="cell", comm=None, part=None, backend=None, hilbert: bool = True):
-        if comm is not None or part is not None:
-            raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
-        if backend is None:
-            backend = HIPBackend()
-        if not isinstance(backend, HIPBackend):
-            raise NotImplementedError("only HIPBackend() is available: the product has no CPU path")
-        super().__init__(dm_or_grid, centers, gravity, coriolis, pv_flux, kinetic_energy)
-        self.backend = backend
-        self._hilbert = hilbert
-        self._handle = None
-        self._create()
-
-    # --- device handle ---------------------------------------------------------------
-    def descriptor(self) -> D.TriskModelDesc:
+    def descriptor(self, hilbert: bool = True) -> D.TriskModelDesc:
         """The C-ABI view of the host tables (valid while the model lives)."""
         g = self.grid
         vk, gm = self.vertex_kinetic, self.gassmann
@@ -153,16 +134,46 @@ class TRiSKShallowWater(ModelTables):
             ptr(gm["edges"], i) if gm else None, ptr(gm["pv_edges"], i) if gm else None,
             ptr(gm["weights"], f) if gm else None, ptr(gm["rhombus"], f) if gm else None,
             ptr(gm["coriolis"], f) if gm else None, D.PV_FLUX[self.pv_flux], 0,
-            ptr(g.centers, f) if self._hilbert else None)
+            ptr(g.centers, f) if hilbert else None,
+            ptr(g.primal_lengths, f), ptr(g.dual_lengths, f), ptr(g.cell_kites, f))
         return desc
+
+    def check_tables(self) -> tuple[bool, str]:
+        """Host-side validation through the C ABI (no device): (cell-pair path available?, reason)."""
+        flag = C.c_int32()
+        desc = self.descriptor()
+        D.check(D.lib().trisk_check_tables(C.byref(desc), C.byref(flag)))
+        return bool(flag.value), D.lib().trisk_last_error().decode()
+
+class TRiSKShallowWater(ModelTables):
+    """Nonlinear rotating shallow water in vector-invariant form on the Voronoi C grid
+    with the TRiSK operators (TRiSKShallowWater.jl:1-66), on a HIP device."""
+
+    def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, pv_flux: str = "trisk",
+                 kinetic_energy="cell", comm=None, part=None, backend=None, hilbert: bool = True,
+                 path: str = "auto"):
+        if comm is not None or part is not None:
+            raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
+        if backend is None:
+            backend = HIPBackend()
+        if not isinstance(backend, HIPBackend):
+            raise NotImplementedError("only HIPBackend() is available: the product has no CPU path")
+        super().__init__(dm_or_grid, centers, gravity, coriolis, pv_flux, kinetic_energy)
+        self.backend = backend
+        self._hilbert = hilbert
+        if path not in D.PATHS:
+            raise ValueError("path must be 'auto', 'tables' or 'pair'")
+        self._path = path
+        self._handle = None
+        self._create()
 
     def _create(self) -> None:
         torch = _torch()
         torch.cuda.set_device(self.backend.device)
         opts = D.TriskOptions(D.TRISK_FP64 if self.backend.precision == "fp64" else D.TRISK_FP32_BULK,
-                              self.backend.device, 0)
+                              self.backend.device, 0, D.PATHS[getattr(self, "_path", "auto")])
         h = C.c_void_p()
-        desc = self.descriptor()
+        desc = self.descriptor(self._hilbert)
         D.check(D.lib().trisk_create(C.byref(desc), C.byref(opts), C.byref(h)))
         self._handle = h
         n = np.zeros(1, dtype=np.int64)
@@ -181,6 +192,13 @@ class TRiSKShallowWater(ModelTables):
     @property
     def handle(self) -> C.c_void_p:
         return self._handle
+
+    def path_info(self) -> tuple[bool, str]:
+        """(cell-pair path?, reason) of the device kernels."""
+        flag = C.c_int32()
+        note = C.c_char_p()
+        D.check(D.lib().trisk_info(self._handle, C.byref(flag), C.byref(note)))
+        return bool(flag.value), note.value.decode()
 
     def permutation(self):
         """(cells, edges, vertices): reference index of every device entity."""

@@ -94,19 +94,39 @@ typedef struct {
     int32_t pv_flux;             /* TRISK_PV_* */
     int32_t reserved;
     const double* cell_points;   /* nc x 3 (NULL: keep the given order) for the Hilbert layout */
+    /* TRiSKGrid fields the coefficients were folded from (TRiSKGrid.jl:37-60), optional;
+     * they enable TRISK_PATH_PAIR, which is only used when `trisk`, `divergence` and
+     * `inverse_dual` agree with their recomputation from these fields. */
+    const double* primal_lengths; /* ne */
+    const double* dual_lengths;   /* ne */
+    const double* cell_kites;     /* kc x nc */
 } trisk_model_desc;
+
+/* Kernel paths.  TABLES streams the reference's coefficient tables and evaluates every
+ * formula in the reference's order (FP64: bit-exact with the reference algorithm).  PAIR
+ * stores the 15 antisymmetric TRiSK weights of each cell's edge pairs, rebuilt from the
+ * grid fields, instead of ten coefficients and indices per edge (fewer bytes; equal to
+ * the tables up to rounding).  AUTO: TABLES for TRISK_FP64, PAIR (when the grid fields
+ * are given and consistent) for TRISK_FP32_BULK. */
+enum { TRISK_PATH_AUTO = 0, TRISK_PATH_TABLES = 1, TRISK_PATH_PAIR = 2 };
 
 typedef struct {
     int32_t precision;           /* TRISK_FP64 or TRISK_FP32_BULK */
     int32_t device;              /* HIP device ordinal */
     int32_t block_size;          /* threads per workgroup (0: library default) */
-    int32_t reserved[5];
+    int32_t path;                /* TRISK_PATH_*: which kernels run the TRiSK/cell-KE step */
+    int32_t reserved[4];
 } trisk_options;
 
 typedef struct trisk_handle trisk_handle;
 
 /* Thread-local text of the last error. */
 const char* trisk_last_error(void);
+
+/* Host-only validation of a descriptor (every index in range, table widths), and
+ * whether TRISK_PATH_PAIR is available (*recomputed); the reason is in
+ * trisk_last_error().  Touches no device. */
+int trisk_check_tables(const trisk_model_desc* desc, int32_t* recomputed);
 
 /* TRiSKShallowWater(grid; ...; backend) (TRiSKShallowWater.jl:94-170): validates and
  * re-lays out the tables and uploads them.  Synchronous. */
@@ -156,9 +176,13 @@ int trisk_set_timing(trisk_handle* h, int32_t on);
 int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double* avg_ms, int64_t* launches,
                        int32_t* count);
 
-/* Algorithmic HBM bytes of one launch of each hot-path kernel (each array element the
- * kernel must touch counted once), in the order of trisk_kernel_times. */
-int trisk_kernel_bytes(trisk_handle* h, int32_t max, double* bytes, int32_t* count);
+/* Algorithmic HBM bytes of one launch of each hot-path kernel of this handle's path
+ * (each array element the kernel must touch counted once), with the kernel names. */
+int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count);
+
+/* Whether this handle runs TRISK_PATH_PAIR (*recomputed = 1) or the table path, and why
+ * (text owned by the handle). */
+int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
 
 /* --- mesh setup on the device ------------------------------------------------------
  * lloyd!(x, triangles; fixed=1:nfixed, tolerance, history, max_iterations)
