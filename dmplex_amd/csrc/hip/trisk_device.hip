@@ -47,7 +47,10 @@ int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg);
             return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-constexpr int kBlock = 256;
+#ifndef TRISK_BLOCK
+#define TRISK_BLOCK 256
+#endif
+constexpr int kBlock = TRISK_BLOCK;
 constexpr int kXcd = 8;
 
 // ---------------------------------------------------------------------------------
@@ -132,7 +135,15 @@ struct Params {
     const C* len;          // ne: l_e
     const C* dual;         // ne: d_e
     I* G;                  // ne: l_e F_e
-    I* T;                  // 2 x ne: PV-flux half sums of the left / right cell
+    I* T;                  // ne x 2: PV-flux half sums of the left / right cell
+    // entity records of the cell-pair path
+    const int32_t* rc_state;  // nc x 6
+    const C* rc_kin;          // nc x 6
+    const int32_t* rv_idx;    // nv x 6
+    const C* rv_coef;         // nv x 8
+    const int32_t* re_cells;  // ne x 2
+    const C* re_lh;           // ne x 2
+    const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;
@@ -162,9 +173,42 @@ __device__ __forceinline__ Tile tile_of3(int64_t b, int64_t t0, int64_t t1, int6
 
 inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
 
+// Streamed tables are read once per launch: nontemporal loads keep them from evicting
+// the gathered state and intermediates from the caches.
 template <class T>
 __device__ __forceinline__ T ld(const T* p) {
+#ifdef TRISK_PLAIN_LOADS
+    return *p;
+#else
     return __builtin_nontemporal_load(p);
+#endif
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// N consecutive elements starting at p (one entity record) with 16-byte (or, for records
+// of a size not divisible by 16, 8-byte) nontemporal vector loads; p must be aligned to
+// that width.
+template <class T, int N>
+__device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
+    constexpr int bytes = N * int(sizeof(T));
+    if constexpr (bytes % 16 == 0) {
+        const f32x4* q = reinterpret_cast<const f32x4*>(p);
+#pragma unroll
+        for (int c = 0; c < bytes / 16; ++c) {
+            f32x4 v = __builtin_nontemporal_load(q + c);
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 16 * c, &v, 16);
+        }
+    } else {
+        static_assert(bytes % 8 == 0, "records are multiples of 8 bytes");
+        const f32x2* q = reinterpret_cast<const f32x2*>(p);
+#pragma unroll
+        for (int c = 0; c < bytes / 8; ++c) {
+            f32x2 v = __builtin_nontemporal_load(q + c);
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 8 * c, &v, 8);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -347,8 +391,10 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 //   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),
 // and the continuity is dh_i = -(sum_k s_k G_k)/A_i (TRiSKShallowWater.jl:275-305).
 
-// W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e)
-template <class P, class UIn>
+// W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e).  REC: the tables as entity
+// records read with vector loads (fewer instructions; faster for the FP32 tables), else
+// slot-major arrays (faster for the FP64 ones, whose records span several cache lines).
+template <class P, class UIn, bool REC>
 __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const UIn* __restrict__ u,
                                                             const double* __restrict__ h, int64_t t0, int64_t t1,
                                                             int64_t t2) {
@@ -358,26 +404,57 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
     if (t.range == 0) {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
-        C s = 0;
-        for (int k = 0; k < p.kc; ++k) {
-            C ue = C(u[ld(p.ce + k * p.nc + i)]);
-            s += ld(p.kin + k * p.nc + i) * (ue * ue);
+        int32_t E[6];
+        C kin[6];
+        if (REC) {
+            ldrec(p.rc_state + 6 * i, E);
+            ldrec(p.rc_kin + 6 * i, kin);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                E[k] = k < p.kc ? ld(p.ce + k * p.nc + i) : 0;
+                kin[k] = k < p.kc ? ld(p.kin + k * p.nc + i) : C(0);
+            }
         }
-        double K = double(s);
-        p.B[i] = K + p.g * h[i];
+        C s = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            C ue = C(u[E[k]]);
+            s += kin[k] * (ue * ue);
+        }
+        p.B[i] = double(s) + p.g * h[i];
     } else if (t.range == 1) {
         int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
+        int32_t idx[6];
+        C cf[8];
+        if (REC) {
+            ldrec(p.rv_idx + 6 * v, idx);
+            ldrec(p.rv_coef + 8 * v, cf);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                idx[k] = ld(p.ve + k * p.nv + v);
+                idx[3 + k] = ld(p.vc + k * p.nv + v);
+                cf[k] = ld(p.curl + k * p.nv + v);
+                cf[3 + k] = ld(p.vh + k * p.nv + v);
+            }
+            cf[6] = ld(p.fv + v);
+        }
         C z = 0, eta = 0;
-        for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
-        for (int k = 0; k < p.kv; ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
-        p.qv[v] = I((z + ld(p.fv + v)) / eta);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) z += cf[k] * C(u[idx[k]]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) eta += cf[3 + k] * C(h[idx[3 + k]]);
+        p.qv[v] = I((z + cf[6]) / eta);
     } else {
         int64_t e = t.first + threadIdx.x;
         if (e >= p.ne) return;
-        int32_t c0 = ld(p.ec + e), c1 = ld(p.ec + p.ne + e);
-        C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
-        p.G[e] = I(ld(p.len + e) * (he * C(u[e])));
+        int32_t c[2];
+        C lh[2];
+        ldrec(p.re_cells + 2 * e, c);
+        ldrec(p.re_lh + 2 * e, lh);
+        p.G[e] = I((lh[0] * C(h[c[0]]) + lh[1] * C(h[c[1]])) * C(u[e]));
     }
 }
 
@@ -394,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
     C W[15];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        E[k] = ld(p.ce + k * nc + i);
+        E[k] = ld(p.ce6 + k * nc + i);
         V[k] = ld(p.ccv + k * nc + i);
     }
 #pragma unroll
@@ -425,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
     for (int k = 0; k < 6; ++k) {
         if (k < n) {
             const int side = (meta >> (4 + k)) & 1;
-            p.T[side * p.ne + E[k]] = typename P::I(H[k]);
+            p.T[2 * int64_t(E[k]) + side] = typename P::I(H[k]);
             div += side ? -double(G[k]) : double(G[k]);
         }
     }
@@ -436,12 +513,17 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
 template <class P>
 __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r) {
     using C = typename P::C;
+    using I = typename P::I;
     using U = typename P::U;
     int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (e >= p.ne) return;
+    int32_t c[2];
+    I hs[2];
+    ldrec(p.re_cells + 2 * e, c);
+    ldrec(p.T + 2 * e, hs);
     const C d = ld(p.dual + e);
-    const C flux = (C(ld(p.T + e)) + C(ld(p.T + p.ne + e))) / (2 * d);
-    const double grad = (p.B[ld(p.ec + p.ne + e)] - p.B[ld(p.ec + e)]) / double(d);
+    const C flux = (C(hs[0]) + C(hs[1])) / (2 * d);
+    const double grad = (p.B[c[1]] - p.B[c[0]]) / double(d);
     rk_update<U>(r, e, double(flux) - grad, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
@@ -634,10 +716,18 @@ static int setup_params(trisk_handle* H) {
     rc |= upload(o, L.area_v, &p.area_v);
     if (L.pair) {
         rc |= upload(o, L.cmeta, &p.cmeta);
+        rc |= coef(L.dual, &p.dual);
+        if (H->precision == TRISK_FP32_BULK) {
+            rc |= upload(o, L.rc_state, &p.rc_state);
+            rc |= coef(L.rc_kin, &p.rc_kin);
+            rc |= upload(o, L.rv_idx, &p.rv_idx);
+            rc |= coef(L.rv_coef, &p.rv_coef);
+        }
+        rc |= upload(o, L.re_cells, &p.re_cells);
+        rc |= coef(L.re_lh, &p.re_lh);
+        rc |= upload(o, L.rc_pce, &p.ce6);
         rc |= upload(o, L.ccv, &p.ccv);
         rc |= coef(L.pw, &p.pw);
-        rc |= coef(L.len, &p.len);
-        rc |= coef(L.dual, &p.dual);
     }
     if (rc) return rc;
     void* q;
@@ -708,8 +798,9 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     {
         int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = tiles_per_xcd(L.ne);
         if ((rc = timed_begin(H, K_PAIR_STATE, s, &stop))) return rc;
-        hipLaunchKernelGGL((pair_state_kernel<P, UIn>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0, s, p,
-                           u, h, t0, t1, t2);
+        constexpr bool rec = P::id == TRISK_FP32_BULK;
+        hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0,
+                           s, p, u, h, t0, t1, t2);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -906,7 +997,7 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
     if (o.precision != TRISK_FP64 && o.precision != TRISK_FP32_BULK)
         return set_err(TRISK_EINVAL, "precision must be TRISK_FP64 or TRISK_FP32_BULK");
     if (o.block_size != 0 && o.block_size != kBlock)
-        return set_err(TRISK_EINVAL, "block_size must be 0 or 256");
+        return set_err(TRISK_EINVAL, "block_size must be 0 or the library's workgroup size");
     auto* H = new trisk_handle;
     H->precision = o.precision;
     H->device = o.device;
@@ -1156,13 +1247,14 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
     const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;  // y, acc r/w, stage
     std::vector<std::pair<const char*, double>> v;
     if (L.pair) {
-        double w1 = nc * (KC * (iw + c) + 8.0 + 8.0)             /* ce, kin, h, B */
-                    + nv * (KV * 2 * (iw + c) + c + c)             /* ve, curl, vc, vh, f, q_v */
-                    + ne * (2 * iw + 2 * c + uw + c + c);          /* ec, eh, u, l, G */
-        double w2 = nc * (6 * 2 * iw + 15 * c + iw + 8.0 + rk_c)  /* ce, corners, weights, meta, A, RK4 */
-                    + ne * c + nv * c                              /* G, q_v */
-                    + 2 * ne * c;                                  /* half sums */
-        double w3 = ne * (2 * c + 2 * iw + c + rk_e) + nc * 8.0;  /* half sums, ec, d, RK4, B */
+        const double vc = f64 ? 7 * c : 8 * c;                  /* FP32: 8-wide vertex records */
+        double w1 = nc * (6 * iw + 6 * c + 8.0 + 8.0)          /* edges, KE weights, h, B */
+                    + nv * (6 * iw + vc + c)                     /* indices, coefficients, q_v */
+                    + ne * (2 * iw + 2 * c + uw + c);            /* cells, l h weights, u, G */
+        double w2 = nc * (12 * iw + 15 * c + iw + 8.0 + rk_c)   /* edges, corners, weights, meta, A, RK4 */
+                    + ne * c + nv * c                            /* G, q_v */
+                    + 2 * ne * c;                                /* half sums */
+        double w3 = ne * (2 * c + 2 * iw + c + rk_e) + nc * 8.0; /* half sums, cells, d, RK4, B */
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
         double p1 = nc * (KC * (iw + c) + 8.0 + 8.0) + ne * uw + nv * (KV * 2 * (iw + c) + c + c);

@@ -365,6 +365,45 @@ void build_pairs(const trisk_model_desc* d, Layout& L) {
         L.len[j] = l[L.eperm[j]];
         L.dual[j] = dl[L.eperm[j]];
     }
+    // entity records
+    const int KV = L.kv;
+    if (KC > 6 || KV != 3) {
+        L.pair_reason = "the cell-pair records need cells of at most six edges and vertices of three";
+        return;
+    }
+    L.rc_state.assign(size_t(nc) * 6, 0);
+    L.rc_kin.assign(size_t(nc) * 6, 0.0);
+    L.rc_pce.assign(size_t(nc) * 6, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < nc; ++j) {
+        for (int k = 0; k < 6; ++k) {
+            int32_t e = k < KC ? L.ce[size_t(k) * nc + j] : L.ce[j];
+            L.rc_state[6 * j + k] = e;
+            L.rc_kin[6 * j + k] = k < KC ? L.kin[size_t(k) * nc + j] : 0.0;
+            L.rc_pce[size_t(k) * nc + j] = e;
+        }
+    }
+    const int64_t nv = L.nv;
+    L.rv_idx.assign(size_t(nv) * 6, 0);
+    L.rv_coef.assign(size_t(nv) * 8, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < nv; ++v) {
+        for (int k = 0; k < 3; ++k) {
+            L.rv_idx[6 * v + k] = L.ve[size_t(k) * nv + v];
+            L.rv_idx[6 * v + 3 + k] = L.vc[size_t(k) * nv + v];
+            L.rv_coef[8 * v + k] = L.curl[size_t(k) * nv + v];
+            L.rv_coef[8 * v + 3 + k] = L.vh[size_t(k) * nv + v];
+        }
+        L.rv_coef[8 * v + 6] = L.fv[v];
+    }
+    L.re_cells.assign(size_t(ne) * 2, 0);
+    L.re_lh.assign(size_t(ne) * 2, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; ++e)
+        for (int s2 = 0; s2 < 2; ++s2) {
+            L.re_cells[2 * e + s2] = L.ec[size_t(s2) * ne + e];
+            L.re_lh[2 * e + s2] = L.len[e] * L.eh[size_t(s2) * ne + e];
+        }
     L.pair = true;
     L.pair_reason = "cell-pair TRiSK weights (no per-edge TRiSK tables)";
 }

@@ -35,6 +35,14 @@ struct Layout {
     std::vector<int32_t> ccv;            // corner k of cell (start of edge k), kc x nc
     std::vector<double> pw;              // 15 x nc pair weights (kc <= 6)
     std::vector<double> len, dual;       // l_e, d_e
+    // the same tables as entity records (AoS) for vector loads
+    std::vector<int32_t> rc_state;       // nc x 6: cell edges
+    std::vector<double> rc_kin;          // nc x 6: kinetic-energy weights
+    std::vector<int32_t> rv_idx;         // nv x 6: vertex edges, vertex cells
+    std::vector<double> rv_coef;         // nv x 8: curl (3), vertex heights (3), f, 0
+    std::vector<int32_t> re_cells;       // ne x 2: left, right cell
+    std::vector<double> re_lh;           // ne x 2: l_e * edge_height
+    std::vector<int32_t> rc_pce;         // 6 x nc: cell edges, pads the first edge
 };
 
 constexpr int kPairSlots = 6;   // cells of up to six edges

@@ -1,0 +1,78 @@
+"""The C ABI: both libraries load, export every function their headers declare, and the
+device library validates tables on the host (no GPU needed for these checks)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from dmplex_amd import ModelTables, create_hexagon_mesh
+from dmplex_amd import _device as D
+from dmplex_amd import _native as N
+from dmplex_amd.synthetic import earth_parameters
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared(header: str, prefix: str):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(rf"\b({prefix}\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,prefix,loader", [("trisk_hip.h", "trisk_", D.lib), ("dmplex_host.h", "dmx_", N.host)])
+def test_every_declared_symbol_is_exported(header, prefix, loader):
+    names = declared(header, prefix)
+    assert len(names) > 5
+    lib = loader()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    names = declared("trisk_hip.h", "trisk_")
+    assert set(names) - {"trisk_last_error"} == set(D.SIGNATURES)
+
+
+def test_desc_struct_matches_header_field_order():
+    text = (ROOT / "include" / "trisk_hip.h").read_text()
+    body = text[text.index("typedef struct {\n    int64_t nc, ne, nv;"):text.index("} trisk_model_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";")[:-1]:
+        decl = decl.replace("typedef struct {", "").strip()
+        names = decl.split()[-1] if "," not in decl else decl
+        for part in re.split(r",", decl):
+            fields.append(re.sub(r"[*\s]", " ", part).split()[-1])
+    assert fields == [f[0] for f in D.TriskModelDesc._fields_]
+
+
+def test_check_tables_accepts_and_rejects_on_the_host():
+    dm = create_hexagon_mesh(2, centroidal=True)
+    t = ModelTables(dm, centers=dm.generators, **earth_parameters())
+    ok, why = t.check_tables()
+    assert ok, why
+    bad = t.cell_edges.copy(order="F")
+    bad[3, 10] = dm.ne + 1
+    t.cell_edges = bad
+    with pytest.raises(ValueError, match="out of range in cell_edges"):
+        t.check_tables()
+    t2 = ModelTables(dm, centers=dm.generators, **earth_parameters())
+    t2.trisk = np.asfortranarray(t2.trisk * (1 + 1e-9))
+    ok, why = t2.check_tables()
+    assert not ok and "trisk" in why
+
+
+def test_create_without_a_device_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    dm = create_hexagon_mesh(1)
+    t = ModelTables(dm)
+    desc = t.descriptor()
+    h = C.c_void_p()
+    rc = D.lib().trisk_create(C.byref(desc), None, C.byref(h))
+    assert rc == -2 and not h.value
+    assert "device" in D.lib().trisk_last_error().decode()

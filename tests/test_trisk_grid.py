@@ -1,0 +1,179 @@
+"""TRiSK geometry and coefficient tables (host code), pinned by the reference's own
+invariants (test/finite_volumes.jl:25-77,168-229) and by closed forms on the level-0
+hexagon mesh (the regular dodecahedron)."""
+import numpy as np
+import pytest
+
+from dmplex_amd import (ModelTables, TRiSKGrid, create_hexagon_mesh, create_spherical_mesh, gassmann_coefficients,
+                        plex_from_polygons, signed_triangle_area, vertex_edge_areas)
+from dmplex_amd.synthetic import williamson2
+
+
+@pytest.fixture(scope="module")
+def hex3():
+    dm = create_hexagon_mesh(3)
+    g = TRiSKGrid(dm)
+    return dm, g, ModelTables(g)
+
+
+def dense_W(g):
+    ne = g.num_edges()
+    W = np.zeros((ne, ne))
+    for k in range(g.trisk_edges.shape[0]):
+        np.add.at(W, (np.arange(ne), g.trisk_edges[k]), g.trisk_weights[k])
+    return W
+
+
+def test_areas_kites_normals(hex3):
+    dm, g, _ = hex3
+    assert np.isclose(g.cell_areas.sum(), 4 * np.pi, rtol=1e-14)
+    assert np.isclose(g.vertex_areas.sum(), 4 * np.pi, rtol=1e-12)
+    assert np.allclose(g.cell_kites.sum(axis=0), g.cell_areas, rtol=0, atol=0)
+    c = g.centers
+    assert np.all(np.sum(g.edge_normals * (c[g.edge_cells[1]] - c[g.edge_cells[0]]), axis=1) > 0)
+
+
+def test_trisk_weights_antisymmetric(hex3):
+    _, g, _ = hex3
+    W = dense_W(g)
+    assert np.max(np.abs(W + W.T)) < 1e-14
+
+
+def test_trisk_property_and_mimetic_identities(hex3):
+    dm, g, m = hex3
+    nc, ne, nv = g.num_cells(), g.num_edges(), g.num_vertices()
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(ne)
+    D = lambda F: np.sum(g.cell_signs * g.primal_lengths[g.cell_edges] * F[g.cell_edges], axis=0) / g.cell_areas
+    tangential = np.sum(m.trisk * u[m.trisk_edges], axis=0)
+    perp = tangential * g.dual_lengths
+    dual_div = np.sum(g.vertex_signs * perp[g.vertex_edges], axis=0) / g.vertex_areas
+    Du = D(u)
+    average = np.sum(g.vertex_kites * Du[g.vertex_cells], axis=0) / g.vertex_areas
+    assert np.max(np.abs(dual_div + average)) < 1e-12 * np.max(np.abs(average))
+    phi, psi = rng.standard_normal(nc), rng.standard_normal(nv)
+    grad = (phi[g.edge_cells[1]] - phi[g.edge_cells[0]]) / g.dual_lengths
+    curl = np.sum(m.curl * grad[m.vertex_edges], axis=0)
+    assert np.max(np.abs(curl)) < 1e-12
+    skew = (psi[g.edge_vertices[1]] - psi[g.edge_vertices[0]]) / g.primal_lengths
+    assert np.max(np.abs(D(skew))) < 1e-11
+
+
+def test_solid_body_vorticity_and_tangential(hex3):
+    _, g, m = hex3
+    w = np.array([0.3, -0.2, 1.0])
+    solid = np.sum(np.cross(w, g.edge_points) * g.edge_normals, axis=1)
+    zeta = np.sum(m.curl * solid[m.vertex_edges], axis=0)
+    assert np.max(np.abs(zeta - 2 * g.vertices @ w)) < 0.03
+    tangential = np.sum(np.cross(w, g.edge_points) * np.cross(g.edge_points, g.edge_normals), axis=1)
+    vt = np.sum(m.trisk * solid[m.trisk_edges], axis=0)
+    assert np.max(np.abs(vt - tangential)) < 0.15
+
+
+def test_gassmann_antisymmetric_and_consistent(hex3):
+    _, g, _ = hex3
+    E, P, Wg = gassmann_coefficients(g)
+    l, d = g.primal_lengths, g.dual_lengths
+    ne = g.num_edges()
+    for e in range(ne):
+        for k in range(12):
+            f = E[k, e]
+            js = [j for j in range(12) if E[j, f] == e and P[j, f] == P[k, e]]
+            assert js, (e, k)
+            assert abs(l[e] * d[e] * Wg[k, e] + l[f] * d[f] * Wg[js[0], f]) < 1e-15
+    w = np.array([0.3, -0.2, 1.0])
+    solid = np.sum(np.cross(w, g.edge_points) * g.edge_normals, axis=1)
+    tangential = np.sum(np.cross(w, g.edge_points) * np.cross(g.edge_points, g.edge_normals), axis=1)
+    assert np.max(np.abs(np.sum(Wg * solid[E], axis=0) - tangential)) < 0.02
+
+
+def test_dodecahedron_closed_forms():
+    """Level 0: 12 regular pentagons.  Every cell has area 4pi/12, every dual triangle
+    4pi/20; primal edges subtend arccos(sqrt(5)/3), dual edges arccos(1/sqrt(5)); all kite
+    fractions are 1/5, so the TRiSK weights of the walk are +-(1/2 - s/5)."""
+    dm = create_hexagon_mesh(0)
+    g = TRiSKGrid(dm)
+    assert (g.num_cells(), g.num_edges(), g.num_vertices()) == (12, 30, 20)
+    assert np.allclose(g.cell_areas, 4 * np.pi / 12, rtol=1e-14)
+    assert np.allclose(g.vertex_areas, 4 * np.pi / 20, rtol=1e-14)
+    assert np.allclose(g.primal_lengths, np.arccos(np.sqrt(5) / 3), rtol=1e-14)
+    assert np.allclose(g.dual_lengths, np.arccos(1 / np.sqrt(5)), rtol=1e-14)
+    assert np.allclose(g.cell_kites / g.cell_areas, 1 / 5, rtol=1e-13)
+    assert np.allclose(g.cell_distances, g.dual_lengths / 2, rtol=1e-14)
+    assert g.trisk_edges.shape == (8, 30)
+    for e in range(30):
+        for side, c in enumerate(sorted(g.edge_cells[:, e])):
+            k = list(g.cell_edges[:, c]).index(e)
+            for s in range(1, 5):
+                k2 = (k + s) % 5
+                pos = side * 4 + s - 1
+                assert g.trisk_edges[pos, e] == g.cell_edges[k2, c]
+                expect = g.cell_signs[k, c] * g.cell_signs[k2, c] * (0.5 - s / 5)
+                assert abs(g.trisk_weights[pos, e] - expect) < 1e-14
+
+
+def test_vertex_edge_areas_and_consistency():
+    tri = create_spherical_mesh(4, centroidal=True)
+    g = TRiSKGrid(create_hexagon_mesh(4, primal=tri), centers=tri.coordinates)
+    by_vertex, by_edge = vertex_edge_areas(g)
+    assert np.allclose(by_vertex.sum(axis=0), g.vertex_areas, rtol=1e-13)
+    assert np.allclose(by_edge.sum(axis=0), g.primal_lengths * g.dual_lengths / 2, rtol=1e-3)
+    for v in range(g.num_vertices()):
+        x = g.vertices[v]
+        n = g.edge_normals[g.vertex_edges[:, v]]
+        T = np.einsum("k,ki,kj->ij", by_vertex[:, v], n, n) / g.vertex_areas[v]
+        P = np.eye(3) - np.outer(x, x)
+        assert np.max(np.abs(P @ T @ P - P / 2)) < 2e-3
+    with pytest.raises(ValueError):
+        ModelTables(g, kinetic_energy="edge")
+
+
+def test_signed_geometry_on_obtuse_triangles():
+    rng = np.random.default_rng(1)
+    dm = create_spherical_mesh(3)
+    h = 0.25 * np.sqrt(4 * np.pi / dm.nc)
+    x = dm.coordinates
+    r = rng.standard_normal(x.shape)
+    t = np.cross(x, r)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    y = x + h * t
+    y /= np.linalg.norm(y, axis=1, keepdims=True)
+    tri = dm.cell_vertex_table()
+    g = TRiSKGrid(plex_from_polygons(list(tri), y))
+    a, b, c = (g.vertices[g.cell_vertices[k]] for k in range(3))
+    # Girard's theorem: area = angle excess (independent of the kite construction)
+    def angle(p, q, r_):
+        t1 = np.cross(p, q)
+        t2 = np.cross(p, r_)
+        return np.arccos(np.clip(np.sum(t1 * t2, axis=1) / np.linalg.norm(t1, axis=1) / np.linalg.norm(t2, axis=1),
+                                 -1, 1))
+    exact = angle(a, b, c) + angle(b, c, a) + angle(c, a, b) - np.pi
+    assert np.any(g.cell_distances < 0)
+    assert np.max(np.abs(g.cell_areas - exact) / exact) < 1e-10
+    assert np.isclose(g.vertex_areas.sum(), 4 * np.pi)
+    by_vertex, by_edge = vertex_edge_areas(g)
+    assert np.allclose(by_vertex.sum(axis=0), g.vertex_areas)
+    assert np.any(by_edge < 0)
+
+
+def test_signed_triangle_area_closed_form():
+    # octant triangle: area pi/2, clockwise negative
+    a, b, c = np.eye(3)
+    assert np.isclose(signed_triangle_area(a, b, c), np.pi / 2, rtol=1e-15)
+    assert np.isclose(signed_triangle_area(a, c, b), -np.pi / 2, rtol=1e-15)
+
+
+def test_model_coefficients_fold_like_the_reference(hex3):
+    _, g, m = hex3
+    l, d, A = g.primal_lengths, g.dual_lengths, g.cell_areas
+    i, k = 7, 2
+    e = g.cell_edges[k, i]
+    side = 0 if g.edge_cells[0, e] == i else 1
+    assert m.divergence[k, i] == g.cell_signs[k, i] * l[e] / A[i]
+    assert m.kinetic[k, i] == 1.0 * l[e] * g.cell_distances[side, e] / (2 * A[i])
+    assert m.trisk[3, 11] == g.trisk_weights[3, 11] * l[g.trisk_edges[3, 11]] / d[11]
+    assert m.inverse_dual[5] == 1 / d[5]
+    assert np.array_equal(m.edge_height, 1.0 * g.cell_distances / d[None, :])
+    assert m.divergence.flags.f_contiguous and m.trisk.flags.f_contiguous
+    ok, why = m.check_tables()
+    assert ok, why
