@@ -45,6 +45,7 @@ def parse():
                    help="both: FP64 (bit-exact tables), FP64 on the cell-pair path, and FP32-bulk")
     p.add_argument("--cpu-steps", type=int, default=100, help="RK4 steps of the CPU baseline sample (0: skip)")
     p.add_argument("--no-kernel-events", action="store_true")
+    p.add_argument("--no-graphs", action="store_true", help="launch the kernels directly instead of graph replay")
     p.add_argument("--path", choices=["auto", "tables", "pair"], default="auto",
                    help="kernel path (auto: bit-exact tables for FP64, cell pairs for FP32-bulk)")
     p.add_argument("--no-scvt", action="store_true",
@@ -99,25 +100,38 @@ def run_gpu(grid, y0, precision, args, params, path=None):
     log(f"[bench] {precision} path: {m.path_info()[1]}")
     y = m.upload_state(y0)
     mass0, energy0, _ = diagnostics(m, y)
-    rk4_steps_(y, args.dt, args.warmup, m)
-    torch.cuda.synchronize()
-    D.check(D.lib().trisk_set_timing(m.handle, 0 if args.no_kernel_events else 1))
     stream = torch.cuda.current_stream()
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter()
-    t0.record(stream)
-    rk4_steps_(y, args.dt, args.steps, m)
-    t1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall
-    ms = t0.elapsed_time(t1)
-    D.check(D.lib().trisk_set_timing(m.handle, 0))
-    stats = kernel_stats(m) if not args.no_kernel_events else []
+
+    def timed(steps):
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter()
+        t0.record(stream)
+        rk4_steps_(y, args.dt, steps, m)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1), time.perf_counter() - wall
+
+    # timed run 1: direct launches with HIP events around every kernel (per-kernel
+    # averages for the roofline); timed run 2: the same K steps replayed as HIP graphs
+    rk4_steps_(y, args.dt, args.warmup, m)
+    stats = []
+    ms_direct = None
+    if not args.no_kernel_events:
+        D.check(D.lib().trisk_set_timing(m.handle, 1))
+        ms_direct, _ = timed(args.steps)
+        D.check(D.lib().trisk_set_timing(m.handle, 0))
+        stats = kernel_stats(m)
+    if not args.no_graphs:
+        m.use_graphs(True)
+        rk4_steps_(y, args.dt, 8, m)  # capture
+    ms, wall = timed(args.steps)
     mass, energy, _ = diagnostics(m, y)
     nc = grid.num_cells()
     value = nc * args.steps / (ms / 1e3)
     out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall, path=m.path_info()[1],
+               graphs=not args.no_graphs,
+               ms_per_step_direct_with_events=None if ms_direct is None else ms_direct / args.steps,
                hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK, path_name=path,
                mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0, kernels=stats)
     if stats:

@@ -645,6 +645,12 @@ struct trisk_handle {
     double* diag_zv = nullptr;
     double* diag_hv = nullptr;
     std::vector<double> host_stage;
+    // graph replay of trisk_rk4_steps: one captured graph of kGraphSteps steps per (y, dt)
+    bool graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* graph_y = nullptr;
+    double graph_dt = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> events;     // pairs (start, stop)
@@ -979,6 +985,8 @@ extern "C" {
 
 const char* trisk_last_error(void) { return g_error.c_str(); }
 
+static void drop_graph(trisk_handle* h);
+
 int trisk_check_tables(const trisk_model_desc* desc, int32_t* recomputed) {
     Layout L;
     std::string err;
@@ -1041,6 +1049,7 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
 int trisk_destroy(trisk_handle* h) {
     if (!h) return 0;
     (void)hipSetDevice(h->device);
+    drop_graph(h);
     for (auto& e : h->events) (void)hipEventDestroy(e);
     for (auto& b : h->owned) (void)hipFree(b.p);
     delete h;
@@ -1113,16 +1122,60 @@ int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* s
     return h->precision == TRISK_FP64 ? tendency<Fp64>(h, y_dev, dy_dev, s) : tendency<Fp32Bulk>(h, y_dev, dy_dev, s);
 }
 
+static int run_steps(trisk_handle* h, double* y, double dt, int64_t n, hipStream_t s) {
+    return h->precision == TRISK_FP64 ? rk4_steps<Fp64>(h, y, dt, n, s) : rk4_steps<Fp32Bulk>(h, y, dt, n, s);
+}
+
+static void drop_graph(trisk_handle* h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_y = nullptr;
+}
+
+constexpr int64_t kGraphSteps = 8;
+
 int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, void* stream) {
     CHECK_HANDLE(h);
     if (!y_dev) return set_err(TRISK_EINVAL, "null state pointer");
     if (nsteps < 0) return set_err(TRISK_EINVAL, "nsteps must be non-negative");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->timing) h->events_used = 0;
-    int rc = h->precision == TRISK_FP64 ? rk4_steps<Fp64>(h, y_dev, dt, nsteps, s)
-                                        : rk4_steps<Fp32Bulk>(h, y_dev, dt, nsteps, s);
-    return rc;
+    if (h->timing) {
+        h->events_used = 0;
+        return run_steps(h, y_dev, dt, nsteps, s);
+    }
+    if (!h->graphs || s == nullptr || nsteps < kGraphSteps) return run_steps(h, y_dev, dt, nsteps, s);
+    // replay a captured graph of kGraphSteps steps; the kernels' arguments (y, dt and the
+    // handle's buffers) are baked in, so the graph is rebuilt when y or dt change
+    if (!h->graph_exec || h->graph_y != y_dev || h->graph_dt != dt) {
+        drop_graph(h);
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = run_steps(h, y_dev, dt, kGraphSteps, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(TRISK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        h->graph = g;
+        HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+        h->graph_y = y_dev;
+        h->graph_dt = dt;
+    }
+    int64_t reps = nsteps / kGraphSteps;
+    for (int64_t r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(h->graph_exec, s));
+    return run_steps(h, y_dev, dt, nsteps - reps * kGraphSteps, s);
+}
+
+int trisk_set_graphs(trisk_handle* h, int32_t on) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    h->graphs = on != 0;
+    if (!h->graphs) drop_graph(h);
+    return 0;
 }
 
 int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream) {

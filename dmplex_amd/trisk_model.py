@@ -193,6 +193,10 @@ class TRiSKShallowWater(ModelTables):
     def handle(self) -> C.c_void_p:
         return self._handle
 
+    def use_graphs(self, on: bool = True) -> None:
+        """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""
+        D.check(D.lib().trisk_set_graphs(self._handle, int(on)))
+
     def path_info(self) -> tuple[bool, str]:
         """(cell-pair path?, reason) of the device kernels."""
         flag = C.c_int32()

@@ -155,6 +155,12 @@ int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* s
  * update fused into the tendency kernels (same arithmetic as the reference broadcasts). */
 int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, void* stream);
 
+/* Replay trisk_rk4_steps through a captured HIP graph of 8 RK4 steps (on = 1; default 0).
+ * The graph is captured on the call's stream (which must not be NULL) the first time and
+ * again whenever y_dev or dt change; runs shorter than 8 steps and timed runs launch the
+ * kernels directly.  Same results as direct launches. */
+int trisk_set_graphs(trisk_handle* h, int32_t on);
+
 /* out[0] = total_mass, out[1] = total_energy, out[2] = potential_enstrophy, summed with
  * Neumaier compensation in a fixed order (bitwise reproducible).  Synchronous. */
 int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void* stream);
