@@ -97,7 +97,7 @@ def run_gpu(grid, y0, precision, args, params, path=None):
     path = path or args.path
     log(f"[bench] building {precision} model ({path} path)")
     m = TRiSKShallowWater(grid, backend=HIPBackend(0, precision), path=path, **params)
-    log(f"[bench] {precision} path: {m.path_info()[1]}")
+    log(f"[bench] {precision} path: {m.path_info()[1]}; int32 (wide) cells/vertices/edges {m.wide_counts()}")
     y = m.upload_state(y0)
     mass0, energy0, _ = diagnostics(m, y)
     stream = torch.cuda.current_stream()

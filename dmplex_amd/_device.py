@@ -28,7 +28,7 @@ class TriskModelDesc(C.Structure):
 
 class TriskOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_size", C.c_int32),
-                ("path", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("path", C.c_int32), ("narrow_limit", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 TRISK_FP64, TRISK_FP32_BULK = 0, 1
@@ -58,6 +58,7 @@ SIGNATURES = {
     "trisk_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
     "trisk_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_info": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_char_p)],
+    "trisk_wide_counts": [vp, i64p],
     "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 

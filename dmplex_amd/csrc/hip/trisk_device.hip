@@ -144,6 +144,11 @@ struct Params {
     const int32_t* re_cells;  // ne x 2
     const C* re_lh;           // ne x 2
     const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
+    // 16-bit deltas (sentinel kWide: read the int32 tables)
+    const int16_t* rc_state16;  // nc x 8
+    const int16_t* rv_idx16;    // nv x 8
+    const int32_t* re_cells16;  // ne
+    const int32_t* pc_ev16;     // 6 x nc
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;
@@ -391,6 +396,20 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 //   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),
 // and the continuity is dh_i = -(sum_k s_k G_k)/A_i (TRiSKShallowWater.jl:275-305).
 
+// The two cells of edge e from the packed 16-bit deltas (base e/3), or the int32 pair.
+template <class P>
+__device__ __forceinline__ void edge_cells_narrow(const Params<P>& p, int64_t e, int32_t (&c)[2]) {
+    const int32_t pk = ld(p.re_cells16 + e);
+    const int16_t d0 = int16_t(pk & 0xffff), d1 = int16_t(uint32_t(pk) >> 16);
+    if (d0 == kWide) {
+        ldrec(p.re_cells + 2 * e, c);
+    } else {
+        const int32_t b = int32_t(e / 3);
+        c[0] = b + d0;
+        c[1] = b + d1;
+    }
+}
+
 // W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e).  REC: the tables as entity
 // records read with vector loads (fewer instructions; faster for the FP32 tables), else
 // slot-major arrays (faster for the FP64 ones, whose records span several cache lines).
@@ -407,7 +426,14 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         int32_t E[6];
         C kin[6];
         if (REC) {
-            ldrec(p.rc_state + 6 * i, E);
+            int16_t d[8];
+            ldrec(p.rc_state16 + 8 * i, d);
+            if (d[0] == kWide) {
+                ldrec(p.rc_state + 6 * i, E);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) E[k] = int32_t(3 * i) + d[k];
+            }
             ldrec(p.rc_kin + 6 * i, kin);
         } else {
 #pragma unroll
@@ -429,7 +455,18 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         int32_t idx[6];
         C cf[8];
         if (REC) {
-            ldrec(p.rv_idx + 6 * v, idx);
+            int16_t d[8];
+            ldrec(p.rv_idx16 + 8 * v, d);
+            if (d[0] == kWide) {
+                ldrec(p.rv_idx + 6 * v, idx);
+            } else {
+                const int32_t b0 = int32_t((3 * v) >> 1), b1 = int32_t(v >> 1);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    idx[k] = b0 + d[k];
+                    idx[3 + k] = b1 + d[3 + k];
+                }
+            }
             ldrec(p.rv_coef + 8 * v, cf);
         } else {
 #pragma unroll
@@ -452,7 +489,10 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         if (e >= p.ne) return;
         int32_t c[2];
         C lh[2];
-        ldrec(p.re_cells + 2 * e, c);
+        if (REC)
+            edge_cells_narrow(p, e, c);
+        else
+            ldrec(p.re_cells + 2 * e, c);
         ldrec(p.re_lh + 2 * e, lh);
         p.G[e] = I((lh[0] * C(h[c[0]]) + lh[1] * C(h[c[1]])) * C(u[e]));
     }
@@ -469,10 +509,28 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
     const int n = meta & 15;
     int32_t E[6], V[6];
     C W[15];
+    constexpr bool narrow = P::id == TRISK_FP32_BULK;
+    bool wide = !narrow;
+    if (narrow) {
+        int32_t pk[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        E[k] = ld(p.ce6 + k * nc + i);
-        V[k] = ld(p.ccv + k * nc + i);
+        for (int k = 0; k < 6; ++k) pk[k] = ld(p.pc_ev16 + k * nc + i);
+        wide = int16_t(pk[0] & 0xffff) == kWide;
+        if (!wide) {
+            const int32_t be = int32_t(3 * i), bv = int32_t(2 * i);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                E[k] = be + int16_t(pk[k] & 0xffff);
+                V[k] = bv + int16_t(uint32_t(pk[k]) >> 16);
+            }
+        }
+    }
+    if (wide) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            E[k] = ld(p.ce6 + k * nc + i);
+            V[k] = ld(p.ccv + k * nc + i);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 15; ++q) W[q] = ld(p.pw + q * nc + i);
@@ -519,7 +577,10 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r) {
     if (e >= p.ne) return;
     int32_t c[2];
     I hs[2];
-    ldrec(p.re_cells + 2 * e, c);
+    if (P::id == TRISK_FP32_BULK)
+        edge_cells_narrow(p, e, c);
+    else
+        ldrec(p.re_cells + 2 * e, c);
     ldrec(p.T + 2 * e, hs);
     const C d = ld(p.dual + e);
     const C flux = (C(hs[0]) + C(hs[1])) / (2 * d);
@@ -728,6 +789,10 @@ static int setup_params(trisk_handle* H) {
             rc |= coef(L.rc_kin, &p.rc_kin);
             rc |= upload(o, L.rv_idx, &p.rv_idx);
             rc |= coef(L.rv_coef, &p.rv_coef);
+            rc |= upload(o, L.rc_state16, &p.rc_state16);
+            rc |= upload(o, L.rv_idx16, &p.rv_idx16);
+            rc |= upload(o, L.re_cells16, &p.re_cells16);
+            rc |= upload(o, L.pc_ev16, &p.pc_ev16);
         }
         rc |= upload(o, L.re_cells, &p.re_cells);
         rc |= coef(L.re_lh, &p.re_lh);
@@ -1020,7 +1085,7 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
     }
     const bool want_pair = o.path == TRISK_PATH_PAIR || (o.path == TRISK_PATH_AUTO && o.precision == TRISK_FP32_BULK);
     if (want_pair) {
-        build_pairs(desc, H->L);
+        build_pairs(desc, H->L, o.narrow_limit);
         if (!H->L.pair && o.path == TRISK_PATH_PAIR) {
             std::string why = H->L.pair_reason;
             delete H;
@@ -1301,13 +1366,16 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
     std::vector<std::pair<const char*, double>> v;
     if (L.pair) {
         const double vc = f64 ? 7 * c : 8 * c;                  /* FP32: 8-wide vertex records */
-        double w1 = nc * (6 * iw + 6 * c + 8.0 + 8.0)          /* edges, KE weights, h, B */
-                    + nv * (6 * iw + vc + c)                     /* indices, coefficients, q_v */
-                    + ne * (2 * iw + 2 * c + uw + c);            /* cells, l h weights, u, G */
-        double w2 = nc * (12 * iw + 15 * c + iw + 8.0 + rk_c)   /* edges, corners, weights, meta, A, RK4 */
+        const double rec8 = f64 ? 6 * iw : 16.0;                 /* FP32: 16-bit deltas, 8-wide records */
+        const double ew = f64 ? 2 * iw : 4.0;                    /* edge cells */
+        const double cw = f64 ? 12 * iw : 24.0;                  /* cell edges and corners */
+        double w1 = nc * (rec8 + 6 * c + 8.0 + 8.0)             /* edges, KE weights, h, B */
+                    + nv * (rec8 + vc + c)                       /* indices, coefficients, q_v */
+                    + ne * (ew + 2 * c + uw + c);                /* cells, l h weights, u, G */
+        double w2 = nc * (cw + 15 * c + iw + 8.0 + rk_c)        /* edges, corners, weights, meta, A, RK4 */
                     + ne * c + nv * c                            /* G, q_v */
                     + 2 * ne * c;                                /* half sums */
-        double w3 = ne * (2 * c + 2 * iw + c + rk_e) + nc * 8.0; /* half sums, cells, d, RK4, B */
+        double w3 = ne * (2 * c + ew + c + rk_e) + nc * 8.0;    /* half sums, cells, d, RK4, B */
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
         double p1 = nc * (KC * (iw + c) + 8.0 + 8.0) + ne * uw + nv * (KV * 2 * (iw + c) + c + c);
@@ -1321,6 +1389,14 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         bytes[k] = v[k].second;
     }
     if (count) *count = k;
+    return 0;
+}
+
+int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]) {
+    CHECK_HANDLE(h);
+    counts[0] = h->L.wide_cells;
+    counts[1] = h->L.wide_vertices;
+    counts[2] = h->L.wide_edges;
     return 0;
 }
 

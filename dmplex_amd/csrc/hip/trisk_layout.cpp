@@ -232,7 +232,7 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     return true;
 }
 
-void build_pairs(const trisk_model_desc* d, Layout& L) {
+void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.pair = false;
     if (!d->primal_lengths || !d->dual_lengths || !d->cell_kites) {
         L.pair_reason = "grid fields (primal_lengths, dual_lengths, cell_kites) not given";
@@ -404,6 +404,61 @@ void build_pairs(const trisk_model_desc* d, Layout& L) {
             L.re_cells[2 * e + s2] = L.ec[size_t(s2) * ne + e];
             L.re_lh[2 * e + s2] = L.len[e] * L.eh[size_t(s2) * ne + e];
         }
+    // 16-bit deltas
+    const int64_t lim = narrow_limit > 0 && narrow_limit < 32767 ? narrow_limit : 32767;
+    auto fits = [lim](int64_t d) { return d >= -lim && d <= lim; };
+    L.rc_state16.assign(size_t(nc) * 8, 0);
+    L.pc_ev16.assign(size_t(6) * nc, 0);
+    int64_t wc = 0, wv = 0, we = 0;
+#pragma omp parallel for schedule(static) reduction(+ : wc)
+    for (int64_t j = 0; j < nc; ++j) {
+        bool ok = true;
+        for (int k = 0; k < 6; ++k)
+            ok = ok && fits(int64_t(L.rc_pce[size_t(k) * nc + j]) - 3 * j) &&
+                 fits(int64_t(L.ccv[size_t(k) * nc + j]) - 2 * j);
+        if (!ok) {
+            L.rc_state16[8 * j] = kWide;
+            L.pc_ev16[j] = uint16_t(kWide);
+            ++wc;
+            continue;
+        }
+        for (int k = 0; k < 6; ++k) {
+            int16_t de = int16_t(int64_t(L.rc_pce[size_t(k) * nc + j]) - 3 * j);
+            int16_t dv = int16_t(int64_t(L.ccv[size_t(k) * nc + j]) - 2 * j);
+            L.rc_state16[8 * j + k] = de;
+            L.pc_ev16[size_t(k) * nc + j] = int32_t(uint32_t(uint16_t(de)) | (uint32_t(uint16_t(dv)) << 16));
+        }
+    }
+    L.rv_idx16.assign(size_t(nv) * 8, 0);
+#pragma omp parallel for schedule(static) reduction(+ : wv)
+    for (int64_t v = 0; v < nv; ++v) {
+        bool ok = true;
+        for (int k = 0; k < 3; ++k)
+            ok = ok && fits(int64_t(L.rv_idx[6 * v + k]) - (3 * v) / 2) && fits(int64_t(L.rv_idx[6 * v + 3 + k]) - v / 2);
+        if (!ok) {
+            L.rv_idx16[8 * v] = kWide;
+            ++wv;
+            continue;
+        }
+        for (int k = 0; k < 3; ++k) {
+            L.rv_idx16[8 * v + k] = int16_t(int64_t(L.rv_idx[6 * v + k]) - (3 * v) / 2);
+            L.rv_idx16[8 * v + 3 + k] = int16_t(int64_t(L.rv_idx[6 * v + 3 + k]) - v / 2);
+        }
+    }
+    L.re_cells16.assign(ne, 0);
+#pragma omp parallel for schedule(static) reduction(+ : we)
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t d0 = int64_t(L.re_cells[2 * e]) - e / 3, d1 = int64_t(L.re_cells[2 * e + 1]) - e / 3;
+        if (!fits(d0) || !fits(d1)) {
+            L.re_cells16[e] = int32_t(uint32_t(uint16_t(kWide)));
+            ++we;
+            continue;
+        }
+        L.re_cells16[e] = int32_t(uint32_t(uint16_t(int16_t(d0))) | (uint32_t(uint16_t(int16_t(d1))) << 16));
+    }
+    L.wide_cells = wc;
+    L.wide_vertices = wv;
+    L.wide_edges = we;
     L.pair = true;
     L.pair_reason = "cell-pair TRiSK weights (no per-edge TRiSK tables)";
 }

@@ -43,7 +43,17 @@ struct Layout {
     std::vector<int32_t> re_cells;       // ne x 2: left, right cell
     std::vector<double> re_lh;           // ne x 2: l_e * edge_height
     std::vector<int32_t> rc_pce;         // 6 x nc: cell edges, pads the first edge
+    // 16-bit index deltas from a linear base (cell j: edges 3j, corners 2j; vertex v: edges
+    // 3v/2, cells v/2; edge e: cells e/3); an entity with a delta outside +-32767 has the
+    // sentinel -32768 in its first slot and is read from the int32 tables instead
+    std::vector<int16_t> rc_state16;     // nc x 8: edges (6), 0, 0
+    std::vector<int16_t> rv_idx16;       // nv x 8: edges (3), cells (3), 0, 0
+    std::vector<int32_t> re_cells16;     // ne: left | right << 16
+    std::vector<int32_t> pc_ev16;        // 6 x nc: edge | corner << 16
+    int64_t wide_cells = 0, wide_vertices = 0, wide_edges = 0;
 };
+
+constexpr int16_t kWide = -32768;
 
 constexpr int kPairSlots = 6;   // cells of up to six edges
 constexpr int kPairs = 15;      // pairs k < k' of six slots, in the order of pair_index()
@@ -60,6 +70,6 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
 // w l_e'/d_e reproduces every entry of the caller's trisk table (to 1e-13 of the row's
 // largest entry), the divergence (s l_e)/A_i and 1/d_e; otherwise L.pair stays false and
 // L.pair_reason says why.
-void build_pairs(const trisk_model_desc* d, Layout& L);
+void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit = 32767);
 
 }  // namespace trisk

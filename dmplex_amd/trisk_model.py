@@ -151,7 +151,7 @@ class TRiSKShallowWater(ModelTables):
 
     def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, pv_flux: str = "trisk",
                  kinetic_energy="cell", comm=None, part=None, backend=None, hilbert: bool = True,
-                 path: str = "auto"):
+                 path: str = "auto", narrow_limit: int = 0):
         if comm is not None or part is not None:
             raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
         if backend is None:
@@ -164,6 +164,7 @@ class TRiSKShallowWater(ModelTables):
         if path not in D.PATHS:
             raise ValueError("path must be 'auto', 'tables' or 'pair'")
         self._path = path
+        self._narrow_limit = int(narrow_limit)
         self._handle = None
         self._create()
 
@@ -171,7 +172,8 @@ class TRiSKShallowWater(ModelTables):
         torch = _torch()
         torch.cuda.set_device(self.backend.device)
         opts = D.TriskOptions(D.TRISK_FP64 if self.backend.precision == "fp64" else D.TRISK_FP32_BULK,
-                              self.backend.device, 0, D.PATHS[getattr(self, "_path", "auto")])
+                              self.backend.device, 0, D.PATHS[getattr(self, "_path", "auto")],
+                              getattr(self, "_narrow_limit", 0))
         h = C.c_void_p()
         desc = self.descriptor(self._hilbert)
         D.check(D.lib().trisk_create(C.byref(desc), C.byref(opts), C.byref(h)))
@@ -192,6 +194,12 @@ class TRiSKShallowWater(ModelTables):
     @property
     def handle(self) -> C.c_void_p:
         return self._handle
+
+    def wide_counts(self) -> tuple[int, int, int]:
+        """Cells, vertices, edges read with int32 instead of 16-bit-delta indices."""
+        n = np.zeros(3, dtype=np.int64)
+        D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
+        return tuple(int(x) for x in n)
 
     def use_graphs(self, on: bool = True) -> None:
         """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""

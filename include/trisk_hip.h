@@ -115,7 +115,9 @@ typedef struct {
     int32_t device;              /* HIP device ordinal */
     int32_t block_size;          /* threads per workgroup (0: library default) */
     int32_t path;                /* TRISK_PATH_*: which kernels run the TRiSK/cell-KE step */
-    int32_t reserved[4];
+    int32_t narrow_limit;        /* FP32-bulk pair path: largest 16-bit index delta used
+                                    (0: 32767); entities beyond it read int32 indices */
+    int32_t reserved[3];
 } trisk_options;
 
 typedef struct trisk_handle trisk_handle;
@@ -185,6 +187,10 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
 /* Algorithmic HBM bytes of one launch of each hot-path kernel of this handle's path
  * (each array element the kernel must touch counted once), with the kernel names. */
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count);
+
+/* Cells, vertices and edges whose indices do not fit the 16-bit deltas of the FP32-bulk
+ * pair path (they are read from int32 tables). */
+int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]);
 
 /* Whether this handle runs TRISK_PATH_PAIR (*recomputed = 1) or the table path, and why
  * (text owned by the handle). */

@@ -181,6 +181,28 @@ def test_pair_path_fp64_matches_oracle(level):
     assert abs(diagnostics(m, yd)[0] - mass0) / mass0 < 1e-14
 
 
+def test_fp32_narrow_and_wide_indices_agree():
+    """16-bit index deltas and the int32 fallback give the same FP32-bulk tendency; a low
+    delta limit forces a mix of narrow and wide entities on a small mesh."""
+    say("narrow vs wide indices level 5")
+    dm = mesh(5)
+    kw = dict(centers=dm.generators, backend=HIPBackend(precision="fp32_bulk"), **earth_parameters())
+    narrow = TRiSKShallowWater(dm, **kw)
+    mixed = TRiSKShallowWater(dm, narrow_limit=60, **kw)
+    wc, wv, we = mixed.wide_counts()
+    assert 0 < wc < dm.nc and 0 < wv < dm.nv and 0 < we < dm.ne
+    assert narrow.wide_counts() == (0, 0, 0)
+    y = seeded_state(narrow.grid)
+    a, b = narrow.zeros(), mixed.zeros()
+    tendency_(a, narrow.upload_state(y), narrow)
+    tendency_(b, mixed.upload_state(y), mixed)
+    assert np.array_equal(gather_state(narrow, a), gather_state(mixed, b))
+    ya, yb = narrow.upload_state(y), mixed.upload_state(y)
+    rk4_steps_(ya, 600.0, 4, narrow)
+    rk4_steps_(yb, 600.0, 4, mixed)
+    assert np.array_equal(gather_state(narrow, ya), gather_state(mixed, yb))
+
+
 def test_pair_path_fp32_close_to_tables():
     say("pair vs tables fp32 level 5")
     dm = mesh(5)
