@@ -50,6 +50,10 @@ def parse():
                    help="kernel path (auto: bit-exact tables for FP64, cell pairs for FP32-bulk)")
     p.add_argument("--no-scvt", action="store_true",
                    help="skip Lloyd (same topology and kernels; for short profiling runs only)")
+    p.add_argument("--accuracy", type=int, default=0, metavar="DAYS",
+                   help="instead of timing: run FP64 (bit-exact), FP64 cell-pair and FP32-bulk for DAYS "
+                        "simulated days from the same state and report their errors against FP64 after "
+                        "day 1 and day DAYS")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="run a plain device copy of MIB MiB first (HBM counter calibration)")
     return p.parse_args()
@@ -165,6 +169,65 @@ def run_cpu(grid, y0, args, params):
                        f"reference's kernel decomposition, int64 column-major tables, OpenMP), {wall:.1f} s")
 
 
+def field_errors(m, a, b):
+    """Max and L2 errors of state a against the reference state b (host, reference order)."""
+    g = m.grid
+    ne = g.num_edges()
+    ua, ub, ha, hb = a[:ne], b[:ne], a[ne:], b[ne:]
+    A = g.cell_areas
+    w = g.primal_lengths * g.dual_lengths
+    return dict(
+        h_max=float(np.max(np.abs(ha - hb)) / np.max(np.abs(hb))),
+        h_l2=float(np.sqrt(np.sum(A * (ha - hb) ** 2) / np.sum(A * hb ** 2))),
+        u_max=float(np.max(np.abs(ua - ub)) / np.max(np.abs(ub))),
+        u_l2=float(np.sqrt(np.sum(w * (ua - ub) ** 2) / np.sum(w * ub ** 2))))
+
+
+def run_accuracy(args, grid, y0, params):
+    import torch
+
+    from dmplex_amd import HIPBackend, TRiSKShallowWater, diagnostics, rk4_steps_, vertex_pv_
+
+    runs = {"fp64": ("fp64", "auto"), "fp64_pair": ("fp64", "pair"), "fp32_bulk": ("fp32_bulk", "auto")}
+    models, states, mass0, energy0 = {}, {}, {}, {}
+    for key, (prec, path) in runs.items():
+        m = TRiSKShallowWater(grid, backend=HIPBackend(0, prec), path=path, **params)
+        m.use_graphs(True)
+        models[key] = m
+        states[key] = m.upload_state(y0)
+        mass0[key], energy0[key], _ = diagnostics(m, states[key])
+    spd = int(round(86400.0 / args.dt))
+    report = {}
+    t0 = time.perf_counter()
+    for day in range(1, args.accuracy + 1):
+        for key, m in models.items():
+            rk4_steps_(states[key], args.dt, spd, m)
+        torch.cuda.synchronize()
+        log(f"[accuracy] day {day} done ({time.perf_counter() - t0:.0f} s)")
+        if day not in (1, args.accuracy):
+            continue
+        ref_m = models["fp64"]
+        ref = ref_m.download_state(states["fp64"])
+        q_ref = vertex_pv_(ref_m, states["fp64"])[0].cpu().numpy()
+        Av = grid.vertex_areas
+        entry = {}
+        for key, m in models.items():
+            mass, energy, _ = diagnostics(m, states[key])
+            e = {"mass_drift": (mass - mass0[key]) / mass0[key],
+                 "energy_drift": (energy - energy0[key]) / energy0[key]}
+            if key != "fp64":
+                e.update(field_errors(m, m.download_state(states[key]), ref))
+                # PV of both states from the same FP64 diagnostic kernel, vertices in device order
+                q = vertex_pv_(ref_m, ref_m.upload_state(m.download_state(states[key])))[0].cpu().numpy()
+                e["q_max"] = float(np.max(np.abs(q - q_ref)) / np.max(np.abs(q_ref)))
+                e["q_l2"] = float(np.sqrt(np.sum(Av[ref_m.permutation()[2]] * (q - q_ref) ** 2) /
+                                          np.sum(Av[ref_m.permutation()[2]] * q_ref ** 2)))
+            entry[key] = e
+        report[f"day_{day}"] = entry
+    return {"accuracy": report, "level": args.level, "dt": args.dt, "steps_per_day": spd,
+            "reference": "FP64 table path (bitwise equal to the FP64 oracle)"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -192,6 +255,9 @@ def main():
     y0 = seeded_state(grid)
     log(f"[bench] mesh + tables + state in {time.perf_counter() - t:.1f} s: "
         f"{grid.num_cells()} cells, {grid.num_edges()} edges, {grid.num_vertices()} vertices")
+    if args.accuracy:
+        print(json.dumps(run_accuracy(args, grid, y0, params)), flush=True)
+        return
     if args.precision == "both":
         runs = [("fp64", "fp64", None), ("fp64_pair", "fp64", "pair"), ("fp32_bulk", "fp32_bulk", None)]
     else:
