@@ -9,11 +9,14 @@ Public API (mirrors the reference's Julia names; ``!`` becomes ``_``):
 * model: ``TRiSKShallowWater`` (``backend=HIPBackend(...)``), ``initial_state``,
   ``tendency_``, ``RK4Cache``, ``rk4_step_``, ``gather_state``, ``total_mass``,
   ``total_energy``, ``potential_enstrophy``, ``relative_vorticity``,
-  ``tangential_velocity``, ``kinetic_energy_``, ``vertex_pv_``.
+  ``tangential_velocity``, ``kinetic_energy_``, ``vertex_pv_``;
+* ensembles: ``TRiSKEnsemble`` (members on meshes of several resolutions, one launch per
+  kernel for all of them, with an autotuned schedule).
 
 Compute runs in ``lib/libtrisk_hip.so`` (gfx950 kernels behind the C ABI of
 ``include/trisk_hip.h``); setup runs in ``lib/libdmplex_host.so``.
 """
+from .ensemble import TRiSKEnsemble
 from .grids import (cell_centers, create_hexagon_mesh, create_spherical_mesh, lloyd_, lloyd_residual,
                     orthogonality_defect, voronoi_generators)
 from .topology import (Plex, check_plex, hilbert_key, hilbert_keys, hilbert_order, hilbert_permutation,

@@ -59,6 +59,23 @@ SIGNATURES = {
     "trisk_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_info": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_char_p)],
     "trisk_wide_counts": [vp, i64p],
+    "trisk_ensemble_create": [C.POINTER(C.POINTER(TriskModelDesc)), C.POINTER(C.c_int32), C.c_int32,
+                              C.POINTER(TriskOptions), C.POINTER(vp)],
+    "trisk_ensemble_destroy": [vp],
+    "trisk_ensemble_state_length": [vp, i64p],
+    "trisk_ensemble_level": [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_int64), C.POINTER(C.c_int32)],
+    "trisk_ensemble_get_member": [vp, C.c_int32, C.c_int32, vp, vp, vp],
+    "trisk_ensemble_set_member": [vp, C.c_int32, C.c_int32, vp, vp, vp],
+    "trisk_ensemble_tendency": [vp, vp, vp, vp],
+    "trisk_ensemble_rk4_steps": [vp, vp, f64p, i64, vp],
+    "trisk_ensemble_set_graphs": [vp, C.c_int32],
+    "trisk_ensemble_diagnostics": [vp, vp, f64p, vp],
+    "trisk_ensemble_set_schedule": [vp, vp],
+    "trisk_ensemble_get_schedule": [vp, vp],
+    "trisk_ensemble_autotune": [vp, vp, f64p, C.c_int32, vp, vp, f64p],
+    "trisk_ensemble_set_timing": [vp, C.c_int32],
+    "trisk_ensemble_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
+    "trisk_ensemble_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 

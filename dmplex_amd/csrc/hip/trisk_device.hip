@@ -27,194 +27,18 @@
 #include <string>
 #include <vector>
 
-#include "trisk_layout.hpp"
+#include "trisk_device.hpp"
 
 namespace trisk {
 
 static thread_local std::string g_error;
 
-static int set_err(int code, const std::string& msg) {
+int set_err(int code, const std::string& msg) {
     g_error = msg;
     return code;
 }
 
 int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg); }
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
-#ifndef TRISK_BLOCK
-#define TRISK_BLOCK 256
-#endif
-constexpr int kBlock = TRISK_BLOCK;
-constexpr int kXcd = 8;
-
-// ---------------------------------------------------------------------------------
-// precision policies
-
-struct Fp64 {
-    using C = double;  // coefficients of the stencils
-    using I = double;  // intermediates F, q_e, q_v, zeta_v, h_v, K_v
-    using U = double;  // stage velocity
-    static constexpr int id = TRISK_FP64;
-};
-struct Fp32Bulk {
-    using C = float;
-    using I = float;
-    using U = float;
-    static constexpr int id = TRISK_FP32_BULK;
-};
-
-template <class I>
-struct Pair {
-    I F, q;
-};
-template <>
-struct alignas(16) Pair<double> {
-    double F, q;
-};
-template <>
-struct alignas(8) Pair<float> {
-    float F, q;
-};
-
-// ---------------------------------------------------------------------------------
-// kernel parameters (device pointers, entity-major tables)
-
-template <class P>
-struct Params {
-    using C = typename P::C;
-    using I = typename P::I;
-    int64_t nc, ne, nv;
-    int kc, kv, kt, kg;
-    double g;
-    // cells
-    const int32_t* ce;
-    const double* div;
-    const C* kin;
-    const int32_t* cv;
-    const C* vkk;
-    const double* area_c;
-    // edges
-    const int32_t* ec;
-    const int32_t* ev;
-    const C* eh;
-    const C* invd;
-    const int32_t* te;
-    const C* tw;
-    const C* vkh;
-    const int32_t* ge;
-    const int32_t* gp;
-    const C* gw;
-    const C* grh;
-    const C* gf;
-    // vertices
-    const int32_t* ve;
-    const C* curl;
-    const int32_t* vc;
-    const C* vh;
-    const C* fv;
-    const C* vkw;
-    const double* area_v;
-    // work
-    double* B;          // nc
-    double* K;          // nc (vertex KE or diagnostics)
-    Pair<I>* Fq;        // ne
-    I* qv;              // nv
-    I* zv;              // nv (Gassmann, diagnostics)
-    I* hv;              // nv (Gassmann, vertex KE, diagnostics)
-    I* Kv;              // nv (vertex KE)
-    // cell-pair path
-    const int32_t* cmeta;  // nc: size | negative-sign bits << 4
-    const int32_t* ccv;    // 6 x nc corners
-    const C* pw;           // 15 x nc pair weights
-    const C* len;          // ne: l_e
-    const C* dual;         // ne: d_e
-    I* G;                  // ne: l_e F_e
-    I* T;                  // ne x 2: PV-flux half sums of the left / right cell
-    // entity records of the cell-pair path
-    const int32_t* rc_state;  // nc x 6
-    const C* rc_kin;          // nc x 6
-    const int32_t* rv_idx;    // nv x 6
-    const C* rv_coef;         // nv x 8
-    const int32_t* re_cells;  // ne x 2
-    const C* re_lh;           // ne x 2
-    const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
-    // 16-bit deltas (sentinel kWide: read the int32 tables)
-    const int16_t* rc_state16;  // nc x 8
-    const int16_t* rv_idx16;    // nv x 8
-    const int32_t* re_cells16;  // ne
-    const int32_t* pc_ev16;     // 6 x nc
-    // optional FP64 outputs of the diagnostics (NULL: not written)
-    double* K_out;
-    double* qv_out;
-    double* zv_out;
-    double* hv_out;
-};
-
-// Block b of a grid of kXcd * (t0 + t1) blocks -> (range, tile).  The hardware deals
-// blocks round-robin to the XCDs; XCD x receives tiles [x*t0, (x+1)*t0) of range 0
-// and [x*t1, (x+1)*t1) of range 1, contiguous in the Hilbert order.
-struct Tile {
-    int range;
-    int64_t first;  // first entity of the tile (may be past the range's end)
-};
-__device__ __forceinline__ Tile tile_of(int64_t b, int64_t t0, int64_t t1) {
-    int64_t x = b % kXcd, j = b / kXcd;
-    if (j < t0) return {0, (x * t0 + j) * kBlock};
-    return {1, (x * t1 + (j - t0)) * kBlock};
-}
-
-__device__ __forceinline__ Tile tile_of3(int64_t b, int64_t t0, int64_t t1, int64_t t2) {
-    int64_t x = b % kXcd, j = b / kXcd;
-    if (j < t0) return {0, (x * t0 + j) * kBlock};
-    if (j < t0 + t1) return {1, (x * t1 + (j - t0)) * kBlock};
-    return {2, (x * t2 + (j - t0 - t1)) * kBlock};
-}
-
-inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
-
-// Streamed tables are read once per launch: nontemporal loads keep them from evicting
-// the gathered state and intermediates from the caches.
-template <class T>
-__device__ __forceinline__ T ld(const T* p) {
-#ifdef TRISK_PLAIN_LOADS
-    return *p;
-#else
-    return __builtin_nontemporal_load(p);
-#endif
-}
-
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// N consecutive elements starting at p (one entity record) with 16-byte (or, for records
-// of a size not divisible by 16, 8-byte) nontemporal vector loads; p must be aligned to
-// that width.
-template <class T, int N>
-__device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
-    constexpr int bytes = N * int(sizeof(T));
-    if constexpr (bytes % 16 == 0) {
-        const f32x4* q = reinterpret_cast<const f32x4*>(p);
-#pragma unroll
-        for (int c = 0; c < bytes / 16; ++c) {
-            f32x4 v = __builtin_nontemporal_load(q + c);
-            __builtin_memcpy(reinterpret_cast<char*>(out) + 16 * c, &v, 16);
-        }
-    } else {
-        static_assert(bytes % 8 == 0, "records are multiples of 8 bytes");
-        const f32x2* q = reinterpret_cast<const f32x2*>(p);
-#pragma unroll
-        for (int c = 0; c < bytes / 8; ++c) {
-            f32x2 v = __builtin_nontemporal_load(q + c);
-            __builtin_memcpy(reinterpret_cast<char*>(out) + 8 * c, &v, 8);
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
@@ -313,41 +137,6 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
 // ---------------------------------------------------------------------------------
 // P3: edges (du) | cells (dh), then the RK4 stage update
 
-struct Rk {
-    int mode;            // 0: store the tendency in dy; 1..4: RK4 stage
-    double* dy;          // mode 0
-    double* y;           // state
-    double* acc;         // k1 + 2k2 + 2k3 accumulator
-    void* su;            // stage velocity (ne, type U)
-    double* sh;          // stage height (nc)
-    double dt, dt2, dt6;
-};
-
-// y + c dt k and the accumulator, in the order of rk4_step!'s broadcasts
-template <class U>
-__device__ __forceinline__ void rk_update(const Rk& r, int64_t j, double k, U* stage) {
-    switch (r.mode) {
-        case 0:
-            r.dy[j] = k;
-            break;
-        case 1:
-            r.acc[j] = k;
-            stage[0] = U(r.y[j] + r.dt2 * k);
-            break;
-        case 2:
-            r.acc[j] = r.acc[j] + 2 * k;
-            stage[0] = U(r.y[j] + r.dt2 * k);
-            break;
-        case 3:
-            r.acc[j] = r.acc[j] + 2 * k;
-            stage[0] = U(r.y[j] + r.dt * k);
-            break;
-        default:
-            r.y[j] = r.y[j] + r.dt6 * (r.acc[j] + k);
-            break;
-    }
-}
-
 template <class P, bool GM, bool GMEAN>
 __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -396,20 +185,6 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 //   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),
 // and the continuity is dh_i = -(sum_k s_k G_k)/A_i (TRiSKShallowWater.jl:275-305).
 
-// The two cells of edge e from the packed 16-bit deltas (base e/3), or the int32 pair.
-template <class P>
-__device__ __forceinline__ void edge_cells_narrow(const Params<P>& p, int64_t e, int32_t (&c)[2]) {
-    const int32_t pk = ld(p.re_cells16 + e);
-    const int16_t d0 = int16_t(pk & 0xffff), d1 = int16_t(uint32_t(pk) >> 16);
-    if (d0 == kWide) {
-        ldrec(p.re_cells + 2 * e, c);
-    } else {
-        const int32_t b = int32_t(e / 3);
-        c[0] = b + d0;
-        c[1] = b + d1;
-    }
-}
-
 // W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e).  REC: the tables as entity
 // records read with vector loads (fewer instructions; faster for the FP32 tables), else
 // slot-major arrays (faster for the FP64 ones, whose records span several cache lines).
@@ -425,76 +200,32 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         if (i >= p.nc) return;
         int32_t E[6];
         C kin[6];
-        if (REC) {
-            int16_t d[8];
-            ldrec(p.rc_state16 + 8 * i, d);
-            if (d[0] == kWide) {
-                ldrec(p.rc_state + 6 * i, E);
-            } else {
+        load_cell_state<REC>(p, i, E, kin);
+        C ue[6];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) E[k] = int32_t(3 * i) + d[k];
-            }
-            ldrec(p.rc_kin + 6 * i, kin);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                E[k] = k < p.kc ? ld(p.ce + k * p.nc + i) : 0;
-                kin[k] = k < p.kc ? ld(p.kin + k * p.nc + i) : C(0);
-            }
-        }
-        C s = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            C ue = C(u[E[k]]);
-            s += kin[k] * (ue * ue);
-        }
-        p.B[i] = double(s) + p.g * h[i];
+        for (int k = 0; k < 6; ++k) ue[k] = C(u[E[k]]);
+        p.B[i] = double(pair_kinetic(kin, ue)) + p.g * h[i];
     } else if (t.range == 1) {
         int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
         int32_t idx[6];
         C cf[8];
-        if (REC) {
-            int16_t d[8];
-            ldrec(p.rv_idx16 + 8 * v, d);
-            if (d[0] == kWide) {
-                ldrec(p.rv_idx + 6 * v, idx);
-            } else {
-                const int32_t b0 = int32_t((3 * v) >> 1), b1 = int32_t(v >> 1);
+        load_vertex_record<REC>(p, v, idx, cf);
+        C uz[3], hz[3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    idx[k] = b0 + d[k];
-                    idx[3 + k] = b1 + d[3 + k];
-                }
-            }
-            ldrec(p.rv_coef + 8 * v, cf);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                idx[k] = ld(p.ve + k * p.nv + v);
-                idx[3 + k] = ld(p.vc + k * p.nv + v);
-                cf[k] = ld(p.curl + k * p.nv + v);
-                cf[3 + k] = ld(p.vh + k * p.nv + v);
-            }
-            cf[6] = ld(p.fv + v);
+        for (int k = 0; k < 3; ++k) {
+            uz[k] = C(u[idx[k]]);
+            hz[k] = C(h[idx[3 + k]]);
         }
-        C z = 0, eta = 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) z += cf[k] * C(u[idx[k]]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) eta += cf[3 + k] * C(h[idx[3 + k]]);
-        p.qv[v] = I((z + cf[6]) / eta);
+        p.qv[v] = I(pair_vertex_pv(cf, uz, hz));
     } else {
         int64_t e = t.first + threadIdx.x;
         if (e >= p.ne) return;
         int32_t c[2];
         C lh[2];
-        if (REC)
-            edge_cells_narrow(p, e, c);
-        else
-            ldrec(p.re_cells + 2 * e, c);
+        load_edge_cells<REC>(p, e, c);
         ldrec(p.re_lh + 2 * e, lh);
-        p.G[e] = I((lh[0] * C(h[c[0]]) + lh[1] * C(h[c[1]])) * C(u[e]));
+        p.G[e] = I(pair_mass_flux(lh, C(h[c[0]]), C(h[c[1]]), C(u[e])));
     }
 }
 
@@ -505,56 +236,18 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
     const int64_t nc = p.nc;
     int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= nc) return;
-    const int32_t meta = ld(p.cmeta + i);
-    const int n = meta & 15;
-    int32_t E[6], V[6];
+    int32_t meta, E[6], V[6];
     C W[15];
-    constexpr bool narrow = P::id == TRISK_FP32_BULK;
-    bool wide = !narrow;
-    if (narrow) {
-        int32_t pk[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) pk[k] = ld(p.pc_ev16 + k * nc + i);
-        wide = int16_t(pk[0] & 0xffff) == kWide;
-        if (!wide) {
-            const int32_t be = int32_t(3 * i), bv = int32_t(2 * i);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                E[k] = be + int16_t(pk[k] & 0xffff);
-                V[k] = bv + int16_t(uint32_t(pk[k]) >> 16);
-            }
-        }
-    }
-    if (wide) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            E[k] = ld(p.ce6 + k * nc + i);
-            V[k] = ld(p.ccv + k * nc + i);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 15; ++q) W[q] = ld(p.pw + q * nc + i);
+    load_cell_pairs<P::id == TRISK_FP32_BULK>(p, i, meta, E, V, W);
+    const int n = meta & 15;
     C G[6], Q[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         G[k] = C(p.G[E[k]]);
         Q[k] = C(p.qv[V[k]]);
     }
-    C qe[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) qe[k] = (Q[k] + (k + 1 < n ? Q[k + 1 < 6 ? k + 1 : 0] : Q[0])) / 2;
-    C H[6] = {0, 0, 0, 0, 0, 0};
-    int q = 0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-#pragma unroll
-        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
-            if (k2 < n) {
-                const C tq = W[q] * (qe[k] + qe[k2]);
-                H[k] += tq * G[k2];
-                H[k2] -= tq * G[k];
-            }
-        }
+    C H[6];
+    pair_half_sums(n, W, G, Q, H);
     double div = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -577,15 +270,10 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r) {
     if (e >= p.ne) return;
     int32_t c[2];
     I hs[2];
-    if (P::id == TRISK_FP32_BULK)
-        edge_cells_narrow(p, e, c);
-    else
-        ldrec(p.re_cells + 2 * e, c);
+    load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
     ldrec(p.T + 2 * e, hs);
-    const C d = ld(p.dual + e);
-    const C flux = (C(hs[0]) + C(hs[1])) / (2 * d);
-    const double grad = (p.B[c[1]] - p.B[c[0]]) / double(d);
-    rk_update<U>(r, e, double(flux) - grad, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    const double du = pair_edge_tendency(C(hs[0]), C(hs[1]), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
+    rk_update<U>(r, e, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------
@@ -603,15 +291,6 @@ __global__ __launch_bounds__(kBlock) void edge_sum_kernel(int64_t n, int K, cons
 
 // ---------------------------------------------------------------------------------
 // compensated sums: Neumaier per thread over a fixed chunk, then in thread order
-
-__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
-    double t = s + x;
-    if (fabs(s) >= fabs(x))
-        c += (s - t) + x;
-    else
-        c += (x - t) + s;
-    s = t;
-}
 
 // term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
 __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
@@ -648,37 +327,6 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, con
 // ---------------------------------------------------------------------------------
 // handle
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-template <class T>
-static int upload(std::vector<DevBuf>& owned, const std::vector<T>& v, const T** out) {
-    if (v.empty()) {
-        *out = nullptr;
-        return 0;
-    }
-    DevBuf b;
-    b.bytes = v.size() * sizeof(T);
-    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a table failed");
-    owned.push_back(b);
-    HIP_TRY(hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice));
-    *out = static_cast<const T*>(b.p);
-    return 0;
-}
-
-static int alloc(std::vector<DevBuf>& owned, size_t bytes, void** out) {
-    DevBuf b;
-    b.bytes = bytes ? bytes : 16;
-    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a work vector failed");
-    owned.push_back(b);
-    *out = b.p;
-    return 0;
-}
-
-static std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
-
 enum KernelId { K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_PAIR_STATE = 3, K_PAIR_CELL = 4, K_PAIR_EDGE = 5,
                 K_COUNT = 6 };
 static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel",      "update_kernel",
@@ -687,38 +335,6 @@ static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel",
 }  // namespace trisk
 
 using namespace trisk;
-
-struct trisk_handle {
-    Layout L;
-    int precision = TRISK_FP64;
-    int device = 0;
-    std::vector<DevBuf> owned;
-    Params<Fp64> p64{};
-    Params<Fp32Bulk> p32{};
-    // RK4 work (acc, stage) and reduction scratch
-    double* acc = nullptr;
-    void* su = nullptr;    // stage velocity (ne, FP64 or FP32)
-    double* sh = nullptr;  // stage height (nc)
-    double* partial = nullptr;
-    int reduce_blocks = 1024;
-    double* diag_K = nullptr;
-    double* diag_qv = nullptr;
-    double* diag_zv = nullptr;
-    double* diag_hv = nullptr;
-    std::vector<double> host_stage;
-    // graph replay of trisk_rk4_steps: one captured graph of kGraphSteps steps per (y, dt)
-    bool graphs = false;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    double* graph_y = nullptr;
-    double graph_dt = 0;
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> events;     // pairs (start, stop)
-    std::vector<int> event_kernel;      // kernel id per pair
-    size_t events_used = 0;
-
-};
 
 namespace trisk {
 

@@ -1,0 +1,488 @@
+// Device-side definitions shared by the TRiSK kernels (trisk_device.hip) and the ensemble
+// kernels (trisk_ensemble.hip): precision policies, kernel parameters, XCD tile mapping,
+// streamed loads, the RK4 stage update and the handle of one model.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "trisk_layout.hpp"
+
+namespace trisk {
+
+// Records `msg` as the thread's last error (trisk_last_error) and returns `code`.
+int set_err(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+#ifndef TRISK_BLOCK
+#define TRISK_BLOCK 256
+#endif
+constexpr int kBlock = TRISK_BLOCK;
+constexpr int kXcd = 8;
+
+// ---------------------------------------------------------------------------------
+// precision policies
+
+struct Fp64 {
+    using C = double;  // coefficients of the stencils
+    using I = double;  // intermediates F, q_e, q_v, zeta_v, h_v, K_v
+    using U = double;  // stage velocity
+    static constexpr int id = TRISK_FP64;
+};
+struct Fp32Bulk {
+    using C = float;
+    using I = float;
+    using U = float;
+    static constexpr int id = TRISK_FP32_BULK;
+};
+
+template <class I>
+struct Pair {
+    I F, q;
+};
+template <>
+struct alignas(16) Pair<double> {
+    double F, q;
+};
+template <>
+struct alignas(8) Pair<float> {
+    float F, q;
+};
+
+// ---------------------------------------------------------------------------------
+// kernel parameters (device pointers, entity-major tables)
+
+template <class P>
+struct Params {
+    using C = typename P::C;
+    using I = typename P::I;
+    int64_t nc, ne, nv;
+    int kc, kv, kt, kg;
+    double g;
+    // cells
+    const int32_t* ce;
+    const double* div;
+    const C* kin;
+    const int32_t* cv;
+    const C* vkk;
+    const double* area_c;
+    // edges
+    const int32_t* ec;
+    const int32_t* ev;
+    const C* eh;
+    const C* invd;
+    const int32_t* te;
+    const C* tw;
+    const C* vkh;
+    const int32_t* ge;
+    const int32_t* gp;
+    const C* gw;
+    const C* grh;
+    const C* gf;
+    // vertices
+    const int32_t* ve;
+    const C* curl;
+    const int32_t* vc;
+    const C* vh;
+    const C* fv;
+    const C* vkw;
+    const double* area_v;
+    // work
+    double* B;          // nc
+    double* K;          // nc (vertex KE or diagnostics)
+    Pair<I>* Fq;        // ne
+    I* qv;              // nv
+    I* zv;              // nv (Gassmann, diagnostics)
+    I* hv;              // nv (Gassmann, vertex KE, diagnostics)
+    I* Kv;              // nv (vertex KE)
+    // cell-pair path
+    const int32_t* cmeta;  // nc: size | negative-sign bits << 4
+    const int32_t* ccv;    // 6 x nc corners
+    const C* pw;           // 15 x nc pair weights
+    const C* len;          // ne: l_e
+    const C* dual;         // ne: d_e
+    I* G;                  // ne: l_e F_e
+    I* T;                  // ne x 2: PV-flux half sums of the left / right cell
+    // entity records of the cell-pair path
+    const int32_t* rc_state;  // nc x 6
+    const C* rc_kin;          // nc x 6
+    const int32_t* rv_idx;    // nv x 6
+    const C* rv_coef;         // nv x 8
+    const int32_t* re_cells;  // ne x 2
+    const C* re_lh;           // ne x 2
+    const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
+    // 16-bit deltas (sentinel kWide: read the int32 tables)
+    const int16_t* rc_state16;  // nc x 8
+    const int16_t* rv_idx16;    // nv x 8
+    const int32_t* re_cells16;  // ne
+    const int32_t* pc_ev16;     // 6 x nc
+    // optional FP64 outputs of the diagnostics (NULL: not written)
+    double* K_out;
+    double* qv_out;
+    double* zv_out;
+    double* hv_out;
+};
+
+// Block b of a grid of kXcd * (t0 + t1) blocks -> (range, tile).  The hardware deals
+// blocks round-robin to the XCDs; XCD x receives tiles [x*t0, (x+1)*t0) of range 0
+// and [x*t1, (x+1)*t1) of range 1, contiguous in the Hilbert order.
+struct Tile {
+    int range;
+    int64_t first;  // first entity of the tile (may be past the range's end)
+};
+__device__ __forceinline__ Tile tile_of(int64_t b, int64_t t0, int64_t t1) {
+    int64_t x = b % kXcd, j = b / kXcd;
+    if (j < t0) return {0, (x * t0 + j) * kBlock};
+    return {1, (x * t1 + (j - t0)) * kBlock};
+}
+
+__device__ __forceinline__ Tile tile_of3(int64_t b, int64_t t0, int64_t t1, int64_t t2) {
+    int64_t x = b % kXcd, j = b / kXcd;
+    if (j < t0) return {0, (x * t0 + j) * kBlock};
+    if (j < t0 + t1) return {1, (x * t1 + (j - t0)) * kBlock};
+    return {2, (x * t2 + (j - t0 - t1)) * kBlock};
+}
+
+inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
+
+// Streamed tables are read once per launch: nontemporal loads keep them from evicting
+// the gathered state and intermediates from the caches.
+template <class T>
+__device__ __forceinline__ T ld(const T* p) {
+#ifdef TRISK_PLAIN_LOADS
+    return *p;
+#else
+    return __builtin_nontemporal_load(p);
+#endif
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// N consecutive elements starting at p (one entity record) with 16-byte (or, for records
+// of a size not divisible by 16, 8-byte) nontemporal vector loads; p must be aligned to
+// that width.
+template <class T, int N>
+__device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
+    constexpr int bytes = N * int(sizeof(T));
+    if constexpr (bytes % 16 == 0) {
+        const f32x4* q = reinterpret_cast<const f32x4*>(p);
+#pragma unroll
+        for (int c = 0; c < bytes / 16; ++c) {
+            f32x4 v = __builtin_nontemporal_load(q + c);
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 16 * c, &v, 16);
+        }
+    } else {
+        static_assert(bytes % 8 == 0, "records are multiples of 8 bytes");
+        const f32x2* q = reinterpret_cast<const f32x2*>(p);
+#pragma unroll
+        for (int c = 0; c < bytes / 8; ++c) {
+            f32x2 v = __builtin_nontemporal_load(q + c);
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 8 * c, &v, 8);
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------------
+// RK4 stage update (rk4_step!, RungeKutta.jl:18-28)
+
+struct Rk {
+    int mode;            // 0: store the tendency in dy; 1..4: RK4 stage
+    double* dy;          // mode 0
+    double* y;           // state
+    double* acc;         // k1 + 2k2 + 2k3 accumulator
+    void* su;            // stage velocity (ne, type U)
+    double* sh;          // stage height (nc)
+    double dt, dt2, dt6;
+};
+
+// y + c dt k and the accumulator, in the order of rk4_step!'s broadcasts
+template <class U>
+__device__ __forceinline__ void rk_update(const Rk& r, int64_t j, double k, U* stage) {
+    switch (r.mode) {
+        case 0:
+            r.dy[j] = k;
+            break;
+        case 1:
+            r.acc[j] = k;
+            stage[0] = U(r.y[j] + r.dt2 * k);
+            break;
+        case 2:
+            r.acc[j] = r.acc[j] + 2 * k;
+            stage[0] = U(r.y[j] + r.dt2 * k);
+            break;
+        case 3:
+            r.acc[j] = r.acc[j] + 2 * k;
+            stage[0] = U(r.y[j] + r.dt * k);
+            break;
+        default:
+            r.y[j] = r.y[j] + r.dt6 * (r.acc[j] + k);
+            break;
+    }
+}
+
+// The two cells of edge e from the packed 16-bit deltas (base e/3), or the int32 pair.
+template <class P>
+__device__ __forceinline__ void edge_cells_narrow(const Params<P>& p, int64_t e, int32_t (&c)[2]) {
+    const int32_t pk = ld(p.re_cells16 + e);
+    const int16_t d0 = int16_t(pk & 0xffff), d1 = int16_t(uint32_t(pk) >> 16);
+    if (d0 == kWide) {
+        ldrec(p.re_cells + 2 * e, c);
+    } else {
+        const int32_t b = int32_t(e / 3);
+        c[0] = b + d0;
+        c[1] = b + d1;
+    }
+}
+
+// Table records of the cell-pair path.  NARROW: the FP32-bulk layout (16-bit index
+// deltas from a linear base, entity records read with vector loads, int32 fallback for
+// entities flagged kWide); otherwise the slot-major int32 tables.
+
+// cell i: its six edge slots and kinetic-energy weights (pads: a valid index, weight 0)
+template <bool NARROW, class P>
+__device__ __forceinline__ void load_cell_state(const Params<P>& p, int64_t i, int32_t (&E)[6],
+                                                typename P::C (&kin)[6]) {
+    using C = typename P::C;
+    if (NARROW) {
+        int16_t d[8];
+        ldrec(p.rc_state16 + 8 * i, d);
+        if (d[0] == kWide) {
+            ldrec(p.rc_state + 6 * i, E);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) E[k] = int32_t(3 * i) + d[k];
+        }
+        ldrec(p.rc_kin + 6 * i, kin);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            E[k] = k < p.kc ? ld(p.ce + k * p.nc + i) : 0;
+            kin[k] = k < p.kc ? ld(p.kin + k * p.nc + i) : C(0);
+        }
+    }
+}
+
+// vertex v: edges (3), cells (3); coefficients curl (3), vertex heights (3), f
+template <bool NARROW, class P>
+__device__ __forceinline__ void load_vertex_record(const Params<P>& p, int64_t v, int32_t (&idx)[6],
+                                                   typename P::C (&cf)[8]) {
+    if (NARROW) {
+        int16_t d[8];
+        ldrec(p.rv_idx16 + 8 * v, d);
+        if (d[0] == kWide) {
+            ldrec(p.rv_idx + 6 * v, idx);
+        } else {
+            const int32_t b0 = int32_t((3 * v) >> 1), b1 = int32_t(v >> 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                idx[k] = b0 + d[k];
+                idx[3 + k] = b1 + d[3 + k];
+            }
+        }
+        ldrec(p.rv_coef + 8 * v, cf);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            idx[k] = ld(p.ve + k * p.nv + v);
+            idx[3 + k] = ld(p.vc + k * p.nv + v);
+            cf[k] = ld(p.curl + k * p.nv + v);
+            cf[3 + k] = ld(p.vh + k * p.nv + v);
+        }
+        cf[6] = ld(p.fv + v);
+        cf[7] = 0;
+    }
+}
+
+// edge e: left and right cell
+template <bool NARROW, class P>
+__device__ __forceinline__ void load_edge_cells(const Params<P>& p, int64_t e, int32_t (&c)[2]) {
+    if (NARROW)
+        edge_cells_narrow(p, e, c);
+    else
+        ldrec(p.re_cells + 2 * e, c);
+}
+
+// cell i: size and side bits, edges and corners (6 slots, pads valid), pair weights
+template <bool NARROW, class P>
+__device__ __forceinline__ void load_cell_pairs(const Params<P>& p, int64_t i, int32_t& meta, int32_t (&E)[6],
+                                                int32_t (&V)[6], typename P::C (&W)[15]) {
+    const int64_t nc = p.nc;
+    meta = ld(p.cmeta + i);
+    bool wide = !NARROW;
+    if (NARROW) {
+        int32_t pk[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pk[k] = ld(p.pc_ev16 + k * nc + i);
+        wide = int16_t(pk[0] & 0xffff) == kWide;
+        if (!wide) {
+            const int32_t be = int32_t(3 * i), bv = int32_t(2 * i);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                E[k] = be + int16_t(pk[k] & 0xffff);
+                V[k] = bv + int16_t(uint32_t(pk[k]) >> 16);
+            }
+        }
+    }
+    if (wide) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            E[k] = ld(p.ce6 + k * nc + i);
+            V[k] = ld(p.ccv + k * nc + i);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 15; ++q) W[q] = ld(p.pw + q * nc + i);
+}
+
+// ---------------------------------------------------------------------------------
+// per-entity arithmetic of the cell-pair path, shared by the single-model kernels and the
+// ensemble kernels so that every member rounds exactly as a model of its own
+
+// K_i = sum_k kin_k u_k^2 over the six slots (pads have a zero weight)
+template <class C>
+__device__ __forceinline__ C pair_kinetic(const C (&kin)[6], const C (&ue)[6]) {
+    C s = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s += kin[k] * (ue[k] * ue[k]);
+    return s;
+}
+
+// q_v = (zeta_v + f_v) / h_v from the vertex record (curl 3, vertex heights 3, f)
+template <class C>
+__device__ __forceinline__ C pair_vertex_pv(const C (&cf)[8], const C (&uz)[3], const C (&hz)[3]) {
+    C z = 0, eta = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) z += cf[k] * uz[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) eta += cf[3 + k] * hz[k];
+    return (z + cf[6]) / eta;
+}
+
+// G_e = l_e h_e u_e
+template <class C>
+__device__ __forceinline__ C pair_mass_flux(const C (&lh)[2], C h0, C h1, C u) {
+    return (lh[0] * h0 + lh[1] * h1) * u;
+}
+
+// the PV-flux half sums H_k = sum_k' w_{k,k'} G_k' (q_k + q_k') of a cell with n edges,
+// q_k the mean of the PV at the two corners of edge k
+template <class C>
+__device__ __forceinline__ void pair_half_sums(int n, const C (&W)[15], const C (&G)[6], const C (&Q)[6],
+                                               C (&H)[6]) {
+    C qe[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) qe[k] = (Q[k] + (k + 1 < n ? Q[k + 1 < 6 ? k + 1 : 0] : Q[0])) / 2;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) H[k] = 0;
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
+            if (k2 < n) {
+                const C tq = W[q] * (qe[k] + qe[k2]);
+                H[k] += tq * G[k2];
+                H[k2] -= tq * G[k];
+            }
+        }
+}
+
+// du_e = (H_left + H_right) / (2 d_e) - (B_right - B_left) / d_e
+template <class C>
+__device__ __forceinline__ double pair_edge_tendency(C hs0, C hs1, C d, double Bl, double Br) {
+    const C flux = (hs0 + hs1) / (2 * d);
+    const double grad = (Br - Bl) / double(d);
+    return double(flux) - grad;
+}
+
+// ---------------------------------------------------------------------------------
+// compensated sums
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+// ---------------------------------------------------------------------------------
+// device memory owned by a handle
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+template <class T>
+inline int upload(std::vector<DevBuf>& owned, const std::vector<T>& v, const T** out) {
+    if (v.empty()) {
+        *out = nullptr;
+        return 0;
+    }
+    DevBuf b;
+    b.bytes = v.size() * sizeof(T);
+    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a table failed");
+    owned.push_back(b);
+    HIP_TRY(hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(b.p);
+    return 0;
+}
+
+inline int alloc(std::vector<DevBuf>& owned, size_t bytes, void** out) {
+    DevBuf b;
+    b.bytes = bytes ? bytes : 16;
+    if (hipMalloc(&b.p, b.bytes) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of a work vector failed");
+    owned.push_back(b);
+    *out = b.p;
+    return 0;
+}
+
+inline std::vector<float> to_float(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+}  // namespace trisk
+
+// One model on one device: its tables, work vectors, RK4 registers, captured graph and timing events.
+struct trisk_handle {
+    trisk::Layout L;
+    int precision = TRISK_FP64;
+    int device = 0;
+    std::vector<trisk::DevBuf> owned;
+    trisk::Params<trisk::Fp64> p64{};
+    trisk::Params<trisk::Fp32Bulk> p32{};
+    // RK4 work (acc, stage) and reduction scratch
+    double* acc = nullptr;
+    void* su = nullptr;    // stage velocity (ne, FP64 or FP32)
+    double* sh = nullptr;  // stage height (nc)
+    double* partial = nullptr;
+    int reduce_blocks = 1024;
+    double* diag_K = nullptr;
+    double* diag_qv = nullptr;
+    double* diag_zv = nullptr;
+    double* diag_hv = nullptr;
+    std::vector<double> host_stage;
+    // graph replay of trisk_rk4_steps: one captured graph of kGraphSteps steps per (y, dt)
+    bool graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* graph_y = nullptr;
+    double graph_dt = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;     // pairs (start, stop)
+    std::vector<int> event_kernel;      // kernel id per pair
+    size_t events_used = 0;
+};
+

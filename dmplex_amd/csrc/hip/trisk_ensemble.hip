@@ -1,0 +1,873 @@
+// Batched ensembles of TRiSK shallow-water models on meshes of several resolutions
+// (trisk_ensemble_* in include/trisk_hip.h).
+//
+// The reference advances one model per rk4_step! loop (RungeKutta.jl:18-28,
+// examples/trisk_shallow_water.jl:92-100).  Here the members of a level share that level's
+// tables, and every work vector is member-interleaved: member m of entity j sits at
+// j * M + m.  A thread owns one (entity, member) item, so the M lanes of one entity read its
+// table record once (a broadcast) and every neighbour gather reads M consecutive values.
+// Each kernel of the cell-pair tendency (state, cell, edge: see trisk_device.hip) is one
+// launch over all members of all levels; its blocks walk a per-XCD list of tiles built on
+// the host (the schedule), optionally as a persistent grid.  The arithmetic is the
+// single-model kernels' (trisk_device.hpp), so every member rounds exactly as a model of
+// its own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "trisk_device.hpp"
+
+namespace trisk {
+
+constexpr int kMaxLevels = 16;
+constexpr int kEnsKinds = 3;  // state, cell, edge kernels
+
+// n / d for n < 2^31 with one 32x32->64-bit product: with l = ceil(log2 d) and
+// mul = ceil(2^(31+l) / d) (< 2^32), n * mul / 2^(31+l) = n/d + n e / (d 2^(31+l)) with
+// 0 <= e < d <= 2^l, so the excess over n/d is below 1/d and the floor is exact.
+struct FastDiv {
+    uint32_t mul = 1u << 31;
+    uint32_t shift = 31;
+    __host__ __device__ uint32_t operator()(uint32_t n) const { return uint32_t((uint64_t(n) * mul) >> shift); }
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    FastDiv f;
+    f.shift = 31 + l;
+    f.mul = uint32_t(((uint64_t(1) << f.shift) + d - 1) / d);
+    return f;
+}
+
+// One level: the tables of its model, the member-interleaved work vectors and its dt.
+template <class P>
+struct EnsLevel {
+    Params<P> p;       // tables; B, qv, G, T point to the level's (entity x member) vectors
+    FastDiv md;        // division by M
+    int32_t M;         // members
+    int64_t yoff;      // first element of the level's block [u (ne M); h (nc M)] of the state
+    double dt, dt2, dt6;
+    double* acc;       // (ne + nc) M  RK4 accumulator
+    void* su;          // ne M         stage velocity (type P::U)
+    double* sh;        // nc M         stage height
+};
+
+// A run of tiles of one range (cells, vertices or edges of one level) on one XCD.
+struct EnsSeg {
+    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges
+    int32_t tile0;        // first tile of the run within its range
+    int32_t ntiles;
+    int32_t cum;          // tiles before the run in its XCD's list
+    int32_t nitems;       // items (entity x member) of the range
+};
+
+struct EnsArgs {
+    const void* levels;          // EnsLevel<P>[nlevels] in device memory
+    const EnsSeg* seg;           // runs of all XCDs
+    int32_t first[kXcd + 1];     // XCD x's runs: seg[first[x] .. first[x + 1])
+    double* y;                   // ensemble state
+    double* dy;                  // tendency output (mode 0)
+    int32_t mode;                // Rk::mode
+    int32_t from_y;              // stage input: the state itself (1) or the stage registers (0)
+};
+
+template <class P>
+__device__ __forceinline__ Rk level_rk(const EnsLevel<P>& L, const EnsArgs& a) {
+    Rk r;
+    r.mode = a.mode;
+    r.dy = a.dy ? a.dy + L.yoff : nullptr;
+    r.y = a.y + L.yoff;
+    r.acc = L.acc;
+    r.su = L.su;
+    r.sh = L.sh;
+    r.dt = L.dt;
+    r.dt2 = L.dt2;
+    r.dt6 = L.dt6;
+    return r;
+}
+
+// body(run, item) for every item of the tiles of this block's XCD (blockIdx mod 8: the
+// hardware deals workgroups to the XCDs round-robin); the XCD's blocks stride over its
+// tile list, so a grid smaller than the list is a persistent launch.
+template <int BLK, class F>
+__device__ __forceinline__ void for_each_item(const EnsArgs& a, F&& body) {
+    const int x = int(blockIdx.x % kXcd);
+    const int nbx = int(gridDim.x / kXcd);
+    const int s0 = a.first[x], s1 = a.first[x + 1];
+    if (s1 <= s0) return;
+    const int total = a.seg[s1 - 1].cum + a.seg[s1 - 1].ntiles;
+    int s = s0;
+    for (int t = int(blockIdx.x / kXcd); t < total; t += nbx) {
+        while (s + 1 < s1 && t >= a.seg[s + 1].cum) ++s;  // t only grows
+        const EnsSeg g = a.seg[s];
+        const int64_t j = int64_t(g.tile0 + (t - g.cum)) * BLK + threadIdx.x;
+        if (j < g.nitems) body(g, uint32_t(j));
+    }
+}
+
+// W1 of all levels: cells (B), vertices (q_v), edges (G); cf. pair_state_kernel
+template <class P, class UIn, int BLK>
+__global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
+    using C = typename P::C;
+    using I = typename P::I;
+    constexpr bool NARROW = P::id == TRISK_FP32_BULK;
+    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
+    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
+        const EnsLevel<P>& L = lv[g.level];
+        const Params<P>& p = L.p;
+        const uint32_t M = uint32_t(L.M);
+        const uint32_t ent = L.md(j);
+        const uint32_t m = j - ent * M;
+        const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
+        const double* h = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
+        if (g.kind == 0) {
+            int32_t E[6];
+            C kin[6], ue[6];
+            load_cell_state<NARROW>(p, ent, E, kin);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) ue[k] = C(u[uint32_t(E[k]) * M + m]);
+            p.B[j] = double(pair_kinetic(kin, ue)) + p.g * h[j];
+        } else if (g.kind == 1) {
+            int32_t idx[6];
+            C cf[8], uz[3], hz[3];
+            load_vertex_record<NARROW>(p, ent, idx, cf);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                uz[k] = C(u[uint32_t(idx[k]) * M + m]);
+                hz[k] = C(h[uint32_t(idx[3 + k]) * M + m]);
+            }
+            p.qv[j] = I(pair_vertex_pv(cf, uz, hz));
+        } else {
+            int32_t c[2];
+            C lh[2];
+            load_edge_cells<NARROW>(p, ent, c);
+            ldrec(p.re_lh + 2 * int64_t(ent), lh);
+            p.G[j] = I(pair_mass_flux(lh, C(h[uint32_t(c[0]) * M + m]), C(h[uint32_t(c[1]) * M + m]), C(u[j])));
+        }
+    });
+}
+
+// W2 of all levels: PV-flux half sums and continuity with its RK4 update; cf. pair_cell_kernel.
+// Half sums are stored side-major per edge: T[(2 e + side) M + m].
+template <class P, int BLK>
+__global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
+    using C = typename P::C;
+    using I = typename P::I;
+    constexpr bool NARROW = P::id == TRISK_FP32_BULK;
+    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
+    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
+        const EnsLevel<P>& L = lv[g.level];
+        const Params<P>& p = L.p;
+        const uint32_t M = uint32_t(L.M);
+        const uint32_t i = L.md(j);
+        const uint32_t m = j - i * M;
+        int32_t meta, E[6], V[6];
+        C W[15], G[6], Q[6], H[6];
+        load_cell_pairs<NARROW>(p, i, meta, E, V, W);
+        const int n = meta & 15;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            G[k] = C(p.G[uint32_t(E[k]) * M + m]);
+            Q[k] = C(p.qv[uint32_t(V[k]) * M + m]);
+        }
+        pair_half_sums(n, W, G, Q, H);
+        double div = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if (k < n) {
+                const int side = (meta >> (4 + k)) & 1;
+                p.T[(2 * int64_t(E[k]) + side) * M + m] = I(H[k]);
+                div += side ? -double(G[k]) : double(G[k]);
+            }
+        }
+        const Rk r = level_rk(L, a);
+        rk_update<double>(r, p.ne * M + j, -div / ld(p.area_c + i),
+                          r.mode >= 1 && r.mode <= 3 ? L.sh + j : nullptr);
+    });
+}
+
+// W3 of all levels: du and its RK4 update; cf. pair_edge_kernel
+template <class P, int BLK>
+__global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
+    using C = typename P::C;
+    using U = typename P::U;
+    constexpr bool NARROW = P::id == TRISK_FP32_BULK;
+    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
+    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
+        const EnsLevel<P>& L = lv[g.level];
+        const Params<P>& p = L.p;
+        const uint32_t M = uint32_t(L.M);
+        const uint32_t e = L.md(j);
+        const uint32_t m = j - e * M;
+        int32_t c[2];
+        load_edge_cells<NARROW>(p, e, c);
+        const int64_t t = 2 * int64_t(e) * M + m;
+        const double du = pair_edge_tendency(C(p.T[t]), C(p.T[t + M]), C(ld(p.dual + e)),
+                                             p.B[uint32_t(c[0]) * M + m], p.B[uint32_t(c[1]) * M + m]);
+        const Rk r = level_rk(L, a);
+        rk_update<U>(r, j, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + j : nullptr);
+    });
+}
+
+// member m of a level block <-> a single-member state (dir 0: get, 1: set)
+__global__ __launch_bounds__(256) void ens_member_kernel(double* __restrict__ block, double* __restrict__ one,
+                                                         int64_t ne, int64_t n, int32_t M, int32_t m, int32_t dir) {
+    const int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int64_t at = k < ne ? k * M + m : ne * M + (k - ne) * M + m;
+    if (dir == 0)
+        one[k] = block[at];
+    else
+        block[at] = one[k];
+}
+
+static const char* kEnsNames[kEnsKinds] = {"ens_state_kernel", "ens_cell_kernel", "ens_edge_kernel"};
+
+}  // namespace trisk
+
+using namespace trisk;
+
+struct trisk_ensemble {
+    int precision = TRISK_FP64;
+    int device = 0;
+    int cus = 256;
+    int nlevels = 0;
+    std::vector<trisk_handle*> models;  // one per level: its tables (device) and numbering
+    std::vector<int32_t> M;
+    std::vector<int64_t> yoff;
+    int64_t n_state = 0;
+    std::vector<DevBuf> owned;
+    std::vector<EnsLevel<Fp64>> l64;
+    std::vector<EnsLevel<Fp32Bulk>> l32;
+    void* levels_dev = nullptr;
+    std::vector<double> dts;
+    // schedules of the three kernels
+    trisk_schedule sched[kEnsKinds]{};
+    EnsSeg* seg_dev[kEnsKinds]{};
+    size_t seg_cap[kEnsKinds]{};
+    int32_t first[kEnsKinds][kXcd + 1]{};
+    int64_t grid[kEnsKinds]{};
+    double* scratch = nullptr;  // one member of the largest level
+    // graph replay
+    bool graphs = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* graph_y = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> event_kernel;
+    size_t events_used = 0;
+};
+
+namespace trisk {
+
+static bool valid_schedule(const trisk_schedule& s) {
+    return (s.block_size == 128 || s.block_size == 256 || s.block_size == 512) && s.blocks_per_cu >= 0 &&
+           s.blocks_per_cu <= 64 && (s.split == TRISK_SPLIT_CHUNKED || s.split == TRISK_SPLIT_BLOCKED);
+}
+
+// The runs of kernel `kind` per XCD under its schedule.
+static int build_schedule(trisk_ensemble* E, int kind) {
+    const trisk_schedule& sc = E->sched[kind];
+    const int64_t blk = sc.block_size;
+    struct Range {
+        int32_t level, kind;
+        int64_t items, tiles;
+    };
+    std::vector<Range> ranges;
+    for (int l = 0; l < E->nlevels; ++l) {
+        const Layout& L = E->models[l]->L;
+        const int64_t M = E->M[l];
+        if (kind == TRISK_ENS_STATE) {
+            ranges.push_back({l, 0, L.nc * M, 0});
+            ranges.push_back({l, 1, L.nv * M, 0});
+            ranges.push_back({l, 2, L.ne * M, 0});
+        } else if (kind == TRISK_ENS_CELL) {
+            ranges.push_back({l, 0, L.nc * M, 0});
+        } else {
+            ranges.push_back({l, 2, L.ne * M, 0});
+        }
+    }
+    int64_t all = 0;
+    for (auto& r : ranges) {
+        r.tiles = (r.items + blk - 1) / blk;
+        all += r.tiles;
+    }
+    std::vector<std::vector<EnsSeg>> per(kXcd);
+    auto add = [&](int x, const Range& r, int64_t lo, int64_t hi) {
+        if (lo >= hi) return;
+        per[x].push_back({r.level, r.kind, int32_t(lo), int32_t(hi - lo), 0, int32_t(r.items)});
+    };
+    if (sc.split == TRISK_SPLIT_CHUNKED) {
+        // XCD x: the x-th contiguous chunk of every range
+        for (int x = 0; x < kXcd; ++x)
+            for (const auto& r : ranges) {
+                const int64_t c = (r.tiles + kXcd - 1) / kXcd;
+                add(x, r, std::min(r.tiles, x * c), std::min(r.tiles, (x + 1) * c));
+            }
+    } else {
+        // XCD x: the x-th contiguous chunk of the ranges laid end to end
+        const int64_t c = (all + kXcd - 1) / kXcd;
+        for (int x = 0; x < kXcd; ++x) {
+            const int64_t lo = x * c, hi = std::min(all, (x + 1) * c);
+            int64_t base = 0;
+            for (const auto& r : ranges) {
+                add(x, r, std::max<int64_t>(lo - base, 0), std::min(r.tiles, hi - base));
+                base += r.tiles;
+            }
+        }
+    }
+    std::vector<EnsSeg> segs;
+    int64_t most = 0;
+    for (int x = 0; x < kXcd; ++x) {
+        E->first[kind][x] = int32_t(segs.size());
+        int64_t cum = 0;
+        for (auto g : per[x]) {
+            g.cum = int32_t(cum);
+            cum += g.ntiles;
+            segs.push_back(g);
+        }
+        most = std::max(most, cum);
+    }
+    E->first[kind][kXcd] = int32_t(segs.size());
+    int64_t nbx = most;
+    if (sc.blocks_per_cu > 0) nbx = std::min<int64_t>(most, int64_t(sc.blocks_per_cu) * ((E->cus + kXcd - 1) / kXcd));
+    E->grid[kind] = kXcd * std::max<int64_t>(nbx, 1);
+    if (segs.empty()) segs.push_back({0, 0, 0, 0, 0, 0});
+    if (segs.size() > E->seg_cap[kind]) {
+        void* q;
+        int rc = alloc(E->owned, segs.size() * sizeof(EnsSeg), &q);
+        if (rc) return rc;
+        E->seg_dev[kind] = static_cast<EnsSeg*>(q);
+        E->seg_cap[kind] = segs.size();
+    }
+    HIP_TRY(hipMemcpy(E->seg_dev[kind], segs.data(), segs.size() * sizeof(EnsSeg), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <class P>
+static std::vector<EnsLevel<P>>& ens_levels(trisk_ensemble* E);
+template <>
+std::vector<EnsLevel<Fp64>>& ens_levels<Fp64>(trisk_ensemble* E) {
+    return E->l64;
+}
+template <>
+std::vector<EnsLevel<Fp32Bulk>>& ens_levels<Fp32Bulk>(trisk_ensemble* E) {
+    return E->l32;
+}
+
+template <class P>
+static Params<P>& model_params(trisk_handle* h) {
+    if constexpr (P::id == TRISK_FP64)
+        return h->p64;
+    else
+        return h->p32;
+}
+
+template <class P>
+static int setup_levels(trisk_ensemble* E) {
+    using I = typename P::I;
+    using U = typename P::U;
+    auto& lv = ens_levels<P>(E);
+    lv.resize(E->nlevels);
+    int64_t biggest = 0;
+    for (int l = 0; l < E->nlevels; ++l) {
+        const Layout& L = E->models[l]->L;
+        const int64_t M = E->M[l];
+        EnsLevel<P>& e = lv[l];
+        e.p = model_params<P>(E->models[l]);
+        e.md = make_fastdiv(uint32_t(M));
+        e.M = int32_t(M);
+        e.yoff = E->yoff[l];
+        e.dt = e.dt2 = e.dt6 = 0;
+        void* q;
+        int rc;
+        auto get = [&](size_t bytes) -> void* {
+            rc = alloc(E->owned, bytes, &q);
+            return rc ? nullptr : q;
+        };
+        e.p.B = static_cast<double*>(get(L.nc * M * sizeof(double)));
+        if (rc) return rc;
+        e.p.qv = static_cast<I*>(get(L.nv * M * sizeof(I)));
+        if (rc) return rc;
+        e.p.G = static_cast<I*>(get(L.ne * M * sizeof(I)));
+        if (rc) return rc;
+        e.p.T = static_cast<I*>(get(2 * L.ne * M * sizeof(I)));
+        if (rc) return rc;
+        e.acc = static_cast<double*>(get((L.ne + L.nc) * M * sizeof(double)));
+        if (rc) return rc;
+        e.su = get(L.ne * M * sizeof(U));
+        if (rc) return rc;
+        e.sh = static_cast<double*>(get(L.nc * M * sizeof(double)));
+        if (rc) return rc;
+        e.p.K = nullptr;
+        e.p.Fq = nullptr;
+        e.p.zv = e.p.hv = e.p.Kv = nullptr;
+        e.p.K_out = e.p.qv_out = e.p.zv_out = e.p.hv_out = nullptr;
+        biggest = std::max(biggest, L.ne + L.nc);
+    }
+    void* q;
+    int rc = alloc(E->owned, E->nlevels * sizeof(EnsLevel<P>), &q);
+    if (rc) return rc;
+    E->levels_dev = q;
+    if ((rc = alloc(E->owned, biggest * sizeof(double), &q))) return rc;
+    E->scratch = static_cast<double*>(q);
+    HIP_TRY(hipMemcpy(E->levels_dev, lv.data(), lv.size() * sizeof(EnsLevel<P>), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Per-level dt into the device copy of the levels (only when they change; the stream is
+// drained first because captured graphs read them).
+template <class P>
+static int set_dts(trisk_ensemble* E, const double* dt, hipStream_t s) {
+    bool same = true;
+    for (int l = 0; l < E->nlevels; ++l) same = same && E->dts[l] == dt[l];
+    if (same) return 0;
+    auto& lv = ens_levels<P>(E);
+    for (int l = 0; l < E->nlevels; ++l) {
+        E->dts[l] = dt[l];
+        lv[l].dt = dt[l];
+        lv[l].dt2 = dt[l] / 2;
+        lv[l].dt6 = dt[l] / 6;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(E->levels_dev, lv.data(), lv.size() * sizeof(EnsLevel<P>), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int ens_timed_begin(trisk_ensemble* E, int kind, hipStream_t s, hipEvent_t* stop) {
+    *stop = nullptr;
+    if (!E->timing) return 0;
+    if (E->events_used + 2 > E->events.size()) {
+        for (int k = 0; k < 256; ++k) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            E->events.push_back(e);
+        }
+        E->event_kernel.resize(E->events.size() / 2);
+    }
+    size_t i = E->events_used;
+    E->events_used += 2;
+    E->event_kernel[i / 2] = kind;
+    HIP_TRY(hipEventRecord(E->events[i], s));
+    *stop = E->events[i + 1];
+    return 0;
+}
+
+template <class P, class UIn, int BLK>
+static void launch_kind(int kind, dim3 grid, const EnsArgs& a, hipStream_t s) {
+    if (kind == TRISK_ENS_STATE)
+        hipLaunchKernelGGL((ens_state_kernel<P, UIn, BLK>), grid, dim3(BLK), 0, s, a);
+    else if (kind == TRISK_ENS_CELL)
+        hipLaunchKernelGGL((ens_cell_kernel<P, BLK>), grid, dim3(BLK), 0, s, a);
+    else
+        hipLaunchKernelGGL((ens_edge_kernel<P, BLK>), grid, dim3(BLK), 0, s, a);
+}
+
+// One tendency of every member: the three kernels with the given stage input and RK4 mode.
+template <class P, class UIn>
+static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool from_y, hipStream_t s) {
+    EnsArgs a{};
+    a.levels = E->levels_dev;
+    a.y = y;
+    a.dy = dy;
+    a.mode = mode;
+    a.from_y = from_y ? 1 : 0;
+    for (int kind = 0; kind < kEnsKinds; ++kind) {
+        a.seg = E->seg_dev[kind];
+        std::memcpy(a.first, E->first[kind], sizeof(a.first));
+        hipEvent_t stop;
+        int rc = ens_timed_begin(E, kind, s, &stop);
+        if (rc) return rc;
+        const dim3 grid(unsigned(E->grid[kind]));
+        switch (E->sched[kind].block_size) {
+            case 128:
+                launch_kind<P, UIn, 128>(kind, grid, a, s);
+                break;
+            case 512:
+                launch_kind<P, UIn, 512>(kind, grid, a, s);
+                break;
+            default:
+                launch_kind<P, UIn, 256>(kind, grid, a, s);
+                break;
+        }
+        HIP_TRY(hipGetLastError());
+        if (stop) HIP_TRY(hipEventRecord(stop, s));
+    }
+    return 0;
+}
+
+template <class P>
+static int ens_steps(trisk_ensemble* E, double* y, int64_t nsteps, hipStream_t s) {
+    using U = typename P::U;
+    int rc;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if ((rc = ens_tendency<P, double>(E, y, nullptr, 1, true, s))) return rc;
+        for (int mode = 2; mode <= 4; ++mode)
+            if ((rc = ens_tendency<P, U>(E, y, nullptr, mode, false, s))) return rc;
+    }
+    return 0;
+}
+
+static int run_ens_steps(trisk_ensemble* E, double* y, int64_t n, hipStream_t s) {
+    return E->precision == TRISK_FP64 ? ens_steps<Fp64>(E, y, n, s) : ens_steps<Fp32Bulk>(E, y, n, s);
+}
+
+static void ens_drop_graph(trisk_ensemble* E) {
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    E->graph_exec = nullptr;
+    E->graph = nullptr;
+    E->graph_y = nullptr;
+}
+
+constexpr int64_t kEnsGraphSteps = 8;
+
+static int ens_member_check(trisk_ensemble* E, int32_t level, int32_t member) {
+    if (level < 0 || level >= E->nlevels) return set_err(TRISK_EINVAL, "level out of range");
+    if (member < 0 || member >= E->M[level]) return set_err(TRISK_EINVAL, "member out of range");
+    return 0;
+}
+
+static int ens_member_copy(trisk_ensemble* E, int32_t level, int32_t member, double* y, double* one, int dir,
+                           hipStream_t s) {
+    const Layout& L = E->models[level]->L;
+    const int64_t n = L.ne + L.nc;
+    hipLaunchKernelGGL(ens_member_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, y + E->yoff[level], one,
+                       L.ne, n, E->M[level], member, dir);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace trisk
+
+#define CHECK_ENS(e) \
+    if (!(e)) return set_err(TRISK_EINVAL, "null ensemble")
+
+extern "C" {
+
+int trisk_ensemble_destroy(trisk_ensemble* e) {
+    if (!e) return 0;
+    (void)hipSetDevice(e->device);
+    ens_drop_graph(e);
+    for (auto& ev : e->events) (void)hipEventDestroy(ev);
+    for (auto& b : e->owned) (void)hipFree(b.p);
+    for (auto* h : e->models) trisk_destroy(h);
+    delete e;
+    return 0;
+}
+
+int trisk_ensemble_create(const trisk_model_desc* const* levels, const int32_t* members, int32_t nlevels,
+                          const trisk_options* opt, trisk_ensemble** out) {
+    if (!out) return set_err(TRISK_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!levels || !members || nlevels < 1 || nlevels > kMaxLevels)
+        return set_err(TRISK_EINVAL, "need 1 to 16 levels with their member counts");
+    trisk_options o{};
+    if (opt) o = *opt;
+    if (o.path != TRISK_PATH_AUTO && o.path != TRISK_PATH_PAIR)
+        return set_err(TRISK_EINVAL, "ensembles run the cell-pair path (TRISK_PATH_AUTO or TRISK_PATH_PAIR)");
+    o.path = TRISK_PATH_PAIR;
+    auto* E = new trisk_ensemble;
+    E->precision = o.precision;
+    E->device = o.device;
+    E->nlevels = nlevels;
+    for (int l = 0; l < nlevels; ++l) {
+        if (members[l] < 1) {
+            trisk_ensemble_destroy(E);
+            return set_err(TRISK_EINVAL, "every level needs at least one member");
+        }
+        trisk_handle* h = nullptr;
+        int rc = trisk_create(levels[l], &o, &h);
+        if (rc) {
+            std::string why = trisk_last_error();
+            trisk_ensemble_destroy(E);
+            return set_err(rc, "level " + std::to_string(l) + ": " + why);
+        }
+        E->models.push_back(h);
+        const int64_t items = (h->L.ne + h->L.nc) * int64_t(members[l]);
+        if (2 * h->L.ne * int64_t(members[l]) >= (int64_t(1) << 31) || items >= (int64_t(1) << 31)) {
+            trisk_ensemble_destroy(E);
+            return set_err(TRISK_EINVAL, "level " + std::to_string(l) + ": entities x members exceed 2^30");
+        }
+        E->M.push_back(members[l]);
+        E->yoff.push_back(E->n_state);
+        E->n_state += items;
+    }
+    E->dts.assign(nlevels, 0.0);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, E->device) == hipSuccess && cus > 0)
+        E->cus = cus;
+    int rc = E->precision == TRISK_FP64 ? setup_levels<Fp64>(E) : setup_levels<Fp32Bulk>(E);
+    for (int k = 0; k < kEnsKinds && !rc; ++k) {
+        E->sched[k] = trisk_schedule{256, 0, TRISK_SPLIT_CHUNKED, 0};
+        rc = build_schedule(E, k);
+    }
+    if (rc) {
+        std::string why = trisk_last_error();
+        trisk_ensemble_destroy(E);
+        return set_err(rc, why);
+    }
+    *out = E;
+    return 0;
+}
+
+int trisk_ensemble_state_length(const trisk_ensemble* e, int64_t* n) {
+    CHECK_ENS(e);
+    if (!n) return set_err(TRISK_EINVAL, "null pointer");
+    *n = e->n_state;
+    return 0;
+}
+
+int trisk_ensemble_level(trisk_ensemble* e, int32_t level, trisk_handle** model, int64_t* offset,
+                         int32_t* members) {
+    CHECK_ENS(e);
+    if (level < 0 || level >= e->nlevels) return set_err(TRISK_EINVAL, "level out of range");
+    if (model) *model = e->models[level];
+    if (offset) *offset = e->yoff[level];
+    if (members) *members = e->M[level];
+    return 0;
+}
+
+int trisk_ensemble_get_member(trisk_ensemble* e, int32_t level, int32_t member, const double* y_dev,
+                              double* y_member_dev, void* stream) {
+    CHECK_ENS(e);
+    int rc = ens_member_check(e, level, member);
+    if (rc) return rc;
+    if (!y_dev || !y_member_dev) return set_err(TRISK_EINVAL, "null state pointer");
+    HIP_TRY(hipSetDevice(e->device));
+    return ens_member_copy(e, level, member, const_cast<double*>(y_dev), y_member_dev, 0,
+                           static_cast<hipStream_t>(stream));
+}
+
+int trisk_ensemble_set_member(trisk_ensemble* e, int32_t level, int32_t member, const double* y_member_dev,
+                              double* y_dev, void* stream) {
+    CHECK_ENS(e);
+    int rc = ens_member_check(e, level, member);
+    if (rc) return rc;
+    if (!y_dev || !y_member_dev) return set_err(TRISK_EINVAL, "null state pointer");
+    HIP_TRY(hipSetDevice(e->device));
+    return ens_member_copy(e, level, member, y_dev, const_cast<double*>(y_member_dev), 1,
+                           static_cast<hipStream_t>(stream));
+}
+
+int trisk_ensemble_tendency(trisk_ensemble* e, const double* y_dev, double* dy_dev, void* stream) {
+    CHECK_ENS(e);
+    if (!y_dev || !dy_dev) return set_err(TRISK_EINVAL, "null state pointer");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    e->events_used = 0;
+    double* y = const_cast<double*>(y_dev);  // mode 0 only reads the state
+    return e->precision == TRISK_FP64 ? ens_tendency<Fp64, double>(e, y, dy_dev, 0, true, s)
+                                      : ens_tendency<Fp32Bulk, double>(e, y, dy_dev, 0, true, s);
+}
+
+int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt, int64_t nsteps, void* stream) {
+    CHECK_ENS(e);
+    if (!y_dev || !dt) return set_err(TRISK_EINVAL, "null pointer");
+    if (nsteps < 0) return set_err(TRISK_EINVAL, "nsteps must be non-negative");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = e->precision == TRISK_FP64 ? set_dts<Fp64>(e, dt, s) : set_dts<Fp32Bulk>(e, dt, s);
+    if (rc) return rc;
+    if (e->timing) {
+        e->events_used = 0;
+        return run_ens_steps(e, y_dev, nsteps, s);
+    }
+    if (!e->graphs || s == nullptr || nsteps < kEnsGraphSteps) return run_ens_steps(e, y_dev, nsteps, s);
+    // the graph bakes in y and the schedules; the per-level dt is read from device memory
+    if (!e->graph_exec || e->graph_y != y_dev) {
+        ens_drop_graph(e);
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = run_ens_steps(e, y_dev, kEnsGraphSteps, s);
+        hipGraph_t g = nullptr;
+        hipError_t he = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (he != hipSuccess) return set_err(TRISK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+        e->graph = g;
+        HIP_TRY(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0));
+        e->graph_y = y_dev;
+    }
+    const int64_t reps = nsteps / kEnsGraphSteps;
+    for (int64_t r = 0; r < reps; ++r) HIP_TRY(hipGraphLaunch(e->graph_exec, s));
+    return run_ens_steps(e, y_dev, nsteps - reps * kEnsGraphSteps, s);
+}
+
+int trisk_ensemble_set_graphs(trisk_ensemble* e, int32_t on) {
+    CHECK_ENS(e);
+    HIP_TRY(hipSetDevice(e->device));
+    e->graphs = on != 0;
+    if (!e->graphs) ens_drop_graph(e);
+    return 0;
+}
+
+int trisk_ensemble_diagnostics(trisk_ensemble* e, const double* y_dev, double* out, void* stream) {
+    CHECK_ENS(e);
+    if (!y_dev || !out) return set_err(TRISK_EINVAL, "null pointer");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t k = 0;
+    for (int l = 0; l < e->nlevels; ++l)
+        for (int m = 0; m < e->M[l]; ++m, ++k) {
+            int rc = ens_member_copy(e, l, m, const_cast<double*>(y_dev), e->scratch, 0, s);
+            if (rc) return rc;
+            if ((rc = trisk_diagnostics(e->models[l], e->scratch, out + 3 * k, stream))) return rc;
+        }
+    return 0;
+}
+
+int trisk_ensemble_set_schedule(trisk_ensemble* e, const trisk_schedule sched[3]) {
+    CHECK_ENS(e);
+    if (!sched) return set_err(TRISK_EINVAL, "null schedule");
+    for (int k = 0; k < kEnsKinds; ++k)
+        if (!valid_schedule(sched[k]))
+            return set_err(TRISK_EINVAL, "schedule: block_size 128/256/512, blocks_per_cu 0..64, split CHUNKED/BLOCKED");
+    HIP_TRY(hipSetDevice(e->device));
+    ens_drop_graph(e);
+    HIP_TRY(hipDeviceSynchronize());  // runs in flight may still read the old tile lists
+    for (int k = 0; k < kEnsKinds; ++k) {
+        e->sched[k] = sched[k];
+        int rc = build_schedule(e, k);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int trisk_ensemble_get_schedule(const trisk_ensemble* e, trisk_schedule sched[3]) {
+    CHECK_ENS(e);
+    if (!sched) return set_err(TRISK_EINVAL, "null schedule");
+    for (int k = 0; k < kEnsKinds; ++k) sched[k] = e->sched[k];
+    return 0;
+}
+
+int trisk_ensemble_set_timing(trisk_ensemble* e, int32_t on) {
+    CHECK_ENS(e);
+    e->timing = on != 0;
+    return 0;
+}
+
+int trisk_ensemble_kernel_times(trisk_ensemble* e, int32_t max, const char** names, double* avg_ms,
+                                int64_t* launches, int32_t* count) {
+    CHECK_ENS(e);
+    HIP_TRY(hipSetDevice(e->device));
+    double ms[kEnsKinds] = {0};
+    int64_t n[kEnsKinds] = {0};
+    for (size_t i = 0; i + 1 < e->events_used; i += 2) {
+        HIP_TRY(hipEventSynchronize(e->events[i + 1]));
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, e->events[i], e->events[i + 1]));
+        ms[e->event_kernel[i / 2]] += t;
+        n[e->event_kernel[i / 2]] += 1;
+    }
+    int c = 0;
+    for (int k = 0; k < kEnsKinds && c < max; ++k) {
+        if (n[k] == 0) continue;
+        if (names) names[c] = kEnsNames[k];
+        if (avg_ms) avg_ms[c] = ms[k] / double(n[k]);
+        if (launches) launches[c] = n[k];
+        ++c;
+    }
+    if (count) *count = c;
+    return 0;
+}
+
+int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** names, double* bytes,
+                                int32_t* count) {
+    CHECK_ENS(e);
+    // Algorithmic HBM bytes of one stage-2/3 launch: each level's tables once (shared by its
+    // members), each member's state, intermediates and RK4 registers once.
+    const bool f64 = e->precision == TRISK_FP64;
+    const double c = f64 ? 8.0 : 4.0, uw = c, iw = 4.0;
+    const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;
+    const double vc = f64 ? 7 * c : 8 * c, rec8 = f64 ? 6 * iw : 16.0, ew = f64 ? 2 * iw : 4.0,
+                 cw = f64 ? 12 * iw : 24.0;
+    double b[kEnsKinds] = {0, 0, 0};
+    for (int l = 0; l < e->nlevels; ++l) {
+        const Layout& L = e->models[l]->L;
+        const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
+        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + ne * (ew + 2 * c) +
+                M * (nc * (8.0 + 8.0) + nv * c + ne * (uw + c));
+        b[1] += nc * (cw + 15 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 2 * ne * c);
+        b[2] += ne * (ew + c) + M * (ne * (2 * c + rk_e) + nc * 8.0);
+    }
+    int k = 0;
+    for (; k < kEnsKinds && k < max; ++k) {
+        if (names) names[k] = kEnsNames[k];
+        bytes[k] = b[k];
+    }
+    if (count) *count = k;
+    return 0;
+}
+
+// Times every candidate schedule on a copy of the state and keeps, per kernel, the one
+// with the shortest mean launch.
+int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double* dt, int32_t steps, void* stream,
+                            trisk_schedule best[3], double best_ms[3]) {
+    CHECK_ENS(e);
+    if (!y_dev || !dt) return set_err(TRISK_EINVAL, "null pointer");
+    if (steps < 1) return set_err(TRISK_EINVAL, "steps must be positive");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = e->precision == TRISK_FP64 ? set_dts<Fp64>(e, dt, s) : set_dts<Fp32Bulk>(e, dt, s);
+    if (rc) return rc;
+    double* work = nullptr;
+    HIP_TRY(hipMalloc(&work, e->n_state * sizeof(double)));
+    struct Free {
+        double* p;
+        ~Free() { (void)hipFree(p); }
+    } guard{work};
+    HIP_TRY(hipMemcpyAsync(work, y_dev, e->n_state * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const bool timing = e->timing;
+    trisk_schedule keep[kEnsKinds];
+    for (int k = 0; k < kEnsKinds; ++k) keep[k] = e->sched[k];
+    trisk_schedule win[kEnsKinds] = {keep[0], keep[1], keep[2]};
+    double win_ms[kEnsKinds] = {1e300, 1e300, 1e300};
+    ens_drop_graph(e);
+    e->timing = true;
+    for (int blk : {128, 256, 512})
+        for (int bpc : {0, 2, 4, 8})
+            for (int split : {TRISK_SPLIT_CHUNKED, TRISK_SPLIT_BLOCKED}) {
+                HIP_TRY(hipStreamSynchronize(s));
+                for (int k = 0; k < kEnsKinds && !rc; ++k) {
+                    e->sched[k] = trisk_schedule{blk, bpc, split, 0};
+                    rc = build_schedule(e, k);
+                }
+                e->events_used = 0;
+                if (!rc) rc = run_ens_steps(e, work, 1, s);  // warm-up
+                e->events_used = 0;
+                if (!rc) rc = run_ens_steps(e, work, steps, s);
+                double ms[kEnsKinds];
+                int32_t cnt = 0;
+                if (!rc) rc = trisk_ensemble_kernel_times(e, kEnsKinds, nullptr, ms, nullptr, &cnt);
+                if (rc) break;
+                for (int k = 0; k < kEnsKinds && k < cnt; ++k)
+                    if (ms[k] < win_ms[k]) {
+                        win_ms[k] = ms[k];
+                        win[k] = e->sched[k];
+                    }
+            }
+    e->timing = timing;
+    e->events_used = 0;
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < kEnsKinds; ++k) {
+        e->sched[k] = rc ? keep[k] : win[k];
+        int rc2 = build_schedule(e, k);
+        if (!rc) rc = rc2;
+    }
+    if (rc) return rc;
+    for (int k = 0; k < kEnsKinds; ++k) {
+        if (best) best[k] = win[k];
+        if (best_ms) best_ms[k] = win_ms[k];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -196,6 +196,75 @@ int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]);
  * (text owned by the handle). */
 int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
 
+/* --- ensembles ----------------------------------------------------------------------
+ * No counterpart in the reference, which advances each model with its own rk4_step! loop
+ * (src/TimeSteppers/RungeKutta.jl:18-28, examples/trisk_shallow_water.jl:92-100).  An
+ * ensemble holds `nlevels` models (meshes of any resolution; TRiSK PV flux and cell
+ * kinetic energy, i.e. TRISK_PATH_PAIR) with members[l] states each, and advances them
+ * together: each of the three kernels of a tendency is one launch over every member of
+ * every level.
+ *
+ * The ensemble state is one device vector of trisk_ensemble_state_length() doubles: the
+ * blocks of the levels in order, level l's block [u (ne x M); h (nc x M)] holding member m
+ * of entity j at j*M + m, entities in the level model's internal order.  Fill and read it
+ * member by member with trisk_ensemble_set_member / get_member and a single-member device
+ * state of the level's model (trisk_ensemble_level, trisk_upload_state,
+ * trisk_download_state).  The level models belong to the ensemble.  Calls on one ensemble
+ * must not overlap. */
+typedef struct trisk_ensemble trisk_ensemble;
+
+/* How one kernel's tiles (block_size items each) are laid on the device. */
+enum { TRISK_SPLIT_CHUNKED = 0,  /* XCD x takes the x-th eighth of every level's range */
+       TRISK_SPLIT_BLOCKED = 1   /* XCD x takes the x-th eighth of all ranges laid end to end */ };
+enum { TRISK_ENS_STATE = 0, TRISK_ENS_CELL = 1, TRISK_ENS_EDGE = 2 };  /* the three kernels */
+
+typedef struct {
+    int32_t block_size;          /* 128, 256 or 512 work-items per workgroup */
+    int32_t blocks_per_cu;       /* persistent grid of blocks_per_cu workgroups per CU; 0: one per tile */
+    int32_t split;               /* TRISK_SPLIT_* */
+    int32_t reserved;
+} trisk_schedule;
+
+/* Creates the level models (trisk_create with TRISK_PATH_PAIR; opt->path must be AUTO or
+ * PAIR) and the member-interleaved work vectors.  Synchronous. */
+int trisk_ensemble_create(const trisk_model_desc* const* levels, const int32_t* members, int32_t nlevels,
+                          const trisk_options* opt, trisk_ensemble** out);
+int trisk_ensemble_destroy(trisk_ensemble* e);
+int trisk_ensemble_state_length(const trisk_ensemble* e, int64_t* n);
+/* Level l's model (owned by the ensemble), the offset of its block and its member count;
+ * any output may be NULL. */
+int trisk_ensemble_level(trisk_ensemble* e, int32_t level, trisk_handle** model, int64_t* offset,
+                         int32_t* members);
+/* Member `member` of level `level`: ensemble state <-> single-model device state. */
+int trisk_ensemble_get_member(trisk_ensemble* e, int32_t level, int32_t member, const double* y_dev,
+                              double* y_member_dev, void* stream);
+int trisk_ensemble_set_member(trisk_ensemble* e, int32_t level, int32_t member, const double* y_member_dev,
+                              double* y_dev, void* stream);
+/* tendency! of every member, dy in the layout of y. */
+int trisk_ensemble_tendency(trisk_ensemble* e, const double* y_dev, double* dy_dev, void* stream);
+/* nsteps RK4 steps of every member, level l with time step dt[l]; bitwise equal to
+ * trisk_rk4_steps of each member on its level's model. */
+int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt, int64_t nsteps, void* stream);
+/* Replay trisk_ensemble_rk4_steps through a captured graph of 8 steps (as trisk_set_graphs). */
+int trisk_ensemble_set_graphs(trisk_ensemble* e, int32_t on);
+/* out[3k .. 3k+2] = trisk_diagnostics of member k, members numbered level by level.  Synchronous. */
+int trisk_ensemble_diagnostics(trisk_ensemble* e, const double* y_dev, double* out, void* stream);
+/* The schedules of the three kernels (TRISK_ENS_*); the default is {256, 0, CHUNKED}. */
+int trisk_ensemble_set_schedule(trisk_ensemble* e, const trisk_schedule sched[3]);
+int trisk_ensemble_get_schedule(const trisk_ensemble* e, trisk_schedule sched[3]);
+/* Times `steps` RK4 steps under each candidate schedule (block size x persistence x
+ * split) on a device copy of y_dev (y_dev is not changed), keeps per kernel the one with
+ * the shortest mean launch, and returns it with that time (either output may be NULL).
+ * Synchronous. */
+int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double* dt, int32_t steps, void* stream,
+                            trisk_schedule best[3], double best_ms[3]);
+/* Kernel timing and algorithmic bytes, as trisk_set_timing / trisk_kernel_times / trisk_kernel_bytes. */
+int trisk_ensemble_set_timing(trisk_ensemble* e, int32_t on);
+int trisk_ensemble_kernel_times(trisk_ensemble* e, int32_t max, const char** names, double* avg_ms,
+                                int64_t* launches, int32_t* count);
+int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** names, double* bytes,
+                                int32_t* count);
+
 /* --- mesh setup on the device ------------------------------------------------------
  * lloyd!(x, triangles; fixed=1:nfixed, tolerance, history, max_iterations)
  * (src/Grids/Icosahedral.jl:126-184) on HIP device `device`: x (host, nv x 3) is moved in
