@@ -56,6 +56,14 @@ def parse():
                         "day 1 and day DAYS")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="run a plain device copy of MIB MiB first (HBM counter calibration)")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    p.add_argument("--ensemble-only", action="store_true", help="only the batched-ensemble measurement")
+    p.add_argument("--ens-members", type=int, default=16, help="ensemble members per level (levels 4-7)")
+    p.add_argument("--ens-steps", type=int, default=96, help="timed RK4 steps of the ensemble")
+    p.add_argument("--ens-schedule", default="",
+                   help="JSON list of three kernel schedules to use instead of autotuning")
+    p.add_argument("--no-member-loop", action="store_true",
+                   help="skip timing the members model by model")
     return p.parse_args()
 
 
@@ -151,6 +159,106 @@ def run_gpu(grid, y0, precision, args, params, path=None):
     return out
 
 
+ENS_LEVELS = (4, 5, 6, 7)
+ENS_DTS = (1200.0, 600.0, 300.0, 150.0)
+
+
+def run_ensemble(args):
+    """Members on SCVT levels 4-7 (per-level dt 1200/600/300/150 s), FP32-bulk, in one
+    launch per kernel; beside it the same members stepped model by model."""
+    import torch
+
+    from dmplex_amd import HIPBackend, ModelTables, TRiSKEnsemble, TRiSKShallowWater, create_hexagon_mesh, rk4_steps_
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+
+    params = earth_parameters()
+    M = args.ens_members
+    t = time.perf_counter()
+    tabs = []
+    for n in ENS_LEVELS:
+        dm = create_hexagon_mesh(n, centroidal=True, device=0)
+        tabs.append(ModelTables(dm, centers=dm.generators, **params))
+    ens = TRiSKEnsemble(tabs, [M] * len(tabs), backend=HIPBackend(0, "fp32_bulk"))
+    y = ens.zeros()
+    seeds = {}
+    j = 0
+    for li, tb in enumerate(tabs):
+        for m in range(M):
+            seeds[li, m] = j
+            ens.set_member(y, li, m, seeded_state(tb.grid, seed_coef=1000 + j, seed_noise=2000 + j))
+            j += 1
+    log(f"[ensemble] {len(seeds)} members on levels {ENS_LEVELS} ready in {time.perf_counter() - t:.1f} s, "
+        f"{ens.cells()} cell-members")
+    d0 = ens.diagnostics(y)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    t = time.perf_counter()
+    if args.ens_schedule:
+        ens.set_schedule(json.loads(args.ens_schedule))
+        sched, tune_ms = ens.schedule(), None
+    else:
+        sched, tune_ms = ens.autotune(y, ENS_DTS, steps=4)
+    log(f"[ensemble] schedule ({time.perf_counter() - t:.1f} s): {sched}")
+    K = args.ens_steps
+    ens.rk4_steps_(y, ENS_DTS, 8)
+    ens.set_timing(True)
+    ms_direct = timed(lambda: ens.rk4_steps_(y, ENS_DTS, K))
+    ens.set_timing(False)
+    stats = ens.kernel_stats()
+    ens.use_graphs(True)
+    ens.rk4_steps_(y, ENS_DTS, 8)
+    ms = timed(lambda: ens.rk4_steps_(y, ENS_DTS, K))
+    d1 = ens.diagnostics(y)
+    cells = ens.cells()
+    value = cells * K / (ms / 1e3)
+    step_bytes = 4 * sum(k["bytes"] for k in stats)
+    dom = max(stats, key=lambda k: k["avg_ms"])
+    achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
+    log(f"[ensemble] {ms / K:.3f} ms/step, {value:.3e} cell-member-steps/s")
+    out = {
+        "workload": f"{len(seeds)} members, {M} each on SCVT levels {list(ENS_LEVELS)}, dt {list(ENS_DTS)} s, "
+                    f"FP32-bulk, TRiSK PV flux, cell KE",
+        "unit": "cell-member-steps/s", "value": value, "ms_per_step": ms / K, "steps": K,
+        "ms_per_step_direct_with_events": ms_direct / K, "graphs": True,
+        "cell_members": cells, "schedule": sched, "autotune_kernel_ms": tune_ms,
+        "model_bytes_per_step": step_bytes, "hbm_frac_model": step_bytes / (ms / K / 1e3) / HBM_PEAK,
+        "roofline": dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
+                         frac=achieved * 1e9 / HBM_PEAK, avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
+        "kernels": stats,
+        "max_member_mass_drift": float(np.max(np.abs((d1[:, 0] - d0[:, 0]) / d0[:, 0]))),
+    }
+    if args.no_member_loop:
+        return out
+    # the same members, one model per level, stepped member after member (graph replay)
+    loop_ms = 0.0
+    for li, tb in enumerate(tabs):
+        m = TRiSKShallowWater(tb.grid, backend=HIPBackend(0, "fp32_bulk"), path="pair", **params)
+        m.use_graphs(True)
+        states = [m.upload_state(seeded_state(tb.grid, seed_coef=1000 + seeds[li, k], seed_noise=2000 + seeds[li, k]))
+                  for k in range(M)]
+        for ys in states:
+            rk4_steps_(ys, ENS_DTS[li], 8, m)  # capture and warm-up
+        loop_ms += timed(lambda: [rk4_steps_(ys, ENS_DTS[li], K, m) for ys in states])
+        del m, states
+    torch.cuda.empty_cache()
+    loop_value = cells * K / (loop_ms / 1e3)
+    log(f"[ensemble] member-by-member loop: {loop_ms / K:.3f} ms/step, {loop_value:.3e} cell-member-steps/s")
+    out["member_loop"] = {"value": loop_value, "ms_per_step": loop_ms / K,
+                          "how": "one FP32-bulk cell-pair model per level, each member stepped K steps in turn "
+                                 "(graph replay)"}
+    out["speedup_vs_member_loop"] = value / loop_value
+    return out
+
+
 def run_cpu(grid, y0, args, params):
     from dmplex_amd import ModelTables
     from oracle.oracle import Oracle
@@ -239,8 +347,6 @@ def main():
     from dmplex_amd import TRiSKGrid, create_hexagon_mesh
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
-    t = time.perf_counter()
-    log(f"[bench] generating SCVT hexagon mesh level {args.level}")
     if args.calibrate_copy:
         n = args.calibrate_copy * 2 ** 20 // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -249,6 +355,11 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibrate_copy} MiB done")
         del a, b
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
+        return
+    t = time.perf_counter()
+    log(f"[bench] generating SCVT hexagon mesh level {args.level}")
     dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0, log=lambda m: log(f"[bench] {m}"))
     grid = TRiSKGrid(dm, centers=dm.generators)
     params = earth_parameters()
@@ -290,6 +401,9 @@ def main():
         line["speedup_fp32_bulk_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
     if "fp64_pair" in res and "fp32_bulk" in res:
         line["speedup_fp32_bulk_vs_fp64_pair"] = res["fp32_bulk"]["value"] / res["fp64_pair"]["value"]
+    if not args.no_ensemble:
+        log("[bench] batched ensemble (levels 4-7)")
+        line["ensemble"] = run_ensemble(args)
     print(json.dumps(line), flush=True)
 
 

@@ -47,7 +47,7 @@ inline FastDiv make_fastdiv(uint32_t d) {
 template <class P>
 struct EnsLevel {
     Params<P> p;       // tables; B, qv, G, T point to the level's (entity x member) vectors
-    FastDiv md;        // division by M
+    FastDiv md[3];     // division by M / V for V = 1, 2, 4 members per lane
     int32_t M;         // members
     int64_t yoff;      // first element of the level's block [u (ne M); h (nc M)] of the state
     double dt, dt2, dt6;
@@ -58,7 +58,7 @@ struct EnsLevel {
 
 // A run of tiles of one range (cells, vertices or edges of one level) on one XCD.
 struct EnsSeg {
-    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges
+    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges, 3 all three interleaved
     int32_t tile0;        // first tile of the run within its range
     int32_t ntiles;
     int32_t cum;          // tiles before the run in its XCD's list
@@ -90,11 +90,16 @@ __device__ __forceinline__ Rk level_rk(const EnsLevel<P>& L, const EnsArgs& a) {
     return r;
 }
 
-// body(run, item) for every item of the tiles of this block's XCD (blockIdx mod 8: the
-// hardware deals workgroups to the XCDs round-robin); the XCD's blocks stride over its
-// tile list, so a grid smaller than the list is a persistent launch.
-template <int BLK, class F>
-__device__ __forceinline__ void for_each_item(const EnsArgs& a, F&& body) {
+// body(level, kind, item) for every item of the tiles of this block's XCD (blockIdx mod 8:
+// the hardware deals workgroups to the XCDs round-robin); the XCD's blocks stride over its
+// tile list, so a grid smaller than the list is a persistent launch.  A run of kind 3 holds
+// the cells, vertices and edges of one level interleaved along the Hilbert curve: tile
+// 6k is cell tile k, tiles 6k+1, 6k+2 vertex tiles 2k, 2k+1 and tiles 6k+3..6k+5 edge
+// tiles 3k..3k+2 (a level has fewer than 2 vertices and 3 edges per cell), so the blocks
+// running together touch one stretch of the mesh; count(level, kind) gives the items of
+// the level's cells (0), vertices (1) or edges (2).
+template <int BLK, class N, class F>
+__device__ __forceinline__ void for_each_item(const EnsArgs& a, N&& count, F&& body) {
     const int x = int(blockIdx.x % kXcd);
     const int nbx = int(gridDim.x / kXcd);
     const int s0 = a.first[x], s1 = a.first[x + 1];
@@ -104,112 +109,230 @@ __device__ __forceinline__ void for_each_item(const EnsArgs& a, F&& body) {
     for (int t = int(blockIdx.x / kXcd); t < total; t += nbx) {
         while (s + 1 < s1 && t >= a.seg[s + 1].cum) ++s;  // t only grows
         const EnsSeg g = a.seg[s];
-        const int64_t j = int64_t(g.tile0 + (t - g.cum)) * BLK + threadIdx.x;
-        if (j < g.nitems) body(g, uint32_t(j));
+        int64_t tile = g.tile0 + (t - g.cum);
+        int kind = g.kind;
+        int64_t nitems = g.nitems;
+        if (kind == 3) {
+            const int64_t sup = tile / 6, r = tile - 6 * sup;
+            kind = r == 0 ? 0 : r <= 2 ? 1 : 2;
+            tile = sup * (kind + 1) + (r - (kind == 0 ? 0 : kind == 1 ? 1 : 3));
+            nitems = count(g.level, kind);
+        }
+        const int64_t j = tile * BLK + threadIdx.x;
+        if (j < nitems) body(g.level, kind, uint32_t(j));
     }
 }
 
+// V consecutive members at an address aligned to V * sizeof(T) (cached vector accesses)
+template <class T, int V>
+struct alignas(sizeof(T) * V) Lanes {
+    T v[V];
+};
+template <class T, int V>
+__device__ __forceinline__ Lanes<T, V> vload(const T* p) {
+    return *reinterpret_cast<const Lanes<T, V>*>(p);
+}
+template <class T, int V>
+__device__ __forceinline__ void vstore(T* p, const Lanes<T, V>& x) {
+    *reinterpret_cast<Lanes<T, V>*>(p) = x;
+}
+
+// rk_update (trisk_device.hpp) for V consecutive elements, with the same arithmetic per element
+template <class U, int V>
+__device__ __forceinline__ void rk_update_lanes(const Rk& r, int64_t j, const Lanes<double, V>& k, U* stage) {
+    Lanes<U, V> st;
+    if (r.mode == 0) {
+        vstore(r.dy + j, k);
+    } else if (r.mode <= 3) {
+        Lanes<double, V> acc = k;
+        if (r.mode > 1) {
+            acc = vload<double, V>(r.acc + j);
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc.v[v] = acc.v[v] + 2 * k.v[v];
+        }
+        vstore(r.acc + j, acc);
+        const Lanes<double, V> y = vload<double, V>(r.y + j);
+        const double c = r.mode == 3 ? r.dt : r.dt2;
+#pragma unroll
+        for (int v = 0; v < V; ++v) st.v[v] = U(y.v[v] + c * k.v[v]);
+        vstore(stage, st);
+    } else {
+        Lanes<double, V> y = vload<double, V>(r.y + j);
+        const Lanes<double, V> acc = vload<double, V>(r.acc + j);
+#pragma unroll
+        for (int v = 0; v < V; ++v) y.v[v] = y.v[v] + r.dt6 * (acc.v[v] + k.v[v]);
+        vstore(r.y + j, y);
+    }
+}
+
+template <int V>
+constexpr int lane_log() {
+    return V == 1 ? 0 : V == 2 ? 1 : 2;
+}
+
+// Item j of a range owns members [m0, m0 + V) of entity ent, elements [j V, j V + V) of the
+// range's member-interleaved vectors (j V = ent M + m0).
+struct Item {
+    uint32_t ent, m0, o;
+};
+template <int V, class P>
+__device__ __forceinline__ Item item_of(const EnsLevel<P>& L, uint32_t j) {
+    const uint32_t ent = L.md[lane_log<V>()](j);
+    const uint32_t o = j * V;
+    return {ent, o - ent * uint32_t(L.M), o};
+}
+
 // W1 of all levels: cells (B), vertices (q_v), edges (G); cf. pair_state_kernel
-template <class P, class UIn, int BLK>
+template <class P, class UIn, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
     const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
-    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
-        const EnsLevel<P>& L = lv[g.level];
+    auto count = [&](int level, int kind) -> int64_t {
+        const EnsLevel<P>& L = lv[level];
+        return (kind == 0 ? L.p.nc : kind == 1 ? L.p.nv : L.p.ne) * (L.M / V);
+    };
+    for_each_item<BLK>(a, count, [&](int level, int kind, uint32_t j) {
+        const EnsLevel<P>& L = lv[level];
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
-        const uint32_t ent = L.md(j);
-        const uint32_t m = j - ent * M;
+        const Item it = item_of<V>(L, j);
         const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
         const double* h = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
-        if (g.kind == 0) {
+        if (kind == 0) {
             int32_t E[6];
-            C kin[6], ue[6];
-            load_cell_state<NARROW>(p, ent, E, kin);
+            C kin[6];
+            load_cell_state<NARROW>(p, it.ent, E, kin);
+            Lanes<UIn, V> uk[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) ue[k] = C(u[uint32_t(E[k]) * M + m]);
-            p.B[j] = double(pair_kinetic(kin, ue)) + p.g * h[j];
-        } else if (g.kind == 1) {
+            for (int k = 0; k < 6; ++k) uk[k] = vload<UIn, V>(u + uint32_t(E[k]) * M + it.m0);
+            const Lanes<double, V> hh = vload<double, V>(h + it.o);
+            Lanes<double, V> B;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                C ue[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) ue[k] = C(uk[k].v[v]);
+                B.v[v] = double(pair_kinetic(kin, ue)) + p.g * hh.v[v];
+            }
+            vstore(p.B + it.o, B);
+        } else if (kind == 1) {
             int32_t idx[6];
-            C cf[8], uz[3], hz[3];
-            load_vertex_record<NARROW>(p, ent, idx, cf);
+            C cf[8];
+            load_vertex_record<NARROW>(p, it.ent, idx, cf);
+            Lanes<UIn, V> uz[3];
+            Lanes<double, V> hz[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                uz[k] = C(u[uint32_t(idx[k]) * M + m]);
-                hz[k] = C(h[uint32_t(idx[3 + k]) * M + m]);
+                uz[k] = vload<UIn, V>(u + uint32_t(idx[k]) * M + it.m0);
+                hz[k] = vload<double, V>(h + uint32_t(idx[3 + k]) * M + it.m0);
             }
-            p.qv[j] = I(pair_vertex_pv(cf, uz, hz));
+            Lanes<I, V> q;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                C uu[3], hh[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    uu[k] = C(uz[k].v[v]);
+                    hh[k] = C(hz[k].v[v]);
+                }
+                q.v[v] = I(pair_vertex_pv(cf, uu, hh));
+            }
+            vstore(p.qv + it.o, q);
         } else {
             int32_t c[2];
             C lh[2];
-            load_edge_cells<NARROW>(p, ent, c);
-            ldrec(p.re_lh + 2 * int64_t(ent), lh);
-            p.G[j] = I(pair_mass_flux(lh, C(h[uint32_t(c[0]) * M + m]), C(h[uint32_t(c[1]) * M + m]), C(u[j])));
+            load_edge_cells<NARROW>(p, it.ent, c);
+            ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
+            const Lanes<double, V> h0 = vload<double, V>(h + uint32_t(c[0]) * M + it.m0);
+            const Lanes<double, V> h1 = vload<double, V>(h + uint32_t(c[1]) * M + it.m0);
+            const Lanes<UIn, V> ue = vload<UIn, V>(u + it.o);
+            Lanes<I, V> G;
+#pragma unroll
+            for (int v = 0; v < V; ++v) G.v[v] = I(pair_mass_flux(lh, C(h0.v[v]), C(h1.v[v]), C(ue.v[v])));
+            vstore(p.G + it.o, G);
         }
     });
 }
 
 // W2 of all levels: PV-flux half sums and continuity with its RK4 update; cf. pair_cell_kernel.
 // Half sums are stored side-major per edge: T[(2 e + side) M + m].
-template <class P, int BLK>
+template <class P, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
     const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
-    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
-        const EnsLevel<P>& L = lv[g.level];
+    auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
+    for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
+        const EnsLevel<P>& L = lv[level];
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
-        const uint32_t i = L.md(j);
-        const uint32_t m = j - i * M;
-        int32_t meta, E[6], V[6];
-        C W[15], G[6], Q[6], H[6];
-        load_cell_pairs<NARROW>(p, i, meta, E, V, W);
+        const Item it = item_of<V>(L, j);
+        int32_t meta, E[6], V6[6];
+        C W[15];
+        load_cell_pairs<NARROW>(p, it.ent, meta, E, V6, W);
         const int n = meta & 15;
+        Lanes<I, V> Gk[6], Qk[6], Hk[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            G[k] = C(p.G[uint32_t(E[k]) * M + m]);
-            Q[k] = C(p.qv[uint32_t(V[k]) * M + m]);
+            Gk[k] = vload<I, V>(p.G + uint32_t(E[k]) * M + it.m0);
+            Qk[k] = vload<I, V>(p.qv + uint32_t(V6[k]) * M + it.m0);
         }
-        pair_half_sums(n, W, G, Q, H);
-        double div = 0.0;
+        const double area = ld(p.area_c + it.ent);
+        Lanes<double, V> dh;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            if (k < n) {
-                const int side = (meta >> (4 + k)) & 1;
-                p.T[(2 * int64_t(E[k]) + side) * M + m] = I(H[k]);
-                div += side ? -double(G[k]) : double(G[k]);
+        for (int v = 0; v < V; ++v) {
+            C G[6], Q[6], H[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                G[k] = C(Gk[k].v[v]);
+                Q[k] = C(Qk[k].v[v]);
             }
+            pair_half_sums(n, W, G, Q, H);
+            double div = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                Hk[k].v[v] = I(H[k]);
+                if (k < n) div += (meta >> (4 + k)) & 1 ? -double(G[k]) : double(G[k]);
+            }
+            dh.v[v] = -div / area;
         }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < n) vstore(p.T + (2 * int64_t(E[k]) + ((meta >> (4 + k)) & 1)) * M + it.m0, Hk[k]);
         const Rk r = level_rk(L, a);
-        rk_update<double>(r, p.ne * M + j, -div / ld(p.area_c + i),
-                          r.mode >= 1 && r.mode <= 3 ? L.sh + j : nullptr);
+        rk_update_lanes<double, V>(r, p.ne * M + it.o, dh, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
     });
 }
 
 // W3 of all levels: du and its RK4 update; cf. pair_edge_kernel
-template <class P, int BLK>
+template <class P, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
     using C = typename P::C;
+    using I = typename P::I;
     using U = typename P::U;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
     const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
-    for_each_item<BLK>(a, [&](const EnsSeg& g, uint32_t j) {
-        const EnsLevel<P>& L = lv[g.level];
+    auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
+    for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
+        const EnsLevel<P>& L = lv[level];
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
-        const uint32_t e = L.md(j);
-        const uint32_t m = j - e * M;
+        const Item it = item_of<V>(L, j);
         int32_t c[2];
-        load_edge_cells<NARROW>(p, e, c);
-        const int64_t t = 2 * int64_t(e) * M + m;
-        const double du = pair_edge_tendency(C(p.T[t]), C(p.T[t + M]), C(ld(p.dual + e)),
-                                             p.B[uint32_t(c[0]) * M + m], p.B[uint32_t(c[1]) * M + m]);
+        load_edge_cells<NARROW>(p, it.ent, c);
+        const int64_t t = 2 * int64_t(it.ent) * M + it.m0;
+        const Lanes<I, V> t0 = vload<I, V>(p.T + t), t1 = vload<I, V>(p.T + t + M);
+        const Lanes<double, V> bl = vload<double, V>(p.B + uint32_t(c[0]) * M + it.m0);
+        const Lanes<double, V> br = vload<double, V>(p.B + uint32_t(c[1]) * M + it.m0);
+        const C d = C(ld(p.dual + it.ent));
+        Lanes<double, V> du;
+#pragma unroll
+        for (int v = 0; v < V; ++v) du.v[v] = pair_edge_tendency(C(t0.v[v]), C(t1.v[v]), d, bl.v[v], br.v[v]);
         const Rk r = level_rk(L, a);
-        rk_update<U>(r, j, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + j : nullptr);
+        rk_update_lanes<U, V>(r, it.o, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + it.o : nullptr);
     });
 }
 
@@ -266,15 +389,33 @@ struct trisk_ensemble {
 
 namespace trisk {
 
-static bool valid_schedule(const trisk_schedule& s) {
-    return (s.block_size == 128 || s.block_size == 256 || s.block_size == 512) && s.blocks_per_cu >= 0 &&
-           s.blocks_per_cu <= 64 && (s.split == TRISK_SPLIT_CHUNKED || s.split == TRISK_SPLIT_BLOCKED);
+static int lanes_of(const trisk_schedule& s) { return s.members_per_lane > 0 ? s.members_per_lane : 1; }
+
+static bool valid_schedule(const trisk_ensemble* E, const trisk_schedule& s) {
+    const int V = lanes_of(s);
+    bool ok = (s.block_size == 128 || s.block_size == 256 || s.block_size == 512) && s.blocks_per_cu >= 0 &&
+              s.blocks_per_cu <= 64 &&
+              (s.split == TRISK_SPLIT_CHUNKED || s.split == TRISK_SPLIT_BLOCKED || s.split == TRISK_SPLIT_INTERLEAVED) &&
+              (V == 1 || V == 2 || V == 4);
+    for (int l = 0; ok && l < E->nlevels; ++l) ok = E->M[l] % V == 0;
+    return ok;
+}
+
+// State vectors must be aligned to the widest member group any kernel loads.
+static int check_alignment(const trisk_ensemble* E, const void* p) {
+    int V = 1;
+    for (const auto& s : E->sched) V = std::max(V, lanes_of(s));
+    if (reinterpret_cast<uintptr_t>(p) % (8 * V) != 0)
+        return set_err(TRISK_EINVAL, "ensemble state vectors must be aligned to " + std::to_string(8 * V) +
+                                         " bytes for this schedule");
+    return 0;
 }
 
 // The runs of kernel `kind` per XCD under its schedule.
 static int build_schedule(trisk_ensemble* E, int kind) {
     const trisk_schedule& sc = E->sched[kind];
     const int64_t blk = sc.block_size;
+    const int64_t V = lanes_of(sc);
     struct Range {
         int32_t level, kind;
         int64_t items, tiles;
@@ -282,7 +423,7 @@ static int build_schedule(trisk_ensemble* E, int kind) {
     std::vector<Range> ranges;
     for (int l = 0; l < E->nlevels; ++l) {
         const Layout& L = E->models[l]->L;
-        const int64_t M = E->M[l];
+        const int64_t M = E->M[l] / V;  // items per entity
         if (kind == TRISK_ENS_STATE) {
             ranges.push_back({l, 0, L.nc * M, 0});
             ranges.push_back({l, 1, L.nv * M, 0});
@@ -303,7 +444,20 @@ static int build_schedule(trisk_ensemble* E, int kind) {
         if (lo >= hi) return;
         per[x].push_back({r.level, r.kind, int32_t(lo), int32_t(hi - lo), 0, int32_t(r.items)});
     };
-    if (sc.split == TRISK_SPLIT_CHUNKED) {
+    if (sc.split == TRISK_SPLIT_INTERLEAVED && kind == TRISK_ENS_STATE) {
+        // XCD x: the x-th contiguous chunk of every level's interleaved cells, vertices and
+        // edges, in whole groups of 6 tiles (one cell tile with its vertex and edge tiles)
+        ranges.clear();
+        for (int l = 0; l < E->nlevels; ++l) {
+            const int64_t items = E->models[l]->L.nc * (E->M[l] / V);
+            ranges.push_back({l, 3, items, 6 * ((items + blk - 1) / blk)});
+        }
+        for (int x = 0; x < kXcd; ++x)
+            for (const auto& r : ranges) {
+                const int64_t groups = r.tiles / 6, c = (groups + kXcd - 1) / kXcd;
+                add(x, r, 6 * std::min(groups, x * c), 6 * std::min(groups, (x + 1) * c));
+            }
+    } else if (sc.split != TRISK_SPLIT_BLOCKED) {
         // XCD x: the x-th contiguous chunk of every range
         for (int x = 0; x < kXcd; ++x)
             for (const auto& r : ranges) {
@@ -381,7 +535,7 @@ static int setup_levels(trisk_ensemble* E) {
         const int64_t M = E->M[l];
         EnsLevel<P>& e = lv[l];
         e.p = model_params<P>(E->models[l]);
-        e.md = make_fastdiv(uint32_t(M));
+        for (int k = 0; k < 3; ++k) e.md[k] = make_fastdiv(uint32_t(M % (1 << k) == 0 ? M >> k : 1));
         e.M = int32_t(M);
         e.yoff = E->yoff[l];
         e.dt = e.dt2 = e.dt6 = 0;
@@ -459,14 +613,24 @@ static int ens_timed_begin(trisk_ensemble* E, int kind, hipStream_t s, hipEvent_
     return 0;
 }
 
-template <class P, class UIn, int BLK>
+template <class P, class UIn, int BLK, int V>
 static void launch_kind(int kind, dim3 grid, const EnsArgs& a, hipStream_t s) {
     if (kind == TRISK_ENS_STATE)
-        hipLaunchKernelGGL((ens_state_kernel<P, UIn, BLK>), grid, dim3(BLK), 0, s, a);
+        hipLaunchKernelGGL((ens_state_kernel<P, UIn, BLK, V>), grid, dim3(BLK), 0, s, a);
     else if (kind == TRISK_ENS_CELL)
-        hipLaunchKernelGGL((ens_cell_kernel<P, BLK>), grid, dim3(BLK), 0, s, a);
+        hipLaunchKernelGGL((ens_cell_kernel<P, BLK, V>), grid, dim3(BLK), 0, s, a);
     else
-        hipLaunchKernelGGL((ens_edge_kernel<P, BLK>), grid, dim3(BLK), 0, s, a);
+        hipLaunchKernelGGL((ens_edge_kernel<P, BLK, V>), grid, dim3(BLK), 0, s, a);
+}
+
+template <class P, class UIn, int BLK>
+static void launch_lanes(int kind, int V, dim3 grid, const EnsArgs& a, hipStream_t s) {
+    if (V == 4)
+        launch_kind<P, UIn, BLK, 4>(kind, grid, a, s);
+    else if (V == 2)
+        launch_kind<P, UIn, BLK, 2>(kind, grid, a, s);
+    else
+        launch_kind<P, UIn, BLK, 1>(kind, grid, a, s);
 }
 
 // One tendency of every member: the three kernels with the given stage input and RK4 mode.
@@ -485,15 +649,16 @@ static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool
         int rc = ens_timed_begin(E, kind, s, &stop);
         if (rc) return rc;
         const dim3 grid(unsigned(E->grid[kind]));
+        const int V = lanes_of(E->sched[kind]);
         switch (E->sched[kind].block_size) {
             case 128:
-                launch_kind<P, UIn, 128>(kind, grid, a, s);
+                launch_lanes<P, UIn, 128>(kind, V, grid, a, s);
                 break;
             case 512:
-                launch_kind<P, UIn, 512>(kind, grid, a, s);
+                launch_lanes<P, UIn, 512>(kind, V, grid, a, s);
                 break;
             default:
-                launch_kind<P, UIn, 256>(kind, grid, a, s);
+                launch_lanes<P, UIn, 256>(kind, V, grid, a, s);
                 break;
         }
         HIP_TRY(hipGetLastError());
@@ -605,7 +770,7 @@ int trisk_ensemble_create(const trisk_model_desc* const* levels, const int32_t* 
         E->cus = cus;
     int rc = E->precision == TRISK_FP64 ? setup_levels<Fp64>(E) : setup_levels<Fp32Bulk>(E);
     for (int k = 0; k < kEnsKinds && !rc; ++k) {
-        E->sched[k] = trisk_schedule{256, 0, TRISK_SPLIT_CHUNKED, 0};
+        E->sched[k] = trisk_schedule{256, 0, TRISK_SPLIT_CHUNKED, 1};
         rc = build_schedule(E, k);
     }
     if (rc) {
@@ -662,6 +827,9 @@ int trisk_ensemble_tendency(trisk_ensemble* e, const double* y_dev, double* dy_d
     HIP_TRY(hipSetDevice(e->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     e->events_used = 0;
+    int rc = check_alignment(e, y_dev);
+    if (!rc) rc = check_alignment(e, dy_dev);
+    if (rc) return rc;
     double* y = const_cast<double*>(y_dev);  // mode 0 only reads the state
     return e->precision == TRISK_FP64 ? ens_tendency<Fp64, double>(e, y, dy_dev, 0, true, s)
                                       : ens_tendency<Fp32Bulk, double>(e, y, dy_dev, 0, true, s);
@@ -673,7 +841,9 @@ int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt,
     if (nsteps < 0) return set_err(TRISK_EINVAL, "nsteps must be non-negative");
     HIP_TRY(hipSetDevice(e->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int rc = e->precision == TRISK_FP64 ? set_dts<Fp64>(e, dt, s) : set_dts<Fp32Bulk>(e, dt, s);
+    int rc = check_alignment(e, y_dev);
+    if (rc) return rc;
+    rc = e->precision == TRISK_FP64 ? set_dts<Fp64>(e, dt, s) : set_dts<Fp32Bulk>(e, dt, s);
     if (rc) return rc;
     if (e->timing) {
         e->events_used = 0;
@@ -728,8 +898,9 @@ int trisk_ensemble_set_schedule(trisk_ensemble* e, const trisk_schedule sched[3]
     CHECK_ENS(e);
     if (!sched) return set_err(TRISK_EINVAL, "null schedule");
     for (int k = 0; k < kEnsKinds; ++k)
-        if (!valid_schedule(sched[k]))
-            return set_err(TRISK_EINVAL, "schedule: block_size 128/256/512, blocks_per_cu 0..64, split CHUNKED/BLOCKED");
+        if (!valid_schedule(e, sched[k]))
+            return set_err(TRISK_EINVAL, "schedule: block_size 128/256/512, blocks_per_cu 0..64, split CHUNKED/BLOCKED/INTERLEAVED, "
+                                         "members_per_lane 1/2/4 dividing every level's member count");
     HIP_TRY(hipSetDevice(e->device));
     ens_drop_graph(e);
     HIP_TRY(hipDeviceSynchronize());  // runs in flight may still read the old tile lists
@@ -832,28 +1003,31 @@ int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double
     double win_ms[kEnsKinds] = {1e300, 1e300, 1e300};
     ens_drop_graph(e);
     e->timing = true;
-    for (int blk : {128, 256, 512})
-        for (int bpc : {0, 2, 4, 8})
-            for (int split : {TRISK_SPLIT_CHUNKED, TRISK_SPLIT_BLOCKED}) {
-                HIP_TRY(hipStreamSynchronize(s));
-                for (int k = 0; k < kEnsKinds && !rc; ++k) {
-                    e->sched[k] = trisk_schedule{blk, bpc, split, 0};
-                    rc = build_schedule(e, k);
-                }
-                e->events_used = 0;
-                if (!rc) rc = run_ens_steps(e, work, 1, s);  // warm-up
-                e->events_used = 0;
-                if (!rc) rc = run_ens_steps(e, work, steps, s);
-                double ms[kEnsKinds];
-                int32_t cnt = 0;
-                if (!rc) rc = trisk_ensemble_kernel_times(e, kEnsKinds, nullptr, ms, nullptr, &cnt);
-                if (rc) break;
-                for (int k = 0; k < kEnsKinds && k < cnt; ++k)
-                    if (ms[k] < win_ms[k]) {
-                        win_ms[k] = ms[k];
-                        win[k] = e->sched[k];
+    for (int lanes : {1, 2, 4})
+        for (int blk : {128, 256, 512})
+            for (int bpc : {0, 2, 4, 8})
+                for (int split : {TRISK_SPLIT_CHUNKED, TRISK_SPLIT_BLOCKED, TRISK_SPLIT_INTERLEAVED}) {
+                    const trisk_schedule cand{blk, bpc, split, lanes};
+                    if (rc || !valid_schedule(e, cand)) continue;
+                    HIP_TRY(hipStreamSynchronize(s));
+                    for (int k = 0; k < kEnsKinds && !rc; ++k) {
+                        e->sched[k] = cand;
+                        rc = build_schedule(e, k);
                     }
-            }
+                    e->events_used = 0;
+                    if (!rc) rc = run_ens_steps(e, work, 1, s);  // warm-up
+                    e->events_used = 0;
+                    if (!rc) rc = run_ens_steps(e, work, steps, s);
+                    double ms[kEnsKinds];
+                    int32_t cnt = 0;
+                    if (!rc) rc = trisk_ensemble_kernel_times(e, kEnsKinds, nullptr, ms, nullptr, &cnt);
+                    if (rc) continue;
+                    for (int k = 0; k < kEnsKinds && k < cnt; ++k)
+                        if (ms[k] < win_ms[k]) {
+                            win_ms[k] = ms[k];
+                            win[k] = e->sched[k];
+                        }
+                }
     e->timing = timing;
     e->events_used = 0;
     HIP_TRY(hipStreamSynchronize(s));

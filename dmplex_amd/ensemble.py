@@ -23,17 +23,17 @@ from . import _device as D
 from ._native import f64p, i64p, ptr
 from .trisk_model import HIPBackend, ModelTables, _stream, _torch
 
-SPLITS = {"chunked": 0, "blocked": 1}
+SPLITS = {"chunked": 0, "blocked": 1, "interleaved": 2}
 KERNELS = ("ens_state_kernel", "ens_cell_kernel", "ens_edge_kernel")
 
 
 class TriskSchedule(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("split", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("members_per_lane", C.c_int32)]
 
     def as_dict(self) -> dict:
         return dict(block_size=self.block_size, blocks_per_cu=self.blocks_per_cu,
-                    split={v: k for k, v in SPLITS.items()}[self.split])
+                    split={v: k for k, v in SPLITS.items()}[self.split], members_per_lane=self.members_per_lane)
 
 
 Schedule3 = TriskSchedule * 3
@@ -178,8 +178,8 @@ class TRiSKEnsemble:
         return [x.as_dict() for x in s]
 
     def set_schedule(self, sched: Sequence[dict]) -> None:
-        s = Schedule3(*[TriskSchedule(int(d["block_size"]), int(d["blocks_per_cu"]), SPLITS[d["split"]], 0)
-                        for d in sched])
+        s = Schedule3(*[TriskSchedule(int(d["block_size"]), int(d["blocks_per_cu"]), SPLITS[d["split"]],
+                                      int(d.get("members_per_lane", 1))) for d in sched])
         D.check(D.lib().trisk_ensemble_set_schedule(self._handle, s))
 
     def autotune(self, y, dts: Sequence[float], steps: int = 4) -> tuple[list[dict], list[float]]:

@@ -214,15 +214,19 @@ int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
 typedef struct trisk_ensemble trisk_ensemble;
 
 /* How one kernel's tiles (block_size items each) are laid on the device. */
-enum { TRISK_SPLIT_CHUNKED = 0,  /* XCD x takes the x-th eighth of every level's range */
-       TRISK_SPLIT_BLOCKED = 1   /* XCD x takes the x-th eighth of all ranges laid end to end */ };
+enum { TRISK_SPLIT_CHUNKED = 0,     /* XCD x takes the x-th eighth of every level's range */
+       TRISK_SPLIT_BLOCKED = 1,     /* XCD x takes the x-th eighth of all ranges laid end to end */
+       TRISK_SPLIT_INTERLEAVED = 2  /* as CHUNKED; the state kernel takes each stretch of cells
+                                       together with its vertices and edges */ };
 enum { TRISK_ENS_STATE = 0, TRISK_ENS_CELL = 1, TRISK_ENS_EDGE = 2 };  /* the three kernels */
 
 typedef struct {
     int32_t block_size;          /* 128, 256 or 512 work-items per workgroup */
     int32_t blocks_per_cu;       /* persistent grid of blocks_per_cu workgroups per CU; 0: one per tile */
     int32_t split;               /* TRISK_SPLIT_* */
-    int32_t reserved;
+    int32_t members_per_lane;    /* 1, 2 or 4 consecutive members per work-item (0: 1); must divide
+                                    every level's member count, and the state vectors passed to the
+                                    ensemble must then be aligned to 8 x members_per_lane bytes */
 } trisk_schedule;
 
 /* Creates the level models (trisk_create with TRISK_PATH_PAIR; opt->path must be AUTO or
@@ -249,11 +253,11 @@ int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt,
 int trisk_ensemble_set_graphs(trisk_ensemble* e, int32_t on);
 /* out[3k .. 3k+2] = trisk_diagnostics of member k, members numbered level by level.  Synchronous. */
 int trisk_ensemble_diagnostics(trisk_ensemble* e, const double* y_dev, double* out, void* stream);
-/* The schedules of the three kernels (TRISK_ENS_*); the default is {256, 0, CHUNKED}. */
+/* The schedules of the three kernels (TRISK_ENS_*); the default is {256, 0, CHUNKED, 1}. */
 int trisk_ensemble_set_schedule(trisk_ensemble* e, const trisk_schedule sched[3]);
 int trisk_ensemble_get_schedule(const trisk_ensemble* e, trisk_schedule sched[3]);
 /* Times `steps` RK4 steps under each candidate schedule (block size x persistence x
- * split) on a device copy of y_dev (y_dev is not changed), keeps per kernel the one with
+ * split x members per lane) on a device copy of y_dev (y_dev is not changed), keeps per kernel the one with
  * the shortest mean launch, and returns it with that time (either output may be NULL).
  * Synchronous. */
 int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double* dt, int32_t steps, void* stream,

@@ -6,6 +6,7 @@ several member counts, including odd ones, under every schedule the autotuner ca
 choose, and with graph replay.  The FP64 members are also checked against the oracle
 to 1e-12 per field.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -13,6 +14,7 @@ import pytest
 
 from dmplex_amd import (HIPBackend, ModelTables, TRiSKEnsemble, TRiSKShallowWater, create_hexagon_mesh, diagnostics,
                         rk4_steps_, tendency_)
+from dmplex_amd import _device as D
 from dmplex_amd.synthetic import earth_parameters, seeded_state
 from oracle.oracle import Oracle
 
@@ -99,17 +101,34 @@ def test_tendency_equals_single_models():
             assert bool((got == ref).all()), (li, m)
 
 
-@pytest.mark.parametrize("block_size", [128, 256, 512])
-@pytest.mark.parametrize("blocks_per_cu", [0, 1, 4])
-@pytest.mark.parametrize("split", ["chunked", "blocked"])
-def test_every_schedule_gives_the_same_bits(block_size, blocks_per_cu, split):
-    say(f"ensemble schedule {block_size} {blocks_per_cu} {split}")
-    ens, y = build("fp32_bulk")
+EVEN = (4, 8, 4)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+@pytest.mark.parametrize("block_size", [128, 512])
+@pytest.mark.parametrize("blocks_per_cu", [0, 4])
+@pytest.mark.parametrize("split", ["chunked", "blocked", "interleaved"])
+def test_every_schedule_gives_the_same_bits(precision, lanes, block_size, blocks_per_cu, split):
+    """Members per lane, block size, persistence and XCD split change only the mapping of
+    work to lanes: the states must be bitwise those of the default schedule and of a
+    single model."""
+    say(f"ensemble schedule {precision} lanes={lanes} {block_size} {blocks_per_cu} {split}")
+    ens = TRiSKEnsemble([tables(n) for n in LEVELS], EVEN, backend=HIPBackend(0, precision))
+    y = ens.zeros()
+    for li, (n, M) in enumerate(zip(LEVELS, EVEN)):
+        for m in range(M):
+            ens.set_member(y, li, m, member_state(tables(n), li, m))
     ref = y.clone()
     ens.rk4_steps_(ref, DTS, 3)
-    ens.set_schedule([dict(block_size=block_size, blocks_per_cu=blocks_per_cu, split=split)] * 3)
+    ens.set_schedule([dict(block_size=block_size, blocks_per_cu=blocks_per_cu, split=split,
+                           members_per_lane=lanes)] * 3)
     ens.rk4_steps_(y, DTS, 3)
     assert bool((y == ref).all())
+    s = make_single(3, precision)
+    yd = s.upload_state(member_state(s, 1, 7))
+    rk4_steps_(yd, DTS[1], 3, s)
+    assert np.array_equal(ens.get_member(y, 1, 7), s.download_state(yd))
 
 
 def test_fp64_members_match_oracle():
@@ -175,5 +194,13 @@ def test_ensemble_rejects_bad_arguments():
         ens.member_device(y, 0, 2)
     with pytest.raises(ValueError, match="schedule"):
         ens.set_schedule([dict(block_size=96, blocks_per_cu=0, split="chunked")] * 3)
+    with pytest.raises(ValueError, match="members_per_lane"):
+        TRiSKEnsemble([tables(2)], [3]).set_schedule([dict(block_size=256, blocks_per_cu=0, split="chunked",
+                                                          members_per_lane=2)] * 3)
+    ens.set_schedule([dict(block_size=256, blocks_per_cu=0, split="chunked", members_per_lane=2)] * 3)
+    dt = np.array([600.0])
+    with pytest.raises(ValueError, match="aligned"):  # refused on the host, before any launch
+        D.check(D.lib().trisk_ensemble_rk4_steps(ens.handle, C.c_void_p(y.data_ptr() + 8),
+                                                 dt.ctypes.data_as(C.POINTER(C.c_double)), 1, None))
     with pytest.raises(ValueError):
         ens.rk4_steps_(y, [1.0, 2.0], 1)
