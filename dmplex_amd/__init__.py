@@ -10,6 +10,8 @@ Public API (mirrors the reference's Julia names; ``!`` becomes ``_``):
   ``tendency_``, ``RK4Cache``, ``rk4_step_``, ``gather_state``, ``total_mass``,
   ``total_energy``, ``potential_enstrophy``, ``relative_vorticity``,
   ``tangential_velocity``, ``kinetic_energy_``, ``vertex_pv_``;
+* adjoint: ``tendency_vjp``, ``rk4_adjoint`` (checkpointed), ``rrule_rk4_steps``,
+  ``rk4_steps`` (a differentiable torch operation), ``height_misfit``;
 * ensembles: ``TRiSKEnsemble`` (members on meshes of several resolutions, one launch per
   kernel for all of them, with an autotuned schedule).
 
@@ -22,10 +24,11 @@ from .grids import (cell_centers, create_hexagon_mesh, create_spherical_mesh, ll
 from .topology import (Plex, check_plex, hilbert_key, hilbert_keys, hilbert_order, hilbert_permutation,
                        plex_from_polygons)
 from .trisk_grid import TRiSKGrid, arc_length, gassmann_coefficients, signed_triangle_area, vertex_edge_areas
-from .trisk_model import (HIPBackend, KineticWork, ModelTables, RK4Cache, TRiSKShallowWater, diagnostics,
-                          gather_state, global_height_dofs, global_velocity_dofs, height_dofs, initial_state,
-                          initial_state_values, kinetic_energy_, potential_enstrophy, relative_vorticity,
-                          rk4_step_, rk4_steps_, tangential_velocity, tendency_, total_energy, total_mass,
+from .trisk_model import (HIPBackend, KineticWork, ModelTables, RK4Cache, TRiSKShallowWater, cell_areas_device,
+                          diagnostics, gather_state, global_height_dofs, global_velocity_dofs, height_dofs,
+                          height_misfit, initial_state, initial_state_values, kinetic_energy_,
+                          potential_enstrophy, relative_vorticity, rk4_adjoint, rk4_step_, rk4_steps, rk4_steps_,
+                          rrule_rk4_steps, tangential_velocity, tendency_, tendency_vjp, total_energy, total_mass,
                           velocity_dofs, vertex_pv_)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -59,6 +59,8 @@ SIGNATURES = {
     "trisk_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_info": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_char_p)],
     "trisk_wide_counts": [vp, i64p],
+    "trisk_tendency_vjp": [vp, vp, vp, vp, vp],
+    "trisk_rk4_adjoint": [vp, vp, C.c_double, i64, i64, vp, vp, vp, vp],
     "trisk_ensemble_create": [C.POINTER(C.POINTER(TriskModelDesc)), C.POINTER(C.c_int32), C.c_int32,
                               C.POINTER(TriskOptions), C.POINTER(vp)],
     "trisk_ensemble_destroy": [vp],
@@ -86,6 +88,13 @@ def lib() -> C.CDLL:
     """The device library (loaded once); raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64 under the same soname: loaded first, it is the one
+        # HIP runtime of the process (loaded after ours, it would be a second runtime and
+        # torch would see no device), so the state tensors and the kernels share it
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         path = LIB_DIR / "libtrisk_hip.so"
         if not path.exists():
             raise NativeError(f"{path} is missing: run `python -m dmplex_amd.build`")

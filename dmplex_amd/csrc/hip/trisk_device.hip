@@ -731,6 +731,7 @@ int trisk_destroy(trisk_handle* h) {
     if (!h) return 0;
     (void)hipSetDevice(h->device);
     drop_graph(h);
+    if (h->adj_release) h->adj_release(h);
     for (auto& e : h->events) (void)hipEventDestroy(e);
     for (auto& b : h->owned) (void)hipFree(b.p);
     delete h;
@@ -806,6 +807,43 @@ int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* s
 static int run_steps(trisk_handle* h, double* y, double dt, int64_t n, hipStream_t s) {
     return h->precision == TRISK_FP64 ? rk4_steps<Fp64>(h, y, dt, n, s) : rk4_steps<Fp32Bulk>(h, y, dt, n, s);
 }
+
+}  // extern "C"
+
+namespace trisk {
+
+template <class P, class UIn>
+static int state_pass(trisk_handle* H, const UIn* u, const double* h, hipStream_t s) {
+    const Layout& L = H->L;
+    const int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = tiles_per_xcd(L.ne);
+    constexpr bool rec = P::id == TRISK_FP32_BULK;
+    hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0, s,
+                       params<P>(H), u, h, t0, t1, t2);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int pair_state_pass(trisk_handle* H, const void* u, bool u_double, const double* h, hipStream_t s) {
+    if (H->precision == TRISK_FP64) return state_pass<Fp64, double>(H, static_cast<const double*>(u), h, s);
+    if (u_double) return state_pass<Fp32Bulk, double>(H, static_cast<const double*>(u), h, s);
+    return state_pass<Fp32Bulk, float>(H, static_cast<const float*>(u), h, s);
+}
+
+int pair_tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r,
+                        hipStream_t s) {
+    if (H->precision == TRISK_FP64) return run_pair<Fp64, double>(H, static_cast<const double*>(u), h, r, s);
+    if (u_double) return run_pair<Fp32Bulk, double>(H, static_cast<const double*>(u), h, r, s);
+    return run_pair<Fp32Bulk, float>(H, static_cast<const float*>(u), h, r, s);
+}
+
+int rk4_steps_direct(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s) {
+    return run_steps(H, y, dt, n, s);
+}
+
+}  // namespace trisk
+
+extern "C" {
+
 
 static void drop_graph(trisk_handle* h) {
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);

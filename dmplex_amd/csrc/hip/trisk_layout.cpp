@@ -459,6 +459,54 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.wide_cells = wc;
     L.wide_vertices = wv;
     L.wide_edges = we;
+    // transposed coefficients (adjoint): the forward coefficient of each (entity, neighbour)
+    // pair, stored at the neighbour
+    L.adj_ecurl.assign(size_t(ne) * 2, 0.0);
+    L.adj_ekin.assign(size_t(ne) * 2, 0.0);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t e = 0; e < ne; ++e)
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int64_t v = L.ev[size_t(s2) * ne + e], c = L.ec[size_t(s2) * ne + e];
+            int k = 0;
+            while (k < 3 && L.ve[size_t(k) * nv + v] != e) ++k;
+            bad |= k == 3;
+            L.adj_ecurl[2 * e + s2] = k < 3 ? L.curl[size_t(k) * nv + v] : 0.0;
+            const int n = L.cmeta[c] & 15;
+            k = 0;
+            while (k < n && L.ce[size_t(k) * nc + c] != e) ++k;
+            bad |= k == n;
+            L.adj_ekin[2 * e + s2] = k < n ? L.kin[size_t(k) * nc + c] : 0.0;
+        }
+    L.adj_cvh.assign(size_t(6) * nc, 0.0);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t i = 0; i < nc; ++i) {
+        const int n = L.cmeta[i] & 15;
+        for (int k = 0; k < n; ++k) {
+            const int64_t v = L.ccv[size_t(k) * nc + i];
+            int j = 0;
+            while (j < 3 && L.vc[size_t(j) * nv + v] != i) ++j;
+            bad |= j == 3;
+            L.adj_cvh[size_t(k) * nc + i] = j < 3 ? L.vh[size_t(j) * nv + v] : 0.0;
+        }
+    }
+    L.adj_vslot.assign(nv, 0);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t v = 0; v < nv; ++v) {
+        int32_t slots = 0;
+        for (int j = 0; j < 3; ++j) {
+            const int64_t c = L.vc[size_t(j) * nv + v];
+            const int n = L.cmeta[c] & 15;
+            int k = 0;
+            while (k < n && L.ccv[size_t(k) * nc + c] != v) ++k;
+            bad |= k == n;
+            slots |= int32_t(k < n ? k : 0) << (4 * j);
+        }
+        L.adj_vslot[v] = slots;
+    }
+    if (bad) {
+        L.pair_reason = "vertex, cell and edge tables are not mutually consistent";
+        return;
+    }
     L.pair = true;
     L.pair_reason = "cell-pair TRiSK weights (no per-edge TRiSK tables)";
 }

@@ -51,6 +51,11 @@ struct Layout {
     std::vector<int32_t> re_cells16;     // ne: left | right << 16
     std::vector<int32_t> pc_ev16;        // 6 x nc: edge | corner << 16
     int64_t wide_cells = 0, wide_vertices = 0, wide_edges = 0;
+    // transposed coefficients for the adjoint of the cell-pair path
+    std::vector<double> adj_ecurl;       // ne x 2: curl coefficient of edge e in vertex ev[s]
+    std::vector<double> adj_ekin;        // ne x 2: kinetic-energy weight of edge e in cell ec[s]
+    std::vector<double> adj_cvh;         // 6 x nc: vertex-height weight of cell i in its corner k
+    std::vector<int32_t> adj_vslot;      // nv: corner index of v in its cell j at bits 4j..4j+3
 };
 
 constexpr int16_t kWide = -32768;

@@ -445,3 +445,97 @@ def tangential_velocity(m: TRiSKShallowWater, u):
     D.check(D.lib().trisk_tangential_velocity(m.handle, C.c_void_p(ud.data_ptr()), C.c_void_p(v.data_ptr()),
                                               _stream(torch)))
     return m.download_field(1, v) if host else v
+
+
+# --- adjoint (the pullbacks behind a ChainRulesCore rrule of tendency! / rk4_step!) ----------
+
+def _check_state(m: TRiSKShallowWater, *ys) -> None:
+    torch = _torch()
+    for y in ys:
+        if y.dtype != torch.float64 or not y.is_contiguous() or y.numel() != m.state_length:
+            raise ValueError(f"device states must be contiguous float64 tensors of {m.state_length} entries")
+
+
+def tendency_vjp(m: TRiSKShallowWater, y, dybar):
+    """(d tendency / dy)^T dybar at the device state ``y`` (the pullback of ``tendency!``,
+    TRiSKShallowWater.jl:379-394; cell-pair path: TRiSK PV flux, cell kinetic energy)."""
+    torch = _torch()
+    _check_state(m, y, dybar)
+    out = m.zeros()
+    D.check(D.lib().trisk_tendency_vjp(m.handle, C.c_void_p(y.data_ptr()), C.c_void_p(dybar.data_ptr()),
+                                       C.c_void_p(out.data_ptr()), _stream(torch)))
+    return out
+
+
+def rk4_adjoint(m: TRiSKShallowWater, y0, dt: float, nsteps: int, ybar, checkpoint_every: int = 8):
+    """(ybar0, yT): the cotangent of ``y0`` for the cotangent ``ybar`` of the state after
+    ``nsteps`` RK4 steps (RungeKutta.jl:18-28), with checkpoints every ``checkpoint_every``
+    steps and recomputation in between; ``yT`` is that state."""
+    torch = _torch()
+    _check_state(m, y0, ybar)
+    out, yT = m.zeros(), m.zeros()
+    D.check(D.lib().trisk_rk4_adjoint(m.handle, C.c_void_p(y0.data_ptr()), float(dt), int(nsteps),
+                                      int(checkpoint_every), C.c_void_p(ybar.data_ptr()), C.c_void_p(out.data_ptr()),
+                                      C.c_void_p(yT.data_ptr()), _stream(torch)))
+    return out, yT
+
+
+def rrule_rk4_steps(m: TRiSKShallowWater, y0, dt: float, nsteps: int, checkpoint_every: int = 8):
+    """The rrule of ``nsteps`` fused ``rk4_step!``: ``(yT, pullback)`` with
+    ``pullback(ybarT) -> ybar0`` (ChainRulesCore's convention)."""
+    yT = y0.clone()
+    rk4_steps_(yT, dt, nsteps, m)
+
+    def pullback(ybar):
+        return rk4_adjoint(m, y0, dt, nsteps, ybar.contiguous(), checkpoint_every)[0]
+
+    return yT, pullback
+
+
+_AUTOGRAD = None
+
+
+def rk4_steps(m: TRiSKShallowWater, y0, dt: float, nsteps: int, checkpoint_every: int = 8):
+    """``nsteps`` RK4 steps from ``y0`` (not in place) as a differentiable torch operation:
+    the backward pass runs ``trisk_rk4_adjoint``."""
+    global _AUTOGRAD
+    torch = _torch()
+    if _AUTOGRAD is None:
+        class RK4Steps(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, y, model, dt_, n_, ck_):
+                ctx.model, ctx.dt, ctx.n, ctx.ck = model, dt_, n_, ck_
+                ctx.save_for_backward(y)
+                yT = y.detach().clone()
+                rk4_steps_(yT, dt_, n_, model)
+                return yT
+
+            @staticmethod
+            def backward(ctx, ybar):
+                (y,) = ctx.saved_tensors
+                g, _ = rk4_adjoint(ctx.model, y.detach(), ctx.dt, ctx.n, ybar.contiguous(), ctx.ck)
+                return g, None, None, None, None
+
+        _AUTOGRAD = RK4Steps
+    return _AUTOGRAD.apply(y0, m, float(dt), int(nsteps), int(checkpoint_every))
+
+
+def cell_areas_device(m: TRiSKShallowWater):
+    """Cell areas as a device tensor in the library's internal order."""
+    torch = _torch()
+    if getattr(m, "_areas_dev", None) is None:
+        perm = m.permutation()[0]
+        m._areas_dev = torch.tensor(m.cell_areas[perm], dtype=torch.float64, device=f"cuda:{m.backend.device}")
+    return m._areas_dev
+
+
+def height_misfit(m: TRiSKShallowWater, y, hstar):
+    """J = sum_i A_i (h_i - h*_i)^2 and its cotangent [0; 2 A (h - h*)] (device order;
+    ``hstar`` a device vector of the cells)."""
+    torch = _torch()
+    A = cell_areas_device(m)
+    ne = m.grid.num_edges()
+    d = y[ne:] - hstar
+    ybar = torch.zeros_like(y)
+    ybar[ne:] = 2 * A * d
+    return float(torch.sum(A * d * d).item()), ybar

@@ -196,6 +196,28 @@ int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]);
  * (text owned by the handle). */
 int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
 
+/* --- adjoint ------------------------------------------------------------------------
+ * No counterpart in the reference (it has no adjoint).  These are the pullbacks of
+ * tendency! (src/FiniteVolumes/TRiSKShallowWater.jl:379-394) and of nsteps rk4_step!
+ * (src/TimeSteppers/RungeKutta.jl:18-28) that a ChainRulesCore rrule calls: the transposed
+ * linearisation of the handle's own forward arithmetic.  They need the cell-pair path
+ * (TRiSK PV flux, cell kinetic energy: a handle created with TRISK_PATH_PAIR, or any
+ * FP32-bulk handle of such a model); other handles return TRISK_EINVAL. */
+
+/* ybar = (d tendency / dy)^T dybar at the state y (device vectors, internal order; ybar
+ * must not alias y or dybar). */
+int trisk_tendency_vjp(trisk_handle* h, const double* y_dev, const double* dybar_dev, double* ybar_dev,
+                       void* stream);
+
+/* The cotangent ybar0 of y0 for the cotangent ybarT of y_T, y_T being y0 after nsteps RK4
+ * steps of dt (as trisk_rk4_steps).  The forward run keeps the state every
+ * checkpoint_every steps (0: 8) and each segment is recomputed from its checkpoint during
+ * the backward sweep.  yT_dev (may be NULL) receives y_T.  ybar0 may alias ybarT.  The
+ * library allocates (ceil(nsteps / checkpoint_every) + checkpoint_every + 4) state vectors
+ * of work memory for the call.  Synchronous. */
+int trisk_rk4_adjoint(trisk_handle* h, const double* y0_dev, double dt, int64_t nsteps, int64_t checkpoint_every,
+                      const double* ybarT_dev, double* ybar0_dev, double* yT_dev, void* stream);
+
 /* --- ensembles ----------------------------------------------------------------------
  * No counterpart in the reference, which advances each model with its own rk4_step! loop
  * (src/TimeSteppers/RungeKutta.jl:18-28, examples/trisk_shallow_water.jl:92-100).  An

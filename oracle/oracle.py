@@ -60,6 +60,8 @@ def lib() -> C.CDLL:
             ("oracle_vertex_pv", [M, vp, vp], None),
             ("oracle_mass_flux", [M, vp, vp], None),
             ("oracle_edge_sum", [i64, i64, vp, vp, vp, vp], None),
+            ("oracle_tendency_vjp", [M, vp, vp, vp], C.c_int),
+            ("oracle_rk4_adjoint", [M, vp, f64, i64, vp], C.c_int),
             ("oracle_threads", [], C.c_int),
             ("oracle_set_threads", [C.c_int], None),
         ]:
@@ -119,6 +121,23 @@ class Oracle:
         k = self._cache
         lib().oracle_rk4_steps(C.byref(self.s), self._ptr(y), dt, nsteps, *(self._ptr(a) for a in k))
         return y
+
+    def tendency_vjp(self, y: np.ndarray, lam: np.ndarray) -> np.ndarray:
+        """(d tendency / dy)^T lam at y (TRiSK PV flux, cell kinetic energy)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.empty_like(y)
+        if lib().oracle_tendency_vjp(C.byref(self.s), self._ptr(y), self._ptr(lam), self._ptr(out)) != 0:
+            raise ValueError("the oracle adjoint covers the TRiSK PV flux with cell kinetic energy only")
+        return out
+
+    def rk4_adjoint(self, y0: np.ndarray, dt: float, nsteps: int, ybar: np.ndarray) -> np.ndarray:
+        """The cotangent of y0 for the cotangent ``ybar`` of the state after nsteps RK4 steps."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        out = np.array(ybar, dtype=np.float64)
+        if lib().oracle_rk4_adjoint(C.byref(self.s), self._ptr(y0), dt, nsteps, self._ptr(out)) != 0:
+            raise ValueError("the oracle adjoint covers the TRiSK PV flux with cell kinetic energy only")
+        return out
 
     def total_mass(self, y):
         return lib().oracle_total_mass(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))

@@ -228,3 +228,128 @@ void oracle_edge_sum(int64_t n, int64_t k, const int64_t* edges, const double* w
         out[j] = s;
     }
 }
+
+/* ---------------------------------------------------------------------------------
+ * Adjoint (vector-Jacobian product) of the tendency with TRiSK PV flux and cell kinetic
+ * energy, and of rk4_step!.  The reference has no adjoint; this is the transpose of the
+ * linearisation of oracle_tendency above, written as plain scatter-adds over the
+ * reference's tables in one thread (an implementation independent of the device's
+ * gather-only kernels).  It is pinned by finite differences in the tests. */
+
+int oracle_tendency_vjp(const oracle_model* m, const double* y, const double* lam, double* ybar) {
+    if (m->cell_vertices || m->g_edges) return -1;
+    const int64_t ne = m->ne, nc = m->nc, nv = m->nv, KC = m->kc, KV = m->kv, KT = m->kt;
+    const double* u = y;
+    const double* h = y + ne;
+    const double* lu = lam;
+    const double* lh = lam + ne;
+    double* ub = ybar;
+    double* hb = ybar + ne;
+    /* forward intermediates */
+    oracle_kinetic_energy(m, u, h);
+    oracle_vertex_pv(m, u, h);
+    oracle_mass_flux(m, u, h);
+    for (int64_t e = 0; e < ne; ++e)
+        m->qe[e] = (m->qv[AT(m->edge_vertices, 2, 0, e)] + m->qv[AT(m->edge_vertices, 2, 1, e)]) / 2;
+    double* Fb = (double*)calloc((size_t)ne, sizeof(double));
+    double* qeb = (double*)calloc((size_t)ne, sizeof(double));
+    double* Bb = (double*)calloc((size_t)nc, sizeof(double));
+    double* qb = (double*)calloc((size_t)nv, sizeof(double));
+    for (int64_t j = 0; j < ne + nc; ++j) ybar[j] = 0.0;
+    /* du_e = 1/2 sum_k w F_f (qe_e + qe_f) - (B_c1 - B_c0) / d_e */
+    for (int64_t e = 0; e < ne; ++e) {
+        for (int64_t k = 0; k < KT; ++k) {
+            int64_t f = AT(m->trisk_edges, KT, k, e);
+            double w = AT(m->trisk, KT, k, e);
+            Fb[f] += 0.5 * lu[e] * w * (m->qe[e] + m->qe[f]);
+            qeb[e] += 0.5 * lu[e] * w * m->F[f];
+            qeb[f] += 0.5 * lu[e] * w * m->F[f];
+        }
+        Bb[AT(m->edge_cells, 2, 1, e)] -= lu[e] * m->inverse_dual[e];
+        Bb[AT(m->edge_cells, 2, 0, e)] += lu[e] * m->inverse_dual[e];
+    }
+    /* dh_i = -sum_k div F */
+    for (int64_t i = 0; i < nc; ++i)
+        for (int64_t k = 0; k < KC; ++k) Fb[AT(m->cell_edges, KC, k, i)] -= AT(m->divergence, KC, k, i) * lh[i];
+    /* qe_e = (q_v0 + q_v1) / 2 */
+    for (int64_t e = 0; e < ne; ++e) {
+        qb[AT(m->edge_vertices, 2, 0, e)] += qeb[e] / 2;
+        qb[AT(m->edge_vertices, 2, 1, e)] += qeb[e] / 2;
+    }
+    /* q_v = (zeta_v + f_v) / h_v */
+    for (int64_t v = 0; v < nv; ++v) {
+        double zb = qb[v] / m->hv[v], eb = -qb[v] * m->qv[v] / m->hv[v];
+        for (int64_t k = 0; k < KV; ++k) {
+            ub[AT(m->vertex_edges, KV, k, v)] += AT(m->curl, KV, k, v) * zb;
+            hb[AT(m->vertex_cells, KV, k, v)] += AT(m->vertex_height, KV, k, v) * eb;
+        }
+    }
+    /* F_e = (eh_0 h_c0 + eh_1 h_c1) u_e */
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t c0 = AT(m->edge_cells, 2, 0, e), c1 = AT(m->edge_cells, 2, 1, e);
+        double e0 = AT(m->edge_height, 2, 0, e), e1 = AT(m->edge_height, 2, 1, e);
+        ub[e] += Fb[e] * (e0 * h[c0] + e1 * h[c1]);
+        hb[c0] += Fb[e] * e0 * u[e];
+        hb[c1] += Fb[e] * e1 * u[e];
+    }
+    /* B_i = sum_k kin u^2 + g h_i */
+    for (int64_t i = 0; i < nc; ++i) {
+        hb[i] += m->gravity * Bb[i];
+        for (int64_t k = 0; k < KC; ++k) {
+            int64_t e = AT(m->cell_edges, KC, k, i);
+            ub[e] += 2 * AT(m->kinetic, KC, k, i) * u[e] * Bb[i];
+        }
+    }
+    free(Fb);
+    free(qeb);
+    free(Bb);
+    free(qb);
+    return 0;
+}
+
+/* ybar (in: the cotangent of y after nsteps rk4_step! from y0; out: the cotangent of y0).
+ * Every step's state is stored (small meshes only); each step's stages are recomputed. */
+int oracle_rk4_adjoint(const oracle_model* m, const double* y0, double dt, int64_t nsteps, double* ybar) {
+    if (m->cell_vertices || m->g_edges) return -1;
+    const int64_t n = m->ne + m->nc;
+    double* ys = (double*)malloc(sizeof(double) * (size_t)n * (size_t)(nsteps > 0 ? nsteps : 1));
+    double* w = (double*)malloc(sizeof(double) * (size_t)n * 11);
+    double *k1 = w, *k2 = w + n, *k3 = w + 2 * n, *k4 = w + 3 * n, *st = w + 4 * n;
+    double *s2 = w + 5 * n, *s3 = w + 6 * n, *s4 = w + 7 * n, *kb = w + 8 * n, *sb = w + 9 * n, *acc = w + 10 * n;
+    for (int64_t j = 0; j < n; ++j) st[j] = y0[j];
+    for (int64_t s = 0; s < nsteps; ++s) {
+        for (int64_t j = 0; j < n; ++j) ys[s * n + j] = st[j];
+        oracle_rk4_step(m, st, dt, k1, k2, k3, k4, s4);
+    }
+    for (int64_t s = nsteps - 1; s >= 0; --s) {
+        const double* y = ys + s * n;
+        oracle_tendency(m, y, k1);
+        for (int64_t j = 0; j < n; ++j) s2[j] = y[j] + dt / 2 * k1[j];
+        oracle_tendency(m, s2, k2);
+        for (int64_t j = 0; j < n; ++j) s3[j] = y[j] + dt / 2 * k2[j];
+        oracle_tendency(m, s3, k3);
+        for (int64_t j = 0; j < n; ++j) s4[j] = y[j] + dt * k3[j];
+        /* y' = y + dt/6 (k1 + 2 k2 + 2 k3 + k4); k4 = f(s4), k3 = f(s3), ... */
+        for (int64_t j = 0; j < n; ++j) kb[j] = dt / 6 * ybar[j];
+        oracle_tendency_vjp(m, s4, kb, sb);
+        for (int64_t j = 0; j < n; ++j) {
+            acc[j] = ybar[j] + sb[j];
+            kb[j] = dt / 3 * ybar[j] + dt * sb[j];
+        }
+        oracle_tendency_vjp(m, s3, kb, sb);
+        for (int64_t j = 0; j < n; ++j) {
+            acc[j] += sb[j];
+            kb[j] = dt / 3 * ybar[j] + dt / 2 * sb[j];
+        }
+        oracle_tendency_vjp(m, s2, kb, sb);
+        for (int64_t j = 0; j < n; ++j) {
+            acc[j] += sb[j];
+            kb[j] = dt / 6 * ybar[j] + dt / 2 * sb[j];
+        }
+        oracle_tendency_vjp(m, y, kb, sb);
+        for (int64_t j = 0; j < n; ++j) ybar[j] = acc[j] + sb[j];
+    }
+    free(ys);
+    free(w);
+    return 0;
+}

@@ -197,3 +197,69 @@ def test_spherical_harmonics_orthonormal():
         ys.append(streamfunction(grid.centers, coef))
     G = np.array([[np.sum(A * p * q) for q in ys] for p in ys])
     assert np.max(np.abs(G - np.eye(len(ys)))) < 5e-3
+
+
+# --- the oracle's adjoint (no reference counterpart: pinned by finite differences) -------------
+
+def _adjoint_case(n=3):
+    dm = create_hexagon_mesh(n, centroidal=True)
+    t = ModelTables(dm, centers=dm.generators, **earth_parameters())
+    y = seeded_state(t.grid)
+    ne = dm.ne
+    scale = np.concatenate([np.full(ne, np.std(y[:ne])), np.full(dm.nc, np.std(y[ne:]))])
+    return t, y, ne, scale
+
+
+def test_oracle_tendency_vjp_dot_product():
+    """<w, J v> by central differences of the tendency equals <J^T w, v>."""
+    t, y, ne, scale = _adjoint_case()
+    o = Oracle(t)
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        v = rng.standard_normal(len(y)) * scale
+        w = rng.standard_normal(len(y)) / scale
+        eps = 1e-4
+        jv = (o.tendency(y + eps * v) - o.tendency(y - eps * v)) / (2 * eps)
+        a, b = w @ jv, o.tendency_vjp(y, w) @ v
+        assert abs(a - b) < 1e-8 * abs(b)
+
+
+def test_oracle_rk4_adjoint_matches_finite_differences():
+    """Gradient of J = sum A (h_T - h*)^2 after 6 RK4 steps against central differences."""
+    t, y, ne, scale = _adjoint_case()
+    o = Oracle(t)
+    dt, N = 1200.0, 6
+    hstar = seeded_state(t.grid, seed_coef=3, seed_noise=4)[ne:]
+    A = t.cell_areas
+
+    def J(y0):
+        yT = o.rk4_steps(y0.copy(), dt, N)
+        return np.sum(A * (yT[ne:] - hstar) ** 2)
+
+    yT = o.rk4_steps(y.copy(), dt, N)
+    ybar = np.zeros_like(y)
+    ybar[ne:] = 2 * A * (yT[ne:] - hstar)
+    g = o.rk4_adjoint(y, dt, N, ybar)
+    rng = np.random.default_rng(12)
+    for _ in range(4):
+        v = rng.standard_normal(len(y)) * scale
+        eps = 1e-5
+        fd = (J(y + eps * v) - J(y - eps * v)) / (2 * eps)
+        assert abs(fd - g @ v) < 1e-7 * abs(g @ v)
+
+
+def test_adjoint_check_is_sharp():
+    """The dot-product check fails for the adjoint of a slightly different model (kinetic
+    weights scaled by 1 + 1e-3), so a dropped or mis-weighted term cannot pass it."""
+    t, y, ne, scale = _adjoint_case()
+    o = Oracle(t)
+    bad = ModelTables(t.grid, **earth_parameters())
+    bad.kinetic = np.asfortranarray(bad.kinetic * (1 + 1e-3))
+    ob = Oracle(bad)
+    rng = np.random.default_rng(13)
+    v = rng.standard_normal(len(y)) * scale
+    w = rng.standard_normal(len(y)) / scale
+    eps = 1e-4
+    a = w @ ((o.tendency(y + eps * v) - o.tendency(y - eps * v)) / (2 * eps))
+    assert abs(a - ob.tendency_vjp(y, w) @ v) > 1e-6 * abs(a)
+    assert abs(a - o.tendency_vjp(y, w) @ v) < 1e-8 * abs(a)
