@@ -64,6 +64,8 @@ def parse():
                    help="JSON list of three kernel schedules to use instead of autotuning")
     p.add_argument("--no-member-loop", action="store_true",
                    help="skip timing the members model by model")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    p.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
     return p.parse_args()
 
 
@@ -259,6 +261,69 @@ def run_ensemble(args):
     return out
 
 
+def run_adjoint(args):
+    """Gradient of J = sum_i A_i (h_i(T) - h*_i)^2 after 40 RK4 steps of 150 s on SCVT level 7,
+    by the checkpointed adjoint (checkpoints every 8 steps), FP64 and FP32-bulk; beside it the
+    plain forward run and four central-difference checks (eps = 1e-6)."""
+    import torch
+
+    from dmplex_amd import (HIPBackend, TRiSKShallowWater, create_hexagon_mesh, height_misfit, rk4_adjoint,
+                            rk4_steps, rk4_steps_)
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+
+    dm = create_hexagon_mesh(7, centroidal=True, device=0)
+    params = earth_parameters()
+    dt, N, ck = 150.0, 40, 8
+    out = {"workload": f"SCVT level 7 ({dm.nc} cells), J = sum A (h_T - h*)^2 after {N} RK4 steps of {dt} s, "
+                       f"checkpoints every {ck} steps", "unit": "ms"}
+    grads = {}
+    for prec in ("fp64", "fp32_bulk"):
+        m = TRiSKShallowWater(dm, centers=dm.generators, path="pair", backend=HIPBackend(0, prec), **params)
+        y0 = m.upload_state(seeded_state(m.grid))
+        hstar = m.upload_state(seeded_state(m.grid, seed_coef=3, seed_noise=4))[m.grid.num_edges():].clone()
+        y = y0.clone()
+        rk4_steps_(y, dt, N, m)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            y = y0.clone()
+            rk4_steps_(y, dt, N, m)
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - t) / reps * 1e3
+        J0, ybar = height_misfit(m, y, hstar)
+        rk4_adjoint(m, y0, dt, N, ybar, ck)  # warm-up (allocates the adjoint work)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            g, _ = rk4_adjoint(m, y0, dt, N, ybar, ck)
+        torch.cuda.synchronize()
+        grad_ms = (time.perf_counter() - t) / reps * 1e3
+        grads[prec] = g
+        fd = []
+        if prec == "fp64":
+            gen = torch.Generator(device="cuda").manual_seed(5)
+            ne = m.grid.num_edges()
+            su, sh = float(torch.std(y0[:ne])), float(torch.std(y0[ne:]))
+            for _ in range(4):
+                v = torch.randn(m.state_length, dtype=torch.float64, device="cuda", generator=gen)
+                v[:ne] *= su
+                v[ne:] *= sh
+                eps = 1e-6
+                Jp = height_misfit(m, rk4_steps(m, y0 + eps * v, dt, N), hstar)[0]
+                Jm = height_misfit(m, rk4_steps(m, y0 - eps * v, dt, N), hstar)[0]
+                ad = float(g @ v)
+                fd.append(abs((Jp - Jm) / (2 * eps) - ad) / abs(ad))
+        out[prec] = {"forward_ms": fwd, "gradient_ms": grad_ms, "gradient_over_forward": grad_ms / fwd, "J": J0,
+                     "fd_max_rel_diff": max(fd) if fd else None}
+        log(f"[adjoint] {prec}: forward {fwd:.2f} ms, gradient {grad_ms:.2f} ms ({grad_ms / fwd:.2f}x)")
+        del m
+    a, b = grads["fp32_bulk"].cpu().numpy(), grads["fp64"].cpu().numpy()
+    out["fp32_bulk_gradient_rel_l2_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_cpu(grid, y0, args, params):
     from dmplex_amd import ModelTables
     from oracle.oracle import Oracle
@@ -358,6 +423,9 @@ def main():
     if args.ensemble_only:
         print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
         return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
+        return
     t = time.perf_counter()
     log(f"[bench] generating SCVT hexagon mesh level {args.level}")
     dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0, log=lambda m: log(f"[bench] {m}"))
@@ -404,6 +472,9 @@ def main():
     if not args.no_ensemble:
         log("[bench] batched ensemble (levels 4-7)")
         line["ensemble"] = run_ensemble(args)
+    if not args.no_adjoint:
+        log("[bench] adjoint (level 7, 40 steps)")
+        line["adjoint"] = run_adjoint(args)
     print(json.dumps(line), flush=True)
 
 
