@@ -54,6 +54,8 @@ def parse():
                    help="instead of timing: run FP64 (bit-exact), FP64 cell-pair and FP32-bulk for DAYS "
                         "simulated days from the same state and report their errors against FP64 after "
                         "day 1 and day DAYS")
+    p.add_argument("--oracle-steps", type=int, default=10,
+                   help="accuracy mode: RK4 steps compared bitwise with the CPU oracle first")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="run a plain device copy of MIB MiB first (HBM counter calibration)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
@@ -64,6 +66,8 @@ def parse():
                    help="JSON list of three kernel schedules to use instead of autotuning")
     p.add_argument("--no-member-loop", action="store_true",
                    help="skip timing the members model by model")
+    p.add_argument("--pv-flux", choices=["trisk", "gassmann", "gassmann_mean"], default="trisk")
+    p.add_argument("--kinetic-energy", choices=["cell", "mixed", "vertex"], default="cell")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
     return p.parse_args()
@@ -143,7 +147,10 @@ def run_gpu(grid, y0, precision, args, params, path=None):
     mass, energy, _ = diagnostics(m, y)
     nc = grid.num_cells()
     value = nc * args.steps / (ms / 1e3)
+    kbytes = sum(k["bytes"] * k["launches"] for k in stats) / args.steps if stats else None
     out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall, path=m.path_info()[1],
+               kernel_model_bytes_per_cell_step=None if kbytes is None else kbytes / nc,
+               hbm_frac_kernel_model=None if kbytes is None else kbytes / (ms / args.steps / 1e3) / HBM_PEAK,
                graphs=not args.no_graphs,
                ms_per_step_direct_with_events=None if ms_direct is None else ms_direct / args.steps,
                hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK, path_name=path,
@@ -362,6 +369,8 @@ def run_accuracy(args, grid, y0, params):
     from dmplex_amd import HIPBackend, TRiSKShallowWater, diagnostics, rk4_steps_, vertex_pv_
 
     runs = {"fp64": ("fp64", "auto"), "fp64_pair": ("fp64", "pair"), "fp32_bulk": ("fp32_bulk", "auto")}
+    if params.get("pv_flux", "trisk") != "trisk" or params.get("kinetic_energy", "cell") != "cell":
+        del runs["fp64_pair"]  # the cell-pair path covers the TRiSK PV flux with cell kinetic energy only
     models, states, mass0, energy0 = {}, {}, {}, {}
     for key, (prec, path) in runs.items():
         m = TRiSKShallowWater(grid, backend=HIPBackend(0, prec), path=path, **params)
@@ -371,6 +380,18 @@ def run_accuracy(args, grid, y0, params):
         mass0[key], energy0[key], _ = diagnostics(m, states[key])
     spd = int(round(86400.0 / args.dt))
     report = {}
+    # the FP64 reference trajectory is the FP64 table path; tie it to the oracle (the CPU
+    # restatement of the reference) on the first steps of the same state
+    from dmplex_amd import ModelTables
+    from oracle.oracle import Oracle
+
+    n_or = args.oracle_steps
+    yo = Oracle(ModelTables(grid, **params)).rk4_steps(y0.copy(), args.dt, n_or)
+    yg = models["fp64"].upload_state(y0)
+    rk4_steps_(yg, args.dt, n_or, models["fp64"])
+    report["fp64_bitwise_equal_to_oracle"] = {"steps": n_or,
+                                              "equal": bool(np.array_equal(models["fp64"].download_state(yg), yo))}
+    log(f"[accuracy] FP64 GPU vs oracle after {n_or} steps: {report['fp64_bitwise_equal_to_oracle']}")
     t0 = time.perf_counter()
     for day in range(1, args.accuracy + 1):
         for key, m in models.items():
@@ -430,7 +451,8 @@ def main():
     log(f"[bench] generating SCVT hexagon mesh level {args.level}")
     dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0, log=lambda m: log(f"[bench] {m}"))
     grid = TRiSKGrid(dm, centers=dm.generators)
-    params = earth_parameters()
+    params = dict(earth_parameters(), pv_flux=args.pv_flux, kinetic_energy=args.kinetic_energy)
+    trisk_cell = args.pv_flux == "trisk" and args.kinetic_energy == "cell"
     y0 = seeded_state(grid)
     log(f"[bench] mesh + tables + state in {time.perf_counter() - t:.1f} s: "
         f"{grid.num_cells()} cells, {grid.num_edges()} edges, {grid.num_vertices()} vertices")
@@ -439,6 +461,8 @@ def main():
         return
     if args.precision == "both":
         runs = [("fp64", "fp64", None), ("fp64_pair", "fp64", "pair"), ("fp32_bulk", "fp32_bulk", None)]
+        if not trisk_cell:  # the cell-pair path covers the TRiSK PV flux with cell kinetic energy only
+            runs = [r for r in runs if r[0] != "fp64_pair"]
     else:
         runs = [(args.precision, args.precision, None)]
     res = {key: run_gpu(grid, y0, prec, args, params, path) for key, prec, path in runs}
@@ -455,10 +479,13 @@ def main():
         "vs_baseline": None, "dtype": "f32" if head_p == "fp32_bulk" else "f64",
         "data": "synthetic: seeded streamfunction state (l<=8 harmonics, max|u|=30 m/s, linear balance, "
                 "N(0,5 m) noise), Earth parameters on the unit sphere",
-        "config": {"workload": f"SCVT hexagon level {args.level}, TRiSK PV flux, cell KE, RK4 dt={args.dt} s",
+        "config": {"workload": f"SCVT hexagon level {args.level}, {args.pv_flux} PV flux, {args.kinetic_energy} KE, "
+                               f"RK4 dt={args.dt} s",
                    "cells": grid.num_cells(), "edges": grid.num_edges(), "vertices": grid.num_vertices(),
-                   "precision": head_p, "model_bytes_per_cell_step": MODEL_BYTES[head_p]},
-        "hbm_frac_model": head["hbm_frac_model"],
+                   "precision": head_p,
+                   "model_bytes_per_cell_step": MODEL_BYTES[head_p] if trisk_cell
+                   else head["kernel_model_bytes_per_cell_step"]},
+        "hbm_frac_model": head["hbm_frac_model"] if trisk_cell else head["hbm_frac_kernel_model"],
         "roofline": head.get("roofline"),
         "cpu_baseline": cpu,
     }
@@ -469,10 +496,10 @@ def main():
         line["speedup_fp32_bulk_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
     if "fp64_pair" in res and "fp32_bulk" in res:
         line["speedup_fp32_bulk_vs_fp64_pair"] = res["fp32_bulk"]["value"] / res["fp64_pair"]["value"]
-    if not args.no_ensemble:
+    if not args.no_ensemble and trisk_cell:
         log("[bench] batched ensemble (levels 4-7)")
         line["ensemble"] = run_ensemble(args)
-    if not args.no_adjoint:
+    if not args.no_adjoint and trisk_cell:
         log("[bench] adjoint (level 7, 40 steps)")
         line["adjoint"] = run_adjoint(args)
     print(json.dumps(line), flush=True)

@@ -1032,9 +1032,18 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         double w3 = ne * (2 * c + ew + c + rk_e) + nc * 8.0;    /* half sums, cells, d, RK4, B */
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
-        double p1 = nc * (KC * (iw + c) + 8.0 + 8.0) + ne * uw + nv * (KV * 2 * (iw + c) + c + c);
-        double p2 = ne * (2 * iw + 2 * c + 2 * iw + uw + 2 * c) + nc * 8.0 + nv * c;
-        double p3 = ne * (KT * (iw + c) + 2 * iw + c + 2 * c + rk_e) + nc * (KC * (iw + 8.0) + 8.0 + rk_c);
+        // Gassmann: zeta_v, h_v written in P1 and gathered in P2 with the rhombus weights and
+        // f at the edges; 12-wide edge, PV-edge (not for the mean) and weight tables in P3.
+        // Vertex kinetic energy: K_v and its weights in P1, the vertex part of h_e in P2, and
+        // the cells' vertex part of K and B (kites, corners, K_v gathers) in P2.
+        const bool vk = L.vertex_ke, gm = L.gassmann;
+        const double KG = L.kg;
+        double p1 = nc * (KC * (iw + c) + 8.0 + 8.0 + (vk ? 8.0 : 0.0)) + ne * uw +
+                    nv * (KV * 2 * (iw + c) + c + c + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
+        double p2 = ne * (2 * iw + 2 * c + 2 * iw + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
+                    nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + iw) + 32.0) + nv * c : 0.0);
+        const double pv = gm ? KG * (iw + c + (L.gassmann_mean ? 0.0 : iw)) : KT * (iw + c);
+        double p3 = ne * (pv + 2 * iw + c + 2 * c + rk_e) + nc * (KC * (iw + 8.0) + 8.0 + rk_c);
         v = {{kKernelNames[K_CELL_VERTEX], p1}, {kKernelNames[K_EDGE], p2}, {kKernelNames[K_UPDATE], p3}};
     }
     int k = 0;
