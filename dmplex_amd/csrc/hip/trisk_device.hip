@@ -229,12 +229,13 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
     }
 }
 
-// W2: cells: PV-flux half sums of their edges and the continuity with its RK4 update
+// W2: cells: PV-flux half sums of their edges and the continuity with its RK4 update (XCD x
+// takes the x-th eighth of the cells, so the G and q_v its tiles share stay in its L2)
 template <class P>
-__global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
+__global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, int64_t t0) {
     using C = typename P::C;
     const int64_t nc = p.nc;
-    int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t i = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;
     if (i >= nc) return;
     int32_t meta, E[6], V[6];
     C W[15];
@@ -262,11 +263,11 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r) {
 
 // W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update
 template <class P>
-__global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r) {
+__global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
     using U = typename P::U;
-    int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t e = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;
     if (e >= p.ne) return;
     int32_t c[2];
     I hs[2];
@@ -493,15 +494,15 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     }
     {
         if ((rc = timed_begin(H, K_PAIR_CELL, s, &stop))) return rc;
-        hipLaunchKernelGGL((pair_cell_kernel<P>), dim3(unsigned((L.nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p,
-                           r);
+        const int64_t t0 = tiles_per_xcd(L.nc);
+        hipLaunchKernelGGL((pair_cell_kernel<P>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
     {
         if ((rc = timed_begin(H, K_PAIR_EDGE, s, &stop))) return rc;
-        hipLaunchKernelGGL((pair_edge_kernel<P>), dim3(unsigned((L.ne + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, p,
-                           r);
+        const int64_t t0 = tiles_per_xcd(L.ne);
+        hipLaunchKernelGGL((pair_edge_kernel<P>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
