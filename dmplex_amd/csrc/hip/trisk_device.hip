@@ -41,9 +41,29 @@ int set_err(int code, const std::string& msg) {
 int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg); }
 
 // ---------------------------------------------------------------------------------
+// Stencil widths of the table kernels: compile-time for hexagon meshes (6 cell edges, 3
+// vertex edges, 10 TRiSK and 12 Gassmann terms), so that every gather loop unrolls and its
+// table loads issue together, or the handle's runtime widths.  The summation order is the
+// same either way.
+
+template <int KC, int KV, int KT, int KG>
+struct Widths {
+    template <class P>
+    __device__ static int kc(const Params<P>& p) { return KC ? KC : p.kc; }
+    template <class P>
+    __device__ static int kv(const Params<P>& p) { return KV ? KV : p.kv; }
+    template <class P>
+    __device__ static int kt(const Params<P>& p) { return KT ? KT : p.kt; }
+    template <class P>
+    __device__ static int kg(const Params<P>& p) { return KG ? KG : p.kg; }
+};
+using AnyWidths = Widths<0, 0, 0, 0>;
+using HexWidths = Widths<6, 3, 10, 12>;
+
+// ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
 
-template <class P, class UIn, class HIn, bool VK, bool ZH>
+template <class P, class UIn, class HIn, bool VK, bool ZH, class W = AnyWidths>
 __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const UIn* __restrict__ u,
                                                              const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -53,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
         C s = 0;
-        for (int k = 0; k < p.kc; ++k) {
+        for (int k = 0; k < W::kc(p); ++k) {
             C ue = C(u[ld(p.ce + k * p.nc + i)]);
             s += ld(p.kin + k * p.nc + i) * (ue * ue);
         }
@@ -70,8 +90,8 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
         C z = 0, eta = 0;
-        for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
-        for (int k = 0; k < p.kv; ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
+        for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
+        for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
         C q = (z + ld(p.fv + v)) / eta;
         p.qv[v] = I(q);
         if (ZH) {
@@ -85,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         }
         if (VK) {
             C s = 0;
-            for (int k = 0; k < p.kv; ++k) {
+            for (int k = 0; k < W::kv(p); ++k) {
                 C ue = C(u[ld(p.ve + k * p.nv + v)]);
                 s += ld(p.vkw + k * p.nv + v) * (ue * ue);
             }
@@ -97,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
 // ---------------------------------------------------------------------------------
 // P2: edges (F, q_e) | cells (vertex part of K, B; only with vertex kinetic energy)
 
-template <class P, class UIn, class HIn, bool VK, bool GM>
+template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths>
 __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __restrict__ u,
                                                       const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -126,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
         C s = 0;
-        for (int k = 0; k < p.kc; ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[ld(p.cv + k * p.nc + i)]);
+        for (int k = 0; k < W::kc(p); ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[ld(p.cv + k * p.nc + i)]);
         double K = p.K[i] + double(s);
         p.B[i] = p.B[i] + double(s);
         p.K[i] = K;
@@ -137,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
 // ---------------------------------------------------------------------------------
 // P3: edges (du) | cells (dh), then the RK4 stage update
 
-template <class P, bool GM, bool GMEAN>
+template <class P, bool GM, bool GMEAN, class W = AnyWidths>
 __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
@@ -151,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         if (!GM) {
             C qe = C(p.Fq[e].q);
             C s = 0;
-            for (int k = 0; k < p.kt; ++k) {
+            for (int k = 0; k < W::kt(p); ++k) {
                 Pair<I> f = p.Fq[ld(p.te + k * p.ne + e)];
                 s += ld(p.tw + k * p.ne + e) * C(f.F) * (qe + C(f.q));
             }
@@ -159,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         } else {
             C qe = C(p.Fq[e].q);
             C s = 0;
-            for (int k = 0; k < p.kg; ++k) {
+            for (int k = 0; k < W::kg(p); ++k) {
                 Pair<I> f = p.Fq[ld(p.ge + k * p.ne + e)];
                 C q = GMEAN ? (qe + C(f.q)) / 2 : C(p.Fq[ld(p.gp + k * p.ne + e)].q);
                 s += ld(p.gw + k * p.ne + e) * C(f.F) * q;
@@ -171,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
         double s = 0.0;
-        for (int k = 0; k < p.kc; ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[ld(p.ce + k * p.nc + i)].F);
+        for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[ld(p.ce + k * p.nc + i)].F);
         rk_update<double>(r, p.ne + i, -s, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
@@ -509,11 +529,10 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     return 0;
 }
 
-template <class P, class UIn>
-static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
+template <class P, class UIn, class W>
+static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
     const Params<P>& p = params<P>(H);
     const Layout& L = H->L;
-    if (L.pair && p.qv_out == nullptr) return run_pair<P, UIn>(H, u, h, r, s);
     const bool vk = L.vertex_ke, gm = L.gassmann;
     const bool zh = gm || vk || p.qv_out != nullptr;
     hipEvent_t stop;
@@ -524,13 +543,13 @@ static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_CELL_VERTEX, s, &stop))) return rc;
         if (vk)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0,
                                t1);
         else if (zh)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true>), grid, dim3(kBlock), 0, s, p, u, h,
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W>), grid, dim3(kBlock), 0, s, p, u, h,
                                t0, t1);
         else
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, false>), grid, dim3(kBlock), 0, s, p, u, h,
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, false, W>), grid, dim3(kBlock), 0, s, p, u, h,
                                t0, t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
@@ -541,13 +560,13 @@ static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_EDGE, s, &stop))) return rc;
         if (vk && gm)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
         else if (vk)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
         else if (gm)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, true>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
         else
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, false>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, false, W>), grid, dim3(kBlock), 0, s, p, u, h, t0,
                                t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
@@ -558,15 +577,24 @@ static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_UPDATE, s, &stop))) return rc;
         if (!gm)
-            hipLaunchKernelGGL((update_kernel<P, false, false>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+            hipLaunchKernelGGL((update_kernel<P, false, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
         else if (L.gassmann_mean)
-            hipLaunchKernelGGL((update_kernel<P, true, true>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+            hipLaunchKernelGGL((update_kernel<P, true, true, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
         else
-            hipLaunchKernelGGL((update_kernel<P, true, false>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+            hipLaunchKernelGGL((update_kernel<P, true, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
     return 0;
+}
+
+template <class P, class UIn>
+static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
+    const Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    if (L.pair && p.qv_out == nullptr) return run_pair<P, UIn>(H, u, h, r, s);
+    const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
+    return hex ? run_widths<P, UIn, HexWidths>(H, u, h, r, s) : run_widths<P, UIn, AnyWidths>(H, u, h, r, s);
 }
 
 template <class P>
