@@ -435,7 +435,7 @@ static int setup_params(trisk_handle* H) {
         rc |= coef(L.re_lh, &p.re_lh);
         rc |= upload(o, L.rc_pce, &p.ce6);
         rc |= upload(o, L.ccv, &p.ccv);
-        rc |= coef(L.pw, &p.pw);
+        rc |= coef(L.pr, &p.pr);
     }
     if (rc) return rc;
     void* q;
@@ -1055,7 +1055,7 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         double w1 = nc * (rec8 + 6 * c + 8.0 + 8.0)             /* edges, KE weights, h, B */
                     + nv * (rec8 + vc + c)                       /* indices, coefficients, q_v */
                     + ne * (ew + 2 * c + uw + c);                /* cells, l h weights, u, G */
-        double w2 = nc * (cw + 15 * c + iw + 8.0 + rk_c)        /* edges, corners, weights, meta, A, RK4 */
+        double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
                     + ne * c + nv * c                            /* G, q_v */
                     + 2 * ne * c;                                /* half sums */
         double w3 = ne * (2 * c + ew + c + rk_e) + nc * 8.0;    /* half sums, cells, d, RK4, B */

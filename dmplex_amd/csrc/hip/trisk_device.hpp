@@ -107,7 +107,7 @@ struct Params {
     // cell-pair path
     const int32_t* cmeta;  // nc: size | negative-sign bits << 4
     const int32_t* ccv;    // 6 x nc corners
-    const C* pw;           // 15 x nc pair weights
+    const C* pr;           // 6 x nc kite fractions (the pair weights are rebuilt from them)
     const C* len;          // ne: l_e
     const C* dual;         // ne: d_e
     I* G;                  // ne: l_e F_e
@@ -313,7 +313,8 @@ __device__ __forceinline__ void load_edge_cells(const Params<P>& p, int64_t e, i
         ldrec(p.re_cells + 2 * e, c);
 }
 
-// cell i: size and side bits, edges and corners (6 slots, pads valid), pair weights
+// cell i: size and side bits, edges and corners (6 slots, pads valid), pair weights (those of
+// slots beyond the size are not used)
 template <bool NARROW, class P>
 __device__ __forceinline__ void load_cell_pairs(const Params<P>& p, int64_t i, int32_t& meta, int32_t (&E)[6],
                                                 int32_t (&V)[6], typename P::C (&W)[15]) {
@@ -341,8 +342,24 @@ __device__ __forceinline__ void load_cell_pairs(const Params<P>& p, int64_t i, i
             V[k] = ld(p.ccv + k * nc + i);
         }
     }
+    // w_{k,k'} = s_k s_k' (1/2 - sum_{j=k+1..k'} R_j) (TRiSKGrid.jl:143-155), in the order and
+    // association of the host construction, so FP64 weights are bitwise the host's
+    using C = typename P::C;
+    C R[6];
 #pragma unroll
-    for (int q = 0; q < 15; ++q) W[q] = ld(p.pw + q * nc + i);
+    for (int k = 0; k < 6; ++k) R[k] = ld(p.pr + k * nc + i);
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const C sk = (meta >> (4 + k)) & 1 ? C(-1) : C(1);
+        C S = 0;
+#pragma unroll
+        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
+            S += R[k2];
+            const C s2 = (meta >> (4 + k2)) & 1 ? C(-1) : C(1);
+            W[q] = sk * s2 * (C(0.5) - S);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------

@@ -966,7 +966,7 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
         const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
         b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + ne * (ew + 2 * c) +
                 M * (nc * (8.0 + 8.0) + nv * c + ne * (uw + c));
-        b[1] += nc * (cw + 15 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 2 * ne * c);
+        b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 2 * ne * c);
         b[2] += ne * (ew + c) + M * (ne * (2 * c + rk_e) + nc * 8.0);
     }
     int k = 0;

@@ -348,9 +348,12 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.cmeta.resize(nc);
     L.ccv.assign(size_t(kPairSlots) * nc, 0);
     L.pw.assign(size_t(kPairs) * nc, 0.0);
+    L.pr.assign(size_t(kPairSlots) * nc, 0.0);
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < nc; ++j) {
         int64_t i = L.cperm[j];
+        for (int k = 0; k < size[i]; ++k)
+            L.pr[size_t(k) * nc + j] = d->cell_kites[k + KC * i] / d->cell_areas[i];
         L.cmeta[j] = size[i] | (neg[i] << 4);
         for (int k = 0; k < kPairSlots; ++k) {
             int64_t v = k < size[i] ? corner[k + size_t(KC) * i] : corner[size_t(KC) * i];
