@@ -48,6 +48,7 @@ int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg);
 
 template <int KC, int KV, int KT, int KG>
 struct Widths {
+    static constexpr int cw = KC, vw = KV, tw = KT, gw = KG;
     template <class P>
     __device__ static int kc(const Params<P>& p) { return KC ? KC : p.kc; }
     template <class P>
@@ -59,6 +60,40 @@ struct Widths {
 };
 using AnyWidths = Widths<0, 0, 0, 0>;
 using HexWidths = Widths<6, 3, 10, 12>;
+
+// Slot k of entity j of a slot-major index table.  With a compile-time width K the row is
+// decoded up front from the 16-bit deltas packed two per word (base + delta; the int32 row
+// when word 0 holds kWide), so all its loads issue together; with K = 0 each slot is read
+// from the int32 table when used.
+template <int K>
+struct Index {
+    int32_t v[K > 0 ? K : 1];
+    const int32_t* t32;
+    int64_t n, j;
+    __device__ __forceinline__ Index(const int32_t* p16, const int32_t* t32_, int64_t n_, int64_t j_, int64_t base)
+        : t32(t32_), n(n_), j(j_) {
+        if constexpr (K > 0) {
+            constexpr int W = (K + 1) / 2;
+            int32_t w[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) w[q] = ld(p16 + q * n + j);
+            if (int16_t(w[0] & 0xffff) == kWide) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[k] = ld(t32 + k * n + j);
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    v[k] = int32_t(base) + (k % 2 ? int16_t(uint32_t(w[k / 2]) >> 16) : int16_t(w[k / 2] & 0xffff));
+            }
+        }
+    }
+    __device__ __forceinline__ int32_t operator[](int k) const {
+        if constexpr (K > 0)
+            return v[k];
+        else
+            return ld(t32 + k * n + j);
+    }
+};
 
 // ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
@@ -72,9 +107,10 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
     if (t.range == 0) {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
+        const Index<W::cw> E(p.ce16p, p.ce, p.nc, i, 3 * i);
         C s = 0;
         for (int k = 0; k < W::kc(p); ++k) {
-            C ue = C(u[ld(p.ce + k * p.nc + i)]);
+            C ue = C(u[E[k]]);
             s += ld(p.kin + k * p.nc + i) * (ue * ue);
         }
         double hi = double(h[i]);
@@ -89,9 +125,10 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
     } else {
         int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
+        const Index<W::vw> VE(p.ve16p, p.ve, p.nv, v, (3 * v) / 2), VC(p.vc16p, p.vc, p.nv, v, v / 2);
         C z = 0, eta = 0;
-        for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
-        for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
+        for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[VE[k]]);
+        for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[VC[k]]);
         C q = (z + ld(p.fv + v)) / eta;
         p.qv[v] = I(q);
         if (ZH) {
@@ -106,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         if (VK) {
             C s = 0;
             for (int k = 0; k < W::kv(p); ++k) {
-                C ue = C(u[ld(p.ve + k * p.nv + v)]);
+                C ue = C(u[VE[k]]);
                 s += ld(p.vkw + k * p.nv + v) * (ue * ue);
             }
             p.Kv[v] = I(s);
@@ -126,8 +163,8 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
     if (t.range == 0) {
         int64_t e = t.first + threadIdx.x;
         if (e >= p.ne) return;
-        int32_t c0 = ld(p.ec + e), c1 = ld(p.ec + p.ne + e);
-        int32_t v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
+        const Index<W::cw ? 2 : 0> EC(p.ec16p, p.ec, p.ne, e, e / 3), EV(p.ev16p, p.ev, p.ne, e, (2 * e) / 3);
+        const int32_t c0 = EC[0], c1 = EC[1], v0 = EV[0], v1 = EV[1];
         C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
         if (VK) he = he + ld(p.vkh + e) * C(p.hv[v0]) + ld(p.vkh + p.ne + e) * C(p.hv[v1]);
         C F = he * C(u[e]);
@@ -145,8 +182,9 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
         if (!VK) return;
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
+        const Index<W::cw> CV(p.cv16p, p.cv, p.nc, i, 2 * i);
         C s = 0;
-        for (int k = 0; k < W::kc(p); ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[ld(p.cv + k * p.nc + i)]);
+        for (int k = 0; k < W::kc(p); ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[CV[k]]);
         double K = p.K[i] + double(s);
         p.B[i] = p.B[i] + double(s);
         p.K[i] = K;
@@ -167,21 +205,25 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         int64_t e = t.first + threadIdx.x;
         if (e >= p.ne) return;
         double du;
-        double grad = (p.B[ld(p.ec + p.ne + e)] - p.B[ld(p.ec + e)]) * double(ld(p.invd + e));
+        const Index<W::cw ? 2 : 0> EC(p.ec16p, p.ec, p.ne, e, e / 3);
+        double grad = (p.B[EC[1]] - p.B[EC[0]]) * double(ld(p.invd + e));
         if (!GM) {
+            const Index<W::tw> TE(p.te16p, p.te, p.ne, e, e);
             C qe = C(p.Fq[e].q);
             C s = 0;
             for (int k = 0; k < W::kt(p); ++k) {
-                Pair<I> f = p.Fq[ld(p.te + k * p.ne + e)];
+                Pair<I> f = p.Fq[TE[k]];
                 s += ld(p.tw + k * p.ne + e) * C(f.F) * (qe + C(f.q));
             }
             du = double(s / 2) - grad;
         } else {
+            const Index<W::gw> GE(p.ge16p, p.ge, p.ne, e, e);
+            const Index<GMEAN ? 0 : W::gw> GP(p.gp16p, p.gp, p.ne, e, e);
             C qe = C(p.Fq[e].q);
             C s = 0;
             for (int k = 0; k < W::kg(p); ++k) {
-                Pair<I> f = p.Fq[ld(p.ge + k * p.ne + e)];
-                C q = GMEAN ? (qe + C(f.q)) / 2 : C(p.Fq[ld(p.gp + k * p.ne + e)].q);
+                Pair<I> f = p.Fq[GE[k]];
+                C q = GMEAN ? (qe + C(f.q)) / 2 : C(p.Fq[GP[k]].q);
                 s += ld(p.gw + k * p.ne + e) * C(f.F) * q;
             }
             du = double(s) - grad;
@@ -190,8 +232,9 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
     } else {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
+        const Index<W::cw> CE(p.ce16p, p.ce, p.nc, i, 3 * i);
         double s = 0.0;
-        for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[ld(p.ce + k * p.nc + i)].F);
+        for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
         rk_update<double>(r, p.ne + i, -s, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
@@ -418,6 +461,15 @@ static int setup_params(trisk_handle* H) {
     rc |= coef(L.fv, &p.fv);
     rc |= coef(L.vkw, &p.vkw);
     rc |= upload(o, L.area_v, &p.area_v);
+    rc |= upload(o, L.ce16p, &p.ce16p);
+    rc |= upload(o, L.cv16p, &p.cv16p);
+    rc |= upload(o, L.ec16p, &p.ec16p);
+    rc |= upload(o, L.ev16p, &p.ev16p);
+    rc |= upload(o, L.te16p, &p.te16p);
+    rc |= upload(o, L.ge16p, &p.ge16p);
+    rc |= upload(o, L.gp16p, &p.gp16p);
+    rc |= upload(o, L.ve16p, &p.ve16p);
+    rc |= upload(o, L.vc16p, &p.vc16p);
     if (L.pair) {
         rc |= upload(o, L.cmeta, &p.cmeta);
         rc |= coef(L.dual, &p.dual);
@@ -1065,14 +1117,17 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         // f at the edges; 12-wide edge, PV-edge (not for the mean) and weight tables in P3.
         // Vertex kinetic energy: K_v and its weights in P1, the vertex part of h_e in P2, and
         // the cells' vertex part of K and B (kites, corners, K_v gathers) in P2.
+        // On hexagon meshes the index tables are read as 16-bit deltas (2 B per index).
         const bool vk = L.vertex_ke, gm = L.gassmann;
         const double KG = L.kg;
-        double p1 = nc * (KC * (iw + c) + 8.0 + 8.0 + (vk ? 8.0 : 0.0)) + ne * uw +
-                    nv * (KV * 2 * (iw + c) + c + c + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
-        double p2 = ne * (2 * iw + 2 * c + 2 * iw + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
-                    nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + iw) + 32.0) + nv * c : 0.0);
-        const double pv = gm ? KG * (iw + c + (L.gassmann_mean ? 0.0 : iw)) : KT * (iw + c);
-        double p3 = ne * (pv + 2 * iw + c + 2 * c + rk_e) + nc * (KC * (iw + 8.0) + 8.0 + rk_c);
+        const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
+        const double ix = hex ? 2.0 : iw;
+        double p1 = nc * (KC * (ix + c) + 8.0 + 8.0 + (vk ? 8.0 : 0.0)) + ne * uw +
+                    nv * (KV * 2 * (ix + c) + c + c + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
+        double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
+                    nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 32.0) + nv * c : 0.0);
+        const double pv = gm ? KG * (ix + c + (L.gassmann_mean ? 0.0 : ix)) : KT * (ix + c);
+        double p3 = ne * (pv + 2 * ix + c + 2 * c + rk_e) + nc * (KC * (ix + 8.0) + 8.0 + rk_c);
         v = {{kKernelNames[K_CELL_VERTEX], p1}, {kKernelNames[K_EDGE], p2}, {kKernelNames[K_UPDATE], p3}};
     }
     int k = 0;

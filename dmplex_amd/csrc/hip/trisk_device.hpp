@@ -120,6 +120,8 @@ struct Params {
     const int32_t* re_cells;  // ne x 2
     const C* re_lh;           // ne x 2
     const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
+    // 16-bit deltas of the slot-major index tables (table kernels; kWide: the int32 row)
+    const int32_t *ce16p, *cv16p, *ec16p, *ev16p, *te16p, *ge16p, *gp16p, *ve16p, *vc16p;
     // 16-bit deltas (sentinel kWide: read the int32 tables)
     const int16_t* rc_state16;  // nc x 8
     const int16_t* rv_idx16;    // nv x 8

@@ -97,6 +97,37 @@ void invert(const std::vector<int64_t>& perm, std::vector<int64_t>& inv) {
 
 }  // namespace
 
+// t16 = t32 - base(j) per slot (slot-major K x n), or kWide in slot 0 of an entity whose
+// deltas do not all fit; returns the number of such entities
+// K-slot table as 16-bit deltas packed two per int32, slot-major: word w of entity j at
+// w * n + j holds slots 2w (low half) and 2w + 1 (high half); kWide in the low half of word
+// 0 for an entity whose deltas do not all fit.
+template <class Base>
+static int64_t narrow_table(const std::vector<int32_t>& t32, int K, int64_t n, Base base, std::vector<int32_t>& out) {
+    const int words = (K + 1) / 2;
+    out.assign(size_t(words) * n, 0);
+    int64_t wide = 0;
+#pragma omp parallel for schedule(static) reduction(+ : wide)
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t b = base(j);
+        bool ok = true;
+        for (int k = 0; k < K && ok; ++k) {
+            const int64_t dlt = int64_t(t32[size_t(k) * n + j]) - b;
+            ok = dlt >= -32767 && dlt <= 32767;
+        }
+        if (!ok) {
+            out[j] = int32_t(uint32_t(uint16_t(kWide)));
+            ++wide;
+            continue;
+        }
+        for (int k = 0; k < K; ++k) {
+            const uint32_t h = uint16_t(int16_t(int64_t(t32[size_t(k) * n + j]) - b));
+            out[size_t(k / 2) * n + j] |= int32_t(k % 2 ? h << 16 : h);
+        }
+    }
+    return wide;
+}
+
 bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     if (!d) {
         err = "null model descriptor";
@@ -229,6 +260,19 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
         remap_value(d->g_rhombus, 2, ne, L.eperm, L.grh);
         remap_value(d->g_coriolis, 1, ne, L.eperm, L.gf);
     }
+    int64_t w = 0;
+    w += narrow_table(L.ce, KC, nc, [](int64_t j) { return 3 * j; }, L.ce16p);
+    w += narrow_table(L.ec, 2, ne, [](int64_t j) { return j / 3; }, L.ec16p);
+    w += narrow_table(L.ev, 2, ne, [](int64_t j) { return (2 * j) / 3; }, L.ev16p);
+    w += narrow_table(L.te, KT, ne, [](int64_t j) { return j; }, L.te16p);
+    w += narrow_table(L.ve, KV, nv, [](int64_t j) { return (3 * j) / 2; }, L.ve16p);
+    w += narrow_table(L.vc, KV, nv, [](int64_t j) { return j / 2; }, L.vc16p);
+    if (vk) w += narrow_table(L.cv, KC, nc, [](int64_t j) { return 2 * j; }, L.cv16p);
+    if (gm) {
+        w += narrow_table(L.ge, KG, ne, [](int64_t j) { return j; }, L.ge16p);
+        w += narrow_table(L.gp, KG, ne, [](int64_t j) { return j; }, L.gp16p);
+    }
+    L.wide_table_entities = w;
     return true;
 }
 

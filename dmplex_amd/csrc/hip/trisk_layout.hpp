@@ -27,6 +27,13 @@ struct Layout {
     // vertices
     std::vector<int32_t> ve, vc;
     std::vector<double> curl, vh, fv, vkw, area_v;
+    // the index tables again as 16-bit deltas from a linear base of the entity (cell i: 3i
+    // for edges, 2i for vertices; vertex v: 3v/2 for edges, v/2 for cells; edge e: e/3 for
+    // cells, 2e/3 for vertices, e for its TRiSK and Gassmann edges), two per int32 word
+    // (slots 2w | 2w+1 << 16), slot-major by word; an entity with any delta outside
+    // +-32767 has kWide in the low half of word 0
+    std::vector<int32_t> ce16p, cv16p, ec16p, ev16p, te16p, ge16p, gp16p, ve16p, vc16p;
+    int64_t wide_table_entities = 0;
     // cell-pair path (TRiSK PV flux and cell kinetic energy only): per cell the 15
     // antisymmetric TRiSK weights w_{k,k'} (k < k') of its edge pairs
     bool pair = false;
