@@ -1100,7 +1100,7 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
     const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;  // y, acc r/w, stage
     std::vector<std::pair<const char*, double>> v;
     if (L.pair) {
-        const double vc = f64 ? 7 * c : 8 * c;                  /* FP32: 8-wide vertex records */
+        const double vc = f64 ? 7 * c : 6 * c;                  /* FP32: f rides in the index record */
         const double rec8 = f64 ? 6 * iw : 16.0;                 /* FP32: 16-bit deltas, 8-wide records */
         const double ew = f64 ? 2 * iw : 4.0;                    /* edge cells */
         const double cw = f64 ? 12 * iw : 24.0;                  /* cell edges and corners */

@@ -116,7 +116,7 @@ struct Params {
     const int32_t* rc_state;  // nc x 6
     const C* rc_kin;          // nc x 6
     const int32_t* rv_idx;    // nv x 6
-    const C* rv_coef;         // nv x 8
+    const C* rv_coef;         // nv x 6 (FP32-bulk)
     const int32_t* re_cells;  // ne x 2
     const C* re_lh;           // ne x 2
     const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
@@ -279,7 +279,7 @@ __device__ __forceinline__ void load_cell_state(const Params<P>& p, int64_t i, i
 template <bool NARROW, class P>
 __device__ __forceinline__ void load_vertex_record(const Params<P>& p, int64_t v, int32_t (&idx)[6],
                                                    typename P::C (&cf)[8]) {
-    if (NARROW) {
+    if constexpr (NARROW) {  // FP32 records: indices and f in 16 bytes, curl and vertex heights in 24
         int16_t d[8];
         ldrec(p.rv_idx16 + 8 * v, d);
         if (d[0] == kWide) {
@@ -292,7 +292,13 @@ __device__ __forceinline__ void load_vertex_record(const Params<P>& p, int64_t v
                 idx[3 + k] = b1 + d[3 + k];
             }
         }
-        ldrec(p.rv_coef + 8 * v, cf);
+        typename P::C c6[6];
+        ldrec(p.rv_coef + 6 * v, c6);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cf[k] = c6[k];
+        static_assert(sizeof(typename P::C) == 4, "the records hold FP32 coefficients");
+        __builtin_memcpy(&cf[6], &d[6], 4);
+        cf[7] = 0;
     } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {

@@ -958,7 +958,7 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
     const bool f64 = e->precision == TRISK_FP64;
     const double c = f64 ? 8.0 : 4.0, uw = c, iw = 4.0;
     const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;
-    const double vc = f64 ? 7 * c : 8 * c, rec8 = f64 ? 6 * iw : 16.0, ew = f64 ? 2 * iw : 4.0,
+    const double vc = f64 ? 7 * c : 6 * c, rec8 = f64 ? 6 * iw : 16.0, ew = f64 ? 2 * iw : 4.0,
                  cw = f64 ? 12 * iw : 24.0;
     double b[kEnsKinds] = {0, 0, 0};
     for (int l = 0; l < e->nlevels; ++l) {

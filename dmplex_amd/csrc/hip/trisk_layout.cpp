@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace trisk {
@@ -432,16 +433,15 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     }
     const int64_t nv = L.nv;
     L.rv_idx.assign(size_t(nv) * 6, 0);
-    L.rv_coef.assign(size_t(nv) * 8, 0.0);
+    L.rv_coef.assign(size_t(nv) * 6, 0.0);
 #pragma omp parallel for schedule(static)
     for (int64_t v = 0; v < nv; ++v) {
         for (int k = 0; k < 3; ++k) {
             L.rv_idx[6 * v + k] = L.ve[size_t(k) * nv + v];
             L.rv_idx[6 * v + 3 + k] = L.vc[size_t(k) * nv + v];
-            L.rv_coef[8 * v + k] = L.curl[size_t(k) * nv + v];
-            L.rv_coef[8 * v + 3 + k] = L.vh[size_t(k) * nv + v];
+            L.rv_coef[6 * v + k] = L.curl[size_t(k) * nv + v];
+            L.rv_coef[6 * v + 3 + k] = L.vh[size_t(k) * nv + v];
         }
-        L.rv_coef[8 * v + 6] = L.fv[v];
     }
     L.re_cells.assign(size_t(ne) * 2, 0);
     L.re_lh.assign(size_t(ne) * 2, 0.0);
@@ -482,6 +482,12 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
         bool ok = true;
         for (int k = 0; k < 3; ++k)
             ok = ok && fits(int64_t(L.rv_idx[6 * v + k]) - (3 * v) / 2) && fits(int64_t(L.rv_idx[6 * v + 3 + k]) - v / 2);
+        // slots 6, 7: the bits of f as FP32 (the records serve the FP32-bulk kernels)
+        const float f32 = float(L.fv[v]);
+        uint32_t fb;
+        std::memcpy(&fb, &f32, 4);
+        L.rv_idx16[8 * v + 6] = int16_t(uint16_t(fb & 0xffff));
+        L.rv_idx16[8 * v + 7] = int16_t(uint16_t(fb >> 16));
         if (!ok) {
             L.rv_idx16[8 * v] = kWide;
             ++wv;

@@ -48,7 +48,7 @@ struct Layout {
     std::vector<int32_t> rc_state;       // nc x 6: cell edges
     std::vector<double> rc_kin;          // nc x 6: kinetic-energy weights
     std::vector<int32_t> rv_idx;         // nv x 6: vertex edges, vertex cells
-    std::vector<double> rv_coef;         // nv x 8: curl (3), vertex heights (3), f, 0
+    std::vector<double> rv_coef;         // nv x 6: curl (3), vertex heights (3)
     std::vector<int32_t> re_cells;       // ne x 2: left, right cell
     std::vector<double> re_lh;           // ne x 2: l_e * edge_height
     std::vector<int32_t> rc_pce;         // 6 x nc: cell edges, pads the first edge
@@ -56,7 +56,7 @@ struct Layout {
     // 3v/2, cells v/2; edge e: cells e/3); an entity with a delta outside +-32767 has the
     // sentinel -32768 in its first slot and is read from the int32 tables instead
     std::vector<int16_t> rc_state16;     // nc x 8: edges (6), 0, 0
-    std::vector<int16_t> rv_idx16;       // nv x 8: edges (3), cells (3), 0, 0
+    std::vector<int16_t> rv_idx16;       // nv x 8: edges (3), cells (3), the FP32 bits of f (2)
     std::vector<int32_t> re_cells16;     // ne: left | right << 16
     std::vector<int32_t> pc_ev16;        // 6 x nc: edge | corner << 16
     int64_t wide_cells = 0, wide_vertices = 0, wide_edges = 0;
