@@ -286,6 +286,7 @@ def run_adjoint(args):
     grads = {}
     for prec in ("fp64", "fp32_bulk"):
         m = TRiSKShallowWater(dm, centers=dm.generators, path="pair", backend=HIPBackend(0, prec), **params)
+        m.use_graphs(True)  # forward steps and the adjoint sweep replay captured graphs
         y0 = m.upload_state(seeded_state(m.grid))
         hstar = m.upload_state(seeded_state(m.grid, seed_coef=3, seed_noise=4))[m.grid.num_edges():].clone()
         y = y0.clone()

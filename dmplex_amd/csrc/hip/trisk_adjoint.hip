@@ -229,16 +229,51 @@ __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjPara
     adj_update(r, p.ne + i, s);
 }
 
+// the arguments a captured adjoint sweep bakes in
+struct SweepKey {
+    const double* y0 = nullptr;
+    const double* ybarT = nullptr;
+    double* ybar0 = nullptr;
+    double* yT = nullptr;
+    double dt = 0;
+    int64_t N = -1, ck = -1;
+    hipStream_t stream = nullptr;
+    bool operator==(const SweepKey& o) const {
+        return y0 == o.y0 && ybarT == o.ybarT && ybar0 == o.ybar0 && yT == o.yT && dt == o.dt && N == o.N &&
+               ck == o.ck && stream == o.stream;
+    }
+};
+
 // per-handle adjoint state
 struct AdjState {
     std::vector<DevBuf> owned;
     AdjParams<Fp64> a64{};
     AdjParams<Fp32Bulk> a32{};
+    // work of trisk_rk4_adjoint, kept (and grown) across calls so a captured sweep stays valid
+    std::vector<DevBuf> work;
+    int64_t work_ck_states = 0, work_seg_states = 0;
+    double *ckp = nullptr, *seg = nullptr, *lam = nullptr, *acc = nullptr, *kbar = nullptr, *run = nullptr;
+    void* su[3] = {nullptr, nullptr, nullptr};
+    double* sh[3] = {nullptr, nullptr, nullptr};
+    // the sweep of the last graph-replayed call
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SweepKey key;
 };
+
+static void drop_sweep(AdjState* st) {
+    if (st->exec) (void)hipGraphExecDestroy(st->exec);
+    if (st->graph) (void)hipGraphDestroy(st->graph);
+    st->exec = nullptr;
+    st->graph = nullptr;
+    st->key = SweepKey{};
+}
 
 static void adj_release(trisk_handle* H) {
     auto* st = static_cast<AdjState*>(H->adj);
     if (!st) return;
+    drop_sweep(st);
+    for (auto& b : st->work) (void)hipFree(b.p);
     for (auto& b : st->owned) (void)hipFree(b.p);
     delete st;
     H->adj = nullptr;
@@ -328,54 +363,64 @@ static int tendency_vjp(trisk_handle* H, const double* y, const double* lam, dou
     return vjp<P, double>(H, y, y + H->L.ne, lam, 1.0, r, s);
 }
 
-struct Scratch {
-    std::vector<void*> bufs;
-    ~Scratch() {
-        for (void* b : bufs) (void)hipFree(b);
-    }
-    int get(size_t bytes, void** out) {
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return set_err(TRISK_ENOMEM, "hipMalloc of adjoint work");
-        bufs.push_back(p);
-        *out = p;
-        return 0;
-    }
-};
-
-// Checkpointed adjoint of nsteps RK4 steps: the forward run keeps the state every ck steps;
-// each segment is recomputed from its checkpoint, and each step's stage states again from
-// its input, before the step is transposed.
+// Work of a sweep with nck checkpoints and segments of up to seg states.
 template <class P>
-static int rk4_adjoint(trisk_handle* H, const double* y0, double dt, int64_t N, int64_t ck, const double* ybarT,
-                       double* ybar0, double* yT, hipStream_t s) {
+static int sweep_work(trisk_handle* H, AdjState* st, int64_t nck, int64_t seg) {
     using U = typename P::U;
-    int rc = adj_setup<P>(H);
-    if (rc) return rc;
+    if (nck <= st->work_ck_states && seg <= st->work_seg_states) return 0;
+    nck = std::max(nck, st->work_ck_states);
+    seg = std::max(seg, st->work_seg_states);
+    drop_sweep(st);
+    HIP_TRY(hipDeviceSynchronize());  // a replay in flight may still use the old work
+    for (auto& b : st->work) (void)hipFree(b.p);
+    st->work.clear();
+    st->work_ck_states = st->work_seg_states = 0;
+    const Layout& L = H->L;
+    const size_t sb = (L.ne + L.nc) * sizeof(double);
+    void* q;
+    int rc;
+    auto get = [&](size_t bytes) -> void* {
+        rc = alloc(st->work, bytes, &q);
+        return rc ? nullptr : q;
+    };
+    if (!(st->ckp = static_cast<double*>(get(nck * sb))) || !(st->seg = static_cast<double*>(get(seg * sb))) ||
+        !(st->lam = static_cast<double*>(get(sb))) || !(st->acc = static_cast<double*>(get(sb))) ||
+        !(st->kbar = static_cast<double*>(get(sb))) || !(st->run = static_cast<double*>(get(sb))))
+        return rc;
+    for (int k = 0; k < 3; ++k)
+        if (!(st->su[k] = get(L.ne * sizeof(U))) || !(st->sh[k] = static_cast<double*>(get(L.nc * sizeof(double)))))
+            return rc;
+    st->work_ck_states = nck;
+    st->work_seg_states = seg;
+    return 0;
+}
+
+// Issues the checkpointed adjoint of N RK4 steps on s: the forward run keeps the state
+// every ck steps; each segment is recomputed from its checkpoint, and each step's stage
+// states again from its input, before the step is transposed.  Nothing synchronises.
+template <class P>
+static int enqueue_sweep(trisk_handle* H, AdjState* st, const double* y0, double dt, int64_t N, int64_t ck,
+                         const double* ybarT, double* ybar0, double* yT, hipStream_t s) {
+    using U = typename P::U;
     const Layout& L = H->L;
     const int64_t n = L.ne + L.nc;
     const size_t sb = n * sizeof(double);
     const int64_t nck = N > 0 ? (N + ck - 1) / ck : 0;
-    Scratch m;
-    void *ckp, *seg, *lam, *acc, *kbar, *run, *su[3], *sh[3];
-    if ((rc = m.get(std::max<int64_t>(nck, 1) * sb, &ckp)) || (rc = m.get(std::min(ck, std::max<int64_t>(N, 1)) * sb, &seg)) ||
-        (rc = m.get(sb, &lam)) || (rc = m.get(sb, &acc)) || (rc = m.get(sb, &kbar)) || (rc = m.get(sb, &run)))
-        return rc;
-    for (int k = 0; k < 3; ++k)
-        if ((rc = m.get(L.ne * sizeof(U), &su[k])) || (rc = m.get(L.nc * sizeof(double), &sh[k]))) return rc;
-    double* C0 = static_cast<double*>(ckp);
-    double* S0 = static_cast<double*>(seg);
-    double* y = yT ? yT : static_cast<double*>(run);
+    int rc;
+    double* C0 = st->ckp;
+    double* S0 = st->seg;
+    double* y = yT ? yT : st->run;
     HIP_TRY(hipMemcpyAsync(y, y0, sb, hipMemcpyDeviceToDevice, s));
     for (int64_t k = 0; k < N; ++k) {
         if (k % ck == 0) HIP_TRY(hipMemcpyAsync(C0 + (k / ck) * n, y, sb, hipMemcpyDeviceToDevice, s));
         if ((rc = rk4_steps_direct(H, y, dt, 1, s))) return rc;
     }
-    double* la = static_cast<double*>(lam);
+    double* la = st->lam;
     HIP_TRY(hipMemcpyAsync(la, ybarT, sb, hipMemcpyDeviceToDevice, s));
     AdjRk a{};
     a.lam = la;
-    a.acc = static_cast<double*>(acc);
-    a.kbar = static_cast<double*>(kbar);
+    a.acc = st->acc;
+    a.kbar = st->kbar;
     a.dt = dt;
     Rk r{};
     r.acc = H->acc;
@@ -393,29 +438,70 @@ static int rk4_adjoint(trisk_handle* H, const double* y0, double dt, int64_t N, 
             double* yn = S0 + k * n;
             // stage states s2, s3, s4 of the step from yn (the forward's own arithmetic)
             r.y = yn;
-            for (int st = 0; st < 3; ++st) {
-                r.mode = st + 1;
-                r.su = su[st];
-                r.sh = static_cast<double*>(sh[st]);
-                const void* uin = st == 0 ? static_cast<const void*>(yn) : su[st - 1];
-                const double* hin = st == 0 ? yn + L.ne : static_cast<const double*>(sh[st - 1]);
-                if ((rc = pair_tendency_stage(H, uin, st == 0, hin, r, s))) return rc;
+            for (int i = 0; i < 3; ++i) {
+                r.mode = i + 1;
+                r.su = st->su[i];
+                r.sh = st->sh[i];
+                const void* uin = i == 0 ? static_cast<const void*>(yn) : st->su[i - 1];
+                const double* hin = i == 0 ? yn + L.ne : st->sh[i - 1];
+                if ((rc = pair_tendency_stage(H, uin, i == 0, hin, r, s))) return rc;
             }
             a.mode = 4;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(su[2]), static_cast<const double*>(sh[2]), la, dt / 6, a, s)))
-                return rc;
+            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[2]), st->sh[2], la, dt / 6, a, s))) return rc;
             a.mode = 3;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(su[1]), static_cast<const double*>(sh[1]), a.kbar, 1.0, a, s)))
-                return rc;
+            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[1]), st->sh[1], a.kbar, 1.0, a, s))) return rc;
             a.mode = 2;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(su[0]), static_cast<const double*>(sh[0]), a.kbar, 1.0, a, s)))
-                return rc;
+            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[0]), st->sh[0], a.kbar, 1.0, a, s))) return rc;
             a.mode = 1;
             if ((rc = vjp<P, double>(H, yn, yn + L.ne, a.kbar, 1.0, a, s))) return rc;
         }
     }
     HIP_TRY(hipMemcpyAsync(ybar0, la, sb, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));  // the scratch is freed on return
+    return 0;
+}
+
+// The adjoint sweep, replayed through one captured HIP graph when the handle has graphs on
+// (trisk_set_graphs) and the arguments match the last capture.
+template <class P>
+static int rk4_adjoint(trisk_handle* H, const double* y0, double dt, int64_t N, int64_t ck, const double* ybarT,
+                       double* ybar0, double* yT, hipStream_t s) {
+    int rc = adj_setup<P>(H);
+    if (rc) return rc;
+    auto* st = static_cast<AdjState*>(H->adj);
+    const int64_t nck = N > 0 ? (N + ck - 1) / ck : 0;
+    if ((rc = sweep_work<P>(H, st, std::max<int64_t>(nck, 1), std::min(ck, std::max<int64_t>(N, 1))))) return rc;
+    const bool graph = H->graphs && !H->timing && s != nullptr;
+    if (!graph) {
+        if ((rc = enqueue_sweep<P>(H, st, y0, dt, N, ck, ybarT, ybar0, yT, s))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        return 0;
+    }
+    SweepKey key;
+    key.y0 = y0;
+    key.ybarT = ybarT;
+    key.ybar0 = ybar0;
+    key.yT = yT;
+    key.dt = dt;
+    key.N = N;
+    key.ck = ck;
+    key.stream = s;
+    if (!st->exec || !(st->key == key)) {
+        drop_sweep(st);
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_sweep<P>(H, st, y0, dt, N, ck, ybarT, ybar0, yT, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(TRISK_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        st->graph = g;
+        HIP_TRY(hipGraphInstantiate(&st->exec, g, nullptr, nullptr, 0));
+        st->key = key;
+    }
+    HIP_TRY(hipGraphLaunch(st->exec, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return 0;
 }
 

@@ -213,8 +213,10 @@ int trisk_tendency_vjp(trisk_handle* h, const double* y_dev, const double* dybar
  * steps of dt (as trisk_rk4_steps).  The forward run keeps the state every
  * checkpoint_every steps (0: 8) and each segment is recomputed from its checkpoint during
  * the backward sweep.  yT_dev (may be NULL) receives y_T.  ybar0 may alias ybarT.  The
- * library allocates (ceil(nsteps / checkpoint_every) + checkpoint_every + 4) state vectors
- * of work memory for the call.  Synchronous. */
+ * handle keeps (ceil(nsteps / checkpoint_every) + checkpoint_every + 4) state vectors of
+ * work memory (grown as needed) until trisk_destroy.  With trisk_set_graphs(h, 1) and a
+ * stream, the whole sweep is captured as one HIP graph and replayed while the pointers,
+ * dt, nsteps and checkpoint_every stay those of the capture.  Synchronous. */
 int trisk_rk4_adjoint(trisk_handle* h, const double* y0_dev, double dt, int64_t nsteps, int64_t checkpoint_every,
                       const double* ybarT_dev, double* ybar0_dev, double* yT_dev, void* stream);
 
