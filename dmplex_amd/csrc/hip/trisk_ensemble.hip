@@ -47,7 +47,7 @@ inline FastDiv make_fastdiv(uint32_t d) {
 template <class P>
 struct EnsLevel {
     Params<P> p;       // tables; B, qv, G, T point to the level's (entity x member) vectors
-    FastDiv md[3];     // division by M / V for V = 1, 2, 4 members per lane
+    FastDiv md[4];     // division by M / V for V = 1, 2, 4, 8 members per lane
     int32_t M;         // members
     int64_t yoff;      // first element of the level's block [u (ne M); h (nc M)] of the state
     double dt, dt2, dt6;
@@ -167,7 +167,7 @@ __device__ __forceinline__ void rk_update_lanes(const Rk& r, int64_t j, const La
 
 template <int V>
 constexpr int lane_log() {
-    return V == 1 ? 0 : V == 2 ? 1 : 2;
+    return V == 1 ? 0 : V == 2 ? 1 : V == 4 ? 2 : 3;
 }
 
 // Item j of a range owns members [m0, m0 + V) of entity ent, elements [j V, j V + V) of the
@@ -396,7 +396,7 @@ static bool valid_schedule(const trisk_ensemble* E, const trisk_schedule& s) {
     bool ok = (s.block_size == 128 || s.block_size == 256 || s.block_size == 512) && s.blocks_per_cu >= 0 &&
               s.blocks_per_cu <= 64 &&
               (s.split == TRISK_SPLIT_CHUNKED || s.split == TRISK_SPLIT_BLOCKED || s.split == TRISK_SPLIT_INTERLEAVED) &&
-              (V == 1 || V == 2 || V == 4);
+              (V == 1 || V == 2 || V == 4 || V == 8);
     for (int l = 0; ok && l < E->nlevels; ++l) ok = E->M[l] % V == 0;
     return ok;
 }
@@ -535,7 +535,7 @@ static int setup_levels(trisk_ensemble* E) {
         const int64_t M = E->M[l];
         EnsLevel<P>& e = lv[l];
         e.p = model_params<P>(E->models[l]);
-        for (int k = 0; k < 3; ++k) e.md[k] = make_fastdiv(uint32_t(M % (1 << k) == 0 ? M >> k : 1));
+        for (int k = 0; k < 4; ++k) e.md[k] = make_fastdiv(uint32_t(M % (1 << k) == 0 ? M >> k : 1));
         e.M = int32_t(M);
         e.yoff = E->yoff[l];
         e.dt = e.dt2 = e.dt6 = 0;
@@ -625,7 +625,9 @@ static void launch_kind(int kind, dim3 grid, const EnsArgs& a, hipStream_t s) {
 
 template <class P, class UIn, int BLK>
 static void launch_lanes(int kind, int V, dim3 grid, const EnsArgs& a, hipStream_t s) {
-    if (V == 4)
+    if (V == 8)
+        launch_kind<P, UIn, BLK, 8>(kind, grid, a, s);
+    else if (V == 4)
         launch_kind<P, UIn, BLK, 4>(kind, grid, a, s);
     else if (V == 2)
         launch_kind<P, UIn, BLK, 2>(kind, grid, a, s);
@@ -900,7 +902,7 @@ int trisk_ensemble_set_schedule(trisk_ensemble* e, const trisk_schedule sched[3]
     for (int k = 0; k < kEnsKinds; ++k)
         if (!valid_schedule(e, sched[k]))
             return set_err(TRISK_EINVAL, "schedule: block_size 128/256/512, blocks_per_cu 0..64, split CHUNKED/BLOCKED/INTERLEAVED, "
-                                         "members_per_lane 1/2/4 dividing every level's member count");
+                                         "members_per_lane 1/2/4/8 dividing every level's member count");
     HIP_TRY(hipSetDevice(e->device));
     ens_drop_graph(e);
     HIP_TRY(hipDeviceSynchronize());  // runs in flight may still read the old tile lists
@@ -1003,7 +1005,7 @@ int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double
     double win_ms[kEnsKinds] = {1e300, 1e300, 1e300};
     ens_drop_graph(e);
     e->timing = true;
-    for (int lanes : {1, 2, 4})
+    for (int lanes : {1, 2, 4, 8})
         for (int blk : {128, 256, 512})
             for (int bpc : {0, 2, 4, 8})
                 for (int split : {TRISK_SPLIT_CHUNKED, TRISK_SPLIT_BLOCKED, TRISK_SPLIT_INTERLEAVED}) {

@@ -248,7 +248,7 @@ typedef struct {
     int32_t block_size;          /* 128, 256 or 512 work-items per workgroup */
     int32_t blocks_per_cu;       /* persistent grid of blocks_per_cu workgroups per CU; 0: one per tile */
     int32_t split;               /* TRISK_SPLIT_* */
-    int32_t members_per_lane;    /* 1, 2 or 4 consecutive members per work-item (0: 1); must divide
+    int32_t members_per_lane;    /* 1, 2, 4 or 8 consecutive members per work-item (0: 1); must divide
                                     every level's member count, and the state vectors passed to the
                                     ensemble must then be aligned to 8 x members_per_lane bytes */
 } trisk_schedule;
