@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 // ---------------------------------------------------------------------------------
 // Cell-pair path (TRiSK PV flux, cell kinetic energy): three launches per tendency.
 // Inside a cell the TRiSK weights are antisymmetric, w_{k',k} = -w_{k,k'} (the kite
-// fractions of a cell add up to one), so a cell stores the 15 weights of its edge
+// fractions of a cell add up to one), so a cell needs only the 15 weights of its edge
 // pairs k < k' (TRiSKGrid.jl:140-155) instead of every edge storing ten coefficients and
 // indices.  With G_e = l_e F_e the PV flux of edge e is
 //   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),

@@ -104,9 +104,9 @@ typedef struct {
 
 /* Kernel paths.  TABLES streams the reference's coefficient tables and evaluates every
  * formula in the reference's order (FP64: bit-exact with the reference algorithm).  PAIR
- * stores the 15 antisymmetric TRiSK weights of each cell's edge pairs, rebuilt from the
- * grid fields, instead of ten coefficients and indices per edge (fewer bytes; equal to
- * the tables up to rounding).  AUTO: TABLES for TRISK_FP64, PAIR (when the grid fields
+ * rebuilds the 15 antisymmetric TRiSK weights of each cell's edge pairs from its kite
+ * fractions (grid fields) instead of streaming ten coefficients and indices per edge (fewer
+ * bytes; equal to the tables up to rounding).  AUTO: TABLES for TRISK_FP64, PAIR (when the grid fields
  * are given and consistent) for TRISK_FP32_BULK. */
 enum { TRISK_PATH_AUTO = 0, TRISK_PATH_TABLES = 1, TRISK_PATH_PAIR = 2 };
 
