@@ -392,7 +392,6 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     // device numbering
     L.cmeta.resize(nc);
     L.ccv.assign(size_t(kPairSlots) * nc, 0);
-    L.pw.assign(size_t(kPairs) * nc, 0.0);
     L.pr.assign(size_t(kPairSlots) * nc, 0.0);
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < nc; ++j) {
@@ -404,7 +403,6 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
             int64_t v = k < size[i] ? corner[k + size_t(KC) * i] : corner[size_t(KC) * i];
             L.ccv[size_t(k) * nc + j] = int32_t(L.vinv[v]);
         }
-        for (int q = 0; q < kPairs; ++q) L.pw[size_t(q) * nc + j] = W[q + size_t(kPairs) * i];
     }
     L.len.resize(ne);
     L.dual.resize(ne);

@@ -40,7 +40,6 @@ struct Layout {
     std::string pair_reason;             // why the table path is used
     std::vector<int32_t> cmeta;          // per cell: size | negative-sign bits << 4
     std::vector<int32_t> ccv;            // corner k of cell (start of edge k), kc x nc
-    std::vector<double> pw;              // 15 x nc pair weights (kc <= 6)
     std::vector<double> pr;              // 6 x nc kite fractions R_k = kite_k / A_i (pads 0), from
                                          // which the kernels rebuild the pair weights
     std::vector<double> len, dual;       // l_e, d_e

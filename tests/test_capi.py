@@ -76,3 +76,23 @@ def test_create_without_a_device_fails_cleanly():
     rc = D.lib().trisk_create(C.byref(desc), None, C.byref(h))
     assert rc == -2 and not h.value
     assert "device" in D.lib().trisk_last_error().decode()
+
+
+def _struct_fields(header_text: str, name: str):
+    body = header_text[:header_text.index(f"}} {name};")]
+    body = body[body.rindex("typedef struct {") + len("typedef struct {"):]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";")[:-1]:
+        for part in decl.split(","):
+            fields.append(re.sub(r"\[.*?\]", "", part).replace("*", " ").split()[-1])
+    return fields
+
+
+def test_options_and_schedule_structs_match_the_header():
+    from dmplex_amd.ensemble import TriskSchedule
+
+    text = (ROOT / "include" / "trisk_hip.h").read_text()
+    assert _struct_fields(text, "trisk_options") == [f[0] for f in D.TriskOptions._fields_]
+    assert _struct_fields(text, "trisk_schedule") == [f[0] for f in TriskSchedule._fields_]
+    assert C.sizeof(D.TriskOptions) == 32 and C.sizeof(TriskSchedule) == 16
