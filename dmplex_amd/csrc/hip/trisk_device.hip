@@ -293,16 +293,47 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
 }
 
 // W2: cells: PV-flux half sums of their edges and the continuity with its RK4 update (XCD x
-// takes the x-th eighth of the cells, so the G and q_v its tiles share stay in its L2)
+// takes the x-th eighth of the cells, so the G and q_v its tiles share stay in its L2).  The
+// FP32-bulk kernel stages its block's cell records through LDS.
 template <class P>
 __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, int64_t t0) {
     using C = typename P::C;
     const int64_t nc = p.nc;
-    const int64_t i = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;
-    if (i >= nc) return;
+    const int64_t first = tile_of(blockIdx.x, t0, 0).first;
+    const int64_t i = first + threadIdx.x;
     int32_t meta, E[6], V[6];
     C W[15];
-    load_cell_pairs<P::id == TRISK_FP32_BULK>(p, i, meta, E, V, W);
+    if constexpr (P::id == TRISK_FP32_BULK) {
+        // the block's cell records (52 B each), staged through LDS with 16-byte coalesced
+        // loads: a quarter of the load instructions of the slot-major arrays
+        __shared__ __attribute__((aligned(16))) int32_t rec[kBlock * 13];
+        if (first >= nc) return;  // the whole block
+        const int64_t ncell = nc - first < kBlock ? nc - first : kBlock;
+        const int nw = int(ncell * 13), nw4 = nw / 4;
+        const int32_t* src = p.rc_pair + first * 13;
+        for (int w = threadIdx.x; w < nw4; w += kBlock)
+            reinterpret_cast<f32x4*>(rec)[w] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + w);
+        for (int w = 4 * nw4 + threadIdx.x; w < nw; w += kBlock) rec[w] = ld(src + w);
+        __syncthreads();
+        if (i >= nc) return;
+        const int32_t* rr = rec + 13 * threadIdx.x;
+        meta = rr[0];
+        const bool wide = int16_t(rr[1] & 0xffff) == kWide;
+        const int32_t be = int32_t(3 * i), bv = int32_t(2 * i);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int32_t pk = rr[1 + k];
+            E[k] = wide ? ld(p.ce6 + k * nc + i) : be + int16_t(pk & 0xffff);
+            V[k] = wide ? ld(p.ccv + k * nc + i) : bv + int16_t(uint32_t(pk) >> 16);
+        }
+        C R[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) __builtin_memcpy(&R[k], &rr[7 + k], 4);
+        pair_weights(meta, R, W);
+    } else {
+        if (i >= nc) return;
+        load_cell_pairs<false>(p, i, meta, E, V, W);
+    }
     const int n = meta & 15;
     C G[6], Q[6];
 #pragma unroll
@@ -354,13 +385,32 @@ __global__ __launch_bounds__(kBlock) void edge_sum_kernel(int64_t n, int K, cons
 }
 
 // ---------------------------------------------------------------------------------
-// compensated sums: Neumaier per thread over a fixed chunk, then in thread order
+// compensated sums: Neumaier per thread over a fixed chunk, then a wavefront butterfly and
+// the waves of the block in order, all on (sum, compensation) pairs combined with Knuth's
+// exact two-sum.  Both partners of a butterfly step compute the same pair (the two-sum
+// error of a + b is that of b + a), so the result does not depend on scheduling.
+
+// s + e = a + b exactly
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (s, c) <- (s, c) + (s2, c2)
+__device__ __forceinline__ void pair_add(double& s, double& c, double s2, double c2) {
+    double t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c = (c + c2) + e;
+}
 
 // term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
 __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
                                                         const double* __restrict__ a, const double* __restrict__ b,
                                                         double g, double* __restrict__ partial) {
-    __shared__ double ss[kBlock], cc[kBlock];
+    constexpr int kWave = 64, kWaves = kBlock / kWave;
+    __shared__ double ws[kWaves], wc[kWaves];
     int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     int64_t lo = int64_t(blockIdx.x) * chunk, hi = lo + chunk < n ? lo + chunk : n;
     double s = 0.0, c = 0.0;
@@ -374,15 +424,17 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, con
             x = area[i] * a[i] * (b[i] * b[i]);
         neumaier(s, c, x);
     }
-    ss[threadIdx.x] = s;
-    cc[threadIdx.x] = c;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) pair_add(s, c, __shfl_xor(s, off), __shfl_xor(c, off));
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        ws[wave] = s;
+        wc[wave] = c;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double S = 0.0, Cc = 0.0;
-        for (int k = 0; k < kBlock; ++k) {
-            neumaier(S, Cc, ss[k]);
-            Cc += cc[k];
-        }
+        double S = ws[0], Cc = wc[0];
+        for (int k = 1; k < kWaves; ++k) pair_add(S, Cc, ws[k], wc[k]);
         partial[2 * blockIdx.x] = S;
         partial[2 * blockIdx.x + 1] = Cc;
     }
@@ -482,6 +534,7 @@ static int setup_params(trisk_handle* H) {
             rc |= upload(o, L.rv_idx16, &p.rv_idx16);
             rc |= upload(o, L.re_cells16, &p.re_cells16);
             rc |= upload(o, L.pc_ev16, &p.pc_ev16);
+            rc |= upload(o, L.rc_pair, &p.rc_pair);
         }
         rc |= upload(o, L.re_cells, &p.re_cells);
         rc |= coef(L.re_lh, &p.re_lh);

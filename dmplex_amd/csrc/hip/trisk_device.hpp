@@ -127,6 +127,7 @@ struct Params {
     const int16_t* rv_idx16;    // nv x 8
     const int32_t* re_cells16;  // ne
     const int32_t* pc_ev16;     // 6 x nc
+    const int32_t* rc_pair;     // nc x 13 (FP32-bulk cell kernel records)
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;
@@ -321,6 +322,25 @@ __device__ __forceinline__ void load_edge_cells(const Params<P>& p, int64_t e, i
         ldrec(p.re_cells + 2 * e, c);
 }
 
+// w_{k,k'} = s_k s_k' (1/2 - sum_{j=k+1..k'} R_j) (TRiSKGrid.jl:143-155) from the kite fractions R
+// and the sign bits of meta, in the order and association of the host construction, so FP64
+// weights are bitwise the host's
+template <class C>
+__device__ __forceinline__ void pair_weights(int32_t meta, const C (&R)[6], C (&W)[15]) {
+    int q = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const C sk = (meta >> (4 + k)) & 1 ? C(-1) : C(1);
+        C S = 0;
+#pragma unroll
+        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
+            S += R[k2];
+            const C s2 = (meta >> (4 + k2)) & 1 ? C(-1) : C(1);
+            W[q] = sk * s2 * (C(0.5) - S);
+        }
+    }
+}
+
 // cell i: size and side bits, edges and corners (6 slots, pads valid), pair weights (those of
 // slots beyond the size are not used)
 template <bool NARROW, class P>
@@ -350,24 +370,10 @@ __device__ __forceinline__ void load_cell_pairs(const Params<P>& p, int64_t i, i
             V[k] = ld(p.ccv + k * nc + i);
         }
     }
-    // w_{k,k'} = s_k s_k' (1/2 - sum_{j=k+1..k'} R_j) (TRiSKGrid.jl:143-155), in the order and
-    // association of the host construction, so FP64 weights are bitwise the host's
-    using C = typename P::C;
-    C R[6];
+    typename P::C R[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) R[k] = ld(p.pr + k * nc + i);
-    int q = 0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const C sk = (meta >> (4 + k)) & 1 ? C(-1) : C(1);
-        C S = 0;
-#pragma unroll
-        for (int k2 = k + 1; k2 < 6; ++k2, ++q) {
-            S += R[k2];
-            const C s2 = (meta >> (4 + k2)) & 1 ? C(-1) : C(1);
-            W[q] = sk * s2 * (C(0.5) - S);
-        }
-    }
+    pair_weights(meta, R, W);
 }
 
 // ---------------------------------------------------------------------------------

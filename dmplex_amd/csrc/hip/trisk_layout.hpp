@@ -58,6 +58,7 @@ struct Layout {
     std::vector<int16_t> rv_idx16;       // nv x 8: edges (3), cells (3), the FP32 bits of f (2)
     std::vector<int32_t> re_cells16;     // ne: left | right << 16
     std::vector<int32_t> pc_ev16;        // 6 x nc: edge | corner << 16
+    std::vector<int32_t> rc_pair;        // nc x 13: meta, 6 x (edge | corner << 16), 6 x FP32 bits of R
     int64_t wide_cells = 0, wide_vertices = 0, wide_edges = 0;
     // transposed coefficients for the adjoint of the cell-pair path
     std::vector<double> adj_ecurl;       // ne x 2: curl coefficient of edge e in vertex ev[s]
