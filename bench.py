@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--no-graphs", action="store_true", help="launch the kernels directly instead of graph replay")
     p.add_argument("--path", choices=["auto", "tables", "pair"], default="auto",
                    help="kernel path (auto: bit-exact tables for FP64, cell pairs for FP32-bulk)")
+    p.add_argument("--grid-file", default="",
+                   help="run on the mesh of an MPAS or ICON grid file (read_grid) instead of generating one")
+    p.add_argument("--grid-dual", action="store_true",
+                   help="with --grid-file: its dual mesh (the hexagons of an ICON file)")
     p.add_argument("--no-scvt", action="store_true",
                    help="skip Lloyd (same topology and kernels; for short profiling runs only)")
     p.add_argument("--accuracy", type=int, default=0, metavar="DAYS",
@@ -449,9 +453,19 @@ def main():
         print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
         return
     t = time.perf_counter()
-    log(f"[bench] generating SCVT hexagon mesh level {args.level}")
-    dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0, log=lambda m: log(f"[bench] {m}"))
-    grid = TRiSKGrid(dm, centers=dm.generators)
+    if args.grid_file:
+        from dmplex_amd import read_grid
+
+        dm = read_grid(args.grid_file, dual=args.grid_dual)
+        log(f"[bench] read {args.grid_file} ({dm.nc} cells) in {time.perf_counter() - t:.2f} s")
+        grid = TRiSKGrid(dm)
+        workload = f"grid file {os.path.basename(args.grid_file)}{' (dual)' if args.grid_dual else ''}"
+    else:
+        log(f"[bench] generating SCVT hexagon mesh level {args.level}")
+        dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0,
+                                 log=lambda m: log(f"[bench] {m}"))
+        grid = TRiSKGrid(dm, centers=dm.generators)
+        workload = f"SCVT hexagon level {args.level}"
     params = dict(earth_parameters(), pv_flux=args.pv_flux, kinetic_energy=args.kinetic_energy)
     trisk_cell = args.pv_flux == "trisk" and args.kinetic_energy == "cell"
     y0 = seeded_state(grid)
@@ -480,8 +494,7 @@ def main():
         "vs_baseline": None, "dtype": "f32" if head_p == "fp32_bulk" else "f64",
         "data": "synthetic: seeded streamfunction state (l<=8 harmonics, max|u|=30 m/s, linear balance, "
                 "N(0,5 m) noise), Earth parameters on the unit sphere",
-        "config": {"workload": f"SCVT hexagon level {args.level}, {args.pv_flux} PV flux, {args.kinetic_energy} KE, "
-                               f"RK4 dt={args.dt} s",
+        "config": {"workload": f"{workload}, {args.pv_flux} PV flux, {args.kinetic_energy} KE, RK4 dt={args.dt} s",
                    "cells": grid.num_cells(), "edges": grid.num_edges(), "vertices": grid.num_vertices(),
                    "precision": head_p,
                    "model_bytes_per_cell_step": MODEL_BYTES[head_p] if trisk_cell

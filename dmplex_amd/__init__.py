@@ -5,6 +5,8 @@ Public API (mirrors the reference's Julia names; ``!`` becomes ``_``):
 * meshes: ``create_spherical_mesh``, ``create_hexagon_mesh``, ``lloyd_``, ``Plex``,
   ``plex_from_polygons``, ``check_plex``, ``hilbert_order``, ``hilbert_permutation``,
   ``cell_centers``, ``orthogonality_defect``;
+* grid files: ``read_grid``, ``read_mpas_grid``, ``read_icon_grid`` (NetCDF classic,
+  64-bit offset, 64-bit data and NetCDF-4), ``NetCDFFile``, ``sort_ring``, ``dual_polygons``;
 * geometry: ``TRiSKGrid``, ``vertex_edge_areas``, ``gassmann_coefficients``;
 * model: ``TRiSKShallowWater`` (``backend=HIPBackend(...)``), ``initial_state``,
   ``tendency_``, ``RK4Cache``, ``rk4_step_``, ``gather_state``, ``total_mass``,
@@ -19,6 +21,8 @@ Compute runs in ``lib/libtrisk_hip.so`` (gfx950 kernels behind the C ABI of
 ``include/trisk_hip.h``); setup runs in ``lib/libdmplex_host.so``.
 """
 from .ensemble import TRiSKEnsemble
+from .file_grids import (NetCDFFile, dual_polygons, lonlat_to_xyz, read_grid, read_icon_grid, read_mpas_grid,
+                         read_variable, rows_per_entity, sort_ring)
 from .grids import (cell_centers, create_hexagon_mesh, create_spherical_mesh, lloyd_, lloyd_residual,
                     orthogonality_defect, voronoi_generators)
 from .topology import (Plex, check_plex, hilbert_key, hilbert_keys, hilbert_order, hilbert_permutation,

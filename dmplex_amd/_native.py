@@ -84,6 +84,15 @@ def host() -> C.CDLL:
             "dmx_hilbert_keys": [i64, u64p, C.c_int, u64p],
             "dmx_hilbert_permutation": [i64, f64p, C.c_int, i64p],
             "dmx_streamfunction": [C.c_int, f64p, i64, f64p, f64p],
+            "dmx_nc_open": [C.c_char_p, C.POINTER(C.c_void_p)],
+            "dmx_nc_format": [C.c_void_p],
+            "dmx_nc_var_name": [C.c_void_p, i64, C.c_char_p, i64],
+            "dmx_nc_has_var": [C.c_void_p, C.c_char_p],
+            "dmx_nc_var_info": [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), i64p, C.c_int],
+            "dmx_nc_read_f64": [C.c_void_p, C.c_char_p, f64p, i64],
+            "dmx_nc_read_i64": [C.c_void_p, C.c_char_p, i64p, i64],
+            "dmx_nc_global_text": [C.c_void_p, C.c_char_p, C.c_char_p, i64, i64p],
+            "dmx_sort_rings": [i64, i64p, i64p, i64, f64p, f64p],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -91,6 +100,10 @@ def host() -> C.CDLL:
             fn.restype = C.c_int
         lib.dmx_plex_free.argtypes = [C.c_void_p]
         lib.dmx_plex_free.restype = None
+        lib.dmx_nc_close.argtypes = [C.c_void_p]
+        lib.dmx_nc_close.restype = None
+        lib.dmx_nc_num_vars.argtypes = [C.c_void_p]
+        lib.dmx_nc_num_vars.restype = i64
         _host = lib
     return _host
 
@@ -102,5 +115,7 @@ def check(rc: int, lib=None, warn_ok: bool = False) -> int:
         msg = lib.dmx_last_error().decode()
         if rc in (-2, -3):
             raise ValueError(msg)
+        if rc == -5:
+            raise FileNotFoundError(msg)
         raise NativeError(f"native call failed ({rc}): {msg}")
     return rc

@@ -16,7 +16,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "lib"
-HOST_SOURCES = ["csrc/host/mesh.cpp", "csrc/host/trisk_grid.cpp"]
+HOST_SOURCES = ["csrc/host/mesh.cpp", "csrc/host/trisk_grid.cpp", "csrc/host/netcdf.cpp"]
 HIP_SOURCES = ["csrc/hip/trisk_device.hip", "csrc/hip/trisk_ensemble.hip", "csrc/hip/trisk_adjoint.hip",
                "csrc/hip/lloyd_device.hip"]
 HIP_HOST_SOURCES = ["csrc/hip/trisk_layout.cpp"]
@@ -43,7 +43,7 @@ def build_host() -> Path:
     out = LIB / "libdmplex_host.so"
     tmp = LIB / "libdmplex_host.so.tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
-          "-Wall", "-o", str(tmp)] + HOST_SOURCES, PKG)
+          "-Wall", "-o", str(tmp)] + HOST_SOURCES + ["-ldl"], PKG)
     os.replace(tmp, out)
     return out
 

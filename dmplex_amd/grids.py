@@ -142,28 +142,47 @@ def _normalize(a: np.ndarray) -> np.ndarray:
     return a / np.linalg.norm(a, axis=-1, keepdims=True)
 
 
+def _edge_cells(dm: Plex) -> tuple[np.ndarray, np.ndarray]:
+    """(left, right) cell of every edge, -1 when missing (get_edge_cells, Plex.jl:210-218)."""
+    nconn = dm.cone_offsets[dm.nc]
+    cone, orient = dm.cones[:nconn], dm.cone_orientations[:nconn]
+    owner = np.repeat(np.arange(dm.nc), np.diff(dm.cone_offsets[:dm.nc + 1]))
+    left = np.full(dm.ne, -1, dtype=np.int64)
+    right = np.full(dm.ne, -1, dtype=np.int64)
+    left[cone[orient == 0] - dm.nc] = owner[orient == 0]
+    right[cone[orient != 0] - dm.nc] = owner[orient != 0]
+    return left, right
+
+
 def voronoi_generators(dm: Plex, iterations: int = 4) -> np.ndarray:
     """Generators of the best-fitting Voronoi diagram by inverse iteration on the
     homogeneous bisector conditions (CellCenters.jl:49-81), with a sparse LU solve."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
 
-    rows, cols, vals = [], [], []
-    nrow = 0
-    for e in dm.edge_range():
-        left, right = dm.get_edge_cells(e)
-        if left < 0 or right < 0:
-            continue
-        a, b = (dm.get_coordinates(v) for v in dm.get_cone(e))
-        n = _normalize(np.cross(a, b))
-        t1 = _normalize(np.cross(n, np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])))
-        t2 = np.cross(n, t1)
-        for w, sj in ((t1, -1), (t2, -1), (n, 1)):
-            for k in range(3):
-                rows += [nrow, nrow]
-                cols += [3 * left + k, 3 * right + k]
-                vals += [w[k], sj * w[k]]
-            nrow += 1
+    left, right = _edge_cells(dm)
+    inner = np.flatnonzero((left >= 0) & (right >= 0))
+    i, j = left[inner], right[inner]
+    first = dm.cones[dm.cone_offsets[dm.nc + inner]]
+    second = dm.cones[dm.cone_offsets[dm.nc + inner] + 1]
+    a = dm.coordinates[first - dm.nc - dm.ne]
+    b = dm.coordinates[second - dm.nc - dm.ne]
+    n = _normalize(np.cross(a, b))
+    pick = np.where((np.abs(n[:, 0]) < 0.9)[:, None], np.array([1.0, 0, 0]), np.array([0, 1.0, 0]))
+    t1 = _normalize(np.cross(n, pick))
+    t2 = np.cross(n, t1)
+    # three rows per edge, (t1, -1), (t2, -1), (n, +1); six entries per row, w_k on g_i
+    # and s w_k on g_j for k = x, y, z
+    m = len(inner)
+    w = np.stack([t1, t2, n], axis=1)                                   # m x 3 rows x 3
+    sgn = np.array([-1.0, -1.0, 1.0])[None, :, None]
+    row = np.broadcast_to(np.arange(3 * m).reshape(m, 3, 1), (m, 3, 3))
+    k = np.arange(3)[None, None, :]
+    rows = np.concatenate([row.ravel(), row.ravel()])
+    cols = np.concatenate([np.broadcast_to(3 * i[:, None, None] + k, (m, 3, 3)).ravel(),
+                           np.broadcast_to(3 * j[:, None, None] + k, (m, 3, 3)).ravel()])
+    vals = np.concatenate([w.ravel(), (sgn * w).ravel()])
+    nrow = 3 * m
     A = sp.csr_matrix((vals, (rows, cols)), shape=(nrow, 3 * dm.nc))
     B = (A.T @ A).tocsc()
     B = B + 1e-10 * abs(B.diagonal()).max() * sp.identity(B.shape[0], format="csc")
@@ -190,10 +209,13 @@ def cell_centers(dm: Plex, method: str = "auto", tolerance: float = 1e-8) -> np.
             return np.array(gens, dtype=np.float64)
         return cell_centers(dm, method="centroid")
     if method == "centroid":
-        out = np.empty((dm.nc, 3))
-        for c in dm.cell_range():
-            out[c] = _normalize(dm.cell_coordinates(c).sum(axis=0))
-        return out
+        sizes = np.diff(dm.cone_offsets[:dm.nc + 1])
+        nconn = dm.cone_offsets[dm.nc]
+        cone, orient = dm.cones[:nconn], dm.cone_orientations[:nconn]
+        start = np.where(orient == 0, dm.cones[dm.cone_offsets[cone]], dm.cones[dm.cone_offsets[cone] + 1])
+        out = np.zeros((dm.nc, 3))
+        np.add.at(out, np.repeat(np.arange(dm.nc), sizes), dm.coordinates[start - dm.nc - dm.ne])
+        return _normalize(out)
     if method == "circumcenter":
         tri = dm.cell_vertex_table()
         if tri.shape[1] != 3:

@@ -167,6 +167,41 @@ int dmx_hilbert_keys(int64_t n, const uint64_t* x, int bits, uint64_t* keys);
  * perm[c] = old index of new entity c (stable sort of the keys). */
 int dmx_hilbert_permutation(int64_t n, const double* points, int bits, int64_t* perm);
 
+/* --- grid files (NetCDF) --------------------------------------------------------- */
+
+/* Read-only NetCDF file: the storage under read_variable / haskey / ds.attrib of the
+ * grid-file readers (src/Grids/FileGrids.jl:8-11, 67, 101-102, which use NCDatasets.jl).
+ * Classic (CDF-1), 64-bit offset (CDF-2) and 64-bit data (CDF-5) files are parsed
+ * natively; NetCDF-4 files go through the HDF5 C library, loaded on first use
+ * (DMPLEX_HDF5_LIBRARY or the system's libhdf5.so).  Error -5: the file cannot be
+ * opened; -2: not a NetCDF file, malformed, or NetCDF-4 without HDF5; -3: no such
+ * variable, a wrong count, or an integer read of a floating-point variable. */
+typedef struct NcFile dmx_netcdf;
+int dmx_nc_open(const char* path, dmx_netcdf** out);
+void dmx_nc_close(dmx_netcdf* f);
+/* 1 classic, 2 64-bit offset, 5 64-bit data, 4 NetCDF-4 (HDF5). */
+int dmx_nc_format(const dmx_netcdf* f);
+int64_t dmx_nc_num_vars(const dmx_netcdf* f);
+int dmx_nc_var_name(const dmx_netcdf* f, int64_t i, char* buf, int64_t cap);
+/* 1 if the file has the variable, else 0 (haskey(ds, name)). */
+int dmx_nc_has_var(const dmx_netcdf* f, const char* name);
+/* NetCDF type code (1 byte, 2 char, 3 short, 4 int, 5 float, 6 double, 7 ubyte,
+ * 8 ushort, 9 uint, 10 int64, 11 uint64) and shape in ncdump (C) order. */
+int dmx_nc_var_info(const dmx_netcdf* f, const char* name, int* type, int* ndims, int64_t* shape,
+                    int max_dims);
+/* The whole variable, raw (no fill-value or scale conversion), in C order, converted to
+ * double / int64; n must equal its number of values. */
+int dmx_nc_read_f64(const dmx_netcdf* f, const char* name, double* out, int64_t n);
+int dmx_nc_read_i64(const dmx_netcdf* f, const char* name, int64_t* out, int64_t n);
+/* Text of a character global attribute; returns 1 when the file has none of that name. */
+int dmx_nc_global_text(const dmx_netcdf* f, const char* name, char* buf, int64_t cap, int64_t* len);
+
+/* sort_ring over many rings (src/Grids/FileGrids.jl:22-41): ring r, the slice
+ * ids[offsets[r], offsets[r+1]) of indices into points (npoints x 3), is sorted in place
+ * counter-clockwise around normalize(axes[r]) (nrings x 3) as seen from outside. */
+int dmx_sort_rings(int64_t nrings, const int64_t* offsets, int64_t* ids, int64_t npoints,
+                   const double* points, const double* axes);
+
 /* --- synthetic initial state ---------------------------------------------------- */
 
 /* psi(x) = sum_{l=1..lmax} sum_{m=-l..l} coef[l*l - 1 + (m + l)] Y_lm(x) with real
