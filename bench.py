@@ -7,6 +7,9 @@ stage updates, 12 kernel launches) with the state resident in HBM.  The FP64 pat
 the FP32-bulk path (FP32 stencils, FP64 state/heights/accumulation) are timed in the
 same run; ``value`` is the FP32-bulk throughput in cell-steps/s.
 
+Beside the headline the line carries the batched ensemble, the adjoint and the ICON
+triangle model (ICONShallowWater at the cell count of ICON's R02B08) measurements.
+
 Prints one JSON line (rank 0).  The CPU baseline is the oracle (FP64 restatement of the
 reference kernels, OpenMP over the host cores) on a bounded sample of the same
 workload.
@@ -74,6 +77,11 @@ def parse():
     p.add_argument("--kinetic-energy", choices=["cell", "mixed", "vertex"], default="cell")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--adjoint-only", action="store_true", help="only the adjoint measurement")
+    p.add_argument("--no-icon", action="store_true", help="skip the ICON triangle-model measurement")
+    p.add_argument("--icon-only", action="store_true", help="only the ICON triangle-model measurement")
+    p.add_argument("--icon-level", type=int, default=9, help="SCVT triangle level of the ICON measurement")
+    p.add_argument("--icon-steps", type=int, default=100, help="timed RK4 steps of the ICON measurement")
+    p.add_argument("--icon-dt", type=float, default=20.0, help="RK4 dt (s) of the ICON measurement")
     return p.parse_args()
 
 
@@ -169,6 +177,79 @@ def run_gpu(grid, y0, precision, args, params, path=None):
         f"model HBM fraction {out['hbm_frac_model']:.3f}")
     del m
     torch.cuda.empty_cache()
+    return out
+
+
+ICON_REFERENCE_MS = 7.8  # ICONShallowWater, TC2, R02B08 (5.2M cells), ms per RK4 step on one A100 (docs/STATUS.md:102-104)
+
+
+def run_icon(args):
+    """ICONShallowWater (FP64) on the SCVT triangle mesh of level 9: 5,242,880 cells, the
+    cell count of ICON's R02B08, where the reference publishes 7.8 ms per RK4 step on one
+    A100.  Timed without divergence damping (the TC2 setting) and with ICON's fourth-order
+    damping (divdamp_fac 0.0025, nu = 0.0025 mean-area^2 / dt)."""
+    import torch
+
+    from dmplex_amd import ICONShallowWater, create_spherical_mesh, diagnostics, rk4_steps_
+    from dmplex_amd import _device as D
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+
+    t = time.perf_counter()
+    dm = create_spherical_mesh(args.icon_level, centroidal=True, device=0)
+    log(f"[bench] icon: SCVT triangle mesh level {args.icon_level} ({dm.nc} cells) in {time.perf_counter() - t:.1f} s")
+    dt, steps = args.icon_dt, args.icon_steps
+    nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
+    out = {"workload": f"ICONShallowWater on SCVT triangles level {args.icon_level} ({dm.nc} cells, "
+                       f"{dm.ne} edges, {dm.nv} vertices), FP64, RK4 dt={dt} s, seeded state, Earth parameters",
+           "reference_a100_ms_per_step": ICON_REFERENCE_MS}
+    stream = torch.cuda.current_stream()
+    grid = None
+    for key, damping in (("no_damping", 0.0), ("damping", nu)):
+        t = time.perf_counter()
+        m = ICONShallowWater(grid if grid is not None else dm, divergence_damping=damping, **earth_parameters())
+        grid = m.grid
+        log(f"[bench] icon {key}: tables and upload in {time.perf_counter() - t:.1f} s")
+        y = m.upload_state(seeded_state(grid))
+        mass0, energy0, _ = diagnostics(m, y)
+        rk4_steps_(y, dt, args.warmup, m)
+
+        def timed(n):
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            t0.record(stream)
+            rk4_steps_(y, dt, n, m)
+            t1.record(stream)
+            torch.cuda.synchronize()
+            return t0.elapsed_time(t1)
+
+        D.check(D.lib().trisk_set_timing(m.handle, 1))
+        ms_direct = timed(steps)
+        D.check(D.lib().trisk_set_timing(m.handle, 0))
+        stats = kernel_stats(m)
+        m.use_graphs(True)
+        rk4_steps_(y, dt, 8, m)
+        ms = timed(steps)
+        mass, energy, _ = diagnostics(m, y)
+        kbytes = sum(k["bytes"] * k["launches"] for k in stats) / steps
+        dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
+        achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
+        r = dict(value=dm.nc * steps / (ms / 1e3), unit="cell-steps/s", ms_per_step=ms / steps,
+                 ms_per_step_direct_with_events=ms_direct / steps,
+                 speedup_vs_reference_a100=ICON_REFERENCE_MS / (ms / steps),
+                 kernel_model_bytes_per_cell_step=kbytes / dm.nc,
+                 hbm_frac_kernel_model=kbytes / (ms / steps / 1e3) / HBM_PEAK,
+                 mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0,
+                 roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
+                               frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], "icon"),
+                               avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
+                 kernels=stats)
+        if damping:
+            r["divergence_damping_nu"] = nu
+        out[key] = r
+        log(f"[bench] icon {key}: {r['ms_per_step']:.3f} ms/step ({r['speedup_vs_reference_a100']:.2f}x the "
+            f"reference's A100), kernel-model HBM fraction {r['hbm_frac_kernel_model']:.3f}")
+        del m
+        torch.cuda.empty_cache()
     return out
 
 
@@ -452,6 +533,9 @@ def main():
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
         return
+    if args.icon_only:
+        print(json.dumps({"icon": run_icon(args)}), flush=True)
+        return
     t = time.perf_counter()
     if args.grid_file:
         from dmplex_amd import read_grid
@@ -516,6 +600,9 @@ def main():
     if not args.no_adjoint and trisk_cell:
         log("[bench] adjoint (level 7, 40 steps)")
         line["adjoint"] = run_adjoint(args)
+    if not args.no_icon and trisk_cell and not args.grid_file:
+        log(f"[bench] ICON triangle model (level {args.icon_level})")
+        line["icon"] = run_icon(args)
     print(json.dumps(line), flush=True)
 
 

@@ -14,6 +14,9 @@ Public API (mirrors the reference's Julia names; ``!`` becomes ``_``):
   ``tangential_velocity``, ``kinetic_energy_``, ``vertex_pv_``;
 * adjoint: ``tendency_vjp``, ``rk4_adjoint`` (checkpointed), ``rrule_rk4_steps``,
   ``rk4_steps`` (a differentiable torch operation), ``height_misfit``;
+* the ICON triangle model: ``ICONShallowWater`` (RBF tangential velocity, bilinear kinetic
+  energy, divergence damping), ``rbf_coefficients``, ``bilinear_weights``; it takes the
+  same state, stepping and diagnostic functions as ``TRiSKShallowWater``;
 * ensembles: ``TRiSKEnsemble`` (members on meshes of several resolutions, one launch per
   kernel for all of them, with an autotuned schedule).
 
@@ -23,6 +26,7 @@ Compute runs in ``lib/libtrisk_hip.so`` (gfx950 kernels behind the C ABI of
 from .ensemble import TRiSKEnsemble
 from .file_grids import (NetCDFFile, dual_polygons, lonlat_to_xyz, read_grid, read_icon_grid, read_mpas_grid,
                          read_variable, rows_per_entity, sort_ring)
+from .icon_model import ICONShallowWater, ICONTables, bilinear_weights, rbf_coefficients
 from .grids import (cell_centers, create_hexagon_mesh, create_spherical_mesh, lloyd_, lloyd_residual,
                     orthogonality_defect, voronoi_generators)
 from .topology import (Plex, check_plex, hilbert_key, hilbert_keys, hilbert_order, hilbert_permutation,

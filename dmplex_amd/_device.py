@@ -26,6 +26,16 @@ class TriskModelDesc(C.Structure):
                 ("cell_kites", f64p)]
 
 
+class IconModelDesc(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("kv", i64), ("kr", i64),
+                ("cell_edges", i64p), ("divergence", f64p), ("bilinear", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_height", f64p), ("gradient", f64p), ("inverse_dual", f64p),
+                ("rbf_edges", i64p), ("rbf", f64p), ("edge_coriolis", f64p), ("vertex_edges", i64p),
+                ("curl", f64p), ("vertex_cells", i64p), ("vertex_height", f64p), ("coriolis", f64p),
+                ("cell_areas", f64p), ("vertex_areas", f64p), ("gravity", C.c_double),
+                ("divergence_damping", f64p), ("cell_points", f64p)]
+
+
 class TriskOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_size", C.c_int32),
                 ("path", C.c_int32), ("narrow_limit", C.c_int32), ("reserved", C.c_int32 * 3)]
@@ -78,6 +88,8 @@ SIGNATURES = {
     "trisk_ensemble_set_timing": [vp, C.c_int32],
     "trisk_ensemble_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
     "trisk_ensemble_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
+    "icon_check_tables": [C.POINTER(IconModelDesc)],
+    "icon_create": [C.POINTER(IconModelDesc), C.POINTER(TriskOptions), C.POINTER(vp)],
     "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 

@@ -93,6 +93,8 @@ def host() -> C.CDLL:
             "dmx_nc_read_i64": [C.c_void_p, C.c_char_p, i64p, i64],
             "dmx_nc_global_text": [C.c_void_p, C.c_char_p, C.c_char_p, i64, i64p],
             "dmx_sort_rings": [i64, i64p, i64p, i64, f64p, f64p],
+            "dmx_rbf_coefficients": [i64, i64, C.POINTER(DmxGridTables), C.c_double, i64, i64p, f64p, i64p],
+            "dmx_bilinear_weights": [i64, C.POINTER(DmxGridTables), f64p, f64p],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)

@@ -522,6 +522,7 @@ extern "C" {
 
 int trisk_tendency_vjp(trisk_handle* h, const double* y_dev, const double* dybar_dev, double* ybar_dev,
                        void* stream) {
+    if (h && h->L.icon) return set_err(TRISK_EINVAL, "no adjoint for the ICON triangle model");
     int rc = adj_check(h);
     if (rc) return rc;
     if (!y_dev || !dybar_dev || !ybar_dev) return set_err(TRISK_EINVAL, "null state pointer");
@@ -534,6 +535,7 @@ int trisk_tendency_vjp(trisk_handle* h, const double* y_dev, const double* dybar
 
 int trisk_rk4_adjoint(trisk_handle* h, const double* y0_dev, double dt, int64_t nsteps, int64_t checkpoint_every,
                       const double* ybarT_dev, double* ybar0_dev, double* yT_dev, void* stream) {
+    if (h && h->L.icon) return set_err(TRISK_EINVAL, "no adjoint for the ICON triangle model");
     int rc = adj_check(h);
     if (rc) return rc;
     if (!y0_dev || !ybarT_dev || !ybar0_dev) return set_err(TRISK_EINVAL, "null state pointer");

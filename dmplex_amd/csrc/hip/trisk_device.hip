@@ -385,68 +385,8 @@ __global__ __launch_bounds__(kBlock) void edge_sum_kernel(int64_t n, int K, cons
 }
 
 // ---------------------------------------------------------------------------------
-// compensated sums: Neumaier per thread over a fixed chunk, then a wavefront butterfly and
-// the waves of the block in order, all on (sum, compensation) pairs combined with Knuth's
-// exact two-sum.  Both partners of a butterfly step compute the same pair (the two-sum
-// error of a + b is that of b + a), so the result does not depend on scheduling.
-
-// s + e = a + b exactly
-__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
-    s = a + b;
-    const double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-// (s, c) <- (s, c) + (s2, c2)
-__device__ __forceinline__ void pair_add(double& s, double& c, double s2, double c2) {
-    double t, e;
-    two_sum(s, s2, t, e);
-    s = t;
-    c = (c + c2) + e;
-}
-
-// term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
-__global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
-                                                        const double* __restrict__ a, const double* __restrict__ b,
-                                                        double g, double* __restrict__ partial) {
-    constexpr int kWave = 64, kWaves = kBlock / kWave;
-    __shared__ double ws[kWaves], wc[kWaves];
-    int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    int64_t lo = int64_t(blockIdx.x) * chunk, hi = lo + chunk < n ? lo + chunk : n;
-    double s = 0.0, c = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        double x;
-        if (term == 0)
-            x = area[i] * a[i];
-        else if (term == 1)
-            x = area[i] * a[i] * (b[i] + g * a[i] / 2);
-        else
-            x = area[i] * a[i] * (b[i] * b[i]);
-        neumaier(s, c, x);
-    }
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) pair_add(s, c, __shfl_xor(s, off), __shfl_xor(c, off));
-    const int wave = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) {
-        ws[wave] = s;
-        wc[wave] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double S = ws[0], Cc = wc[0];
-        for (int k = 1; k < kWaves; ++k) pair_add(S, Cc, ws[k], wc[k]);
-        partial[2 * blockIdx.x] = S;
-        partial[2 * blockIdx.x + 1] = Cc;
-    }
-}
-
-// ---------------------------------------------------------------------------------
 // handle
 
-enum KernelId { K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_PAIR_STATE = 3, K_PAIR_CELL = 4, K_PAIR_EDGE = 5,
-                K_COUNT = 6 };
-static const char* kKernelNames[K_COUNT] = {"cell_vertex_kernel", "edge_kernel",      "update_kernel",
-                                            "pair_state_kernel",  "pair_cell_kernel", "pair_edge_kernel"};
 
 }  // namespace trisk
 
@@ -577,29 +517,6 @@ static int setup_params(trisk_handle* H) {
     return 0;
 }
 
-static int timed_begin(trisk_handle* H, int kernel, hipStream_t s, hipEvent_t* stop) {
-    *stop = nullptr;
-    if (!H->timing) return 0;
-    if (H->events_used + 2 > H->events.size()) {
-        for (int k = 0; k < 256; ++k) {
-            hipEvent_t e;
-            HIP_TRY(hipEventCreate(&e));
-            H->events.push_back(e);
-        }
-        H->event_kernel.resize(H->events.size() / 2);
-    }
-    size_t i = H->events_used;
-    H->events_used += 2;
-    H->event_kernel[i / 2] = kernel;
-    HIP_TRY(hipEventRecord(H->events[i], s));
-    *stop = H->events[i + 1];
-    return 0;
-}
-
-static int timed_end(hipEvent_t stop, hipStream_t s) {
-    if (stop) HIP_TRY(hipEventRecord(stop, s));
-    return 0;
-}
 
 // One tendency: P1, P2, P3 with the given stage input and update mode.
 template <class P, class UIn>
@@ -769,32 +686,13 @@ static int diagnostic_pass(trisk_handle* H, const double* y, double* K_out, doub
     return 0;
 }
 
-static int reduce(trisk_handle* H, int term, int64_t n, const double* area, const double* a, const double* b,
-                  double* out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_kernel, dim3(unsigned(H->reduce_blocks)), dim3(kBlock), 0, s, term, n, area, a, b,
-                       H->L.gravity, H->partial);
-    HIP_TRY(hipGetLastError());
-    std::vector<double> part(2 * H->reduce_blocks);
-    HIP_TRY(hipMemcpyAsync(part.data(), H->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    double S = 0.0, Cc = 0.0;
-    for (int b2 = 0; b2 < H->reduce_blocks; ++b2) {
-        double x = part[2 * b2], t = S + x;
-        if (std::fabs(S) >= std::fabs(x))
-            Cc += (S - t) + x;
-        else
-            Cc += (x - t) + S;
-        S = t;
-        Cc += part[2 * b2 + 1];
-    }
-    *out = S + Cc;
-    return 0;
-}
 
 }  // namespace trisk
 
 #define CHECK_HANDLE(h) \
     if (!(h)) return set_err(TRISK_EINVAL, "null handle")
+#define CHECK_TRISK(h) \
+    if ((h)->L.icon) return set_err(TRISK_EINVAL, "not available for the ICON triangle model")
 
 extern "C" {
 
@@ -866,6 +764,7 @@ int trisk_destroy(trisk_handle* h) {
     (void)hipSetDevice(h->device);
     drop_graph(h);
     if (h->adj_release) h->adj_release(h);
+    if (h->icon_release) h->icon_release(h);
     for (auto& e : h->events) (void)hipEventDestroy(e);
     for (auto& b : h->owned) (void)hipFree(b.p);
     delete h;
@@ -935,10 +834,12 @@ int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* s
     if (!y_dev || !dy_dev) return set_err(TRISK_EINVAL, "null state pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->L.icon) return icon_tendency(h, y_dev, dy_dev, s);
     return h->precision == TRISK_FP64 ? tendency<Fp64>(h, y_dev, dy_dev, s) : tendency<Fp32Bulk>(h, y_dev, dy_dev, s);
 }
 
 static int run_steps(trisk_handle* h, double* y, double dt, int64_t n, hipStream_t s) {
+    if (h->L.icon) return icon_run_steps(h, y, dt, n, s);
     return h->precision == TRISK_FP64 ? rk4_steps<Fp64>(h, y, dt, n, s) : rk4_steps<Fp32Bulk>(h, y, dt, n, s);
 }
 
@@ -1033,6 +934,7 @@ int trisk_set_graphs(trisk_handle* h, int32_t on) {
 
 int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream) {
     CHECK_HANDLE(h);
+    CHECK_TRISK(h);
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc = h->precision == TRISK_FP64
@@ -1052,6 +954,7 @@ int trisk_vertex_pv(trisk_handle* h, const double* y_dev, double* qv_dev, double
     CHECK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->L.icon) return icon_vertex_pv(h, y_dev, qv_dev, zv_dev, hv_dev, s);
     double* q = qv_dev ? qv_dev : h->diag_qv;
     double* z = zv_dev ? zv_dev : h->diag_zv;
     double* e = hv_dev ? hv_dev : h->diag_hv;
@@ -1064,6 +967,7 @@ int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void*
     if (!y_dev || !out) return set_err(TRISK_EINVAL, "null pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->L.icon) return icon_diagnostics(h, y_dev, out, s);
     const Layout& L = h->L;
     int rc = trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);  // also fills diag_K
     if (rc) return rc;
@@ -1080,6 +984,7 @@ int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_
     CHECK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->L.icon) return icon_relative_vorticity(h, y_dev, zeta_dev, s);
     const Layout& L = h->L;
     // FP64 curl table: the FP64 parameters exist only in FP64 mode, so gather from host copies
     const int32_t* idx = h->precision == TRISK_FP64 ? h->p64.ve : h->p32.ve;
@@ -1097,6 +1002,7 @@ int trisk_tangential_velocity(trisk_handle* h, const double* u_dev, double* vt_d
     CHECK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->L.icon) return icon_tangential_velocity(h, u_dev, vt_dev, s);
     const Layout& L = h->L;
     if (h->precision != TRISK_FP64)
         return set_err(TRISK_EINVAL, "tangential_velocity is available on FP64 handles");
@@ -1152,7 +1058,9 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
     const double KC = L.kc, KV = L.kv, KT = L.kt;
     const double rk_e = 8.0 + 16.0 + uw, rk_c = 8.0 + 16.0 + 8.0;  // y, acc r/w, stage
     std::vector<std::pair<const char*, double>> v;
-    if (L.pair) {
+    if (L.icon) {
+        v = icon_kernel_bytes(h);
+    } else if (L.pair) {
         const double vc = f64 ? 7 * c : 6 * c;                  /* FP32: f rides in the index record */
         const double rec8 = f64 ? 6 * iw : 16.0;                 /* FP32: 16-bit deltas, 8-wide records */
         const double ew = f64 ? 2 * iw : 4.0;                    /* edge cells */
@@ -1194,6 +1102,7 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
 
 int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]) {
     CHECK_HANDLE(h);
+    CHECK_TRISK(h);
     counts[0] = h->L.wide_cells;
     counts[1] = h->L.wide_vertices;
     counts[2] = h->L.wide_edges;

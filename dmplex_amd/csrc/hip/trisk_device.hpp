@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "trisk_layout.hpp"
@@ -450,6 +451,62 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double x) {
 }
 
 // ---------------------------------------------------------------------------------
+// compensated sums: Neumaier per thread over a fixed chunk, then a wavefront butterfly and
+// the waves of the block in order, all on (sum, compensation) pairs combined with Knuth's
+// exact two-sum.  Both partners of a butterfly step compute the same pair (the two-sum
+// error of a + b is that of b + a), so the result does not depend on scheduling.
+
+// s + e = a + b exactly
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (s, c) <- (s, c) + (s2, c2)
+__device__ __forceinline__ void pair_add(double& s, double& c, double s2, double c2) {
+    double t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c = (c + c2) + e;
+}
+
+// term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
+static __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
+                                                        const double* __restrict__ a, const double* __restrict__ b,
+                                                        double g, double* __restrict__ partial) {
+    constexpr int kWave = 64, kWaves = kBlock / kWave;
+    __shared__ double ws[kWaves], wc[kWaves];
+    int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    int64_t lo = int64_t(blockIdx.x) * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0, c = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        double x;
+        if (term == 0)
+            x = area[i] * a[i];
+        else if (term == 1)
+            x = area[i] * a[i] * (b[i] + g * a[i] / 2);
+        else
+            x = area[i] * a[i] * (b[i] * b[i]);
+        neumaier(s, c, x);
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) pair_add(s, c, __shfl_xor(s, off), __shfl_xor(c, off));
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        ws[wave] = s;
+        wc[wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ws[0], Cc = wc[0];
+        for (int k = 1; k < kWaves; ++k) pair_add(S, Cc, ws[k], wc[k]);
+        partial[2 * blockIdx.x] = S;
+        partial[2 * blockIdx.x + 1] = Cc;
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // device memory owned by a handle
 
 struct DevBuf {
@@ -486,6 +543,23 @@ inline std::vector<float> to_float(const std::vector<double>& v) { return std::v
 }  // namespace trisk
 
 // One model on one device: its tables, work vectors, RK4 registers, captured graph and timing events.
+namespace trisk {
+
+// Hot-path kernels, for the timing events and the byte model.
+enum KernelId {
+    K_CELL_VERTEX = 0, K_EDGE = 1, K_UPDATE = 2, K_PAIR_STATE = 3, K_PAIR_CELL = 4, K_PAIR_EDGE = 5,
+    K_ICON_EDGE = 6, K_ICON_CELL_VERTEX = 7, K_ICON_DAMP_EDGE = 8, K_ICON_DAMP_CELL = 9, K_ICON_MOMENTUM = 10,
+    K_COUNT = 11
+};
+inline constexpr const char* kKernelNames[K_COUNT] = {
+    "cell_vertex_kernel",      "edge_kernel",           "update_kernel",          "pair_state_kernel",
+    "pair_cell_kernel",        "pair_edge_kernel",      "icon_edge_kernel",       "icon_cell_vertex_kernel",
+    "icon_damp_edge_kernel",   "icon_damp_cell_kernel", "icon_momentum_kernel"};
+
+struct IconState;  // device tables and work of an ICON handle (icon_device.hip)
+
+}  // namespace trisk
+
 struct trisk_handle {
     trisk::Layout L;
     int precision = TRISK_FP64;
@@ -518,6 +592,9 @@ struct trisk_handle {
     // adjoint work (trisk_adjoint.hip), allocated on first use and released with the handle
     void* adj = nullptr;
     void (*adj_release)(trisk_handle*) = nullptr;
+    // the ICON triangle model (L.icon): its device tables and work, released with the handle
+    trisk::IconState* icon = nullptr;
+    void (*icon_release)(trisk_handle*) = nullptr;
 };
 
 namespace trisk {
@@ -531,6 +608,64 @@ int pair_tendency_stage(trisk_handle* H, const void* u, bool u_double, const dou
                         hipStream_t s);
 // n fused RK4 steps of y with direct launches.
 int rk4_steps_direct(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s);
+
+// Timing events around one launch (when the handle's timing is on).
+inline int timed_begin(trisk_handle* H, int kernel, hipStream_t s, hipEvent_t* stop) {
+    *stop = nullptr;
+    if (!H->timing) return 0;
+    if (H->events_used + 2 > H->events.size()) {
+        for (int k = 0; k < 256; ++k) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            H->events.push_back(e);
+        }
+        H->event_kernel.resize(H->events.size() / 2);
+    }
+    size_t i = H->events_used;
+    H->events_used += 2;
+    H->event_kernel[i / 2] = kernel;
+    HIP_TRY(hipEventRecord(H->events[i], s));
+    *stop = H->events[i + 1];
+    return 0;
+}
+
+inline int timed_end(hipEvent_t stop, hipStream_t s) {
+    if (stop) HIP_TRY(hipEventRecord(stop, s));
+    return 0;
+}
+
+// Compensated sum of a diagnostic term (reduce_kernel) over n entities: a fixed-order
+// Neumaier combination of the block partials on the host.  Synchronous.
+inline int reduce(trisk_handle* H, int term, int64_t n, const double* area, const double* a, const double* b,
+                  double* out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(unsigned(H->reduce_blocks)), dim3(kBlock), 0, s, term, n, area, a, b,
+                       H->L.gravity, H->partial);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(2 * H->reduce_blocks);
+    HIP_TRY(hipMemcpyAsync(part.data(), H->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double S = 0.0, Cc = 0.0;
+    for (int b2 = 0; b2 < H->reduce_blocks; ++b2) {
+        double x = part[2 * b2], t = S + x;
+        if (std::fabs(S) >= std::fabs(x))
+            Cc += (S - t) + x;
+        else
+            Cc += (x - t) + S;
+        S = t;
+        Cc += part[2 * b2 + 1];
+    }
+    *out = S + Cc;
+    return 0;
+}
+
+// The generic calls on an ICON handle (icon_device.hip).
+int icon_tendency(trisk_handle* H, const double* y, double* dy, hipStream_t s);
+int icon_run_steps(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s);
+int icon_diagnostics(trisk_handle* H, const double* y, double out[3], hipStream_t s);
+int icon_vertex_pv(trisk_handle* H, const double* y, double* qv, double* zv, double* hv, hipStream_t s);
+int icon_relative_vorticity(trisk_handle* H, const double* y, double* zeta, hipStream_t s);
+int icon_tangential_velocity(trisk_handle* H, const double* u, double* vt, hipStream_t s);
+std::vector<std::pair<const char*, double>> icon_kernel_bytes(const trisk_handle* H);
 
 }  // namespace trisk
 

@@ -277,6 +277,103 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     return true;
 }
 
+bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err) {
+    if (!d) {
+        err = "null model descriptor";
+        return false;
+    }
+    if (d->nc <= 0 || d->ne <= 0 || d->nv <= 0 || d->kv <= 0 || d->kv > 16 || d->kr <= 0 || d->kr > 32 ||
+        d->nc >= (int64_t(1) << 31) || d->ne >= (int64_t(1) << 31) || d->nv >= (int64_t(1) << 31)) {
+        err = "bad sizes or table widths";
+        return false;
+    }
+    if (!d->cell_edges || !d->divergence || !d->bilinear || !d->edge_cells || !d->edge_vertices ||
+        !d->edge_height || !d->gradient || !d->inverse_dual || !d->rbf_edges || !d->rbf || !d->edge_coriolis ||
+        !d->vertex_edges || !d->curl || !d->vertex_cells || !d->vertex_height || !d->coriolis || !d->cell_areas ||
+        !d->vertex_areas) {
+        err = "missing table";
+        return false;
+    }
+    const int64_t nc = d->nc, ne = d->ne, nv = d->nv;
+    const int KV = int(d->kv), KR = int(d->kr);
+    struct Check {
+        const int64_t* t;
+        int64_t count, n;
+        const char* name;
+    } checks[] = {{d->cell_edges, 3 * nc, ne, "cell_edges"},      {d->edge_cells, 2 * ne, nc, "edge_cells"},
+                  {d->edge_vertices, 2 * ne, nv, "edge_vertices"}, {d->rbf_edges, KR * ne, ne, "rbf_edges"},
+                  {d->vertex_edges, KV * nv, ne, "vertex_edges"},  {d->vertex_cells, KV * nv, nc, "vertex_cells"}};
+    for (auto& c : checks)
+        if (!in_range(c.t, c.count, c.n)) {
+            err = std::string("index out of range in ") + c.name;
+            return false;
+        }
+    L = Layout();
+    L.icon = true;
+    L.nc = nc;
+    L.ne = ne;
+    L.nv = nv;
+    L.kc = 3;
+    L.kv = KV;
+    L.kr = KR;
+    L.gravity = d->gravity;
+    if (d->cell_points) {
+        hilbert_cells(nc, d->cell_points, L.cperm);
+    } else {
+        L.cperm.resize(nc);
+        std::iota(L.cperm.begin(), L.cperm.end(), int64_t(0));
+    }
+    std::vector<char> seen(ne, 0);
+    L.eperm.reserve(ne);
+    for (int64_t j = 0; j < nc; ++j)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t e = d->cell_edges[k + 3 * L.cperm[j]];
+            if (!seen[e]) {
+                seen[e] = 1;
+                L.eperm.push_back(e);
+            }
+        }
+    for (int64_t e = 0; e < ne; ++e)
+        if (!seen[e]) L.eperm.push_back(e);
+    std::vector<char> vseen(nv, 0);
+    L.vperm.reserve(nv);
+    for (int64_t j = 0; j < ne; ++j)
+        for (int s = 0; s < 2; ++s) {
+            const int64_t v = d->edge_vertices[s + 2 * L.eperm[j]];
+            if (!vseen[v]) {
+                vseen[v] = 1;
+                L.vperm.push_back(v);
+            }
+        }
+    for (int64_t v = 0; v < nv; ++v)
+        if (!vseen[v]) L.vperm.push_back(v);
+    invert(L.cperm, L.cinv);
+    invert(L.eperm, L.einv);
+    invert(L.vperm, L.vinv);
+
+    remap_index(d->cell_edges, 3, nc, L.cperm, L.einv, L.ce);
+    remap_value(d->divergence, 3, nc, L.cperm, L.div);
+    remap_value(d->bilinear, 3, nc, L.cperm, L.bln);
+    remap_value(d->cell_areas, 1, nc, L.cperm, L.area_c);
+    remap_index(d->edge_cells, 2, ne, L.eperm, L.cinv, L.ec);
+    remap_index(d->edge_vertices, 2, ne, L.eperm, L.vinv, L.ev);
+    remap_value(d->edge_height, 2, ne, L.eperm, L.eh);
+    remap_value(d->gradient, 2, ne, L.eperm, L.grad);
+    remap_value(d->inverse_dual, 1, ne, L.eperm, L.invd);
+    remap_index(d->rbf_edges, KR, ne, L.eperm, L.einv, L.re);
+    remap_value(d->rbf, KR, ne, L.eperm, L.rw);
+    remap_value(d->edge_coriolis, 1, ne, L.eperm, L.fe);
+    if (d->divergence_damping) remap_value(d->divergence_damping, 1, ne, L.eperm, L.damp);
+    remap_index(d->vertex_edges, KV, nv, L.vperm, L.einv, L.ve);
+    remap_value(d->curl, KV, nv, L.vperm, L.curl);
+    remap_index(d->vertex_cells, KV, nv, L.vperm, L.cinv, L.vc);
+    remap_value(d->vertex_height, KV, nv, L.vperm, L.vh);
+    remap_value(d->coriolis, 1, nv, L.vperm, L.fv);
+    remap_value(d->vertex_areas, 1, nv, L.vperm, L.area_v);
+    L.pair_reason = "ICON triangle model: RBF tangential velocity, bilinear kinetic energy";
+    return true;
+}
+
 void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.pair = false;
     if (!d->primal_lengths || !d->dual_lengths || !d->cell_kites) {

@@ -65,6 +65,16 @@ struct Layout {
     std::vector<double> adj_ekin;        // ne x 2: kinetic-energy weight of edge e in cell ec[s]
     std::vector<double> adj_cvh;         // 6 x nc: vertex-height weight of cell i in its corner k
     std::vector<int32_t> adj_vslot;      // nv: corner index of v in its cell j at bits 4j..4j+3
+    // ICON triangle model (icon_create): kc = 3; ce, div, ec, ev, eh (c_lin_e), invd, ve,
+    // curl, vc, vh, fv and the areas as above, plus
+    bool icon = false;
+    int kr = 0;                          // RBF stencil width
+    std::vector<double> bln;             // 3 x nc: bilinear edge-to-cell weights
+    std::vector<double> grad;            // 2 x ne: kinetic-energy gradient coefficients
+    std::vector<int32_t> re;             // kr x ne: RBF stencil edges
+    std::vector<double> rw;              // kr x ne: RBF weights
+    std::vector<double> fe;              // ne: Coriolis parameter at the edges
+    std::vector<double> damp;            // ne: divergence damping (empty: none)
 };
 
 constexpr int16_t kWide = -32768;
@@ -78,6 +88,11 @@ inline int pair_index(int k, int k2) {  // k < k2 < 6
 // Validates `d` (every index in range, widths positive) and fills `L`.  Returns false
 // with a message on a bad table.
 bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
+
+// Validates an ICON descriptor (triangles, every index in range) and fills the generic
+// fields of `L` (sizes, permutations, ce, div, ec, ev, eh, invd, ve, curl, vc, vh, fv,
+// areas) and the ICON ones, in the same numbering as build_layout.
+bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err);
 
 // Builds the cell-pair tables from the grid fields: w_{k,k'} = s_k s_k' (1/2 - sum of the
 // kite fractions of the corners k+1..k') (TRiSKGrid.jl:140-155), and checks that

@@ -145,38 +145,16 @@ class ModelTables:
         D.check(D.lib().trisk_check_tables(C.byref(desc), C.byref(flag)))
         return bool(flag.value), D.lib().trisk_last_error().decode()
 
-class TRiSKShallowWater(ModelTables):
-    """Nonlinear rotating shallow water in vector-invariant form on the Voronoi C grid
-    with the TRiSK operators (TRiSKShallowWater.jl:1-66), on a HIP device."""
 
-    def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, pv_flux: str = "trisk",
-                 kinetic_energy="cell", comm=None, part=None, backend=None, hilbert: bool = True,
-                 path: str = "auto", narrow_limit: int = 0):
-        if comm is not None or part is not None:
-            raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
-        if backend is None:
-            backend = HIPBackend()
-        if not isinstance(backend, HIPBackend):
-            raise NotImplementedError("only HIPBackend() is available: the product has no CPU path")
-        super().__init__(dm_or_grid, centers, gravity, coriolis, pv_flux, kinetic_energy)
-        self.backend = backend
-        self._hilbert = hilbert
-        if path not in D.PATHS:
-            raise ValueError("path must be 'auto', 'tables' or 'pair'")
-        self._path = path
-        self._narrow_limit = int(narrow_limit)
-        self._handle = None
-        self._create()
+class CGridShallowWater:
+    """Device state and the generic calls of the C-grid shallow-water models
+    (``CGridShallowWater``, TRiSKShallowWater.jl:172-198): the handle of the device library,
+    the state layout [u; h] and the transfers between the reference's global order and the
+    library's internal order.  ``TRiSKShallowWater`` and ``ICONShallowWater`` derive from it."""
 
-    def _create(self) -> None:
-        torch = _torch()
-        torch.cuda.set_device(self.backend.device)
-        opts = D.TriskOptions(D.TRISK_FP64 if self.backend.precision == "fp64" else D.TRISK_FP32_BULK,
-                              self.backend.device, 0, D.PATHS[getattr(self, "_path", "auto")],
-                              getattr(self, "_narrow_limit", 0))
-        h = C.c_void_p()
-        desc = self.descriptor(self._hilbert)
-        D.check(D.lib().trisk_create(C.byref(desc), C.byref(opts), C.byref(h)))
+    _handle = None
+
+    def _finish_create(self, h: C.c_void_p) -> None:
         self._handle = h
         n = np.zeros(1, dtype=np.int64)
         D.check(D.lib().trisk_state_length(h, ptr(n, i64p)))
@@ -195,22 +173,9 @@ class TRiSKShallowWater(ModelTables):
     def handle(self) -> C.c_void_p:
         return self._handle
 
-    def wide_counts(self) -> tuple[int, int, int]:
-        """Cells, vertices, edges read with int32 instead of 16-bit-delta indices."""
-        n = np.zeros(3, dtype=np.int64)
-        D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
-        return tuple(int(x) for x in n)
-
     def use_graphs(self, on: bool = True) -> None:
         """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""
         D.check(D.lib().trisk_set_graphs(self._handle, int(on)))
-
-    def path_info(self) -> tuple[bool, str]:
-        """(cell-pair path?, reason) of the device kernels."""
-        flag = C.c_int32()
-        note = C.c_char_p()
-        D.check(D.lib().trisk_info(self._handle, C.byref(flag), C.byref(note)))
-        return bool(flag.value), note.value.decode()
 
     def permutation(self):
         """(cells, edges, vertices): reference index of every device entity."""
@@ -220,6 +185,7 @@ class TRiSKShallowWater(ModelTables):
         return c, e, v
 
     # --- state transfer -----------------------------------------------------------
+
     def zeros(self):
         torch = _torch()
         return torch.zeros(self.state_length, dtype=torch.float64, device=f"cuda:{self.backend.device}")
@@ -265,25 +231,73 @@ class TRiSKShallowWater(ModelTables):
         return y, False
 
 
+class TRiSKShallowWater(ModelTables, CGridShallowWater):
+    """Nonlinear rotating shallow water in vector-invariant form on the Voronoi C grid
+    with the TRiSK operators (TRiSKShallowWater.jl:1-66), on a HIP device."""
+
+    def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, pv_flux: str = "trisk",
+                 kinetic_energy="cell", comm=None, part=None, backend=None, hilbert: bool = True,
+                 path: str = "auto", narrow_limit: int = 0):
+        if comm is not None or part is not None:
+            raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
+        if backend is None:
+            backend = HIPBackend()
+        if not isinstance(backend, HIPBackend):
+            raise NotImplementedError("only HIPBackend() is available: the product has no CPU path")
+        super().__init__(dm_or_grid, centers, gravity, coriolis, pv_flux, kinetic_energy)
+        self.backend = backend
+        self._hilbert = hilbert
+        if path not in D.PATHS:
+            raise ValueError("path must be 'auto', 'tables' or 'pair'")
+        self._path = path
+        self._narrow_limit = int(narrow_limit)
+        self._handle = None
+        self._create()
+
+    def _create(self) -> None:
+        torch = _torch()
+        torch.cuda.set_device(self.backend.device)
+        opts = D.TriskOptions(D.TRISK_FP64 if self.backend.precision == "fp64" else D.TRISK_FP32_BULK,
+                              self.backend.device, 0, D.PATHS[getattr(self, "_path", "auto")],
+                              getattr(self, "_narrow_limit", 0))
+        h = C.c_void_p()
+        desc = self.descriptor(self._hilbert)
+        D.check(D.lib().trisk_create(C.byref(desc), C.byref(opts), C.byref(h)))
+        self._finish_create(h)
+
+    def wide_counts(self) -> tuple[int, int, int]:
+        """Cells, vertices, edges read with int32 instead of 16-bit-delta indices."""
+        n = np.zeros(3, dtype=np.int64)
+        D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
+        return tuple(int(x) for x in n)
+
+    def path_info(self) -> tuple[bool, str]:
+        """(cell-pair path?, reason) of the device kernels."""
+        flag = C.c_int32()
+        note = C.c_char_p()
+        D.check(D.lib().trisk_info(self._handle, C.byref(flag), C.byref(note)))
+        return bool(flag.value), note.value.decode()
+
+
 # --- state layout (TRiSKShallowWater.jl:172-195) ---------------------------------------
 
-def velocity_dofs(m: TRiSKShallowWater) -> slice:
+def velocity_dofs(m: CGridShallowWater) -> slice:
     return slice(0, m.grid.num_edges())
 
 
-def height_dofs(m: TRiSKShallowWater) -> slice:
+def height_dofs(m: CGridShallowWater) -> slice:
     return slice(m.grid.num_edges(), m.grid.num_edges() + m.grid.num_cells())
 
 
 global_velocity_dofs, global_height_dofs = velocity_dofs, height_dofs
 
 
-def initial_state_values(m: TRiSKShallowWater, u: np.ndarray, h: np.ndarray):
+def initial_state_values(m: CGridShallowWater, u: np.ndarray, h: np.ndarray):
     """Device state from host edge-normal velocities and cell heights (reference order)."""
     return m.upload_state(np.concatenate([np.asarray(u, dtype=np.float64), np.asarray(h, dtype=np.float64)]))
 
 
-def initial_state(m: TRiSKShallowWater, velocity, height):
+def initial_state(m: CGridShallowWater, velocity, height):
     """Normal components of ``velocity(x)`` at the edge points and ``height(x)`` at the
     generators (TRiSKShallowWater.jl:184-188); both callables take an (n, 3) array."""
     g = m.grid
@@ -293,7 +307,7 @@ def initial_state(m: TRiSKShallowWater, velocity, height):
     return initial_state_values(m, u, h)
 
 
-def gather_state(m: TRiSKShallowWater, y) -> np.ndarray:
+def gather_state(m: CGridShallowWater, y) -> np.ndarray:
     """Global state [u; h] on the host (TRiSKShallowWater.jl:191-195)."""
     if isinstance(y, np.ndarray):
         return y.copy()
@@ -302,7 +316,7 @@ def gather_state(m: TRiSKShallowWater, y) -> np.ndarray:
 
 # --- tendency and time stepping -----------------------------------------------------
 
-def tendency_(dy, y, m: TRiSKShallowWater):
+def tendency_(dy, y, m: CGridShallowWater):
     """``tendency!(dy, y, m)`` (TRiSKShallowWater.jl:379-394): writes and returns ``dy``."""
     torch = _torch()
     if isinstance(y, np.ndarray):
@@ -342,11 +356,11 @@ class RK4Cache:
 def rk4_step_(y, dt: float, tendency, cache: RK4Cache, *args):
     """Classical RK4 step in place (RungeKutta.jl:18-28).
 
-    With ``tendency_`` and a ``TRiSKShallowWater`` the whole step runs on the device as
-    twelve fused launches (``trisk_rk4_steps``); otherwise the reference's broadcasts
-    are evaluated on whatever array type ``y`` is.
+    With ``tendency_`` and a ``TRiSKShallowWater`` or ``ICONShallowWater`` the whole step
+    runs on the device as fused launches (``trisk_rk4_steps``); otherwise the reference's
+    broadcasts are evaluated on whatever array type ``y`` is.
     """
-    if tendency is tendency_ and len(args) == 1 and isinstance(args[0], TRiSKShallowWater) \
+    if tendency is tendency_ and len(args) == 1 and isinstance(args[0], CGridShallowWater) \
             and not isinstance(y, np.ndarray):
         return rk4_steps_(y, dt, 1, args[0])
     k1, k2, k3, k4, stage = cache.vectors()
@@ -361,7 +375,7 @@ def rk4_step_(y, dt: float, tendency, cache: RK4Cache, *args):
     return y
 
 
-def rk4_steps_(y, dt: float, nsteps: int, m: TRiSKShallowWater):
+def rk4_steps_(y, dt: float, nsteps: int, m: CGridShallowWater):
     """``nsteps`` fused RK4 steps of the model on the device state ``y``, in place."""
     torch = _torch()
     if y.dtype != torch.float64 or not y.is_contiguous() or y.numel() != m.state_length:
@@ -372,7 +386,7 @@ def rk4_steps_(y, dt: float, nsteps: int, m: TRiSKShallowWater):
 
 # --- diagnostics (TRiSKShallowWater.jl:396-433) -----------------------------------------
 
-def diagnostics(m: TRiSKShallowWater, y) -> tuple[float, float, float]:
+def diagnostics(m: CGridShallowWater, y) -> tuple[float, float, float]:
     """(total mass, total energy, potential enstrophy), compensated sums."""
     torch = _torch()
     yd, _ = m._device_state(y)
@@ -381,17 +395,17 @@ def diagnostics(m: TRiSKShallowWater, y) -> tuple[float, float, float]:
     return float(out[0]), float(out[1]), float(out[2])
 
 
-def total_mass(m: TRiSKShallowWater, y) -> float:
+def total_mass(m: CGridShallowWater, y) -> float:
     """Sum_i A_i h_i (TRiSKShallowWater.jl:397-399)."""
     return diagnostics(m, y)[0]
 
 
-def total_energy(m: TRiSKShallowWater, y) -> float:
+def total_energy(m: CGridShallowWater, y) -> float:
     """Sum_i A_i (h_i K_i + g h_i^2 / 2) (TRiSKShallowWater.jl:406-411)."""
     return diagnostics(m, y)[1]
 
 
-def potential_enstrophy(m: TRiSKShallowWater, y) -> float:
+def potential_enstrophy(m: CGridShallowWater, y) -> float:
     """Sum_v A_v h_v q_v^2 / 2 (TRiSKShallowWater.jl:414-419)."""
     return diagnostics(m, y)[2]
 
@@ -414,7 +428,7 @@ def kinetic_energy_(m: TRiSKShallowWater, y) -> KineticWork:
     return KineticWork(K, B)
 
 
-def vertex_pv_(m: TRiSKShallowWater, y):
+def vertex_pv_(m: CGridShallowWater, y):
     """(q_v, zeta_v, h_v) at the vertices (vertex_pv!, TRiSKShallowWater.jl:355-360)."""
     torch = _torch()
     yd, host = m._device_state(y)
@@ -426,7 +440,7 @@ def vertex_pv_(m: TRiSKShallowWater, y):
     return tuple(out)
 
 
-def relative_vorticity(m: TRiSKShallowWater, y):
+def relative_vorticity(m: CGridShallowWater, y):
     """zeta_v at the vertices (TRiSKShallowWater.jl:422-426)."""
     torch = _torch()
     yd, host = m._device_state(y)
@@ -436,7 +450,7 @@ def relative_vorticity(m: TRiSKShallowWater, y):
     return m.download_field(2, z) if host else z
 
 
-def tangential_velocity(m: TRiSKShallowWater, u):
+def tangential_velocity(m: CGridShallowWater, u):
     """(1/d_e) Sum w_{e,e'} l_e' u_e' at the edges (TRiSKShallowWater.jl:429-433)."""
     torch = _torch()
     host = isinstance(u, np.ndarray)

@@ -158,6 +158,20 @@ int dmx_gassmann_coefficients(int64_t nc, int64_t ne, int64_t nv, const dmx_grid
                               const double* vertices, int64_t* edges, int64_t* pv_edges,
                               double* weights);
 
+/* --- ICON triangle model tables -------------------------------------------------- */
+
+/* rbf_coefficients(g; scale) (src/FiniteVolumes/TRiSKGrid.jl:254-281): for every edge the
+ * other edges of its two cells, the inverse-multiquadric vector-RBF weights of the
+ * tangential velocity, normalized along x_e x n_e.  edges, weights: kr x ne (pads index
+ * 0, weight 0).  With edges == NULL only *width (the longest stencil) is returned.
+ * Error -2 when an RBF matrix is not positive definite. */
+int dmx_rbf_coefficients(int64_t nc, int64_t ne, const dmx_grid_tables* g, double scale, int64_t kr,
+                         int64_t* edges, double* weights, int64_t* width);
+/* bilinear_weights(g) (src/FiniteVolumes/ICONShallowWater.jl:79-96): 3 x nc weights with
+ * sum 1 and no first moment in the tangent plane of the cell center.  Error -3 unless
+ * every cell is a triangle. */
+int dmx_bilinear_weights(int64_t nc, const dmx_grid_tables* g, const double* centers, double* weights);
+
 /* --- ordering -------------------------------------------------------------------- */
 
 /* Hilbert key of integer points (hilbert_key, src/Topology/Ordering.jl:10-41):

@@ -196,6 +196,54 @@ int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]);
  * (text owned by the handle). */
 int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
 
+/* --- ICON triangle model ----------------------------------------------------------
+ * ICONShallowWater(grid; gravity, coriolis, rbf_scale, divergence_damping, kinetic_gradient)
+ *   src/FiniteVolumes/ICONShallowWater.jl:96-143                            -> icon_create
+ * The triangular C grid of ICON's dynamical core: RBF tangential velocity, bilinear
+ * edge-to-cell kinetic energy, ICON's kinetic-energy gradient, vertex vorticity averaged to
+ * the edges and optional fourth-order divergence damping.  icon_create returns a
+ * trisk_handle, and the generic calls take it as they take a TRiSK handle (both models
+ * are the reference's CGridShallowWater, with the same state layout [u; h]):
+ * trisk_destroy, trisk_state_length, trisk_permutation, trisk_upload/download_state and
+ * _field, trisk_tendency (tendency!, ICONShallowWater.jl:191-207), trisk_rk4_steps,
+ * trisk_set_graphs, trisk_diagnostics, trisk_vertex_pv, trisk_relative_vorticity,
+ * trisk_tangential_velocity (the RBF tangential velocity, :219-224), trisk_set_timing,
+ * trisk_kernel_times, trisk_kernel_bytes and trisk_info.  The TRiSK-only calls
+ * (trisk_kinetic_energy, trisk_wide_counts, trisk_tendency_vjp, trisk_rk4_adjoint) return
+ * TRISK_EINVAL on an ICON handle.  Only TRISK_FP64 is available for this model. */
+typedef struct {
+    int64_t nc, ne, nv;
+    int64_t kv;                        /* vertex table width (6 on icosahedral meshes) */
+    int64_t kr;                        /* RBF stencil width (4 on triangles) */
+    const int64_t* cell_edges;         /* 3 x nc */
+    const double* divergence;          /* 3 x nc: n l / A */
+    const double* bilinear;            /* 3 x nc: e_bln_c_s (bilinear_weights, :82-94) */
+    const int64_t* edge_cells;         /* 2 x ne */
+    const int64_t* edge_vertices;      /* 2 x ne */
+    const double* edge_height;         /* 2 x ne: c_lin_e */
+    const double* gradient;            /* 2 x ne: coeff_gradekin */
+    const double* inverse_dual;        /* ne */
+    const int64_t* rbf_edges;          /* kr x ne (rbf_coefficients, TRiSKGrid.jl:254-281) */
+    const double* rbf;                 /* kr x ne */
+    const double* edge_coriolis;       /* ne */
+    const int64_t* vertex_edges;       /* kv x nv */
+    const double* curl;                /* kv x nv: t d / A_v */
+    const int64_t* vertex_cells;       /* kv x nv */
+    const double* vertex_height;       /* kv x nv: K_iv / A_v (diagnostics) */
+    const double* coriolis;            /* nv (diagnostics) */
+    const double* cell_areas;          /* nc */
+    const double* vertex_areas;        /* nv */
+    double gravity;
+    const double* divergence_damping;  /* ne (nu_4 per edge), or NULL without damping */
+    const double* cell_points;         /* nc x 3: Hilbert order of the cells, or NULL */
+} icon_model_desc;
+
+/* Host-only validation of an ICON descriptor (every index in range).  Touches no device. */
+int icon_check_tables(const icon_model_desc* desc);
+/* ICONShallowWater(grid; ...; backend): validates, re-lays out and uploads the tables.
+ * Synchronous. */
+int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_handle** out);
+
 /* --- adjoint ------------------------------------------------------------------------
  * No counterpart in the reference (it has no adjoint).  These are the pullbacks of
  * tendency! (src/FiniteVolumes/TRiSKShallowWater.jl:379-394) and of nsteps rk4_step!

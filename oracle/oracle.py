@@ -1,5 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``liboracle.so``, the FP64 CPU
-restatement of the reference TRiSK path (trisk_oracle.c).
+restatement of the reference TRiSK path (trisk_oracle.c) and of the ICON triangle model
+(icon_oracle.c).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it,
 as the checker; the product (``dmplex_amd``) never imports it.
@@ -29,6 +30,17 @@ class OracleModel(C.Structure):
                 ("g_coriolis", f64p), ("g_mean", C.c_int),
                 ("K", f64p), ("B", f64p), ("F", f64p), ("qe", f64p), ("zv", f64p), ("hv", f64p), ("qv", f64p),
                 ("Kv", f64p)]
+
+
+class IconOracleModel(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("kv", i64), ("kr", i64),
+                ("cell_edges", i64p), ("divergence", f64p), ("bilinear", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_height", f64p), ("gradient", f64p), ("inverse_dual", f64p),
+                ("rbf_edges", i64p), ("rbf", f64p), ("edge_coriolis", f64p), ("vertex_edges", i64p),
+                ("curl", f64p), ("vertex_cells", i64p), ("vertex_height", f64p), ("coriolis", f64p),
+                ("cell_areas", f64p), ("vertex_areas", f64p), ("gravity", f64), ("damping", f64p),
+                ("v", f64p), ("Ke", f64p), ("K", f64p), ("F", f64p), ("zv", f64p), ("hv", f64p), ("qv", f64p),
+                ("D", f64p), ("G", f64p)]
 
 
 _lib = None
@@ -62,6 +74,13 @@ def lib() -> C.CDLL:
             ("oracle_edge_sum", [i64, i64, vp, vp, vp, vp], None),
             ("oracle_tendency_vjp", [M, vp, vp, vp], C.c_int),
             ("oracle_rk4_adjoint", [M, vp, f64, i64, vp], C.c_int),
+            ("icon_oracle_tendency", [C.POINTER(IconOracleModel), vp, vp], None),
+            ("icon_oracle_rk4_steps", [C.POINTER(IconOracleModel), vp, f64, i64, vp, vp, vp, vp, vp], None),
+            ("icon_oracle_total_mass", [C.POINTER(IconOracleModel), vp], f64),
+            ("icon_oracle_total_energy", [C.POINTER(IconOracleModel), vp], f64),
+            ("icon_oracle_potential_enstrophy", [C.POINTER(IconOracleModel), vp], f64),
+            ("icon_oracle_kinetic_energy", [C.POINTER(IconOracleModel), vp, vp], None),
+            ("icon_oracle_vertex_pv", [C.POINTER(IconOracleModel), vp, vp], None),
             ("oracle_threads", [], C.c_int),
             ("oracle_set_threads", [C.c_int], None),
         ]:
@@ -179,3 +198,69 @@ class Oracle:
     @staticmethod
     def set_threads(n: int) -> None:
         lib().oracle_set_threads(n)
+
+
+class IconOracle:
+    """The reference ICON triangle model (ICONShallowWater.jl) on the host tables of an
+    ``ICONTables``/``ICONShallowWater`` (reference layout)."""
+
+    def __init__(self, m):
+        self.m = m
+        g = m.grid
+        nc, ne, nv = g.num_cells(), g.num_edges(), g.num_vertices()
+        self.nc, self.ne, self.nv = nc, ne, nv
+        self.work = {k: np.zeros(n) for k, n in dict(v=ne, Ke=ne, K=nc, F=ne, zv=nv, hv=nv, qv=nv, D=nc, G=ne).items()}
+        w = self.work
+        self.s = IconOracleModel(
+            nc, ne, nv, m.vertex_edges.shape[0], m.rbf_edges.shape[0],
+            _p(m.cell_edges, i64p), _p(m.divergence, f64p), _p(m.bilinear, f64p), _p(m.edge_cells, i64p),
+            _p(m.edge_vertices, i64p), _p(m.edge_height, f64p), _p(m.gradient, f64p), _p(m.inverse_dual, f64p),
+            _p(m.rbf_edges, i64p), _p(m.rbf, f64p), _p(m.edge_coriolis, f64p), _p(m.vertex_edges, i64p),
+            _p(m.curl, f64p), _p(m.vertex_cells, i64p), _p(m.vertex_height, f64p), _p(m.coriolis, f64p),
+            _p(m.cell_areas, f64p), _p(m.vertex_areas, f64p), m.gravity, _p(m.divergence_damping, f64p),
+            *(_p(w[k], f64p) for k in ("v", "Ke", "K", "F", "zv", "hv", "qv", "D", "G")))
+        self._cache = None
+
+    def _ptr(self, a):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data_as(C.c_void_p)
+
+    def tendency(self, y: np.ndarray) -> np.ndarray:
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        dy = np.empty_like(y)
+        lib().icon_oracle_tendency(C.byref(self.s), self._ptr(y), self._ptr(dy))
+        return dy
+
+    def rk4_steps(self, y: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
+        """nsteps RK4 steps in place on the host vector y."""
+        if self._cache is None or len(self._cache[0]) != len(y):
+            self._cache = [np.empty_like(y) for _ in range(5)]
+        lib().icon_oracle_rk4_steps(C.byref(self.s), self._ptr(y), dt, nsteps, *(self._ptr(a) for a in self._cache))
+        return y
+
+    def total_mass(self, y):
+        return lib().icon_oracle_total_mass(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def total_energy(self, y):
+        return lib().icon_oracle_total_energy(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def potential_enstrophy(self, y):
+        return lib().icon_oracle_potential_enstrophy(C.byref(self.s), self._ptr(np.ascontiguousarray(y)))
+
+    def kinetic_energy(self, y):
+        """(v, K_e, F at the edges; K at the cells) of the state y."""
+        y = np.ascontiguousarray(y)
+        lib().icon_oracle_kinetic_energy(C.byref(self.s), self._ptr(y), self._ptr(y[self.ne:]))
+        w = self.work
+        return w["v"].copy(), w["Ke"].copy(), w["F"].copy(), w["K"].copy()
+
+    def vertex_pv(self, y):
+        y = np.ascontiguousarray(y)
+        lib().icon_oracle_vertex_pv(C.byref(self.s), self._ptr(y), self._ptr(y[self.ne:]))
+        return self.work["qv"].copy(), self.work["zv"].copy(), self.work["hv"].copy()
+
+    def relative_vorticity(self, y):
+        return Oracle.edge_sum(self, self.m.vertex_edges, self.m.curl, y[:self.ne])
+
+    def tangential_velocity(self, u):
+        return Oracle.edge_sum(self, self.m.rbf_edges, self.m.rbf, u)

@@ -21,7 +21,8 @@ def declared(header: str, prefix: str):
     return sorted(set(re.findall(rf"\b({prefix}\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,prefix,loader", [("trisk_hip.h", "trisk_", D.lib), ("dmplex_host.h", "dmx_", N.host)])
+@pytest.mark.parametrize("header,prefix,loader", [("trisk_hip.h", "(?:trisk|icon)_", D.lib),
+                                                  ("dmplex_host.h", "dmx_", N.host)])
 def test_every_declared_symbol_is_exported(header, prefix, loader):
     names = declared(header, prefix)
     assert len(names) > 5
@@ -31,21 +32,22 @@ def test_every_declared_symbol_is_exported(header, prefix, loader):
 
 
 def test_python_binding_covers_the_header():
-    names = declared("trisk_hip.h", "trisk_")
+    names = declared("trisk_hip.h", "(?:trisk|icon)_")
     assert set(names) - {"trisk_last_error"} == set(D.SIGNATURES)
 
 
-def test_desc_struct_matches_header_field_order():
+@pytest.mark.parametrize("name,struct", [("trisk_model_desc", D.TriskModelDesc), ("icon_model_desc", D.IconModelDesc)])
+def test_desc_struct_matches_header_field_order(name, struct):
     text = (ROOT / "include" / "trisk_hip.h").read_text()
-    body = text[text.index("typedef struct {\n    int64_t nc, ne, nv;"):text.index("} trisk_model_desc;")]
+    end = text.index("} " + name + ";")
+    body = text[text.rindex("typedef struct {", 0, end):end]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = []
     for decl in body.split(";")[:-1]:
         decl = decl.replace("typedef struct {", "").strip()
-        names = decl.split()[-1] if "," not in decl else decl
         for part in re.split(r",", decl):
             fields.append(re.sub(r"[*\s]", " ", part).split()[-1])
-    assert fields == [f[0] for f in D.TriskModelDesc._fields_]
+    assert fields == [f[0] for f in struct._fields_]
 
 
 def test_check_tables_accepts_and_rejects_on_the_host():
