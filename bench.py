@@ -195,7 +195,7 @@ def run_icon(args):
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
     t = time.perf_counter()
-    dm = create_spherical_mesh(args.icon_level, centroidal=True, device=0)
+    dm = create_spherical_mesh(args.icon_level, centroidal=not args.no_scvt, device=0)
     log(f"[bench] icon: SCVT triangle mesh level {args.icon_level} ({dm.nc} cells) in {time.perf_counter() - t:.1f} s")
     dt, steps = args.icon_dt, args.icon_steps
     nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
