@@ -7,8 +7,9 @@ stage updates, 12 kernel launches) with the state resident in HBM.  The FP64 pat
 the FP32-bulk path (FP32 stencils, FP64 state/heights/accumulation) are timed in the
 same run; ``value`` is the FP32-bulk throughput in cell-steps/s.
 
-Beside the headline the line carries the batched ensemble, the adjoint and the ICON
-triangle model (ICONShallowWater at the cell count of ICON's R02B08) measurements.
+Beside the headline the line carries the batched ensemble, the adjoint, the ICON triangle
+model (ICONShallowWater at the cell count of ICON's R02B08) and the MPAS-Gassmann stage
+operators (hexagon 7 x 30 levels) measurements.
 
 Prints one JSON line (rank 0).  The CPU baseline is the oracle (FP64 restatement of the
 reference kernels, OpenMP over the host cores) on a bounded sample of the same
@@ -82,6 +83,10 @@ def parse():
     p.add_argument("--icon-level", type=int, default=9, help="SCVT triangle level of the ICON measurement")
     p.add_argument("--icon-steps", type=int, default=100, help="timed RK4 steps of the ICON measurement")
     p.add_argument("--icon-dt", type=float, default=20.0, help="RK4 dt (s) of the ICON measurement")
+    p.add_argument("--no-mpas", action="store_true", help="skip the MPAS-Gassmann operator measurement")
+    p.add_argument("--mpas-only", action="store_true", help="only the MPAS-Gassmann operator measurement")
+    p.add_argument("--mpas-level", type=int, default=7, help="hexagon level of the MPAS-Gassmann measurement")
+    p.add_argument("--mpas-calls", type=int, default=50, help="timed slow-tendency calls")
     return p.parse_args()
 
 
@@ -250,6 +255,69 @@ def run_icon(args):
             f"reference's A100), kernel-model HBM fraction {r['hbm_frac_kernel_model']:.3f}")
         del m
         torch.cuda.empty_cache()
+    return out
+
+
+MPAS_REFERENCE_VN_MS = 4.5  # vn_tendency per step (two calls), hexagon 7, 30 layers, one A100 (docs/EXAMPLES.md:238-249)
+
+
+def run_mpas(args):
+    """The explicit stage tendencies of MPAS-Gassmann (slow_tendencies!, pressure 0 as in the
+    ICON-IAP step) on the SCVT hexagon mesh of level 7 with 30 layers to 30 km, FP64.  The
+    reference publishes 4.5 ms of vn_tendency per step (two calls) on one A100 there."""
+    import torch
+
+    from dmplex_amd import TRiSKGrid, create_hexagon_mesh
+    from dmplex_amd.mpas_ops import MPASOperators, vertical_grid
+
+    t = time.perf_counter()
+    dm = create_hexagon_mesh(args.mpas_level, centroidal=not args.no_scvt, device=0)
+    g = TRiSKGrid(dm, centers=dm.generators)
+    v = vertical_grid(30, 30000.0)
+    m = MPASOperators(g, v)
+    log(f"[bench] mpas: hexagon {args.mpas_level} x 30 levels, tables and upload in {time.perf_counter() - t:.1f} s")
+    rng = np.random.default_rng(5)
+    nl, nc, ne = 30, g.num_cells(), g.num_edges()
+    rho = np.exp(-v.z_full / 8000.0)[:, None] * (1 + 0.05 * rng.random((nl, nc)))
+    rt = rho * 300.0 * np.exp(v.z_full / 20000.0)[:, None]
+    w = 0.5 * rng.standard_normal((nl + 1, nc))
+    w[0] = w[-1] = 0
+    dev = m.upload_state(20 * rng.standard_normal((nl, ne)), w, rho, rt)
+    for _ in range(3):
+        m.slow_tendencies(*dev, pressure=0.0)
+    m.set_timing(True)
+    for _ in range(args.mpas_calls):
+        m.slow_tendencies(*dev, pressure=0.0)
+    stats = m.kernel_stats()
+    m.set_timing(False)
+    stream = torch.cuda.current_stream()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(args.mpas_calls):
+        m.slow_tendencies(*dev, pressure=0.0)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.mpas_calls
+    vn_ms = next(s["avg_ms"] for s in stats if s["name"] == "mpas_vn_tendency_kernel")
+    kbytes = sum(s["bytes"] for s in stats)
+    dom = max(stats, key=lambda k: k["avg_ms"])
+    achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
+    out = dict(workload=f"MPAS-Gassmann slow_tendencies! (ICON-IAP PV, mass-flux vertical coupling, five-band "
+                        f"solver's K_w order), SCVT hexagon {args.mpas_level} ({nc} cells, {ne} edges) x 30 levels, "
+                        f"FP64, flat ground", ms_per_call=ms, unit="ms per slow_tendencies! call",
+               vn_tendency_ms_per_call=vn_ms, vn_tendency_ms_per_iap_step=2 * vn_ms,
+               reference_a100_vn_tendency_ms_per_step=MPAS_REFERENCE_VN_MS,
+               vn_tendency_speedup_vs_reference_a100=MPAS_REFERENCE_VN_MS / (2 * vn_ms),
+               hbm_frac_kernel_model=kbytes / (ms / 1e3) / HBM_PEAK,
+               roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
+                             frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], "mpas"),
+                             avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
+               kernels=stats)
+    log(f"[bench] mpas: {ms:.3f} ms per call, vn_tendency {vn_ms:.3f} ms ({out['vn_tendency_speedup_vs_reference_a100']:.1f}x "
+        f"the reference's A100 per step)")
+    del m
+    torch.cuda.empty_cache()
     return out
 
 
@@ -536,6 +604,9 @@ def main():
     if args.icon_only:
         print(json.dumps({"icon": run_icon(args)}), flush=True)
         return
+    if args.mpas_only:
+        print(json.dumps({"mpas": run_mpas(args)}), flush=True)
+        return
     t = time.perf_counter()
     if args.grid_file:
         from dmplex_amd import read_grid
@@ -603,6 +674,9 @@ def main():
     if not args.no_icon and trisk_cell and not args.grid_file:
         log(f"[bench] ICON triangle model (level {args.icon_level})")
         line["icon"] = run_icon(args)
+    if not args.no_mpas and trisk_cell and not args.grid_file:
+        log(f"[bench] MPAS-Gassmann stage operators (hexagon {args.mpas_level} x 30 levels)")
+        line["mpas"] = run_mpas(args)
     print(json.dumps(line), flush=True)
 
 

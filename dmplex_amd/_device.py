@@ -36,6 +36,18 @@ class IconModelDesc(C.Structure):
                 ("divergence_damping", f64p), ("cell_points", f64p)]
 
 
+class MpasOpsDesc(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("nlev", i64), ("kc", i64), ("kv", i64), ("kt", i64),
+                ("cell_edges", i64p), ("kinetic", f64p), ("inner", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_weight", f64p), ("inverse_dual", f64p), ("vertex_edges", i64p),
+                ("curl", f64p), ("vertex_cells", i64p), ("vertex_weight", f64p), ("coriolis", f64p),
+                ("trisk_index", C.POINTER(C.c_int32)), ("trisk", f64p), ("rhombus", f64p), ("edge_coriolis", f64p),
+                ("dz", f64p), ("dzh", f64p), ("wlow", f64p), ("edge_dz", f64p), ("edge_dzh", f64p),
+                ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p),
+                ("iap_pv", C.c_int32), ("vertical", C.c_int32), ("five_band", C.c_int32), ("reserved", C.c_int32),
+                ("cell_points", f64p)]
+
+
 class TriskOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_size", C.c_int32),
                 ("path", C.c_int32), ("narrow_limit", C.c_int32), ("reserved", C.c_int32 * 3)]
@@ -90,6 +102,17 @@ SIGNATURES = {
     "trisk_ensemble_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "icon_check_tables": [C.POINTER(IconModelDesc)],
     "icon_create": [C.POINTER(IconModelDesc), C.POINTER(TriskOptions), C.POINTER(vp)],
+    "mpas_ops_check": [C.POINTER(MpasOpsDesc)],
+    "mpas_ops_create": [C.POINTER(MpasOpsDesc), C.c_int32, C.POINTER(vp)],
+    "mpas_ops_destroy": [vp],
+    "mpas_ops_permutation": [vp, i64p, i64p, i64p],
+    "mpas_ops_upload_field": [vp, C.c_int32, i64, f64p, vp, vp],
+    "mpas_ops_download_field": [vp, C.c_int32, i64, vp, f64p, vp],
+    "mpas_slow_tendencies": [vp, vp, vp, vp, vp, C.c_double, vp],
+    "mpas_ops_field": [vp, C.c_int32, C.POINTER(vp), i64p, i64p],
+    "mpas_ops_set_timing": [vp, C.c_int32],
+    "mpas_ops_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
+    "mpas_ops_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_lloyd": [C.c_int32, f64p, i64, i64p, i64, i64p, i64p, i64, C.c_double, C.c_int32, i64, i64p, f64p],
 }
 

@@ -95,6 +95,8 @@ def host() -> C.CDLL:
             "dmx_sort_rings": [i64, i64p, i64p, i64, f64p, f64p],
             "dmx_rbf_coefficients": [i64, i64, C.POINTER(DmxGridTables), C.c_double, i64, i64p, f64p, i64p],
             "dmx_bilinear_weights": [i64, C.POINTER(DmxGridTables), f64p, f64p],
+            "dmx_trisk_pv_edges": [i64, i64, C.POINTER(DmxGridTables), i64p],
+            "dmx_iap_trisk_weights": [i64, i64, C.POINTER(DmxGridTables), f64p, f64p],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)

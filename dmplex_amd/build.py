@@ -17,9 +17,9 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "lib"
 HOST_SOURCES = ["csrc/host/mesh.cpp", "csrc/host/trisk_grid.cpp", "csrc/host/netcdf.cpp",
-                "csrc/host/icon_grid.cpp"]
+                "csrc/host/icon_grid.cpp", "csrc/host/mpas_tables.cpp"]
 HIP_SOURCES = ["csrc/hip/trisk_device.hip", "csrc/hip/trisk_ensemble.hip", "csrc/hip/trisk_adjoint.hip",
-               "csrc/hip/icon_device.hip", "csrc/hip/lloyd_device.hip"]
+               "csrc/hip/icon_device.hip", "csrc/hip/mpas_device.hip", "csrc/hip/lloyd_device.hip"]
 HIP_HOST_SOURCES = ["csrc/hip/trisk_layout.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

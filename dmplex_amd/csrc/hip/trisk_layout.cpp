@@ -374,6 +374,142 @@ bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err) {
     return true;
 }
 
+bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) {
+    if (!d) {
+        err = "null descriptor";
+        return false;
+    }
+    if (d->nc <= 0 || d->ne <= 0 || d->nv <= 0 || d->nlev <= 0 || d->nlev > 4096 || d->kc <= 0 || d->kc > 16 ||
+        d->kv <= 0 || d->kv > 16 || d->kt <= 0 || d->kt > 32 || d->nc >= (int64_t(1) << 31) ||
+        d->ne >= (int64_t(1) << 31) || d->nv >= (int64_t(1) << 31) ||
+        (d->nlev + 1) * std::max(d->ne, d->nc) >= (int64_t(1) << 40)) {
+        err = "bad sizes or table widths";
+        return false;
+    }
+    if (!d->cell_edges || !d->kinetic || !d->inner || !d->edge_cells || !d->edge_vertices || !d->edge_weight ||
+        !d->inverse_dual || !d->vertex_edges || !d->curl || !d->vertex_cells || !d->vertex_weight || !d->coriolis ||
+        !d->trisk_index || !d->trisk || !d->rhombus || !d->edge_coriolis || !d->dz || !d->dzh || !d->wlow ||
+        !d->edge_dz || !d->edge_dzh) {
+        err = "missing table";
+        return false;
+    }
+    if (d->vertical < 0 || d->vertical > 2) {
+        err = "vertical must be 0 (advective), 1 (vector invariant) or 2 (ICON-IAP)";
+        return false;
+    }
+    const bool vk = d->cell_vertices != nullptr;
+    if (vk && (!d->vk_kites || !d->vk_weights)) {
+        err = "vertex kinetic energy needs cell_vertices, vk_kites and vk_weights";
+        return false;
+    }
+    const int64_t nc = d->nc, ne = d->ne, nv = d->nv;
+    const int KC = int(d->kc), KV = int(d->kv), KT = int(d->kt), NL = int(d->nlev);
+    struct Check {
+        const int64_t* t;
+        int64_t count, n;
+        const char* name;
+    } checks[] = {{d->cell_edges, KC * nc, ne, "cell_edges"},      {d->edge_cells, 2 * ne, nc, "edge_cells"},
+                  {d->edge_vertices, 2 * ne, nv, "edge_vertices"}, {d->vertex_edges, KV * nv, ne, "vertex_edges"},
+                  {d->vertex_cells, KV * nv, nc, "vertex_cells"},  {vk ? d->cell_vertices : nullptr, KC * nc, nv,
+                                                                     "cell_vertices"}};
+    for (auto& c : checks)
+        if (c.t && !in_range(c.t, c.count, c.n)) {
+            err = std::string("index out of range in ") + c.name;
+            return false;
+        }
+    for (int64_t t = 0; t < 3 * KT * ne; ++t)
+        if (d->trisk_index[t] < 0 || d->trisk_index[t] >= ne) {
+            err = "index out of range in trisk_index";
+            return false;
+        }
+    L = MpasLayout();
+    L.nc = nc;
+    L.ne = ne;
+    L.nv = nv;
+    L.nlev = NL;
+    L.kc = KC;
+    L.kv = KV;
+    L.kt = KT;
+    L.iap_pv = d->iap_pv != 0;
+    L.vertical = d->vertical;
+    L.five_band = d->five_band != 0;
+    L.vk = vk;
+    if (d->cell_points) {
+        hilbert_cells(nc, d->cell_points, L.cperm);
+    } else {
+        L.cperm.resize(nc);
+        std::iota(L.cperm.begin(), L.cperm.end(), int64_t(0));
+    }
+    std::vector<char> seen(ne, 0);
+    L.eperm.reserve(ne);
+    for (int64_t j = 0; j < nc; ++j)
+        for (int k = 0; k < KC; ++k) {
+            if (d->kinetic[k + KC * L.cperm[j]] == 0.0 && d->inner[k + KC * L.cperm[j]] == 0.0) continue;  // pad
+            const int64_t e = d->cell_edges[k + KC * L.cperm[j]];
+            if (!seen[e]) {
+                seen[e] = 1;
+                L.eperm.push_back(e);
+            }
+        }
+    for (int64_t e = 0; e < ne; ++e)
+        if (!seen[e]) L.eperm.push_back(e);
+    std::vector<char> vseen(nv, 0);
+    L.vperm.reserve(nv);
+    for (int64_t j = 0; j < ne; ++j)
+        for (int s = 0; s < 2; ++s) {
+            const int64_t v = d->edge_vertices[s + 2 * L.eperm[j]];
+            if (!vseen[v]) {
+                vseen[v] = 1;
+                L.vperm.push_back(v);
+            }
+        }
+    for (int64_t v = 0; v < nv; ++v)
+        if (!vseen[v]) L.vperm.push_back(v);
+    invert(L.cperm, L.cinv);
+    invert(L.eperm, L.einv);
+    invert(L.vperm, L.vinv);
+    // k x n column-major tables -> slot-major k * n + j
+    remap_index(d->cell_edges, KC, nc, L.cperm, L.einv, L.ce);
+    remap_value(d->kinetic, KC, nc, L.cperm, L.kin);
+    remap_value(d->inner, KC, nc, L.cperm, L.inner);
+    remap_index(d->edge_cells, 2, ne, L.eperm, L.cinv, L.ec);
+    remap_index(d->edge_vertices, 2, ne, L.eperm, L.vinv, L.ev);
+    remap_value(d->edge_weight, 2, ne, L.eperm, L.ew);
+    remap_value(d->inverse_dual, 1, ne, L.eperm, L.invd);
+    remap_value(d->rhombus, 2, ne, L.eperm, L.rh);
+    remap_value(d->edge_coriolis, 1, ne, L.eperm, L.fe);
+    remap_index(d->vertex_edges, KV, nv, L.vperm, L.einv, L.ve);
+    remap_value(d->curl, KV, nv, L.vperm, L.curl);
+    remap_index(d->vertex_cells, KV, nv, L.vperm, L.cinv, L.vc);
+    remap_value(d->vertex_weight, KV, nv, L.vperm, L.vw);
+    remap_value(d->coriolis, 1, nv, L.vperm, L.fv);
+    if (vk) {
+        remap_index(d->cell_vertices, KC, nc, L.cperm, L.vinv, L.cv);
+        remap_value(d->vk_kites, KC, nc, L.cperm, L.vkk);
+        remap_value(d->vk_weights, KV, nv, L.vperm, L.vkw);
+    }
+    // edge-first tables (entry (e, j) at e + ne * j) and points x levels fields: the point
+    // index is renumbered, the slot and level axes stay
+    auto points = [](const double* old, int64_t n, int64_t m, const std::vector<int64_t>& perm,
+                     std::vector<double>& out) {
+        out.resize(size_t(n * m));
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < m; ++k)
+            for (int64_t p = 0; p < n; ++p) out[size_t(p + n * k)] = old[perm[p] + n * k];
+    };
+    points(d->trisk, ne, KT, L.eperm, L.trw);
+    L.ti.resize(size_t(3 * KT * ne));
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < 3 * KT; ++m)
+        for (int64_t e = 0; e < ne; ++e) L.ti[size_t(e + ne * m)] = int32_t(L.einv[d->trisk_index[L.eperm[e] + ne * m]]);
+    points(d->dz, nc, NL, L.cperm, L.dz);
+    points(d->dzh, nc, NL + 1, L.cperm, L.dzh);
+    points(d->wlow, nc, NL + 1, L.cperm, L.wlow);
+    points(d->edge_dz, ne, NL, L.eperm, L.edz);
+    points(d->edge_dzh, ne, NL + 1, L.eperm, L.edzh);
+    return true;
+}
+
 void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.pair = false;
     if (!d->primal_lengths || !d->dual_lengths || !d->cell_kites) {

@@ -94,6 +94,22 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
 // areas) and the ICON ones, in the same numbering as build_layout.
 bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err);
 
+// Tables of the MPAS-Gassmann operators (mpas_ops_create) in the device numbering: cells
+// along the Hilbert curve, edges and vertices by first appearance; entity-major tables
+// (slot k of entity j at k * n + j, int32 indices) and metric fields point-fastest per level.
+struct MpasLayout {
+    int64_t nc = 0, ne = 0, nv = 0;
+    int nlev = 0, kc = 0, kv = 0, kt = 0;
+    int iap_pv = 0, vertical = 0, five_band = 0;
+    bool vk = false;
+    std::vector<int64_t> cperm, eperm, vperm, cinv, einv, vinv;
+    std::vector<int32_t> ce, cv, ec, ev, ti, ve, vc;  // ti: slot c (0..2) of TRiSK entry j at (c * kt + j) * ne + e
+    std::vector<double> kin, inner, vkk, dz, dzh, wlow;
+    std::vector<double> ew, invd, trw, rh, fe, edz, edzh;
+    std::vector<double> curl, vw, fv, vkw;
+};
+bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err);
+
 // Builds the cell-pair tables from the grid fields: w_{k,k'} = s_k s_k' (1/2 - sum of the
 // kite fractions of the corners k+1..k') (TRiSKGrid.jl:140-155), and checks that
 // w l_e'/d_e reproduces every entry of the caller's trisk table (to 1e-13 of the row's

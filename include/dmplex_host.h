@@ -172,6 +172,17 @@ int dmx_rbf_coefficients(int64_t nc, int64_t ne, const dmx_grid_tables* g, doubl
  * every cell is a triangle. */
 int dmx_bilinear_weights(int64_t nc, const dmx_grid_tables* g, const double* centers, double* weights);
 
+/* --- ICON-IAP TRiSK tables of the MPAS-Gassmann operators ------------------------ */
+
+/* trisk_pv_edges(g) (src/FiniteVolumes/TRiSKGrid.jl:318-362): kt x ne, for TRiSK slot k of
+ * edge e the edge whose rhombus PV replaces the mean (q_e + q_e')/2 (i_cori_method = 4),
+ * -1 for the mean.  Error -3 if the TRiSK table is not in the order of the cell walk. */
+int dmx_trisk_pv_edges(int64_t nc, int64_t ne, const dmx_grid_tables* g, int64_t* pv_edges);
+/* iap_trisk_weights(g) (TRiSKGrid.jl:364-403): kt x ne TRiSK weights from the kite fractions
+ * of planar right triangles; vertices nv x 3. */
+int dmx_iap_trisk_weights(int64_t nc, int64_t ne, const dmx_grid_tables* g, const double* vertices,
+                          double* weights);
+
 /* --- ordering -------------------------------------------------------------------- */
 
 /* Hilbert key of integer points (hilbert_key, src/Topology/Ordering.jl:10-41):

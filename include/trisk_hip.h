@@ -244,6 +244,102 @@ int icon_check_tables(const icon_model_desc* desc);
  * Synchronous. */
 int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_handle** out);
 
+/* --- MPAS-Gassmann TRiSK operators with levels ---------------------------------------
+ * The explicit tendencies of one stage of MPASGassmannNonhydrostatic on flat ground,
+ *   slow_tendencies!(m, ys, pressure)  src/Nonhydrostatic/MPAS/Gassmann/Stepping.jl:22-56
+ * with the TRiSK operators of src/Nonhydrostatic/MPAS/Operators.jl:6-134 (diagnose, vertex
+ * and cell-from-vertex kinetic energy, w kinetic energy, eta, edge density and mass flux,
+ * vertex and edge PV, the w tendencies) and vn_tendency_kernel! with edge_w
+ * (Gassmann/Tendencies.jl:4-67, the pressure gradient along flat levels).  Fields are (points x levels) with the point index
+ * fastest, as the reference's (n, nlev) arrays: vn (ne x nlev), w (nc x nlev+1), rho and
+ * rhotheta (nc x nlev).  The library renumbers the points (cells along the Hilbert curve,
+ * edges and vertices by first appearance); mpas_ops_upload_field / _download_field convert.
+ * Terrain-following levels (orography), the contravariant correction and the pressure
+ * gradient at constant height are not covered. */
+typedef struct mpas_ops mpas_ops;
+
+typedef struct {
+    int64_t nc, ne, nv, nlev;
+    int64_t kc, kv, kt;             /* cell, vertex and TRiSK table widths */
+    /* fields of MPASGassmannNonhydrostatic (MPASGassmannNonhydrostatic.jl:133-176) */
+    const int64_t* cell_edges;      /* kc x nc */
+    const double* kinetic;          /* kc x nc: alpha l d_ei / 2A */
+    const double* inner;            /* kc x nc: l d_ei / A */
+    const int64_t* edge_cells;      /* 2 x ne */
+    const int64_t* edge_vertices;   /* 2 x ne */
+    const double* edge_weight;      /* 2 x ne: d_ei / d_e */
+    const double* inverse_dual;     /* ne: 1 / (a d_e) */
+    const int64_t* vertex_edges;    /* kv x nv */
+    const double* curl;             /* kv x nv: t d / (a A_v) */
+    const int64_t* vertex_cells;    /* kv x nv */
+    const double* vertex_weight;    /* kv x nv: K_iv / A_v */
+    const double* coriolis;         /* nv */
+    const int32_t* trisk_index;     /* ne x kt x 3, edge first: e', and the edges a, b of (q_a + q_b)/2 */
+    const double* trisk;            /* ne x kt, edge first: w l' / 2d */
+    const double* rhombus;          /* 2 x ne */
+    const double* edge_coriolis;    /* ne */
+    /* vertical metrics (flat ground): cells dz (nc x nlev), dzh and wlow (nc x nlev+1);
+     * edges dz (ne x nlev), dzh (ne x nlev+1) (metric.dz, metric.dzh) */
+    const double* dz;
+    const double* dzh;
+    const double* wlow;
+    const double* edge_dz;
+    const double* edge_dzh;
+    /* vertex kinetic energy (kinetic_energy != :cell), else NULL */
+    const int64_t* cell_vertices;   /* kc x nc */
+    const double* vk_kites;         /* kc x nc: (1 - alpha) K_iv / A_i */
+    const double* vk_weights;       /* kv x nv: A_ev / A_v */
+    int32_t iap_pv;                 /* 1: rhombus PV (pv_flux = :iap), 0: mean of the vertex PV */
+    int32_t vertical;               /* 0 advective, 1 vector invariant, 2 ICON-IAP mass fluxes */
+    int32_t five_band;              /* vertical_solver = :five_band */
+    int32_t reserved;
+    const double* cell_points;      /* nc x 3: Hilbert order of the cells, or NULL */
+} mpas_ops_desc;
+
+/* Work fields of a slow-tendency call (mpas_ops_field): kind, points x levels. */
+enum {
+    MPAS_THETA = 0,  /* theta, cells x nlev */
+    MPAS_EXNER = 1,  /* pi, cells x nlev */
+    MPAS_K = 2,      /* K (horizontal, vertex and w parts as the options add them), cells x nlev */
+    MPAS_QV = 3,     /* q_v, vertices x nlev */
+    MPAS_ZETA = 4,   /* zeta_v, vertices x nlev */
+    MPAS_RHOV = 5,   /* rho_v, vertices x nlev */
+    MPAS_QE = 6,     /* q_e, edges x nlev */
+    MPAS_RHOE = 7,   /* rho_e, edges x nlev */
+    MPAS_FS = 8,     /* rho_e vn, edges x nlev */
+    MPAS_ETA = 9,    /* eta, edges x nlev+1 */
+    MPAS_TW = 10,    /* w tendency, cells x nlev+1 */
+    MPAS_TVN = 11,   /* vn tendency, edges x nlev */
+    MPAS_KV = 12,    /* vertex kinetic energy, vertices x nlev */
+    MPAS_FIELDS = 13
+};
+
+/* Host-only validation of a descriptor (index ranges, widths).  Touches no device. */
+int mpas_ops_check(const mpas_ops_desc* desc);
+/* Validates, renumbers and uploads the tables and allocates the work fields.  Synchronous. */
+int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out);
+int mpas_ops_destroy(mpas_ops* h);
+/* new -> reference index maps of cells, edges and vertices; any may be NULL. */
+int mpas_ops_permutation(const mpas_ops* h, int64_t* cells, int64_t* edges, int64_t* vertices);
+/* A host field on cells (kind 0), edges (1) or vertices (2) with `levels` levels (point
+ * fastest, reference numbering) -> device (internal numbering), and back.  Synchronous. */
+int mpas_ops_upload_field(mpas_ops* h, int32_t kind, int64_t levels, const double* host, double* dev, void* stream);
+int mpas_ops_download_field(mpas_ops* h, int32_t kind, int64_t levels, const double* dev, double* host,
+                            void* stream);
+/* slow_tendencies!(m, ys, pressure): theta, pi, K, the PV, eta, tw and tvn of the stage
+ * state (vn, w, rho, rhotheta), device fields in the internal numbering, into the work
+ * fields. */
+int mpas_slow_tendencies(mpas_ops* h, const double* vn, const double* w, const double* rho, const double* rhotheta,
+                         double pressure, void* stream);
+/* Device pointer and shape (points, levels) of a work field (MPAS_*). */
+int mpas_ops_field(mpas_ops* h, int32_t which, double** dev, int64_t* points, int64_t* levels);
+/* HIP-event timing of the kernels of mpas_slow_tendencies (as trisk_set_timing and
+ * trisk_kernel_times), and their algorithmic HBM bytes per launch. */
+int mpas_ops_set_timing(mpas_ops* h, int32_t on);
+int mpas_ops_kernel_times(mpas_ops* h, int32_t max, const char** names, double* avg_ms, int64_t* launches,
+                          int32_t* count);
+int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* bytes, int32_t* count);
+
 /* --- adjoint ------------------------------------------------------------------------
  * No counterpart in the reference (it has no adjoint).  These are the pullbacks of
  * tendency! (src/FiniteVolumes/TRiSKShallowWater.jl:379-394) and of nsteps rk4_step!

@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``liboracle.so``, the FP64 CPU
-restatement of the reference TRiSK path (trisk_oracle.c) and of the ICON triangle model
-(icon_oracle.c).
+restatement of the reference TRiSK path (trisk_oracle.c), of the ICON triangle model
+(icon_oracle.c) and of the MPAS-Gassmann stage tendencies (mpas_oracle.c).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it,
 as the checker; the product (``dmplex_amd``) never imports it.
@@ -81,6 +81,8 @@ def lib() -> C.CDLL:
             ("icon_oracle_potential_enstrophy", [C.POINTER(IconOracleModel), vp], f64),
             ("icon_oracle_kinetic_energy", [C.POINTER(IconOracleModel), vp, vp], None),
             ("icon_oracle_vertex_pv", [C.POINTER(IconOracleModel), vp, vp], None),
+            ("mpas_oracle_slow_tendencies", [C.c_void_p, vp, vp, vp, vp, f64], None),
+            ("mpas_oracle_kernel", [C.c_void_p, C.c_int, vp, vp, vp, vp, f64], C.c_int),
             ("oracle_threads", [], C.c_int),
             ("oracle_set_threads", [C.c_int], None),
         ]:
@@ -264,3 +266,63 @@ class IconOracle:
 
     def tangential_velocity(self, u):
         return Oracle.edge_sum(self, self.m.rbf_edges, self.m.rbf, u)
+
+
+class MpasOracleModel(C.Structure):
+    _fields_ = [("nc", i64), ("ne", i64), ("nv", i64), ("nlev", i64), ("kc", i64), ("kv", i64), ("kt", i64),
+                ("cell_edges", i64p), ("kinetic", f64p), ("inner", f64p), ("edge_cells", i64p),
+                ("edge_vertices", i64p), ("edge_weight", f64p), ("inverse_dual", f64p), ("vertex_edges", i64p),
+                ("curl", f64p), ("vertex_cells", i64p), ("vertex_weight", f64p), ("coriolis", f64p),
+                ("trisk_index", C.POINTER(C.c_int32)), ("trisk", f64p), ("rhombus", f64p), ("edge_coriolis", f64p),
+                ("dz", f64p), ("dzh", f64p), ("wlow", f64p), ("edge_dz", f64p), ("edge_dzh", f64p),
+                ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p),
+                ("iap_pv", C.c_int32), ("vertical", C.c_int32), ("five_band", C.c_int32), ("reserved", C.c_int32)] + \
+               [(n, f64p) for n in ("theta", "exner", "K", "qv", "zeta", "rhov", "qe", "rhoe", "Fs", "eta", "tw",
+                                    "tvn", "Kv")]
+
+
+class MpasOracle:
+    """The reference's explicit MPAS-Gassmann stage tendencies on flat ground
+    (slow_tendencies!) on the host tables of an ``MPASOperatorTables``.  Fields are
+    (levels, points) arrays in the reference numbering."""
+
+    SHAPES = dict(theta="c", exner="c", K="c", qv="v", zeta="v", rhov="v", qe="e", rhoe="e", Fs="e", eta="e+",
+                  tw="c+", tvn="e", Kv="v")
+
+    def __init__(self, m):
+        self.m = m
+        g = m.grid
+        n = dict(c=g.num_cells(), e=g.num_edges(), v=g.num_vertices())
+        L = m.nlev
+        self.work = {k: np.zeros((L + 1 if s.endswith("+") else L, n[s[0]])) for k, s in self.SHAPES.items()}
+        vk = m.vertex_kinetic
+        w = self.work
+        self.s = MpasOracleModel(
+            n["c"], n["e"], n["v"], L, m.cell_edges.shape[0], m.vertex_edges.shape[0], m.trisk.shape[1],
+            _p(m.cell_edges, i64p), _p(m.kinetic, f64p), _p(m.inner, f64p), _p(m.edge_cells, i64p),
+            _p(m.edge_vertices, i64p), _p(m.edge_weight, f64p), _p(m.inverse_dual, f64p), _p(m.vertex_edges, i64p),
+            _p(m.curl, f64p), _p(m.vertex_cells, i64p), _p(m.vertex_weight, f64p), _p(m.coriolis, f64p),
+            _p(m.trisk_index, C.POINTER(C.c_int32)), _p(m.trisk, f64p), _p(m.rhombus, f64p),
+            _p(m.edge_coriolis, f64p), _p(m.dz, f64p), _p(m.dzh, f64p), _p(m.wlow, f64p), _p(m.edge_dz, f64p),
+            _p(m.edge_dzh, f64p),
+            _p(vk["cell_vertices"], i64p) if vk else None, _p(vk["kites"], f64p) if vk else None,
+            _p(vk["weights"], f64p) if vk else None,
+            int(m.iap_pv), m.vertical, int(m.five_band), 0,
+            *(_p(w[k], f64p) for k in ("theta", "exner", "K", "qv", "zeta", "rhov", "qe", "rhoe", "Fs", "eta", "tw",
+                                       "tvn", "Kv")))
+
+    def slow_tendencies(self, vn, w, rho, rhotheta, pressure: float = 1.0) -> dict:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (vn, w, rho, rhotheta)]
+        lib().mpas_oracle_slow_tendencies(C.byref(self.s), *(a.ctypes.data_as(C.c_void_p) for a in arrs),
+                                          float(pressure))
+        return {k: v.copy() for k, v in self.work.items()}
+
+    KERNELS = dict(diagnose=0, vertex_kinetic=1, cell_from_vertex=2, w_kinetic=3, vertex_pv=4, edge_pv=5,
+                   edge_density=6, eta=7, w_tendency_vi=8, w_tendency=9, vn_tendency=10)
+
+    def kernel(self, name: str, vn=None, w=None, rho=None, rhotheta=None, pressure: float = 1.0) -> None:
+        """One reference kernel on the current work fields (``self.work``, edited in place)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (vn, w, rho, rhotheta)]
+        ptrs = [None if a is None else a.ctypes.data_as(C.c_void_p) for a in keep]
+        if lib().mpas_oracle_kernel(C.byref(self.s), self.KERNELS[name], *ptrs, float(pressure)) != 0:
+            raise ValueError(name)

@@ -21,7 +21,7 @@ def declared(header: str, prefix: str):
     return sorted(set(re.findall(rf"\b({prefix}\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,prefix,loader", [("trisk_hip.h", "(?:trisk|icon)_", D.lib),
+@pytest.mark.parametrize("header,prefix,loader", [("trisk_hip.h", "(?:trisk|icon|mpas)_", D.lib),
                                                   ("dmplex_host.h", "dmx_", N.host)])
 def test_every_declared_symbol_is_exported(header, prefix, loader):
     names = declared(header, prefix)
@@ -32,11 +32,12 @@ def test_every_declared_symbol_is_exported(header, prefix, loader):
 
 
 def test_python_binding_covers_the_header():
-    names = declared("trisk_hip.h", "(?:trisk|icon)_")
+    names = declared("trisk_hip.h", "(?:trisk|icon|mpas)_")
     assert set(names) - {"trisk_last_error"} == set(D.SIGNATURES)
 
 
-@pytest.mark.parametrize("name,struct", [("trisk_model_desc", D.TriskModelDesc), ("icon_model_desc", D.IconModelDesc)])
+@pytest.mark.parametrize("name,struct", [("trisk_model_desc", D.TriskModelDesc), ("icon_model_desc", D.IconModelDesc),
+                                         ("mpas_ops_desc", D.MpasOpsDesc)])
 def test_desc_struct_matches_header_field_order(name, struct):
     text = (ROOT / "include" / "trisk_hip.h").read_text()
     end = text.index("} " + name + ";")
