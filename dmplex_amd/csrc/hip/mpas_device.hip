@@ -61,6 +61,7 @@ struct Params {
     const int32_t* ec;
     const int32_t* ev;
     const int32_t* ti;
+    const int32_t* ti2;  // paired TRiSK entries (a, x or ~x), or NULL
     const double* ew;
     const double* invd;
     const double* trw;
@@ -255,21 +256,43 @@ __global__ __launch_bounds__(kBlock) void mpas_w_tendency_kernel(Params p, const
         }
     }
     const bool mw = p.vertical == 2;
+    // vn, rho_e of the cell's edges and K of the layer below the interface, carried from one
+    // interface to the next (the layer above interface j is the one below interface j + 1)
+    double VP[N], RP[N], Kp = 0.0;
+    if (vi && k0 >= 1) {
+        for (int n = 0; n < kc; ++n) {
+            VP[n] = vn[E[n] + p.ne * (k0 - 1)];
+            RP[n] = mw ? p.f[MPAS_RHOE][E[n] + p.ne * (k0 - 1)] : 1.0;
+        }
+        Kp = p.f[MPAS_K][i + p.nc * (k0 - 1)];
+    }
 #pragma unroll
     for (int q = 0; q <= kLevels; ++q) {
         const int j = k0 + q;
         if (j >= j1) break;
         double tw = 0.0;
+        if (vi && j == 0) {
+            for (int n = 0; n < kc; ++n) {
+                VP[n] = vn[E[n]];
+                RP[n] = mw ? p.f[MPAS_RHOE][E[n]] : 1.0;
+            }
+            Kp = p.f[MPAS_K][i];
+        }
         if (j >= 1 && j <= p.nlev - 1) {
             double s = 0.0;
             if (vi) {
                 const double wl = p.wlow[i + p.nc * j];
                 for (int n = 0; n < kc; ++n) {
-                    const int64_t lo = E[n] + p.ne * (j - 1), up = E[n] + p.ne * j;
-                    const double rl = mw ? p.f[MPAS_RHOE][lo] : 1.0, ru = mw ? p.f[MPAS_RHOE][up] : 1.0;
-                    s += Wn[n] * p.f[MPAS_ETA][up] * (rl * vn[lo] + ru * vn[up]) / (2 * (wl * rl + (1 - wl) * ru));
+                    const int64_t up = E[n] + p.ne * j;
+                    const double rl = RP[n], vl = VP[n];
+                    const double ru = mw ? p.f[MPAS_RHOE][up] : 1.0, vu = vn[up];
+                    s += Wn[n] * p.f[MPAS_ETA][up] * (rl * vl + ru * vu) / (2 * (wl * rl + (1 - wl) * ru));
+                    VP[n] = vu;
+                    RP[n] = ru;
                 }
-                tw = s - (p.f[MPAS_K][i + p.nc * j] - p.f[MPAS_K][i + p.nc * (j - 1)]) / p.dzh[i + p.nc * j];
+                const double Ku = p.f[MPAS_K][i + p.nc * j];
+                tw = s - (Ku - Kp) / p.dzh[i + p.nc * j];
+                Kp = Ku;
             } else {
                 for (int n = 0; n < kc; ++n) {
                     const double vm = (vn[E[n] + p.ne * (j - 1)] + vn[E[n] + p.ne * j]) / 2;
@@ -309,7 +332,7 @@ __device__ __forceinline__ double edge_w(const Params& p, const double* __restri
     return M / (ew1 * rhl + ew2 * rhr);
 }
 
-template <int KT>
+template <int KT, bool PAIRED>
 __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, const double* __restrict__ vn,
                                                                   const double* __restrict__ w,
                                                                   const double* __restrict__ rho, double pressure,
@@ -323,10 +346,15 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     int32_t I0[N], I1[N], I2[N];
     double Tw[N];
     for (int j = 0; j < kt; ++j) {
-        I0[j] = ld(p.ti + (0 * kt + j) * p.ne + e);
-        I1[j] = ld(p.ti + (1 * kt + j) * p.ne + e);
-        I2[j] = ld(p.ti + (2 * kt + j) * p.ne + e);
         Tw[j] = ld(p.trw + j * p.ne + e);
+        if constexpr (PAIRED) {
+            I0[j] = ld(p.ti2 + j * p.ne + e);
+            I1[j] = ld(p.ti2 + (kt + j) * p.ne + e);
+        } else {
+            I0[j] = ld(p.ti + (0 * kt + j) * p.ne + e);
+            I1[j] = ld(p.ti + (1 * kt + j) * p.ne + e);
+            I2[j] = ld(p.ti + (2 * kt + j) * p.ne + e);
+        }
     }
     const int32_t l = ld(p.ec + e), r = ld(p.ec + p.ne + e);
     const double ew1 = ld(p.ew + e), ew2 = ld(p.ew + p.ne + e), invd = ld(p.invd + e);
@@ -344,7 +372,18 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         const int k = k0 + q;
         if (k >= k1) break;
         double s = 0.0;
-        for (int j = 0; j < kt; ++j) s += Tw[j] * Fs[I0[j] + p.ne * k] * (qe[I1[j] + p.ne * k] + qe[I2[j] + p.ne * k]);
+        if constexpr (PAIRED) {
+            // (q_b + q_c) with b = e: (q_e + q_x); with b = c: (q_x + q_x)
+            const double qs = qe[e + p.ne * k];
+            for (int j = 0; j < kt; ++j) {
+                const bool twice = I1[j] < 0;
+                const double qx = qe[(twice ? ~I1[j] : I1[j]) + p.ne * k];
+                s += Tw[j] * Fs[I0[j] + p.ne * k] * ((twice ? qx : qs) + qx);
+            }
+        } else {
+            for (int j = 0; j < kt; ++j)
+                s += Tw[j] * Fs[I0[j] + p.ne * k] * (qe[I1[j] + p.ne * k] + qe[I2[j] + p.ne * k]);
+        }
         const double te = ew1 * th[l + p.nc * k] + ew2 * th[r + p.nc * k];
         double tt = s - invd * (K[r + p.nc * k] - K[l + p.nc * k]);
         if (pressure > 0) {
@@ -468,12 +507,15 @@ static int slow_tendencies(mpas_ops* H, const double* vn, const double* w, const
     }
     if ((rc = timed_begin(H, K_D, s, &stop))) return rc;
     const unsigned vchunks = unsigned((p.nlev + kLevelsVn - 1) / kLevelsVn);
-    if (p.kt == 10)
-        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10>), dim3(unsigned(kXcd * te), vchunks), dim3(kBlock), 0, s, p,
-                           vn, w, rho, pressure, te);
+    const dim3 gd(unsigned(kXcd * te), vchunks);
+    if (p.kt == 10 && p.ti2)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, false>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
+    else if (p.ti2)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<0, true>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
     else
-        hipLaunchKernelGGL((mpas_vn_tendency_kernel<0>), dim3(unsigned(kXcd * te), vchunks), dim3(kBlock), 0, s, p,
-                           vn, w, rho, pressure, te);
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<0, false>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
     HIP_TRY(hipGetLastError());
     return timed_end(stop, s);
 }
@@ -530,6 +572,7 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
     rc |= trisk::upload(o, L.ec, &p.ec);
     rc |= trisk::upload(o, L.ev, &p.ev);
     rc |= trisk::upload(o, L.ti, &p.ti);
+    if (L.paired) rc |= trisk::upload(o, L.ti2, &p.ti2);
     rc |= trisk::upload(o, L.ew, &p.ew);
     rc |= trisk::upload(o, L.invd, &p.invd);
     rc |= trisk::upload(o, L.trw, &p.trw);
@@ -687,7 +730,7 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
                 : ne * L * c + nc * L1 * c + nc * L * c + nc * L1 * c + nc * KC * (ix + c) + ne * (2 * ix + c);
     v[K_E] = vi && h->five_band ? nc * (L1 * c * 2 + L * c * 3) : 0.0;
     // D: F, q_e, theta, pi, K, [eta, w, rho, wlow, edge dz, dzh | w, vn, edge dz] -> tvn; tables
-    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * (3 * ix + c) + 2 * ix + 3 * c) +
+    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
              (vi ? ne * L1 * c * 2 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c);
     int k2 = 0;
     for (int k = 0; k < K_N && k2 < max; ++k) {

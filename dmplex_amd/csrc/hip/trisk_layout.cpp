@@ -502,6 +502,24 @@ bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) 
 #pragma omp parallel for schedule(static)
     for (int64_t m = 0; m < 3 * KT; ++m)
         for (int64_t e = 0; e < ne; ++e) L.ti[size_t(e + ne * m)] = int32_t(L.einv[d->trisk_index[L.eperm[e] + ne * m]]);
+    // two gathers per TRiSK entry where every entry has b = e or b = c (the reference's tables)
+    L.paired = true;
+    L.ti2.resize(size_t(2 * KT * ne));
+    for (int64_t j = 0; j < KT && L.paired; ++j)
+        for (int64_t e = 0; e < ne; ++e) {
+            const int32_t a = L.ti[size_t(e + ne * j)], b = L.ti[size_t(e + ne * (KT + j))],
+                          c = L.ti[size_t(e + ne * (2 * KT + j))];
+            L.ti2[size_t(e + ne * j)] = a;
+            if (b == int32_t(e)) {
+                L.ti2[size_t(e + ne * (KT + j))] = c;
+            } else if (b == c) {
+                L.ti2[size_t(e + ne * (KT + j))] = ~c;
+            } else {
+                L.paired = false;
+                break;
+            }
+        }
+    if (!L.paired) L.ti2.clear();
     points(d->dz, nc, NL, L.cperm, L.dz);
     points(d->dzh, nc, NL + 1, L.cperm, L.dzh);
     points(d->wlow, nc, NL + 1, L.cperm, L.wlow);

@@ -104,6 +104,10 @@ struct MpasLayout {
     bool vk = false;
     std::vector<int64_t> cperm, eperm, vperm, cinv, einv, vinv;
     std::vector<int32_t> ce, cv, ec, ev, ti, ve, vc;  // ti: slot c (0..2) of TRiSK entry j at (c * kt + j) * ne + e
+    // the TRiSK entries (a, b, c) with b the edge itself or b == c, as two indices: a, and x
+    // (b == e: the PV pair is (q_e, q_x)) or ~x (b == c: (q_x, q_x)); at (c * kt + j) * ne + e
+    bool paired = false;
+    std::vector<int32_t> ti2;
     std::vector<double> kin, inner, vkk, dz, dzh, wlow;
     std::vector<double> ew, invd, trw, rh, fe, edz, edzh;
     std::vector<double> curl, vw, fv, vkw;
