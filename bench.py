@@ -123,7 +123,10 @@ def load_traffic(kernel: str, precision: str):
         return None
 
 
-def run_gpu(grid, y0, precision, args, params, path=None):
+def run_gpu(grid, y0, precision, args, params, path=None, trisk_cell=True, profiled=False):
+    """One precision on the GPU.  ``trisk_cell``: TRiSK PV flux with cell kinetic energy, the
+    workload the fixed byte model counts; ``profiled``: the level-9 workload whose measured
+    HBM traffic per kernel is committed (profiles/traffic.json)."""
     import torch
 
     from dmplex_amd import HIPBackend, TRiSKShallowWater, diagnostics, rk4_steps_
@@ -170,16 +173,19 @@ def run_gpu(grid, y0, precision, args, params, path=None):
                hbm_frac_kernel_model=None if kbytes is None else kbytes / (ms / args.steps / 1e3) / HBM_PEAK,
                graphs=not args.no_graphs,
                ms_per_step_direct_with_events=None if ms_direct is None else ms_direct / args.steps,
-               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK, path_name=path,
+               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK if trisk_cell else None, path_name=path,
                mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0, kernels=stats)
     if stats:
         dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
         achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
         out["roofline"] = dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                               frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], precision),
+                               frac=achieved * 1e9 / HBM_PEAK,
+                               traffic=load_traffic(dom["name"], "fp64_pair" if precision == "fp64" and
+                                                    path == "pair" else precision) if profiled else None,
                                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"])
+    frac = out["hbm_frac_model"] if trisk_cell else out["hbm_frac_kernel_model"]
     log(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/step, {value:.3e} cell-steps/s, "
-        f"model HBM fraction {out['hbm_frac_model']:.3f}")
+        f"model HBM fraction {'n/a' if frac is None else f'{frac:.3f}'}")
     del m
     torch.cuda.empty_cache()
     return out
@@ -245,7 +251,9 @@ def run_icon(args):
                  hbm_frac_kernel_model=kbytes / (ms / steps / 1e3) / HBM_PEAK,
                  mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0,
                  roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                               frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], "icon"),
+                               frac=achieved * 1e9 / HBM_PEAK,
+                               traffic=load_traffic(dom["name"], "icon_damping" if damping else "icon")
+                               if args.icon_level == 9 else None,
                                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
                  kernels=stats)
         if damping:
@@ -311,7 +319,8 @@ def run_mpas(args):
                vn_tendency_speedup_vs_reference_a100=MPAS_REFERENCE_VN_MS / (2 * vn_ms),
                hbm_frac_kernel_model=kbytes / (ms / 1e3) / HBM_PEAK,
                roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                             frac=achieved * 1e9 / HBM_PEAK, traffic=load_traffic(dom["name"], "mpas"),
+                             frac=achieved * 1e9 / HBM_PEAK,
+                             traffic=load_traffic(dom["name"], "mpas") if args.mpas_level == 7 else None,
                              avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
                kernels=stats)
     log(f"[bench] mpas: {ms:.3f} ms per call, vn_tendency {vn_ms:.3f} ms ({out['vn_tendency_speedup_vs_reference_a100']:.1f}x "
@@ -635,7 +644,8 @@ def main():
             runs = [r for r in runs if r[0] != "fp64_pair"]
     else:
         runs = [(args.precision, args.precision, None)]
-    res = {key: run_gpu(grid, y0, prec, args, params, path) for key, prec, path in runs}
+    profiled = trisk_cell and args.level == 9 and not args.grid_file and not args.no_scvt
+    res = {key: run_gpu(grid, y0, prec, args, params, path, trisk_cell, profiled) for key, prec, path in runs}
     head = res.get("fp32_bulk", res.get("fp64"))
     head_p = "fp32_bulk" if "fp32_bulk" in res else "fp64"
     cpu = None

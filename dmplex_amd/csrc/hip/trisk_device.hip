@@ -195,7 +195,10 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
 // ---------------------------------------------------------------------------------
 // P3: edges (du) | cells (dh), then the RK4 stage update
 
-template <class P, bool GM, bool GMEAN, class W = AnyWidths>
+// GSLOT (12 Gassmann slots in the order of gassmann_coefficients): the PV of slot k is the q
+// of the pair already gathered for slot gassmann_pv_slot(k), so neither the PV-edge table nor
+// a second gather is needed; the products and their order are those of the table form.
+template <class P, bool GM, bool GMEAN, class W = AnyWidths, bool GSLOT = false>
 __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
@@ -216,6 +219,16 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
                 s += ld(p.tw + k * p.ne + e) * C(f.F) * (qe + C(f.q));
             }
             du = double(s / 2) - grad;
+        } else if constexpr (GSLOT) {
+            static_assert(W::gw == 12 && !GMEAN, "the slot form needs the 12 hexagon-mesh slots");
+            const Index<12> GE(p.ge16p, p.ge, p.ne, e, e);
+            Pair<I> f[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) f[k] = p.Fq[GE[k]];
+            C s = 0;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) s += ld(p.gw + k * p.ne + e) * C(f[k].F) * C(f[gassmann_pv_slot(k)].q);
+            du = double(s) - grad;
         } else {
             const Index<W::gw> GE(p.ge16p, p.ge, p.ne, e, e);
             const Index<GMEAN ? 0 : W::gw> GP(p.gp16p, p.gp, p.ne, e, e);
@@ -602,7 +615,12 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
             hipLaunchKernelGGL((update_kernel<P, false, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
         else if (L.gassmann_mean)
             hipLaunchKernelGGL((update_kernel<P, true, true, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
-        else
+        else if constexpr (W::gw == 12) {
+            if (L.gassmann_slots)
+                hipLaunchKernelGGL((update_kernel<P, true, false, W, true>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+            else
+                hipLaunchKernelGGL((update_kernel<P, true, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        } else
             hipLaunchKernelGGL((update_kernel<P, true, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
@@ -1075,7 +1093,8 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
         // Gassmann: zeta_v, h_v written in P1 and gathered in P2 with the rhombus weights and
-        // f at the edges; 12-wide edge, PV-edge (not for the mean) and weight tables in P3.
+        // f at the edges; 12-wide edge, PV-edge (not for the mean, nor in the slot form) and
+        // weight tables in P3.
         // Vertex kinetic energy: K_v and its weights in P1, the vertex part of h_e in P2, and
         // the cells' vertex part of K and B (kites, corners, K_v gathers) in P2.
         // On hexagon meshes the index tables are read as 16-bit deltas (2 B per index).
@@ -1087,7 +1106,8 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
                     nv * (KV * 2 * (ix + c) + c + c + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
         double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
                     nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 32.0) + nv * c : 0.0);
-        const double pv = gm ? KG * (ix + c + (L.gassmann_mean ? 0.0 : ix)) : KT * (ix + c);
+        const bool gp_table = !L.gassmann_mean && !(hex && L.gassmann_slots);
+        const double pv = gm ? KG * (ix + c + (gp_table ? ix : 0.0)) : KT * (ix + c);
         double p3 = ne * (pv + 2 * ix + c + 2 * c + rk_e) + nc * (KC * (ix + 8.0) + 8.0 + rk_c);
         v = {{kKernelNames[K_CELL_VERTEX], p1}, {kKernelNames[K_EDGE], p2}, {kKernelNames[K_UPDATE], p3}};
     }

@@ -271,7 +271,18 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     if (vk) w += narrow_table(L.cv, KC, nc, [](int64_t j) { return 2 * j; }, L.cv16p);
     if (gm) {
         w += narrow_table(L.ge, KG, ne, [](int64_t j) { return j; }, L.ge16p);
-        w += narrow_table(L.gp, KG, ne, [](int64_t j) { return j; }, L.gp16p);
+        L.gassmann_slots = !L.gassmann_mean && KG == 12;
+        for (int64_t e = 0; e < ne && L.gassmann_slots; ++e)
+            for (int k = 0; k < 12; ++k)
+                if (L.gp[k * ne + e] != L.ge[gassmann_pv_slot(k) * ne + e]) {
+                    L.gassmann_slots = false;
+                    break;
+                }
+        if (L.gassmann_slots) {
+            L.gp.clear();
+        } else {
+            w += narrow_table(L.gp, KG, ne, [](int64_t j) { return j; }, L.gp16p);
+        }
     }
     L.wide_table_entities = w;
     return true;

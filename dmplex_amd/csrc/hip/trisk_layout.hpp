@@ -9,12 +9,24 @@
 
 namespace trisk {
 
+// Slot whose edge carries the PV of Gassmann slot k (k < 12): an inner edge takes the other
+// inner edge of the shared vertex, an outer edge its inner edge (TRiSKGrid.jl:233-247), which
+// gives 3, 0, 0, 0, 3, 3 for the first vertex and 9, 6, 6, 6, 9, 9 for the second.  (constexpr:
+// callable in device code too.)
+constexpr int gassmann_pv_slot(int k) {
+    return (k >= 6 ? 6 : 0) + ((k % 6 == 0 || k % 6 >= 4) ? 3 : 0);
+}
+
 // Tables in the device numbering (cells along the Hilbert curve, edges and vertices by
 // first appearance), entity-major: slot k of entity j at k * n + j.
 struct Layout {
     int64_t nc = 0, ne = 0, nv = 0;
     int kc = 0, kv = 0, kt = 0, kg = 0;
     bool vertex_ke = false, gassmann = false, gassmann_mean = false;
+    // Gassmann with 12 slots in the order of gassmann_coefficients (per vertex of e: inner f1,
+    // the two outer edges of f1, inner f2, the two outer edges of f2): slot k takes the PV of
+    // its slot gassmann_pv_slot(k), so the PV-edge table is not needed on the device
+    bool gassmann_slots = false;
     double gravity = 0;
     std::vector<int64_t> cperm, eperm, vperm;  // new -> reference
     std::vector<int64_t> cinv, einv, vinv;     // reference -> new

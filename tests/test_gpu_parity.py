@@ -53,7 +53,7 @@ def test_tendency_bitwise(pv_flux, kinetic_energy):
     assert np.array_equal(got, ref), rel(got, ref)
 
 
-@pytest.mark.parametrize("pv_flux", ["trisk", "gassmann_mean"])
+@pytest.mark.parametrize("pv_flux", ["trisk", "gassmann", "gassmann_mean"])
 def test_rk4_bitwise(pv_flux):
     say(f"rk4 parity level 5 {pv_flux}")
     dm = mesh(5)
@@ -65,6 +65,58 @@ def test_rk4_bitwise(pv_flux):
     rk4_steps_(yd, 600.0, 12, m)
     got = gather_state(m, yd)
     assert np.array_equal(got, ref), rel(got, ref)
+
+
+def _update_bytes(m):
+    import ctypes as C
+
+    from dmplex_amd import _device as D
+    names = (C.c_char_p * 8)()
+    by = np.zeros(8)
+    n = C.c_int32()
+    D.check(D.lib().trisk_kernel_bytes(m.handle, 8, names, by.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+    return dict(zip([names[k].decode() for k in range(n.value)], by))["update_kernel"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+def test_gassmann_slot_and_table_forms(precision):
+    """Gassmann tables in the slot order of gassmann_coefficients run without the PV-edge
+    table (each slot's PV comes from the pair gathered for its inner edge); the same tables
+    with the slots reversed fall back to reading it.  Both agree with the oracle on the
+    tables they were given: bit for bit in FP64, to FP32 rounding in FP32-bulk."""
+    say(f"gassmann slot and table forms level 4 {precision}")
+    from dmplex_amd import _device as D
+    dm = mesh(4)
+    m = TRiSKShallowWater(dm, centers=dm.generators, pv_flux="gassmann", backend=HIPBackend(0, precision),
+                          **earth_parameters())
+    y = seeded_state(m.grid)
+    results = []
+    for form in ("slot", "table"):
+        if form == "table":
+            slot_bytes = _update_bytes(m)
+            for key in ("edges", "pv_edges", "weights"):
+                m.gassmann[key] = np.asfortranarray(m.gassmann[key][::-1])  # k x n column-major
+            D.check(D.lib().trisk_destroy(m.handle))
+            m._handle = None
+            m._create()
+            # the PV-edge table is read again: 12 indices of 2 bytes per edge and tendency
+            assert _update_bytes(m) == slot_bytes + 24 * m.grid.num_edges()
+        ref = Oracle(m).tendency(y)
+        dy = m.zeros()
+        tendency_(dy, m.upload_state(y), m)
+        got = gather_state(m, dy)
+        if precision == "fp64":
+            assert np.array_equal(got, ref), (form, rel(got, ref))
+        else:
+            ne = m.grid.num_edges()
+            # FP32 stencils: u to a few 1e-7, h (a divergence with cancellation) to a few 1e-5
+            assert rel(got[:ne], ref[:ne]) < 2e-6 and rel(got[ne:], ref[ne:]) < 1e-4, form
+        results.append(got)
+    if precision == "fp32_bulk":
+        ne = m.grid.num_edges()
+        assert not np.array_equal(results[0][:ne], results[1][:ne])  # another summation order
+        assert rel(results[1][:ne], results[0][:ne]) < 2e-6
+        assert np.array_equal(results[0][ne:], results[1][ne:])  # dh does not use the PV flux
 
 
 def test_config0_level5_144_steps():

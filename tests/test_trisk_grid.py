@@ -87,6 +87,22 @@ def test_gassmann_antisymmetric_and_consistent(hex3):
     assert np.max(np.abs(np.sum(Wg * solid[E], axis=0) - tangential)) < 0.02
 
 
+def test_gassmann_slot_order(hex3):
+    """Per vertex of e the slots are: inner edge f1, the two outer edges of f1, inner f2,
+    the two outer edges of f2 (TRiSKGrid.jl:233-247).  An inner edge carries the PV of the
+    other inner edge and an outer edge that of its inner edge, so the PV-edge table is the
+    edge table at slots 3, 0, 0, 0, 3, 3, 9, 6, 6, 6, 9, 9, which the device kernel uses."""
+    _, g, _ = hex3
+    E, P, _ = gassmann_coefficients(g)
+    slots = [3, 0, 0, 0, 3, 3, 9, 6, 6, 6, 9, 9]
+    assert np.array_equal(P, E[slots])
+    # inner edges share a vertex with e, outer edges do not
+    ev = g.edge_vertices
+    for k in range(12):
+        touches = np.any(ev[:, E[k]][:, None, :] == ev[None, :, :], axis=(0, 1))
+        assert np.all(touches) == (k % 3 == 0), k
+
+
 def test_dodecahedron_closed_forms():
     """Level 0: 12 regular pentagons.  Every cell has area 4pi/12, every dual triangle
     4pi/20; primal edges subtend arccos(sqrt(5)/3), dual edges arccos(1/sqrt(5)); all kite
