@@ -106,6 +106,7 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
     if (t.range == 0) {
         const int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
+        const RkIn rin = rk_load(r, p.ne + i);
         int32_t E[3];
         double wb[3], wd[3];
 #pragma unroll
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
             for (int k = 0; k < 3; ++k) d += wd[k] * u[E[k]];
             p.D[i] = -d;
         }
-        rk_update<double>(r, p.ne + i, -s, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     } else {
         const int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParams p, con
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
+    const RkIn rin = rk_load(r, e);
     const int32_t l = ld(p.ec + e), rr = ld(p.ec + p.ne + e);
     const int32_t v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
     const double g1 = ld(p.grad + e), g2 = ld(p.grad + p.ne + e);
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParams p, con
         const double G = (-1.0 * (p.D[rr] - p.D[l])) * invd;
         du = du + ld(p.nu + e) * G;
     }
-    rk_update<double>(r, e, du, r.mode >= 1 && r.mode <= 3 ? static_cast<double*>(r.su) + e : nullptr);
+    rk_store<double>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<double*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------

@@ -207,6 +207,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
     if (t.range == 0) {
         int64_t e = t.first + threadIdx.x;
         if (e >= p.ne) return;
+        const RkIn rin = rk_load(r, e);
         double du;
         const Index<W::cw ? 2 : 0> EC(p.ec16p, p.ec, p.ne, e, e / 3);
         double grad = (p.B[EC[1]] - p.B[EC[0]]) * double(ld(p.invd + e));
@@ -241,14 +242,15 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
             }
             du = double(s) - grad;
         }
-        rk_update<U>(r, e, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+        rk_store<U>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
     } else {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
+        const RkIn rin = rk_load(r, p.ne + i);
         const Index<W::cw> CE(p.ce16p, p.ce, p.nc, i, 3 * i);
         double s = 0.0;
         for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
-        rk_update<double>(r, p.ne + i, -s, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
 
@@ -314,6 +316,9 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
     const int64_t nc = p.nc;
     const int64_t first = tile_of(blockIdx.x, t0, 0).first;
     const int64_t i = first + threadIdx.x;
+    // the streaming loads of the RK4 update first: they do not wait for the gathers
+    const RkIn rin = i < nc ? rk_load(r, p.ne + i) : RkIn{0.0, 0.0};
+    const double area = i < nc ? ld(p.area_c + i) : 1.0;
     int32_t meta, E[6], V[6];
     C W[15];
     if constexpr (P::id == TRISK_FP32_BULK) {
@@ -365,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
             div += side ? -double(G[k]) : double(G[k]);
         }
     }
-    rk_update<double>(r, p.ne + i, -div / ld(p.area_c + i), r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+    rk_store<double>(r, p.ne + i, -div / area, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
 }
 
 // W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update
@@ -376,12 +381,13 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     using U = typename P::U;
     const int64_t e = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;
     if (e >= p.ne) return;
+    const RkIn rin = rk_load(r, e);
     int32_t c[2];
     I hs[2];
     load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
     ldrec(p.T + 2 * e, hs);
     const double du = pair_edge_tendency(C(hs[0]), C(hs[1]), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
-    rk_update<U>(r, e, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    rk_store<U>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------

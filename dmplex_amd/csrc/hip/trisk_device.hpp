@@ -210,27 +210,42 @@ struct Rk {
     double dt, dt2, dt6;
 };
 
-// y + c dt k and the accumulator, in the order of rk4_step!'s broadcasts
+// The RK4 stage update of entity j with tendency k: y + c dt k and the accumulator, in the
+// order of rk4_step!'s broadcasts (RungeKutta.jl:18-28).  rk_load reads y_j and acc_j, which
+// a kernel issues first so that these streaming loads do not wait for its gathers; rk_store
+// writes the accumulator and the stage (modes 1-3), the new state (mode 4) or the tendency
+// (mode 0).
+struct RkIn {
+    double y, acc;
+};
+
+__device__ __forceinline__ RkIn rk_load(const Rk& r, int64_t j) {
+    RkIn v{0.0, 0.0};
+    if (r.mode >= 1) v.y = r.y[j];
+    if (r.mode >= 2) v.acc = r.acc[j];
+    return v;
+}
+
 template <class U>
-__device__ __forceinline__ void rk_update(const Rk& r, int64_t j, double k, U* stage) {
+__device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
     switch (r.mode) {
         case 0:
             r.dy[j] = k;
             break;
         case 1:
             r.acc[j] = k;
-            stage[0] = U(r.y[j] + r.dt2 * k);
+            stage[0] = U(v.y + r.dt2 * k);
             break;
         case 2:
-            r.acc[j] = r.acc[j] + 2 * k;
-            stage[0] = U(r.y[j] + r.dt2 * k);
+            r.acc[j] = v.acc + 2 * k;
+            stage[0] = U(v.y + r.dt2 * k);
             break;
         case 3:
-            r.acc[j] = r.acc[j] + 2 * k;
-            stage[0] = U(r.y[j] + r.dt * k);
+            r.acc[j] = v.acc + 2 * k;
+            stage[0] = U(v.y + r.dt * k);
             break;
         default:
-            r.y[j] = r.y[j] + r.dt6 * (r.acc[j] + k);
+            r.y[j] = v.y + r.dt6 * (v.acc + k);
             break;
     }
 }

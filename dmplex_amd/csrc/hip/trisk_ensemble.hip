@@ -137,30 +137,44 @@ __device__ __forceinline__ void vstore(T* p, const Lanes<T, V>& x) {
     *reinterpret_cast<Lanes<T, V>*>(p) = x;
 }
 
-// rk_update (trisk_device.hpp) for V consecutive elements, with the same arithmetic per element
+// rk_load / rk_store (trisk_device.hpp) for V consecutive elements, with the same arithmetic
+// per element: y and acc are loaded before the gathers
+template <int V>
+struct RkInLanes {
+    Lanes<double, V> y, acc;
+};
+
+template <int V>
+__device__ __forceinline__ RkInLanes<V> rk_load_lanes(const Rk& r, int64_t j) {
+    RkInLanes<V> x;
+#pragma unroll
+    for (int v = 0; v < V; ++v) x.y.v[v] = x.acc.v[v] = 0.0;
+    if (r.mode >= 1) x.y = vload<double, V>(r.y + j);
+    if (r.mode >= 2) x.acc = vload<double, V>(r.acc + j);
+    return x;
+}
+
 template <class U, int V>
-__device__ __forceinline__ void rk_update_lanes(const Rk& r, int64_t j, const Lanes<double, V>& k, U* stage) {
+__device__ __forceinline__ void rk_store_lanes(const Rk& r, int64_t j, const Lanes<double, V>& k,
+                                               const RkInLanes<V>& x, U* stage) {
     Lanes<U, V> st;
     if (r.mode == 0) {
         vstore(r.dy + j, k);
     } else if (r.mode <= 3) {
         Lanes<double, V> acc = k;
         if (r.mode > 1) {
-            acc = vload<double, V>(r.acc + j);
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc.v[v] = acc.v[v] + 2 * k.v[v];
+            for (int v = 0; v < V; ++v) acc.v[v] = x.acc.v[v] + 2 * k.v[v];
         }
         vstore(r.acc + j, acc);
-        const Lanes<double, V> y = vload<double, V>(r.y + j);
         const double c = r.mode == 3 ? r.dt : r.dt2;
 #pragma unroll
-        for (int v = 0; v < V; ++v) st.v[v] = U(y.v[v] + c * k.v[v]);
+        for (int v = 0; v < V; ++v) st.v[v] = U(x.y.v[v] + c * k.v[v]);
         vstore(stage, st);
     } else {
-        Lanes<double, V> y = vload<double, V>(r.y + j);
-        const Lanes<double, V> acc = vload<double, V>(r.acc + j);
+        Lanes<double, V> y;
 #pragma unroll
-        for (int v = 0; v < V; ++v) y.v[v] = y.v[v] + r.dt6 * (acc.v[v] + k.v[v]);
+        for (int v = 0; v < V; ++v) y.v[v] = x.y.v[v] + r.dt6 * (x.acc.v[v] + k.v[v]);
         vstore(r.y + j, y);
     }
 }
@@ -270,6 +284,8 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
         const Item it = item_of<V>(L, j);
+        const Rk r = level_rk(L, a);
+        const RkInLanes<V> rin = rk_load_lanes<V>(r, p.ne * M + it.o);
         int32_t meta, E[6], V6[6];
         C W[15];
         load_cell_pairs<NARROW>(p, it.ent, meta, E, V6, W);
@@ -302,8 +318,7 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k)
             if (k < n) vstore(p.T + (2 * int64_t(E[k]) + ((meta >> (4 + k)) & 1)) * M + it.m0, Hk[k]);
-        const Rk r = level_rk(L, a);
-        rk_update_lanes<double, V>(r, p.ne * M + it.o, dh, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
+        rk_store_lanes<double, V>(r, p.ne * M + it.o, dh, rin, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
     });
 }
 
@@ -321,6 +336,8 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
         const Item it = item_of<V>(L, j);
+        const Rk r = level_rk(L, a);
+        const RkInLanes<V> rin = rk_load_lanes<V>(r, it.o);
         int32_t c[2];
         load_edge_cells<NARROW>(p, it.ent, c);
         const int64_t t = 2 * int64_t(it.ent) * M + it.m0;
@@ -331,8 +348,7 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         Lanes<double, V> du;
 #pragma unroll
         for (int v = 0; v < V; ++v) du.v[v] = pair_edge_tendency(C(t0.v[v]), C(t1.v[v]), d, bl.v[v], br.v[v]);
-        const Rk r = level_rk(L, a);
-        rk_update_lanes<U, V>(r, it.o, du, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + it.o : nullptr);
+        rk_store_lanes<U, V>(r, it.o, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + it.o : nullptr);
     });
 }
 
