@@ -63,31 +63,38 @@ struct AdjRk {
     double dt;
 };
 
-__device__ __forceinline__ void adj_update(const AdjRk& r, int64_t j, double s) {
+// lam_j and acc_j are loaded first (adj_load), so that these streaming loads do not wait for
+// the kernel's gathers; adj_store then applies the update above.
+struct AdjIn {
+    double lam, acc;
+};
+
+__device__ __forceinline__ AdjIn adj_load(const AdjRk& r, int64_t j) {
+    AdjIn v{0.0, 0.0};
+    if (r.mode >= 2) v.lam = r.lam[j];
+    if (r.mode == 1 || r.mode == 2 || r.mode == 3) v.acc = r.acc[j];
+    return v;
+}
+
+__device__ __forceinline__ void adj_store(const AdjRk& r, int64_t j, double s, const AdjIn& v) {
     switch (r.mode) {
         case 0:
             r.out[j] = s;
             break;
-        case 4: {
-            const double l = r.lam[j];
-            r.acc[j] = l + s;
-            r.kbar[j] = (r.dt / 3) * l + r.dt * s;
+        case 4:
+            r.acc[j] = v.lam + s;
+            r.kbar[j] = (r.dt / 3) * v.lam + r.dt * s;
             break;
-        }
-        case 3: {
-            const double l = r.lam[j];
-            r.acc[j] = r.acc[j] + s;
-            r.kbar[j] = (r.dt / 3) * l + (r.dt / 2) * s;
+        case 3:
+            r.acc[j] = v.acc + s;
+            r.kbar[j] = (r.dt / 3) * v.lam + (r.dt / 2) * s;
             break;
-        }
-        case 2: {
-            const double l = r.lam[j];
-            r.acc[j] = r.acc[j] + s;
-            r.kbar[j] = (r.dt / 6) * l + (r.dt / 2) * s;
+        case 2:
+            r.acc[j] = v.acc + s;
+            r.kbar[j] = (r.dt / 6) * v.lam + (r.dt / 2) * s;
             break;
-        }
         default:
-            r.lam[j] = r.acc[j] + s;
+            r.lam[j] = v.acc + s;
             break;
     }
 }
@@ -188,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams
     using I = typename P::I;
     const int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (e >= p.ne) return;
+    const AdjIn rin = adj_load(r, e);
     int32_t c[2];
     C lh[2], cu[2], kn[2];
     ldrec(p.re_cells + 2 * e, c);
@@ -201,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams
     const double s = double(Gb * he) + double(cu[0] * C(w.zbar[v0]) + cu[1] * C(w.zbar[v1])) +
                      2.0 * double(ue) * (double(kn[0]) * w.Bbar[c[0]] + double(kn[1]) * w.Bbar[c[1]]);
     w.Gbar[e] = I(Gb);
-    adj_update(r, e, s);
+    adj_store(r, e, s, rin);
 }
 
 // A4: cells
@@ -211,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjPara
     using C = typename P::C;
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= p.nc) return;
+    const AdjIn rin = adj_load(r, p.ne + i);
     const int32_t meta = ld(p.cmeta + i);
     const int n = meta & 15;
     double s = p.g * w.Bbar[i];
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjPara
     for (int k = 0; k < 6; ++k) {
         if (k < n) s += double(ld(w.cvh + k * p.nc + i) * C(w.ebar[ld(p.ccv + k * p.nc + i)]));
     }
-    adj_update(r, p.ne + i, s);
+    adj_store(r, p.ne + i, s, rin);
 }
 
 // the arguments a captured adjoint sweep bakes in
