@@ -332,7 +332,8 @@ __device__ __forceinline__ double edge_w(const Params& p, const double* __restri
     return M / (ew1 * rhl + ew2 * rhr);
 }
 
-template <int KT, bool PAIRED>
+// VERT, PRESS: p.vertical and pressure > 0 fixed at compile time (-1: at run time)
+template <int KT, bool PAIRED, int VERT = -1, int PRESS = -1>
 __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, const double* __restrict__ vn,
                                                                   const double* __restrict__ w,
                                                                   const double* __restrict__ rho, double pressure,
@@ -364,16 +365,28 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     const double* pe = p.f[MPAS_EXNER];
     const double* K = p.f[MPAS_K];
     const double* eta = p.f[MPAS_ETA];
-    const bool mw = p.vertical == 2;
+    const int vert = VERT >= 0 ? VERT : p.vertical;
+    const bool press = PRESS >= 0 ? PRESS == 1 : pressure > 0;
+    const bool mw = vert == 2;
     // w~ at the lower interface of the level; the upper one is the next level's lower
-    double Wnext = p.vertical > 0 && k0 > 0 ? edge_w(p, w, rho, l, r, k0, ew1, ew2, mw) : 0.0;
+    double Wnext = vert > 0 && k0 > 0 ? edge_w(p, w, rho, l, r, k0, ew1, ew2, mw) : 0.0;
 #pragma unroll
     for (int q = 0; q < kLevelsVn; ++q) {
         const int k = k0 + q;
         if (k >= k1) break;
         double s = 0.0;
-        if constexpr (PAIRED) {
-            // (q_b + q_c) with b = e: (q_e + q_x); with b = c: (q_x + q_x)
+        if constexpr (PAIRED && KT > 0) {
+            // (q_b + q_c) with b = e: (q_e + q_x); with b = c: (q_x + q_x).  Every gather of
+            // the level is issued before the sum, so they are in flight together.
+            double Fv[KT], Qv[KT];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) Fv[j] = Fs[I0[j] + p.ne * k];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) Qv[j] = qe[(I1[j] < 0 ? ~I1[j] : I1[j]) + p.ne * k];
+            const double qs = qe[e + p.ne * k];
+#pragma unroll
+            for (int j = 0; j < KT; ++j) s += Tw[j] * Fv[j] * ((I1[j] < 0 ? Qv[j] : qs) + Qv[j]);
+        } else if constexpr (PAIRED) {
             const double qs = qe[e + p.ne * k];
             for (int j = 0; j < kt; ++j) {
                 const bool twice = I1[j] < 0;
@@ -386,12 +399,12 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         }
         const double te = ew1 * th[l + p.nc * k] + ew2 * th[r + p.nc * k];
         double tt = s - invd * (K[r + p.nc * k] - K[l + p.nc * k]);
-        if (pressure > 0) {
+        if (press) {
             const double dpi = (pe[r + p.nc * k] - pe[l + p.nc * k]) * invd;
             tt -= pressure * (Cpd * te * dpi);
         }
         double out;
-        if (p.vertical > 0) {
+        if (vert > 0) {
             const double Wl = Wnext;
             const double Wu = k < p.nlev - 1 ? edge_w(p, w, rho, l, r, k + 1, ew1, ew2, mw) : 0.0;
             Wnext = Wu;
@@ -508,8 +521,20 @@ static int slow_tendencies(mpas_ops* H, const double* vn, const double* w, const
     if ((rc = timed_begin(H, K_D, s, &stop))) return rc;
     const unsigned vchunks = unsigned((p.nlev + kLevelsVn - 1) / kLevelsVn);
     const dim3 gd(unsigned(kXcd * te), vchunks);
-    if (p.kt == 10 && p.ti2)
-        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
+    const bool press = pressure > 0;
+    const dim3 b(kBlock);
+    if (p.kt == 10 && p.ti2 && p.vertical == 0 && press)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 0, 1>), gd, b, 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10 && p.ti2 && p.vertical == 0)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 0, 0>), gd, b, 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10 && p.ti2 && p.vertical == 1 && press)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 1, 1>), gd, b, 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10 && p.ti2 && p.vertical == 1)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 1, 0>), gd, b, 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10 && p.ti2 && press)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 2, 1>), gd, b, 0, s, p, vn, w, rho, pressure, te);
+    else if (p.kt == 10 && p.ti2)
+        hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, true, 2, 0>), gd, b, 0, s, p, vn, w, rho, pressure, te);
     else if (p.kt == 10)
         hipLaunchKernelGGL((mpas_vn_tendency_kernel<10, false>), gd, dim3(kBlock), 0, s, p, vn, w, rho, pressure, te);
     else if (p.ti2)
