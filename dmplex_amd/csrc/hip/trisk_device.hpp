@@ -158,28 +158,31 @@ __device__ __forceinline__ Tile tile_of3(int64_t b, int64_t t0, int64_t t1, int6
 
 inline int64_t tiles_per_xcd(int64_t n) { return ((n + kBlock - 1) / kBlock + kXcd - 1) / kXcd; }
 
+// A pointer into global memory (device allocations, never LDS or scratch): loads through it
+// are global_load instructions.  Kernels that read their pointers from a table in memory
+// (the ensemble's level records) rather than from their arguments would otherwise issue
+// flat loads, whose completion is tracked together with the LDS and scalar counters.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
 // Streamed tables are read once per launch: nontemporal loads keep them from evicting
-// the gathered state and intermediates from the caches.
+// the gathered state and intermediates from the caches.  p points to global memory.
 template <class T>
 __device__ __forceinline__ T ld(const T* p) {
-#ifdef TRISK_PLAIN_LOADS
-    return *p;
-#else
-    return __builtin_nontemporal_load(p);
-#endif
+    return __builtin_nontemporal_load((gptr<T>)p);
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// N consecutive elements starting at p (one entity record) with 16-byte (or, for records
-// of a size not divisible by 16, 8-byte) nontemporal vector loads; p must be aligned to
-// that width.
+// N consecutive elements starting at p (one entity record, global memory) with 16-byte
+// (or, for records of a size not divisible by 16, 8-byte) nontemporal vector loads; p must
+// be aligned to that width.
 template <class T, int N>
 __device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
     constexpr int bytes = N * int(sizeof(T));
     if constexpr (bytes % 16 == 0) {
-        const f32x4* q = reinterpret_cast<const f32x4*>(p);
+        gptr<f32x4> q = (gptr<f32x4>)p;
 #pragma unroll
         for (int c = 0; c < bytes / 16; ++c) {
             f32x4 v = __builtin_nontemporal_load(q + c);
@@ -187,7 +190,7 @@ __device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
         }
     } else {
         static_assert(bytes % 8 == 0, "records are multiples of 8 bytes");
-        const f32x2* q = reinterpret_cast<const f32x2*>(p);
+        gptr<f32x2> q = (gptr<f32x2>)p;
 #pragma unroll
         for (int c = 0; c < bytes / 8; ++c) {
             f32x2 v = __builtin_nontemporal_load(q + c);

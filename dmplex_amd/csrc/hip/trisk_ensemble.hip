@@ -98,17 +98,43 @@ __device__ __forceinline__ Rk level_rk(const EnsLevel<P>& L, const EnsArgs& a) {
 // tiles 3k..3k+2 (a level has fewer than 2 vertices and 3 edges per cell), so the blocks
 // running together touch one stretch of the mesh; count(level, kind) gives the items of
 // the level's cells (0), vertices (1) or edges (2).
+// The level records and tile runs are written by the host before a launch and only read by
+// the kernels: they are loaded through the constant address space, which lets the compiler
+// use scalar loads for these block-uniform reads.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <class P>
+__device__ __forceinline__ EnsLevel<P> load_level(const void* levels, int level) {
+    static_assert(sizeof(EnsLevel<P>) % 8 == 0 && alignof(EnsLevel<P>) >= 8, "levels are arrays of 8-byte words");
+    constexpr int W = int(sizeof(EnsLevel<P>) / 8);
+    cptr<uint64_t> src = (cptr<uint64_t>)levels + int64_t(level) * W;
+    uint64_t w[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = src[k];
+    EnsLevel<P> L;
+    __builtin_memcpy(&L, w, sizeof(L));
+    return L;
+}
+
 template <int BLK, class N, class F>
 __device__ __forceinline__ void for_each_item(const EnsArgs& a, N&& count, F&& body) {
     const int x = int(blockIdx.x % kXcd);
     const int nbx = int(gridDim.x / kXcd);
     const int s0 = a.first[x], s1 = a.first[x + 1];
     if (s1 <= s0) return;
-    const int total = a.seg[s1 - 1].cum + a.seg[s1 - 1].ntiles;
+    const cptr<EnsSeg> seg = (cptr<EnsSeg>)a.seg;
+    const int total = seg[s1 - 1].cum + seg[s1 - 1].ntiles;
     int s = s0;
     for (int t = int(blockIdx.x / kXcd); t < total; t += nbx) {
-        while (s + 1 < s1 && t >= a.seg[s + 1].cum) ++s;  // t only grows
-        const EnsSeg g = a.seg[s];
+        while (s + 1 < s1 && t >= seg[s + 1].cum) ++s;  // t only grows
+        EnsSeg g;
+        g.level = seg[s].level;
+        g.kind = seg[s].kind;
+        g.tile0 = seg[s].tile0;
+        g.ntiles = seg[s].ntiles;
+        g.cum = seg[s].cum;
+        g.nitems = seg[s].nitems;
         int64_t tile = g.tile0 + (t - g.cum);
         int kind = g.kind;
         int64_t nitems = g.nitems;
@@ -128,13 +154,21 @@ template <class T, int V>
 struct alignas(sizeof(T) * V) Lanes {
     T v[V];
 };
+// p points to global memory; element by element through a global pointer, which the compiler
+// merges into global (not flat) vector accesses
 template <class T, int V>
 __device__ __forceinline__ Lanes<T, V> vload(const T* p) {
-    return *reinterpret_cast<const Lanes<T, V>*>(p);
+    const gptr<T> q = (gptr<T>)p;
+    Lanes<T, V> x;
+#pragma unroll
+    for (int v = 0; v < V; ++v) x.v[v] = q[v];
+    return x;
 }
 template <class T, int V>
 __device__ __forceinline__ void vstore(T* p, const Lanes<T, V>& x) {
-    *reinterpret_cast<Lanes<T, V>*>(p) = x;
+    __attribute__((address_space(1))) T* q = (__attribute__((address_space(1))) T*)p;
+#pragma unroll
+    for (int v = 0; v < V; ++v) q[v] = x.v[v];
 }
 
 // rk_load / rk_store (trisk_device.hpp) for V consecutive elements, with the same arithmetic
@@ -202,13 +236,12 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
-    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
     auto count = [&](int level, int kind) -> int64_t {
-        const EnsLevel<P>& L = lv[level];
+        const EnsLevel<P> L = load_level<P>(a.levels, level);
         return (kind == 0 ? L.p.nc : kind == 1 ? L.p.nv : L.p.ne) * (L.M / V);
     };
     for_each_item<BLK>(a, count, [&](int level, int kind, uint32_t j) {
-        const EnsLevel<P>& L = lv[level];
+        const EnsLevel<P> L = load_level<P>(a.levels, level);
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
         const Item it = item_of<V>(L, j);
@@ -277,10 +310,9 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
-    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
     auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
     for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
-        const EnsLevel<P>& L = lv[level];
+        const EnsLevel<P> L = load_level<P>(a.levels, level);
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
         const Item it = item_of<V>(L, j);
@@ -329,10 +361,9 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
     using I = typename P::I;
     using U = typename P::U;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
-    const EnsLevel<P>* lv = static_cast<const EnsLevel<P>*>(a.levels);
     auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
     for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
-        const EnsLevel<P>& L = lv[level];
+        const EnsLevel<P> L = load_level<P>(a.levels, level);
         const Params<P>& p = L.p;
         const uint32_t M = uint32_t(L.M);
         const Item it = item_of<V>(L, j);
