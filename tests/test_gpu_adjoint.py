@@ -134,17 +134,17 @@ def test_graph_replayed_sweep_equals_direct(precision):
         assert bool((out == ref[k]).all())
 
 
-def config4(precision):
+def level7_misfit(precision):
     m = model(7, precision)
     y0 = seeded_state(m.grid)
     hstar = m.upload_state(seeded_state(m.grid, seed_coef=3, seed_noise=4))[m.grid.num_edges():].clone()
     return m, y0, hstar
 
 
-def test_config4_gradient_matches_finite_differences():
+def test_level7_gradient_matches_finite_differences():
     """Level 7, 40 RK4 steps of 150 s, FP64: dJ/dy0 against 32 central differences."""
-    say("config 4: level 7, 40 steps, 32 FD directions")
-    m, y0, hstar = config4("fp64")
+    say("adjoint: level 7, 40 steps, 32 FD directions")
+    m, y0, hstar = level7_misfit("fp64")
     dt, N = 150.0, 40
     yd = m.upload_state(y0)
     yT, pullback = rrule_rk4_steps(m, yd, dt, N, checkpoint_every=8)
@@ -170,10 +170,10 @@ def test_config4_gradient_matches_finite_differences():
 
 
 def test_fp32_bulk_gradient_close_to_fp64():
-    say("config 4: FP32-bulk gradient vs FP64")
+    say("adjoint: level 7 FP32-bulk gradient vs FP64")
     grads = {}
     for p in ("fp64", "fp32_bulk"):
-        m, y0, hstar = config4(p)
+        m, y0, hstar = level7_misfit(p)
         yT, pullback = rrule_rk4_steps(m, m.upload_state(y0), 150.0, 40)
         grads[p] = m.download_state(pullback(height_misfit(m, yT, hstar)[1]))
     ne = m.grid.num_edges()

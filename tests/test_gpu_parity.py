@@ -119,9 +119,9 @@ def test_gassmann_slot_and_table_forms(precision):
         assert np.array_equal(results[0][ne:], results[1][ne:])  # dh does not use the PV flux
 
 
-def test_config0_level5_144_steps():
-    """Configuration 0: level 5, TRiSK, cell KE, dt = 600 s, 144 steps (one day)."""
-    say("config 0: level 5, 144 steps")
+def test_level5_one_day_144_steps():
+    """Level 5, TRiSK, cell KE, dt = 600 s, 144 steps (one day)."""
+    say("level 5, 144 steps of 600 s")
     dm = mesh(5)
     m = TRiSKShallowWater(dm, centers=dm.generators, **earth_parameters())
     y0 = seeded_state(m.grid)
