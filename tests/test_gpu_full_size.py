@@ -1,0 +1,91 @@
+"""The step at the benchmark's full sizes: SCVT level 9 (2,621,442 cells, TRiSK PV flux, cell
+kinetic energy, dt = 45 s) and level 10 (10,485,762 cells, Gassmann PV flux, vertex kinetic
+energy, without Lloyd iterations).  The oracle finishes one tendency in about a second at these
+sizes, so the FP64 path is checked bit for bit against it; longer runs are checked through
+size-independent properties: mass conservation, agreement between the FP64 table and cell-pair
+paths, the FP32-bulk error, and bitwise determinism of direct launches against graph replay."""
+import gc
+import sys
+
+import numpy as np
+import pytest
+
+from dmplex_amd import (HIPBackend, TRiSKGrid, TRiSKShallowWater, create_hexagon_mesh, diagnostics, gather_state,
+                        rk4_steps_, tendency_)
+from dmplex_amd.synthetic import earth_parameters, seeded_state
+from oracle.oracle import Oracle
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def level9():
+    say("level 9 SCVT mesh (Lloyd on the GPU)")
+    dm = create_hexagon_mesh(9, centroidal=True, device=0)
+    grid = TRiSKGrid(dm, centers=dm.generators)
+    return grid, seeded_state(grid)
+
+
+def test_level9_fp64_bitwise_with_oracle(level9):
+    grid, y = level9
+    say("level 9 FP64: tendency and 2 RK4 steps against the oracle")
+    m = TRiSKShallowWater(grid, **earth_parameters())
+    o = Oracle(m)
+    dy = m.zeros()
+    tendency_(dy, m.upload_state(y), m)
+    assert np.array_equal(gather_state(m, dy), o.tendency(y))
+    yd = m.upload_state(y)
+    rk4_steps_(yd, 45.0, 2, m)
+    assert np.array_equal(gather_state(m, yd), o.rk4_steps(y.copy(), 45.0, 2))
+
+
+def test_level9_paths_conservation_and_determinism(level9):
+    grid, y = level9
+    steps = 20
+    say(f"level 9: {steps} steps on the FP64 table and pair paths and in FP32-bulk")
+    out = {}
+    for key, precision, path in (("fp64", "fp64", "tables"), ("fp64_pair", "fp64", "pair"),
+                                 ("fp32_bulk", "fp32_bulk", "auto")):
+        m = TRiSKShallowWater(grid, backend=HIPBackend(0, precision), path=path, **earth_parameters())
+        yd = m.upload_state(y)
+        mass0 = diagnostics(m, yd)[0]
+        rk4_steps_(yd, 45.0, steps, m)
+        mass = diagnostics(m, yd)[0]
+        assert abs(mass - mass0) <= 1e-14 * mass0, key
+        out[key] = m.download_state(yd)
+        if key == "fp32_bulk":
+            # the same steps replayed from captured graphs give the same bits
+            m.use_graphs(True)
+            yg = m.upload_state(y)
+            rk4_steps_(yg, 45.0, steps, m)
+            assert np.array_equal(m.download_state(yg), out[key])
+        del m
+    ne = grid.num_edges()
+    for sl in (slice(0, ne), slice(ne, None)):
+        assert rel(out["fp64_pair"][sl], out["fp64"][sl]) < 1e-12
+    assert rel(out["fp32_bulk"][ne:], out["fp64"][ne:]) < 1e-6
+    assert rel(out["fp32_bulk"][:ne], out["fp64"][:ne]) < 1e-4
+
+
+def test_level10_gassmann_vertex_ke_tendency():
+    say("level 10 (no Lloyd): Gassmann PV flux and vertex kinetic energy, one FP64 tendency")
+    dm = create_hexagon_mesh(10, centroidal=False)
+    grid = TRiSKGrid(dm, centers=dm.generators)
+    del dm
+    y = seeded_state(grid)
+    m = TRiSKShallowWater(grid, pv_flux="gassmann", kinetic_energy="vertex", **earth_parameters())
+    ref = Oracle(m).tendency(y)
+    dy = m.zeros()
+    tendency_(dy, m.upload_state(y), m)
+    assert np.array_equal(gather_state(m, dy), ref)
+    del m, dy, ref, grid
+    gc.collect()

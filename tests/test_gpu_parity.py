@@ -3,8 +3,7 @@
 The FP64 kernels evaluate every formula in the reference's order without FMA
 contraction, so tendencies, RK4 states and the gathered diagnostics must agree with
 the oracle bit for bit; the compensated sums agree to a few ulps.  Sizes are the small
-meshes the oracle finishes in seconds; the level-9 workload is covered by
-size-independent properties (conservation, determinism).
+meshes the oracle finishes in seconds; test_gpu_full_size.py covers levels 9 and 10.
 """
 import sys
 
