@@ -1,6 +1,7 @@
 """The step at the benchmark's full sizes: SCVT level 9 (2,621,442 cells, TRiSK PV flux, cell
 kinetic energy, dt = 45 s) and level 10 (10,485,762 cells, Gassmann PV flux, vertex kinetic
-energy, without Lloyd iterations).  The oracle finishes one tendency in about a second at these
+energy, without Lloyd iterations); the ICON model on triangle level 9 and the MPAS-Gassmann
+stage operators on hexagon 7 x 30 levels, the sizes they are measured at.  The oracle finishes one tendency in about a second at these
 sizes, so the FP64 path is checked bit for bit against it; longer runs are checked through
 size-independent properties: mass conservation, agreement between the FP64 table and cell-pair
 paths, the FP32-bulk error, and bitwise determinism of direct launches against graph replay."""
@@ -89,3 +90,54 @@ def test_level10_gassmann_vertex_ke_tendency():
     assert np.array_equal(gather_state(m, dy), ref)
     del m, dy, ref, grid
     gc.collect()
+
+
+def test_icon_level9_bitwise_with_oracle():
+    """ICONShallowWater on the SCVT triangle mesh of level 9 (5,242,880 cells), with ICON's
+    divergence damping: one tendency and two RK4 steps of 20 s bit for bit."""
+    from dmplex_amd import ICONShallowWater, create_spherical_mesh
+    from oracle.oracle import IconOracle
+
+    say("ICON level 9 triangles: tendency and 2 RK4 steps against the oracle")
+    dm = create_spherical_mesh(9, centroidal=True, device=0)
+    dt = 20.0
+    nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
+    m = ICONShallowWater(dm, divergence_damping=nu, **earth_parameters())
+    y = seeded_state(m.grid)
+    o = IconOracle(m)
+    dy = m.zeros()
+    tendency_(dy, m.upload_state(y), m)
+    assert np.array_equal(gather_state(m, dy), o.tendency(y))
+    yd = m.upload_state(y)
+    rk4_steps_(yd, dt, 2, m)
+    assert np.array_equal(gather_state(m, yd), o.rk4_steps(y.copy(), dt, 2))
+
+
+def test_mpas_hexagon7_30_levels_with_oracle():
+    """The MPAS-Gassmann stage tendencies on SCVT hexagon 7 x 30 levels (the measured
+    workload): every field bit for bit, the Exner pressure to 1 ulp."""
+    from dmplex_amd.mpas_ops import FIELDS, MPASOperators, vertical_grid
+    from oracle.oracle import MpasOracle
+
+    say("MPAS hexagon 7 x 30 levels: one slow_tendencies call against the oracle")
+    dm = create_hexagon_mesh(7, centroidal=True, device=0)
+    g = TRiSKGrid(dm, centers=dm.generators)
+    v = vertical_grid(30, 30000.0)
+    m = MPASOperators(g, v)
+    rng = np.random.default_rng(5)
+    nl, nc, ne = 30, g.num_cells(), g.num_edges()
+    rho = np.exp(-v.z_full / 8000.0)[:, None] * (1 + 0.05 * rng.random((nl, nc)))
+    rt = rho * 300.0 * np.exp(v.z_full / 20000.0)[:, None]
+    w = 0.5 * rng.standard_normal((nl + 1, nc))
+    w[0] = w[-1] = 0
+    vn = 20 * rng.standard_normal((nl, ne))
+    ref = MpasOracle(m).slow_tendencies(vn, w, rho, rt, 0.0)
+    m.slow_tendencies(*m.upload_state(vn, w, rho, rt), pressure=0.0)
+    for f in FIELDS:
+        if (f == "Kv" and m.vertex_kinetic is None) or (f == "eta" and m.vertical == 0):
+            continue
+        got = m.field(f)
+        if f == "exner":
+            np.testing.assert_allclose(got, ref[f], rtol=5e-16, atol=0, err_msg=f)
+        else:
+            assert np.array_equal(got, ref[f]), f
