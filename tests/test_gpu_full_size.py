@@ -1,7 +1,8 @@
 """The step at the benchmark's full sizes: SCVT level 9 (2,621,442 cells, TRiSK PV flux, cell
 kinetic energy, dt = 45 s) and level 10 (10,485,762 cells, Gassmann PV flux, vertex kinetic
-energy, without Lloyd iterations); the ICON model on triangle level 9 and the MPAS-Gassmann
-stage operators on hexagon 7 x 30 levels, the sizes they are measured at.  The oracle finishes one tendency in about a second at these
+energy, without Lloyd iterations); the ICON model on triangle level 9, the MPAS-Gassmann stage
+operators on hexagon 7 x 30 levels and the 64-member ensemble on levels 4-7, the sizes they are
+measured at.  The oracle finishes one tendency in about a second at these
 sizes, so the FP64 path is checked bit for bit against it; longer runs are checked through
 size-independent properties: mass conservation, agreement between the FP64 table and cell-pair
 paths, the FP32-bulk error, and bitwise determinism of direct launches against graph replay."""
@@ -141,3 +142,38 @@ def test_mpas_hexagon7_30_levels_with_oracle():
             np.testing.assert_allclose(got, ref[f], rtol=5e-16, atol=0, err_msg=f)
         else:
             assert np.array_equal(got, ref[f]), f
+
+
+def test_ensemble_64_members_levels_4_to_7():
+    """The measured ensemble: 16 members on each of SCVT levels 4-7, FP32-bulk, per-level dt
+    1200/600/300/150 s, under the autotuned schedule with graph replay.  After 8 steps every
+    member equals a single model of its level bit for bit and has kept its mass."""
+    from dmplex_amd import ModelTables, TRiSKEnsemble
+
+    say("ensemble: 64 members on levels 4-7, autotuned, 8 steps against single models")
+    levels, dts, M = (4, 5, 6, 7), [1200.0, 600.0, 300.0, 150.0], 16
+    tabs = []
+    for n in levels:
+        dm = create_hexagon_mesh(n, centroidal=True, device=0)
+        tabs.append(ModelTables(dm, centers=dm.generators, **earth_parameters()))
+    ens = TRiSKEnsemble(tabs, [M] * 4, backend=HIPBackend(0, "fp32_bulk"))
+    y = ens.zeros()
+
+    def state(li, m):
+        return seeded_state(tabs[li].grid, seed_coef=1000 + M * li + m, seed_noise=2000 + M * li + m)
+
+    for li in range(4):
+        for m in range(M):
+            ens.set_member(y, li, m, state(li, m))
+    ens.autotune(y, dts)
+    ens.use_graphs(True)
+    mass0 = ens.diagnostics(y)[:, 0]
+    ens.rk4_steps_(y, dts, 8)
+    mass = ens.diagnostics(y)[:, 0]
+    assert np.all(np.abs(mass - mass0) <= 1e-14 * mass0)
+    for li in range(4):
+        s = TRiSKShallowWater(tabs[li].grid, path="pair", backend=HIPBackend(0, "fp32_bulk"), **earth_parameters())
+        for m in range(M):
+            yd = s.upload_state(state(li, m))
+            rk4_steps_(yd, dts[li], 8, s)
+            assert np.array_equal(ens.get_member(y, li, m), s.download_state(yd)), (levels[li], m)
