@@ -11,6 +11,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -53,16 +54,17 @@ def build_device() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libtrisk_hip.so"
     tmp = LIB / "libtrisk_hip.so.tmp"
-    objs = []
+    cmds = []
     for src in HIP_SOURCES:
-        obj = LIB / (Path(src).stem + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", "-o", str(obj), src], PKG)
-        objs.append(str(obj))
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", "-o", str(LIB / (Path(src).stem + ".o")), src])
     for src in HIP_HOST_SOURCES:
-        obj = LIB / (Path(src).stem + ".o")
-        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
-              "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-c", "-o", str(obj), src], PKG)
-        objs.append(str(obj))
+        cmds.append(["g++", "-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
+                     "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-c", "-o", str(LIB / (Path(src).stem + ".o")),
+                     src])
+    objs = [c[c.index("-o") + 1] for c in cmds]
+    # one compiler per source, at most as many at once as there are cores (and 8)
+    with ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1))) as pool:
+        list(pool.map(lambda c: _run(c, PKG), cmds))
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", str(tmp), *objs], PKG)
     for o in objs:
         os.remove(o)
