@@ -50,6 +50,10 @@ struct IconParams {
     const int32_t* vc;
     const double* vh;
     const double* fv;
+    // the index tables as 16-bit deltas (two per word; kWide: read the int32 row): cell edges
+    // from 3i/2, edge cells from 2e/3, edge vertices from e/3, RBF edges from e, vertex edges
+    // from 3v
+    const int32_t *ce16p, *ec16p, *ev16p, *re16p, *ve16p;
     // work
     double* v;    // ne: RBF tangential velocity
     KeF* KF;      // ne: edge kinetic energy and mass flux
@@ -75,20 +79,18 @@ __global__ __launch_bounds__(kBlock) void icon_edge_kernel(IconParams p, const d
     if (e >= p.ne) return;
     double s = 0.0;
     if constexpr (KR > 0) {
-        int32_t idx[KR];
+        const Index<KR> RE(p.re16p, p.re, p.ne, e, e);
         double w[KR];
 #pragma unroll
-        for (int k = 0; k < KR; ++k) {
-            idx[k] = ld(p.re + k * p.ne + e);
-            w[k] = ld(p.rw + k * p.ne + e);
-        }
+        for (int k = 0; k < KR; ++k) w[k] = ld(p.rw + k * p.ne + e);
 #pragma unroll
-        for (int k = 0; k < KR; ++k) s += w[k] * u[idx[k]];
+        for (int k = 0; k < KR; ++k) s += w[k] * u[RE[k]];
     } else {
         for (int k = 0; k < p.kr; ++k) s += ld(p.rw + k * p.ne + e) * u[ld(p.re + k * p.ne + e)];
     }
     const double ue = u[e];
-    const int32_t l = ld(p.ec + e), r = ld(p.ec + p.ne + e);
+    const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
+    const int32_t l = EC[0], r = EC[1];
     const double F = (ld(p.eh + e) * h[l] + ld(p.eh + p.ne + e) * h[r]) * ue;
     p.v[e] = s;
     p.KF[e] = KeF{(ue * ue + s * s) / 2, F};
@@ -107,11 +109,10 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
         const int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
         const RkIn rin = rk_load(r, p.ne + i);
-        int32_t E[3];
+        const Index<3> E(p.ce16p, p.ce, p.nc, i, (3 * i) / 2);
         double wb[3], wd[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            E[k] = ld(p.ce + k * p.nc + i);
             wb[k] = ld(p.bln + k * p.nc + i);
             wd[k] = ld(p.div + k * p.nc + i);
         }
@@ -136,15 +137,12 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
         if (v >= p.nv) return;
         double z = 0.0;
         if constexpr (KV > 0) {
-            int32_t idx[KV];
+            const Index<KV> VE(p.ve16p, p.ve, p.nv, v, 3 * v);
             double w[KV];
 #pragma unroll
-            for (int k = 0; k < KV; ++k) {
-                idx[k] = ld(p.ve + k * p.nv + v);
-                w[k] = ld(p.curl + k * p.nv + v);
-            }
+            for (int k = 0; k < KV; ++k) w[k] = ld(p.curl + k * p.nv + v);
 #pragma unroll
-            for (int k = 0; k < KV; ++k) z += w[k] * u[idx[k]];
+            for (int k = 0; k < KV; ++k) z += w[k] * u[VE[k]];
         } else {
             for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * u[ld(p.ve + k * p.nv + v)];
         }
@@ -160,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void icon_damp_edge_kernel(IconParams p, in
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
-    const int32_t l = ld(p.ec + e), r = ld(p.ec + p.ne + e);
+    const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
+    const int32_t l = EC[0], r = EC[1];
     p.G[e] = (1.0 * (p.D[r] - p.D[l])) * ld(p.invd + e);
 }
 
@@ -168,9 +167,10 @@ __global__ __launch_bounds__(kBlock) void icon_damp_cell_kernel(IconParams p, in
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t i = t.first + threadIdx.x;
     if (i >= p.nc) return;
+    const Index<3> E(p.ce16p, p.ce, p.nc, i, (3 * i) / 2);
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) s += ld(p.div + k * p.nc + i) * p.G[ld(p.ce + k * p.nc + i)];
+    for (int k = 0; k < 3; ++k) s += ld(p.div + k * p.nc + i) * p.G[E[k]];
     p.D[i] = -s;
 }
 
@@ -185,8 +185,9 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParams p, con
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
     const RkIn rin = rk_load(r, e);
-    const int32_t l = ld(p.ec + e), rr = ld(p.ec + p.ne + e);
-    const int32_t v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
+    const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3), EV(p.ev16p, p.ev, p.ne, e, e / 3);
+    const int32_t l = EC[0], rr = EC[1];
+    const int32_t v0 = EV[0], v1 = EV[1];
     const double g1 = ld(p.grad + e), g2 = ld(p.grad + p.ne + e);
     const double invd = ld(p.invd + e);
     const double zeta = (p.zeta[v0] + p.zeta[v1]) / 2;
@@ -381,14 +382,17 @@ std::vector<std::pair<const char*, double>> icon_kernel_bytes(const trisk_handle
     const Layout& L = H->L;
     const bool damp = !L.damp.empty();
     const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), KR = L.kr, KV = L.kv;
-    const double ix = 4.0, c = 8.0, rk = 8.0 + 16.0 + 8.0;
+    // index tables as 16-bit deltas (2 B per index); the RBF and vertex-edge tables of other
+    // widths than the triangle mesh's 4 and 6 are read as int32
+    const double ix = 2.0, ixr = L.kr == 4 ? 2.0 : 4.0, ixv = L.kv == 6 ? 2.0 : 4.0, c = 8.0,
+                 rk = 8.0 + 16.0 + 8.0;
     std::vector<std::pair<const char*, double>> v;
     // E: RBF table, u, cells, c_lin_e, h, (K_e, F_e), v
-    v.push_back({kKernelNames[K_ICON_EDGE], ne * (KR * (ix + c) + 8.0 + 2 * ix + 2 * c + 16.0 + 8.0) + nc * 8.0});
+    v.push_back({kKernelNames[K_ICON_EDGE], ne * (KR * (ixr + c) + 8.0 + 2 * ix + 2 * c + 16.0 + 8.0) + nc * 8.0});
     // C: cell edges, bilinear and divergence weights, (K_e, F_e), K, RK4 of h [u, D];
     //    vertex edges, curl, u, zeta
     v.push_back({kKernelNames[K_ICON_CELL_VERTEX], nc * (3 * ix + 6 * c + 8.0 + rk + (damp ? 8.0 : 0.0)) +
-                                                       ne * (16.0 + 8.0) + nv * (KV * (ix + c) + 8.0)});
+                                                       ne * (16.0 + 8.0) + nv * (KV * (ixv + c) + 8.0)});
     if (damp) {
         v.push_back({kKernelNames[K_ICON_DAMP_EDGE], ne * (2 * ix + c + 8.0) + nc * 8.0});
         v.push_back({kKernelNames[K_ICON_DAMP_CELL], nc * (3 * ix + 3 * c + 8.0) + ne * 8.0});
@@ -434,6 +438,11 @@ static int icon_setup(trisk_handle* H) {
     rc |= upload(o, L.vc, &p.vc);
     rc |= upload(o, L.vh, &p.vh);
     rc |= upload(o, L.fv, &p.fv);
+    rc |= upload(o, L.ce16p, &p.ce16p);
+    rc |= upload(o, L.ec16p, &p.ec16p);
+    rc |= upload(o, L.ev16p, &p.ev16p);
+    rc |= upload(o, L.re16p, &p.re16p);
+    rc |= upload(o, L.ve16p, &p.ve16p);
     rc |= upload(o, L.area_c, &H->p64.area_c);
     rc |= upload(o, L.area_v, &H->p64.area_v);
     if (rc) return rc;
@@ -488,7 +497,7 @@ int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_han
     H->precision = TRISK_FP64;
     H->device = o.device;
     std::string err;
-    if (!build_icon_layout(desc, H->L, err)) {
+    if (!build_icon_layout(desc, H->L, err, o.narrow_limit)) {
         delete H;
         return set_err(TRISK_EINVAL, err);
     }

@@ -61,40 +61,6 @@ struct Widths {
 using AnyWidths = Widths<0, 0, 0, 0>;
 using HexWidths = Widths<6, 3, 10, 12>;
 
-// Slot k of entity j of a slot-major index table.  With a compile-time width K the row is
-// decoded up front from the 16-bit deltas packed two per word (base + delta; the int32 row
-// when word 0 holds kWide), so all its loads issue together; with K = 0 each slot is read
-// from the int32 table when used.
-template <int K>
-struct Index {
-    int32_t v[K > 0 ? K : 1];
-    const int32_t* t32;
-    int64_t n, j;
-    __device__ __forceinline__ Index(const int32_t* p16, const int32_t* t32_, int64_t n_, int64_t j_, int64_t base)
-        : t32(t32_), n(n_), j(j_) {
-        if constexpr (K > 0) {
-            constexpr int W = (K + 1) / 2;
-            int32_t w[W];
-#pragma unroll
-            for (int q = 0; q < W; ++q) w[q] = ld(p16 + q * n + j);
-            if (int16_t(w[0] & 0xffff) == kWide) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) v[k] = ld(t32 + k * n + j);
-            } else {
-#pragma unroll
-                for (int k = 0; k < K; ++k)
-                    v[k] = int32_t(base) + (k % 2 ? int16_t(uint32_t(w[k / 2]) >> 16) : int16_t(w[k / 2] & 0xffff));
-            }
-        }
-    }
-    __device__ __forceinline__ int32_t operator[](int k) const {
-        if constexpr (K > 0)
-            return v[k];
-        else
-            return ld(t32 + k * n + j);
-    }
-};
-
 // ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
 
@@ -1128,7 +1094,6 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
 
 int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]) {
     CHECK_HANDLE(h);
-    CHECK_TRISK(h);
     counts[0] = h->L.wide_cells;
     counts[1] = h->L.wide_vertices;
     counts[2] = h->L.wide_edges;

@@ -104,7 +104,8 @@ void invert(const std::vector<int64_t>& perm, std::vector<int64_t>& inv) {
 // w * n + j holds slots 2w (low half) and 2w + 1 (high half); kWide in the low half of word
 // 0 for an entity whose deltas do not all fit.
 template <class Base>
-static int64_t narrow_table(const std::vector<int32_t>& t32, int K, int64_t n, Base base, std::vector<int32_t>& out) {
+static int64_t narrow_table(const std::vector<int32_t>& t32, int K, int64_t n, Base base, std::vector<int32_t>& out,
+                            int64_t lim = 32767) {
     const int words = (K + 1) / 2;
     out.assign(size_t(words) * n, 0);
     int64_t wide = 0;
@@ -114,7 +115,7 @@ static int64_t narrow_table(const std::vector<int32_t>& t32, int K, int64_t n, B
         bool ok = true;
         for (int k = 0; k < K && ok; ++k) {
             const int64_t dlt = int64_t(t32[size_t(k) * n + j]) - b;
-            ok = dlt >= -32767 && dlt <= 32767;
+            ok = dlt >= -lim && dlt <= lim;
         }
         if (!ok) {
             out[j] = int32_t(uint32_t(uint16_t(kWide)));
@@ -288,7 +289,7 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     return true;
 }
 
-bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err) {
+bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err, int32_t narrow_limit) {
     if (!d) {
         err = "null model descriptor";
         return false;
@@ -381,6 +382,15 @@ bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err) {
     remap_value(d->vertex_height, KV, nv, L.vperm, L.vh);
     remap_value(d->coriolis, 1, nv, L.vperm, L.fv);
     remap_value(d->vertex_areas, 1, nv, L.vperm, L.area_v);
+    // 16-bit deltas from the linear bases of triangle meshes (ne = 3 nc / 2 = 3 nv): cell i's
+    // edges near 3i/2, edge e's cells near 2e/3 and vertices near e/3, its RBF edges near e,
+    // vertex v's edges near 3v
+    const int64_t lim = narrow_limit > 0 && narrow_limit < 32767 ? narrow_limit : 32767;
+    L.wide_cells = narrow_table(L.ce, 3, nc, [](int64_t j) { return (3 * j) / 2; }, L.ce16p, lim);
+    L.wide_edges = narrow_table(L.ec, 2, ne, [](int64_t j) { return (2 * j) / 3; }, L.ec16p, lim);
+    L.wide_edges += narrow_table(L.ev, 2, ne, [](int64_t j) { return j / 3; }, L.ev16p, lim);
+    L.wide_edges += narrow_table(L.re, KR, ne, [](int64_t j) { return j; }, L.re16p, lim);
+    L.wide_vertices = narrow_table(L.ve, KV, nv, [](int64_t j) { return 3 * j; }, L.ve16p, lim);
     L.pair_reason = "ICON triangle model: RBF tangential velocity, bilinear kinetic energy";
     return true;
 }

@@ -45,6 +45,7 @@ struct Layout {
     // (slots 2w | 2w+1 << 16), slot-major by word; an entity with any delta outside
     // +-32767 has kWide in the low half of word 0
     std::vector<int32_t> ce16p, cv16p, ec16p, ev16p, te16p, ge16p, gp16p, ve16p, vc16p;
+    std::vector<int32_t> re16p;  // ICON: RBF edges (base e)
     int64_t wide_table_entities = 0;
     // cell-pair path (TRiSK PV flux and cell kinetic energy only): per cell the 15
     // antisymmetric TRiSK weights w_{k,k'} (k < k') of its edge pairs
@@ -104,7 +105,7 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
 // Validates an ICON descriptor (triangles, every index in range) and fills the generic
 // fields of `L` (sizes, permutations, ce, div, ec, ev, eh, invd, ve, curl, vc, vh, fv,
 // areas) and the ICON ones, in the same numbering as build_layout.
-bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err);
+bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err, int32_t narrow_limit = 32767);
 
 // Tables of the MPAS-Gassmann operators (mpas_ops_create) in the device numbering: cells
 // along the Hilbert curve, edges and vertices by first appearance; entity-major tables

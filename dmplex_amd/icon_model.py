@@ -117,7 +117,7 @@ class ICONShallowWater(ICONTables, CGridShallowWater):
 
     def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, rbf_scale: float = 0.5,
                  divergence_damping=0.0, kinetic_gradient: str = "icon", comm=None, part=None, backend=None,
-                 hilbert: bool = True):
+                 hilbert: bool = True, narrow_limit: int = 0):
         if comm is not None or part is not None:
             raise NotImplementedError("the distributed model (comm/part) is not part of this one-GPU build")
         if backend is None:
@@ -130,7 +130,7 @@ class ICONShallowWater(ICONTables, CGridShallowWater):
         self.backend = backend
         torch = _torch()
         torch.cuda.set_device(backend.device)
-        opts = D.TriskOptions(D.TRISK_FP64, backend.device, 0, 0, 0)
+        opts = D.TriskOptions(D.TRISK_FP64, backend.device, 0, 0, int(narrow_limit))
         h = C.c_void_p()
         desc = self.descriptor(hilbert)
         D.check(D.lib().icon_create(C.byref(desc), C.byref(opts), C.byref(h)))

@@ -173,6 +173,12 @@ class CGridShallowWater:
     def handle(self) -> C.c_void_p:
         return self._handle
 
+    def wide_counts(self) -> tuple[int, int, int]:
+        """Cells, vertices, edges read with int32 instead of 16-bit-delta indices."""
+        n = np.zeros(3, dtype=np.int64)
+        D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
+        return tuple(int(x) for x in n)
+
     def use_graphs(self, on: bool = True) -> None:
         """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""
         D.check(D.lib().trisk_set_graphs(self._handle, int(on)))
@@ -264,12 +270,6 @@ class TRiSKShallowWater(ModelTables, CGridShallowWater):
         desc = self.descriptor(self._hilbert)
         D.check(D.lib().trisk_create(C.byref(desc), C.byref(opts), C.byref(h)))
         self._finish_create(h)
-
-    def wide_counts(self) -> tuple[int, int, int]:
-        """Cells, vertices, edges read with int32 instead of 16-bit-delta indices."""
-        n = np.zeros(3, dtype=np.int64)
-        D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
-        return tuple(int(x) for x in n)
 
     def path_info(self) -> tuple[bool, str]:
         """(cell-pair path?, reason) of the device kernels."""

@@ -115,8 +115,9 @@ typedef struct {
     int32_t device;              /* HIP device ordinal */
     int32_t block_size;          /* threads per workgroup (0: library default) */
     int32_t path;                /* TRISK_PATH_*: which kernels run the TRiSK/cell-KE step */
-    int32_t narrow_limit;        /* FP32-bulk pair path: largest 16-bit index delta used
-                                    (0: 32767); entities beyond it read int32 indices */
+    int32_t narrow_limit;        /* FP32-bulk pair path and ICON model: largest 16-bit index
+                                    delta used (0: 32767); entities beyond it read int32
+                                    indices */
     int32_t reserved[3];
 } trisk_options;
 
@@ -189,7 +190,8 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count);
 
 /* Cells, vertices and edges whose indices do not fit the 16-bit deltas of the FP32-bulk
- * pair path (they are read from int32 tables). */
+ * pair path or of the ICON model's tables (they are read from int32 tables; for ICON the
+ * edge count adds up its cell, vertex and RBF tables). */
 int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]);
 
 /* Whether this handle runs TRISK_PATH_PAIR (*recomputed = 1) or the table path, and why
@@ -208,8 +210,8 @@ int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
  * _field, trisk_tendency (tendency!, ICONShallowWater.jl:191-207), trisk_rk4_steps,
  * trisk_set_graphs, trisk_diagnostics, trisk_vertex_pv, trisk_relative_vorticity,
  * trisk_tangential_velocity (the RBF tangential velocity, :219-224), trisk_set_timing,
- * trisk_kernel_times, trisk_kernel_bytes and trisk_info.  The TRiSK-only calls
- * (trisk_kinetic_energy, trisk_wide_counts, trisk_tendency_vjp, trisk_rk4_adjoint) return
+ * trisk_kernel_times, trisk_kernel_bytes, trisk_wide_counts and trisk_info.  The TRiSK-only
+ * calls (trisk_kinetic_energy, trisk_tendency_vjp, trisk_rk4_adjoint) return
  * TRISK_EINVAL on an ICON handle.  Only TRISK_FP64 is available for this model. */
 typedef struct {
     int64_t nc, ne, nv;

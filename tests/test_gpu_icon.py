@@ -145,8 +145,21 @@ def test_trisk_only_calls_and_options():
         tendency_vjp(m, y, y)
     with pytest.raises(NotImplementedError, match="FP64"):
         ICONShallowWater(mesh(2), backend=HIPBackend(precision="fp32_bulk"))
-    from dmplex_amd import _device as D
+    assert m.wide_counts() == (0, 0, 0)
 
-    import ctypes as C
-    with pytest.raises(ValueError, match="ICON"):
-        D.check(D.lib().trisk_wide_counts(m.handle, (C.c_int64 * 3)()))
+
+def test_narrow_and_wide_indices_agree():
+    """The 16-bit index deltas and the int32 rows they fall back to give the same bits: a
+    model whose deltas are limited to 40 reads many rows from the int32 tables."""
+    say("icon narrow and wide indices, level 4")
+    kw = dict(gravity=9.80616, divergence_damping=1e-4)
+    narrow = ICONShallowWater(mesh(4), **kw)
+    wide = ICONShallowWater(mesh(4), narrow_limit=40, **kw)
+    assert narrow.wide_counts() == (0, 0, 0)
+    assert all(n > 0 for n in wide.wide_counts())
+    y = seeded(narrow)
+    ref = IconOracle(narrow).tendency(y)
+    for m in (narrow, wide):
+        dy = m.zeros()
+        tendency_(dy, m.upload_state(y), m)
+        assert np.array_equal(gather_state(m, dy), ref)
