@@ -24,10 +24,8 @@ int set_err(int code, const std::string& msg);
             return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-#ifndef TRISK_BLOCK
-#define TRISK_BLOCK 256
-#endif
-constexpr int kBlock = TRISK_BLOCK;
+// workgroup size of the single-model kernels (4 waves; 128 and 512 measured no faster)
+constexpr int kBlock = 256;
 constexpr int kXcd = 8;
 
 // ---------------------------------------------------------------------------------
