@@ -24,7 +24,7 @@ int set_err(int code, const std::string& msg);
             return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-// workgroup size of the single-model kernels (4 waves; 128 and 512 measured no faster)
+// workgroup size of the single-model kernels (4 waves of 64)
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;
 
