@@ -195,13 +195,13 @@ ICON_REFERENCE_MS = 7.8  # ICONShallowWater, TC2, R02B08 (5.2M cells), ms per RK
 
 
 def run_icon(args):
-    """ICONShallowWater (FP64) on the SCVT triangle mesh of level 9: 5,242,880 cells, the
-    cell count of ICON's R02B08, where the reference publishes 7.8 ms per RK4 step on one
-    A100.  Timed without divergence damping (the TC2 setting) and with ICON's fourth-order
-    damping (divdamp_fac 0.0025, nu = 0.0025 mean-area^2 / dt)."""
+    """ICONShallowWater on the SCVT triangle mesh of level 9: 5,242,880 cells, the cell count
+    of ICON's R02B08, where the reference publishes 7.8 ms per RK4 step (FP64) on one A100.
+    Timed in FP64 and in FP32-bulk, each without divergence damping (the TC2 setting) and
+    with ICON's fourth-order damping (divdamp_fac 0.0025, nu = 0.0025 mean-area^2 / dt)."""
     import torch
 
-    from dmplex_amd import ICONShallowWater, create_spherical_mesh, diagnostics, rk4_steps_
+    from dmplex_amd import HIPBackend, ICONShallowWater, create_spherical_mesh, diagnostics, rk4_steps_
     from dmplex_amd import _device as D
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
@@ -211,13 +211,17 @@ def run_icon(args):
     dt, steps = args.icon_dt, args.icon_steps
     nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
     out = {"workload": f"ICONShallowWater on SCVT triangles level {args.icon_level} ({dm.nc} cells, "
-                       f"{dm.ne} edges, {dm.nv} vertices), FP64, RK4 dt={dt} s, seeded state, Earth parameters",
+                       f"{dm.ne} edges, {dm.nv} vertices), FP64 and FP32-bulk, RK4 dt={dt} s, seeded state, "
+                       "Earth parameters",
            "reference_a100_ms_per_step": ICON_REFERENCE_MS}
     stream = torch.cuda.current_stream()
     grid = None
-    for key, damping in (("no_damping", 0.0), ("damping", nu)):
+    runs = (("no_damping", 0.0, "fp64"), ("damping", nu, "fp64"), ("fp32_bulk_no_damping", 0.0, "fp32_bulk"),
+            ("fp32_bulk_damping", nu, "fp32_bulk"))
+    for key, damping, precision in runs:
         t = time.perf_counter()
-        m = ICONShallowWater(grid if grid is not None else dm, divergence_damping=damping, **earth_parameters())
+        m = ICONShallowWater(grid if grid is not None else dm, divergence_damping=damping,
+                             backend=HIPBackend(0, precision), **earth_parameters())
         grid = m.grid
         log(f"[bench] icon {key}: tables and upload in {time.perf_counter() - t:.1f} s")
         y = m.upload_state(seeded_state(grid))
@@ -244,7 +248,7 @@ def run_icon(args):
         kbytes = sum(k["bytes"] * k["launches"] for k in stats) / steps
         dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
         achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
-        r = dict(value=dm.nc * steps / (ms / 1e3), unit="cell-steps/s", ms_per_step=ms / steps,
+        r = dict(precision=precision, value=dm.nc * steps / (ms / 1e3), unit="cell-steps/s", ms_per_step=ms / steps,
                  ms_per_step_direct_with_events=ms_direct / steps,
                  speedup_vs_reference_a100=ICON_REFERENCE_MS / (ms / steps),
                  kernel_model_bytes_per_cell_step=kbytes / dm.nc,
@@ -253,7 +257,7 @@ def run_icon(args):
                  roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
                                frac=achieved * 1e9 / HBM_PEAK,
                                traffic=load_traffic(dom["name"], "icon_damping" if damping else "icon")
-                               if args.icon_level == 9 else None,
+                               if args.icon_level == 9 and precision == "fp64" else None,
                                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
                  kernels=stats)
         if damping:

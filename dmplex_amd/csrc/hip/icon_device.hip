@@ -16,84 +16,104 @@
 // reads the stage's input heights while C writes the next stage's, so the height stage
 // alternates between two buffers.  Every FP64 formula keeps the reference's order of
 // operations (no contraction), so the FP64 results equal the reference algorithm bit for bit.
+//
+// FP32-bulk (TRISK_FP32_BULK): the same kernels with FP32 coefficients, intermediates and
+// stage velocity; the state, the stage heights, the RK4 accumulator, the pressure gradient
+// g (h_r - h_l) / d and the continuity stay FP64.  The edge kernel then stores G_e = l_e F_e
+// (its height weights carry l_e) and a cell's height tendency is -(sum of its edges' +-G_e)
+// / A_i, summed in FP64: every G_e enters its two cells with opposite signs, so the global
+// mass changes only by the rounding of the division.  Diagnostics run the FP64 kernels.
+#include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "trisk_device.hpp"
 
 namespace trisk {
 
-struct alignas(16) KeF {
-    double Ke, F;
+// (K_e, F_e) of an edge, one aligned pair (FP32-bulk: (K_e, l_e F_e))
+template <class T>
+struct alignas(2 * sizeof(T)) KeFT {
+    T Ke, F;
 };
+using KeF = KeFT<double>;
 
-struct IconParams {
+// Device tables and work of one precision: C = double (FP64) or float (FP32-bulk
+// coefficients and intermediates).
+template <class C>
+struct IconParamsT {
     int64_t nc, ne, nv;
     int kv, kr;
     double g;
     // cells
     const int32_t* ce;
-    const double* div;
-    const double* bln;
+    const C* div;
+    const C* bln;
     // edges
     const int32_t* ec;
     const int32_t* ev;
-    const double* eh;
-    const double* grad;
-    const double* invd;
+    const C* eh;  // FP32-bulk: l_e times the height weights
+    const C* grad;
+    const C* invd;
     const int32_t* re;
-    const double* rw;
-    const double* fe;
-    const double* nu;
+    const C* rw;
+    const C* fe;
+    const C* nu;
     // vertices
     const int32_t* ve;
-    const double* curl;
+    const C* curl;
     const int32_t* vc;
-    const double* vh;
-    const double* fv;
+    const C* vh;
+    const C* fv;
     // the index tables as 16-bit deltas (two per word; kWide: read the int32 row): cell edges
     // from 3i/2, edge cells from 2e/3, edge vertices from e/3, RBF edges from e, vertex edges
     // from 3v
     const int32_t *ce16p, *ec16p, *ev16p, *re16p, *ve16p;
+    // FP32-bulk continuity: bit k set when the normal of cell edge k points inwards, and A_i
+    const uint8_t* cneg;
+    const double* area;
     // work
-    double* v;    // ne: RBF tangential velocity
-    KeF* KF;      // ne: edge kinetic energy and mass flux
-    double* K;    // nc: cell kinetic energy
-    double* zeta; // nv: relative vorticity
-    double* D;    // nc: divergence-damping work
-    double* G;    // ne: divergence-damping work
+    C* v;           // ne: RBF tangential velocity
+    KeFT<C>* KF;    // ne: edge kinetic energy and mass flux
+    C* K;           // nc: cell kinetic energy
+    C* zeta;        // nv: relative vorticity
+    C* D;           // nc: divergence-damping work
+    C* G;           // ne: divergence-damping work
 };
+using IconParams = IconParamsT<double>;
 
 struct IconState {
-    IconParams p{};
-    double* sh2 = nullptr;  // second height stage buffer
+    IconParams p{};                  // FP64 tables (the FP64 model and every diagnostic)
+    IconParamsT<float> p32{};        // FP32-bulk tables (stepping of an FP32-bulk handle)
+    double* sh2 = nullptr;           // second height stage buffer
 };
 
 // ---------------------------------------------------------------------------------
 // E: icon_edge_kernel! (ICONShallowWater.jl:149-159)
 
-template <int KR>
-__global__ __launch_bounds__(kBlock) void icon_edge_kernel(IconParams p, const double* __restrict__ u,
+template <class C, class UIn, int KR>
+__global__ __launch_bounds__(kBlock) void icon_edge_kernel(IconParamsT<C> p, const UIn* __restrict__ u,
                                                            const double* __restrict__ h, int64_t t0) {
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
-    double s = 0.0;
+    C s = 0;
     if constexpr (KR > 0) {
         const Index<KR> RE(p.re16p, p.re, p.ne, e, e);
-        double w[KR];
+        C w[KR];
 #pragma unroll
         for (int k = 0; k < KR; ++k) w[k] = ld(p.rw + k * p.ne + e);
 #pragma unroll
-        for (int k = 0; k < KR; ++k) s += w[k] * u[RE[k]];
+        for (int k = 0; k < KR; ++k) s += w[k] * C(u[RE[k]]);
     } else {
-        for (int k = 0; k < p.kr; ++k) s += ld(p.rw + k * p.ne + e) * u[ld(p.re + k * p.ne + e)];
+        for (int k = 0; k < p.kr; ++k) s += ld(p.rw + k * p.ne + e) * C(u[ld(p.re + k * p.ne + e)]);
     }
-    const double ue = u[e];
+    const C ue = C(u[e]);
     const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
     const int32_t l = EC[0], r = EC[1];
-    const double F = (ld(p.eh + e) * h[l] + ld(p.eh + p.ne + e) * h[r]) * ue;
+    const C F = (ld(p.eh + e) * C(h[l]) + ld(p.eh + p.ne + e) * C(h[r])) * ue;
     p.v[e] = s;
-    p.KF[e] = KeF{(ue * ue + s * s) / 2, F};
+    p.KF[e] = KeFT<C>{(ue * ue + s * s) / 2, F};
 }
 
 // ---------------------------------------------------------------------------------
@@ -101,8 +121,8 @@ __global__ __launch_bounds__(kBlock) void icon_edge_kernel(IconParams p, const d
 // (continuity_kernel!, TRiSKShallowWater.jl:298-305) with the RK4 stage of h, and with
 // damping D = -div u; vertices: zeta (edge_sum_kernel! with the curl weights)
 
-template <int KV, bool DAMP>
-__global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, const double* __restrict__ u, Rk r,
+template <class C, class UIn, int KV, bool DAMP>
+__global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParamsT<C> p, const UIn* __restrict__ u, Rk r,
                                                                   int64_t t0, int64_t t1) {
     const Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
@@ -110,41 +130,52 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
         if (i >= p.nc) return;
         const RkIn rin = rk_load(r, p.ne + i);
         const Index<3> E(p.ce16p, p.ce, p.nc, i, (3 * i) / 2);
-        double wb[3], wd[3];
+        C wb[3], wd[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             wb[k] = ld(p.bln + k * p.nc + i);
-            wd[k] = ld(p.div + k * p.nc + i);
+            if constexpr (std::is_same_v<C, double> || DAMP) wd[k] = ld(p.div + k * p.nc + i);
         }
-        KeF kf[3];
+        KeFT<C> kf[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) kf[k] = p.KF[E[k]];
-        double K = 0.0, s = 0.0;
+        C K = 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) K += wb[k] * kf[k].Ke;
+        double dh;
+        if constexpr (std::is_same_v<C, double>) {
+            double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s += wd[k] * kf[k].F;
+            for (int k = 0; k < 3; ++k) s += wd[k] * kf[k].F;
+            dh = -s;
+        } else {
+            const int neg = ld(p.cneg + i);
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s += (neg >> k) & 1 ? -double(kf[k].F) : double(kf[k].F);
+            dh = -s / ld(p.area + i);
+        }
         p.K[i] = K;
         if constexpr (DAMP) {
-            double d = 0.0;
+            C d = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) d += wd[k] * u[E[k]];
+            for (int k = 0; k < 3; ++k) d += wd[k] * C(u[E[k]]);
             p.D[i] = -d;
         }
-        rk_store<double>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double>(r, p.ne + i, dh, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     } else {
         const int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
-        double z = 0.0;
+        C z = 0;
         if constexpr (KV > 0) {
             const Index<KV> VE(p.ve16p, p.ve, p.nv, v, 3 * v);
-            double w[KV];
+            C w[KV];
 #pragma unroll
             for (int k = 0; k < KV; ++k) w[k] = ld(p.curl + k * p.nv + v);
 #pragma unroll
-            for (int k = 0; k < KV; ++k) z += w[k] * u[VE[k]];
+            for (int k = 0; k < KV; ++k) z += w[k] * C(u[VE[k]]);
         } else {
-            for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * u[ld(p.ve + k * p.nv + v)];
+            for (int k = 0; k < p.kv; ++k) z += ld(p.curl + k * p.nv + v) * C(u[ld(p.ve + k * p.nv + v)]);
         }
         p.zeta[v] = z;
     }
@@ -154,21 +185,23 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParams p, 
 // divergence damping (divergence_damping!, ICONShallowWater.jl:209-217):
 // G = grad D (edge_gradient_kernel!, :173-176, scale 1), D = -div G
 
-__global__ __launch_bounds__(kBlock) void icon_damp_edge_kernel(IconParams p, int64_t t0) {
+template <class C>
+__global__ __launch_bounds__(kBlock) void icon_damp_edge_kernel(IconParamsT<C> p, int64_t t0) {
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
     const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
     const int32_t l = EC[0], r = EC[1];
-    p.G[e] = (1.0 * (p.D[r] - p.D[l])) * ld(p.invd + e);
+    p.G[e] = (C(1) * (p.D[r] - p.D[l])) * ld(p.invd + e);
 }
 
-__global__ __launch_bounds__(kBlock) void icon_damp_cell_kernel(IconParams p, int64_t t0) {
+template <class C>
+__global__ __launch_bounds__(kBlock) void icon_damp_cell_kernel(IconParamsT<C> p, int64_t t0) {
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t i = t.first + threadIdx.x;
     if (i >= p.nc) return;
     const Index<3> E(p.ce16p, p.ce, p.nc, i, (3 * i) / 2);
-    double s = 0.0;
+    C s = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) s += ld(p.div + k * p.nc + i) * p.G[E[k]];
     p.D[i] = -s;
@@ -178,9 +211,10 @@ __global__ __launch_bounds__(kBlock) void icon_damp_cell_kernel(IconParams p, in
 // M: icon_momentum_kernel! (:161-171), with damping the last edge_gradient_kernel!
 // (scale -1) and du += nu G, then the RK4 stage of u
 
-template <bool DAMP>
-__global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParams p, const double* __restrict__ h, Rk r,
+template <class C, bool DAMP>
+__global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParamsT<C> p, const double* __restrict__ h, Rk r,
                                                                int64_t t0) {
+    using U = C;  // stage velocity
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
@@ -188,16 +222,18 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParams p, con
     const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3), EV(p.ev16p, p.ev, p.ne, e, e / 3);
     const int32_t l = EC[0], rr = EC[1];
     const int32_t v0 = EV[0], v1 = EV[1];
-    const double g1 = ld(p.grad + e), g2 = ld(p.grad + p.ne + e);
-    const double invd = ld(p.invd + e);
-    const double zeta = (p.zeta[v0] + p.zeta[v1]) / 2;
-    const double grad_K = p.KF[e].Ke * (g1 - g2) + g2 * p.K[rr] - g1 * p.K[l];
-    double du = (ld(p.fe + e) + zeta) * ld(p.v + e) - grad_K - p.g * (h[rr] - h[l]) * invd;
+    const C g1 = ld(p.grad + e), g2 = ld(p.grad + p.ne + e);
+    const C invd = ld(p.invd + e);
+    const C zeta = (p.zeta[v0] + p.zeta[v1]) / 2;
+    const C grad_K = p.KF[e].Ke * (g1 - g2) + g2 * p.K[rr] - g1 * p.K[l];
+    // the pressure gradient in FP64 in both precisions (for FP64 this is the reference's
+    // left-to-right order of (f + zeta) v - grad K - g (h_r - h_l) / d)
+    double du = double((ld(p.fe + e) + zeta) * ld(p.v + e) - grad_K) - p.g * (h[rr] - h[l]) * double(invd);
     if constexpr (DAMP) {
-        const double G = (-1.0 * (p.D[rr] - p.D[l])) * invd;
-        du = du + ld(p.nu + e) * G;
+        const C G = (C(-1) * (p.D[rr] - p.D[l])) * invd;
+        du = du + double(ld(p.nu + e) * G);
     }
-    rk_store<double>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<double*>(r.su) + e : nullptr);
+    rk_store<U>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------
@@ -235,17 +271,19 @@ static unsigned grid1(int64_t n) { return unsigned(kXcd * tiles_per_xcd(n)); }
 
 // One tendency with the stage epilogue r: u, h are the stage's input; r.sh receives the
 // next stage's heights (modes 1-3).
-static int icon_stage(trisk_handle* H, const double* u, const double* h, const Rk& r, hipStream_t s) {
-    IconParams& p = iparams(H);
+template <class C, class UIn>
+static int icon_stage(trisk_handle* H, const IconParamsT<C>& p, const UIn* u, const double* h, const Rk& r,
+                      hipStream_t s) {
     const bool damp = p.nu != nullptr;
     const int64_t te = tiles_per_xcd(p.ne), tc = tiles_per_xcd(p.nc), tv = tiles_per_xcd(p.nv);
+    const dim3 b(kBlock);
     hipEvent_t stop;
     int rc;
     if ((rc = timed_begin(H, K_ICON_EDGE, s, &stop))) return rc;
     if (p.kr == 4)
-        hipLaunchKernelGGL((icon_edge_kernel<4>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, u, h, te);
+        hipLaunchKernelGGL((icon_edge_kernel<C, UIn, 4>), dim3(grid1(p.ne)), b, 0, s, p, u, h, te);
     else
-        hipLaunchKernelGGL((icon_edge_kernel<0>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, u, h, te);
+        hipLaunchKernelGGL((icon_edge_kernel<C, UIn, 0>), dim3(grid1(p.ne)), b, 0, s, p, u, h, te);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(stop, s))) return rc;
 
@@ -253,34 +291,34 @@ static int icon_stage(trisk_handle* H, const double* u, const double* h, const R
     const dim3 gcv(unsigned(kXcd * (tc + tv)));
     if (p.kv == 6) {
         if (damp)
-            hipLaunchKernelGGL((icon_cell_vertex_kernel<6, true>), gcv, dim3(kBlock), 0, s, p, u, r, tc, tv);
+            hipLaunchKernelGGL((icon_cell_vertex_kernel<C, UIn, 6, true>), gcv, b, 0, s, p, u, r, tc, tv);
         else
-            hipLaunchKernelGGL((icon_cell_vertex_kernel<6, false>), gcv, dim3(kBlock), 0, s, p, u, r, tc, tv);
+            hipLaunchKernelGGL((icon_cell_vertex_kernel<C, UIn, 6, false>), gcv, b, 0, s, p, u, r, tc, tv);
     } else {
         if (damp)
-            hipLaunchKernelGGL((icon_cell_vertex_kernel<0, true>), gcv, dim3(kBlock), 0, s, p, u, r, tc, tv);
+            hipLaunchKernelGGL((icon_cell_vertex_kernel<C, UIn, 0, true>), gcv, b, 0, s, p, u, r, tc, tv);
         else
-            hipLaunchKernelGGL((icon_cell_vertex_kernel<0, false>), gcv, dim3(kBlock), 0, s, p, u, r, tc, tv);
+            hipLaunchKernelGGL((icon_cell_vertex_kernel<C, UIn, 0, false>), gcv, b, 0, s, p, u, r, tc, tv);
     }
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(stop, s))) return rc;
 
     if (damp) {
         if ((rc = timed_begin(H, K_ICON_DAMP_EDGE, s, &stop))) return rc;
-        hipLaunchKernelGGL(icon_damp_edge_kernel, dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, te);
+        hipLaunchKernelGGL((icon_damp_edge_kernel<C>), dim3(grid1(p.ne)), b, 0, s, p, te);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
         if ((rc = timed_begin(H, K_ICON_DAMP_CELL, s, &stop))) return rc;
-        hipLaunchKernelGGL(icon_damp_cell_kernel, dim3(grid1(p.nc)), dim3(kBlock), 0, s, p, tc);
+        hipLaunchKernelGGL((icon_damp_cell_kernel<C>), dim3(grid1(p.nc)), b, 0, s, p, tc);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
 
     if ((rc = timed_begin(H, K_ICON_MOMENTUM, s, &stop))) return rc;
     if (damp)
-        hipLaunchKernelGGL((icon_momentum_kernel<true>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, h, r, te);
+        hipLaunchKernelGGL((icon_momentum_kernel<C, true>), dim3(grid1(p.ne)), b, 0, s, p, h, r, te);
     else
-        hipLaunchKernelGGL((icon_momentum_kernel<false>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, h, r, te);
+        hipLaunchKernelGGL((icon_momentum_kernel<C, false>), dim3(grid1(p.ne)), b, 0, s, p, h, r, te);
     HIP_TRY(hipGetLastError());
     return timed_end(stop, s);
 }
@@ -289,12 +327,14 @@ int icon_tendency(trisk_handle* H, const double* y, double* dy, hipStream_t s) {
     Rk r{};
     r.mode = 0;
     r.dy = dy;
-    return icon_stage(H, y, y + H->L.ne, r, s);
+    if (H->precision == TRISK_FP32_BULK) return icon_stage(H, H->icon->p32, y, y + H->L.ne, r, s);
+    return icon_stage(H, H->icon->p, y, y + H->L.ne, r, s);
 }
 
-int icon_run_steps(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s) {
+template <class C>
+static int icon_steps(trisk_handle* H, const IconParamsT<C>& p, double* y, double dt, int64_t n, hipStream_t s) {
     const int64_t ne = H->L.ne;
-    double* su = static_cast<double*>(H->su);
+    C* su = static_cast<C*>(H->su);
     double* sh[2] = {H->sh, H->icon->sh2};
     Rk r{};
     r.y = y;
@@ -309,16 +349,25 @@ int icon_run_steps(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t
         for (int mode = 1; mode <= 4; ++mode) {
             r.mode = mode;
             r.sh = mode <= 3 ? sh[(mode + 1) % 2] : nullptr;
-            const double* u = mode == 1 ? y : su;
-            const double* h = mode == 1 ? y + ne : sh[mode % 2];
-            if ((rc = icon_stage(H, u, h, r, s))) return rc;
+            if (mode == 1) {
+                rc = icon_stage(H, p, static_cast<const double*>(y), y + ne, r, s);
+            } else {
+                rc = icon_stage(H, p, static_cast<const C*>(su), sh[mode % 2], r, s);
+            }
+            if (rc) return rc;
         }
     }
     return 0;
 }
 
-// K at the cells and zeta at the vertices of the state y: the stage kernels with the
-// tendency epilogue into the RK4 accumulator, which holds nothing between steps
+int icon_run_steps(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s) {
+    if (H->precision == TRISK_FP32_BULK) return icon_steps(H, H->icon->p32, y, dt, n, s);
+    return icon_steps(H, H->icon->p, y, dt, n, s);
+}
+
+// K at the cells and zeta at the vertices of the state y (FP64 tables in both precisions):
+// the stage kernels with the tendency epilogue into the RK4 accumulator, which holds nothing
+// between steps
 static int icon_kinetic_pass(trisk_handle* H, const double* y, hipStream_t s) {
     Rk r{};
     r.mode = 0;
@@ -327,10 +376,10 @@ static int icon_kinetic_pass(trisk_handle* H, const double* y, hipStream_t s) {
     const int64_t te = tiles_per_xcd(p.ne), tc = tiles_per_xcd(p.nc), tv = tiles_per_xcd(p.nv);
     const double* u = y;
     const double* h = y + p.ne;
-    hipLaunchKernelGGL((icon_edge_kernel<0>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, u, h, te);
+    hipLaunchKernelGGL((icon_edge_kernel<double, double, 0>), dim3(grid1(p.ne)), dim3(kBlock), 0, s, p, u, h, te);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((icon_cell_vertex_kernel<0, false>), dim3(unsigned(kXcd * (tc + tv))), dim3(kBlock), 0, s, p,
-                       u, r, tc, tv);
+    hipLaunchKernelGGL((icon_cell_vertex_kernel<double, double, 0, false>), dim3(unsigned(kXcd * (tc + tv))),
+                       dim3(kBlock), 0, s, p, u, r, tc, tv);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -384,22 +433,27 @@ std::vector<std::pair<const char*, double>> icon_kernel_bytes(const trisk_handle
     const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), KR = L.kr, KV = L.kv;
     // index tables as 16-bit deltas (2 B per index); the RBF and vertex-edge tables of other
     // widths than the triangle mesh's 4 and 6 are read as int32
-    const double ix = 2.0, ixr = L.kr == 4 ? 2.0 : 4.0, ixv = L.kv == 6 ? 2.0 : 4.0, c = 8.0,
-                 rk = 8.0 + 16.0 + 8.0;
+    const bool f32 = H->precision == TRISK_FP32_BULK;
+    const double ix = 2.0, ixr = L.kr == 4 ? 2.0 : 4.0, ixv = L.kv == 6 ? 2.0 : 4.0;
+    // coefficients, intermediates and stage velocity (FP32-bulk: 4 B); heights, y and the
+    // accumulator are FP64.  u: stage velocity; rk: y, accumulator read and written, stage
+    const double c = f32 ? 4.0 : 8.0, u = c, rk_h = 8.0 + 16.0 + 8.0, rk_u = 8.0 + 16.0 + u;
     std::vector<std::pair<const char*, double>> v;
     // E: RBF table, u, cells, c_lin_e, h, (K_e, F_e), v
-    v.push_back({kKernelNames[K_ICON_EDGE], ne * (KR * (ixr + c) + 8.0 + 2 * ix + 2 * c + 16.0 + 8.0) + nc * 8.0});
-    // C: cell edges, bilinear and divergence weights, (K_e, F_e), K, RK4 of h [u, D];
+    v.push_back({kKernelNames[K_ICON_EDGE], ne * (KR * (ixr + c) + u + 2 * ix + 2 * c + 2 * c + c) + nc * 8.0});
+    // C: cell edges, bilinear and divergence weights (FP32-bulk: bilinear weights, inward
+    //    bits and A_i), (K_e, F_e), K, RK4 of h [u, D, the divergence weights];
     //    vertex edges, curl, u, zeta
-    v.push_back({kKernelNames[K_ICON_CELL_VERTEX], nc * (3 * ix + 6 * c + 8.0 + rk + (damp ? 8.0 : 0.0)) +
-                                                       ne * (16.0 + 8.0) + nv * (KV * (ixv + c) + 8.0)});
+    const double cw = f32 ? 3 * c + 1.0 + 8.0 + (damp ? 3 * c : 0.0) : 6 * c;
+    v.push_back({kKernelNames[K_ICON_CELL_VERTEX], nc * (3 * ix + cw + c + rk_h + (damp ? c : 0.0)) +
+                                                       ne * (2 * c + u) + nv * (KV * (ixv + c) + c)});
     if (damp) {
-        v.push_back({kKernelNames[K_ICON_DAMP_EDGE], ne * (2 * ix + c + 8.0) + nc * 8.0});
-        v.push_back({kKernelNames[K_ICON_DAMP_CELL], nc * (3 * ix + 3 * c + 8.0) + ne * 8.0});
+        v.push_back({kKernelNames[K_ICON_DAMP_EDGE], ne * (2 * ix + c + c) + nc * c});
+        v.push_back({kKernelNames[K_ICON_DAMP_CELL], nc * (3 * ix + 3 * c + c) + ne * c});
     }
     // M: cells, vertices, gradient, 1/d, f_e, v, K_e, K, h, zeta, RK4 of u [nu, D]
-    v.push_back({kKernelNames[K_ICON_MOMENTUM], ne * (4 * ix + 2 * c + c + c + 8.0 + 8.0 + rk + (damp ? c : 0.0)) +
-                                                    nc * (16.0 + (damp ? 8.0 : 0.0)) + nv * 8.0});
+    v.push_back({kKernelNames[K_ICON_MOMENTUM], ne * (4 * ix + 2 * c + c + c + c + c + rk_u + (damp ? c : 0.0)) +
+                                                    nc * (c + 8.0 + (damp ? c : 0.0)) + nv * c});
     return v;
 }
 
@@ -446,6 +500,59 @@ static int icon_setup(trisk_handle* H) {
     rc |= upload(o, L.area_c, &H->p64.area_c);
     rc |= upload(o, L.area_v, &H->p64.area_v);
     if (rc) return rc;
+    const bool f32 = H->precision == TRISK_FP32_BULK;
+    if (f32) {
+        // FP32-bulk: FP32 coefficients; the height weights times l_e = |div| A of the edge's
+        // first cell slot; per cell the inward-normal bits of its edges
+        IconParamsT<float>& q = H->icon->p32;
+        q.nc = p.nc;
+        q.ne = p.ne;
+        q.nv = p.nv;
+        q.kv = p.kv;
+        q.kr = p.kr;
+        q.g = p.g;
+        q.ce = p.ce;
+        q.ec = p.ec;
+        q.ev = p.ev;
+        q.re = p.re;
+        q.ve = p.ve;
+        q.vc = p.vc;
+        q.ce16p = p.ce16p;
+        q.ec16p = p.ec16p;
+        q.ev16p = p.ev16p;
+        q.re16p = p.re16p;
+        q.ve16p = p.ve16p;
+        q.area = H->p64.area_c;
+        std::vector<double> len(L.ne, 0.0);
+        std::vector<char> seen(L.ne, 0);
+        std::vector<uint8_t> neg(L.nc, 0);
+        for (int64_t i = 0; i < L.nc; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const double d = L.div[size_t(k) * L.nc + i];
+                const int32_t e = L.ce[size_t(k) * L.nc + i];
+                if (d < 0) neg[i] |= uint8_t(1u << k);
+                if (!seen[e]) {
+                    seen[e] = 1;
+                    len[e] = std::fabs(d) * L.area_c[i];
+                }
+            }
+        std::vector<double> lh(L.eh.size());
+        for (int64_t e = 0; e < L.ne; ++e) {
+            lh[e] = len[e] * L.eh[e];
+            lh[L.ne + e] = len[e] * L.eh[L.ne + e];
+        }
+        rc |= upload(o, to_float(L.div), &q.div);
+        rc |= upload(o, to_float(L.bln), &q.bln);
+        rc |= upload(o, to_float(lh), &q.eh);
+        rc |= upload(o, to_float(L.grad), &q.grad);
+        rc |= upload(o, to_float(L.invd), &q.invd);
+        rc |= upload(o, to_float(L.rw), &q.rw);
+        rc |= upload(o, to_float(L.fe), &q.fe);
+        if (!L.damp.empty()) rc |= upload(o, to_float(L.damp), &q.nu);
+        rc |= upload(o, to_float(L.curl), &q.curl);
+        rc |= upload(o, neg, &q.cneg);
+        if (rc) return rc;
+    }
     void* q;
     struct Buf {
         void** out;
@@ -457,7 +564,7 @@ static int icon_setup(trisk_handle* H) {
                 {reinterpret_cast<void**>(&p.D), L.nc * sizeof(double)},
                 {reinterpret_cast<void**>(&p.G), L.ne * sizeof(double)},
                 {reinterpret_cast<void**>(&H->acc), (L.ne + L.nc) * sizeof(double)},
-                {&H->su, L.ne * sizeof(double)},
+                {&H->su, L.ne * (f32 ? sizeof(float) : sizeof(double))},
                 {reinterpret_cast<void**>(&H->sh), L.nc * sizeof(double)},
                 {reinterpret_cast<void**>(&H->icon->sh2), L.nc * sizeof(double)},
                 {reinterpret_cast<void**>(&H->partial), 2 * H->reduce_blocks * sizeof(double)},
@@ -468,6 +575,19 @@ static int icon_setup(trisk_handle* H) {
     for (auto& b : bufs) {
         if ((rc = alloc(o, b.bytes, &q))) return rc;
         *b.out = q;
+    }
+    if (f32) {
+        IconParamsT<float>& w = H->icon->p32;
+        Buf wb[] = {{reinterpret_cast<void**>(&w.v), L.ne * sizeof(float)},
+                    {reinterpret_cast<void**>(&w.KF), L.ne * sizeof(KeFT<float>)},
+                    {reinterpret_cast<void**>(&w.K), L.nc * sizeof(float)},
+                    {reinterpret_cast<void**>(&w.zeta), L.nv * sizeof(float)},
+                    {reinterpret_cast<void**>(&w.D), L.nc * sizeof(float)},
+                    {reinterpret_cast<void**>(&w.G), L.ne * sizeof(float)}};
+        for (auto& b : wb) {
+            if ((rc = alloc(o, b.bytes, &q))) return rc;
+            *b.out = q;
+        }
     }
     return 0;
 }
@@ -490,11 +610,12 @@ int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_han
     *out = nullptr;
     trisk_options o{};
     if (opt) o = *opt;
-    if (o.precision != TRISK_FP64) return set_err(TRISK_EINVAL, "the ICON triangle model runs in TRISK_FP64");
+    if (o.precision != TRISK_FP64 && o.precision != TRISK_FP32_BULK)
+        return set_err(TRISK_EINVAL, "precision must be TRISK_FP64 or TRISK_FP32_BULK");
     if (o.block_size != 0 && o.block_size != kBlock)
         return set_err(TRISK_EINVAL, "block_size must be 0 or the library's workgroup size");
     auto* H = new trisk_handle;
-    H->precision = TRISK_FP64;
+    H->precision = o.precision;
     H->device = o.device;
     std::string err;
     if (!build_icon_layout(desc, H->L, err, o.narrow_limit)) {

@@ -113,7 +113,9 @@ class ICONTables:
 
 class ICONShallowWater(ICONTables, CGridShallowWater):
     """Nonlinear rotating shallow water on the triangular C grid of ICON
-    (ICONShallowWater.jl:1-48), on a HIP device, in FP64."""
+    (ICONShallowWater.jl:1-48), on a HIP device: FP64 (the reference's arithmetic, bit for
+    bit) or FP32-bulk (``HIPBackend(precision="fp32_bulk")``: FP32 stencils, FP64 state,
+    heights, pressure gradient, continuity and RK4 accumulation)."""
 
     def __init__(self, dm_or_grid, centers=None, gravity: float = 1.0, coriolis=None, rbf_scale: float = 0.5,
                  divergence_damping=0.0, kinetic_gradient: str = "icon", comm=None, part=None, backend=None,
@@ -124,13 +126,12 @@ class ICONShallowWater(ICONTables, CGridShallowWater):
             backend = HIPBackend()
         if not isinstance(backend, HIPBackend):
             raise NotImplementedError("only HIPBackend() is available: the product has no CPU path")
-        if backend.precision != "fp64":
-            raise NotImplementedError("the ICON triangle model runs in FP64")
         super().__init__(dm_or_grid, centers, gravity, coriolis, rbf_scale, divergence_damping, kinetic_gradient)
         self.backend = backend
         torch = _torch()
         torch.cuda.set_device(backend.device)
-        opts = D.TriskOptions(D.TRISK_FP64, backend.device, 0, 0, int(narrow_limit))
+        opts = D.TriskOptions(D.TRISK_FP64 if backend.precision == "fp64" else D.TRISK_FP32_BULK, backend.device, 0,
+                              0, int(narrow_limit))
         h = C.c_void_p()
         desc = self.descriptor(hilbert)
         D.check(D.lib().icon_create(C.byref(desc), C.byref(opts), C.byref(h)))

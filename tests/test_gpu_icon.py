@@ -1,7 +1,8 @@
 """Parity of the ICON triangle model (icon_create and the generic C ABI) with the FP64
 oracle.  The FP64 kernels keep every formula of ICONShallowWater.jl in the reference's
 order without contraction, so tendencies and RK4 states must agree bit for bit; the
-diagnostics are compensated sums and agree to a few ulps."""
+diagnostics are compensated sums and agree to a few ulps.  FP32-bulk agrees to FP32
+rounding and conserves mass exactly."""
 import sys
 
 import numpy as np
@@ -143,8 +144,6 @@ def test_trisk_only_calls_and_options():
     y = m.upload_state(seeded(m))
     with pytest.raises(ValueError, match="ICON"):
         tendency_vjp(m, y, y)
-    with pytest.raises(NotImplementedError, match="FP64"):
-        ICONShallowWater(mesh(2), backend=HIPBackend(precision="fp32_bulk"))
     assert m.wide_counts() == (0, 0, 0)
 
 
@@ -163,3 +162,58 @@ def test_narrow_and_wide_indices_agree():
         dy = m.zeros()
         tendency_(dy, m.upload_state(y), m)
         assert np.array_equal(gather_state(m, dy), ref)
+
+
+@pytest.mark.parametrize("damping", [False, True])
+def test_fp32_bulk_close_to_the_oracle(damping):
+    """FP32-bulk (FP32 stencils; FP64 state, heights, pressure gradient, continuity and RK4
+    accumulation) against the FP64 oracle on the seeded Earth state, level 4: the tendency
+    (the height tendency is a small residual of the flux divergence), 20 RK4 steps, exact
+    mass conservation, and graph replay bitwise equal to direct launches."""
+    say(f"icon fp32_bulk level 4 damping={damping}")
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+
+    dt = 640.0
+    nu = 0.0025 * (4 * np.pi / mesh(4).nc) ** 2 / dt if damping else 0.0
+    m64 = ICONShallowWater(mesh(4), divergence_damping=nu, **earth_parameters())
+    m = ICONShallowWater(m64.grid, divergence_damping=nu, backend=HIPBackend(0, "fp32_bulk"), **earth_parameters())
+    y = seeded_state(m.grid)
+    o = IconOracle(m64)
+    ne = m.grid.num_edges()
+    ref = o.tendency(y)
+    dy = m.zeros()
+    tendency_(dy, m.upload_state(y), m)
+    got = gather_state(m, dy)
+    # measured: u 3e-7, h 5e-5
+    assert rel(got[:ne], ref[:ne]) < 2e-6 and rel(got[ne:], ref[ne:]) < 5e-4
+    ref = o.rk4_steps(y.copy(), dt, 20)
+    yd = m.upload_state(y)
+    mass0 = diagnostics(m, yd)[0]
+    rk4_steps_(yd, dt, 20, m)
+    got = gather_state(m, yd)
+    # measured after 50 steps: u 2e-7, h 2e-8
+    assert rel(got[:ne], ref[:ne]) < 2e-6 and rel(got[ne:], ref[ne:]) < 2e-7
+    assert abs(diagnostics(m, yd)[0] - mass0) <= 1e-15 * mass0
+    m.use_graphs(True)
+    yg = m.upload_state(y)
+    rk4_steps_(yg, dt, 20, m)
+    assert bool((yg == yd).all())
+
+
+def test_fp32_bulk_tc2():
+    """The reference's steady-state test in FP32-bulk: the same 8.5e-5 error as FP64 at
+    1280 cells, mass to round-off."""
+    say("icon tc2 level 3 fp32_bulk")
+    m = ICONShallowWater(create_spherical_mesh(3), backend=HIPBackend(0, "fp32_bulk"))
+    g = m.grid
+    y = initial_state(m, lambda p: 0.2 * np.stack([-p[:, 1], p[:, 0], np.zeros(len(p))], axis=1),
+                      lambda p: 1 - (0.2 + 0.02) * p[:, 2] ** 2)
+    h = 1 - (0.2 + 0.02) * g.centers[:, 2] ** 2
+    mass0, energy0, _ = diagnostics(m, y)
+    dt = 0.2 * g.dual_lengths.min()
+    rk4_steps_(y, dt, int(np.ceil(1 / dt)), m)
+    mass, energy, _ = diagnostics(m, y)
+    assert abs(mass - mass0) < 1e-13 and abs(energy - energy0) / energy0 < 1e-5
+    dh = gather_state(m, y)[g.num_edges():] - h
+    err = float(np.sqrt(np.sum(g.cell_areas * dh ** 2) / (4 * np.pi)))
+    assert err == pytest.approx(8.5e-5, abs=0.05e-5)
