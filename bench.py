@@ -83,6 +83,8 @@ def parse():
     p.add_argument("--icon-level", type=int, default=9, help="SCVT triangle level of the ICON measurement")
     p.add_argument("--icon-steps", type=int, default=100, help="timed RK4 steps of the ICON measurement")
     p.add_argument("--icon-dt", type=float, default=20.0, help="RK4 dt (s) of the ICON measurement")
+    p.add_argument("--icon-accuracy", type=int, default=0, metavar="DAYS",
+                   help="instead of timing: FP32-bulk ICON against FP64 after 1 and DAYS simulated days")
     p.add_argument("--no-mpas", action="store_true", help="skip the MPAS-Gassmann operator measurement")
     p.add_argument("--mpas-only", action="store_true", help="only the MPAS-Gassmann operator measurement")
     p.add_argument("--mpas-level", type=int, default=7, help="hexagon level of the MPAS-Gassmann measurement")
@@ -530,6 +532,58 @@ def field_errors(m, a, b):
         u_l2=float(np.sqrt(np.sum(w * (ua - ub) ** 2) / np.sum(w * ub ** 2))))
 
 
+def run_icon_accuracy(args):
+    """FP32-bulk ICON (with ICON's divergence damping) against the FP64 trajectory from the
+    seeded state after 1 and ``args.icon_accuracy`` simulated days; the FP64 trajectory is
+    first checked bitwise equal to the oracle over a few steps."""
+    import torch
+
+    from dmplex_amd import HIPBackend, ICONShallowWater, create_spherical_mesh, diagnostics, rk4_steps_
+    from dmplex_amd.synthetic import earth_parameters, seeded_state
+    from oracle.oracle import IconOracle
+
+    dm = create_spherical_mesh(args.icon_level, centroidal=not args.no_scvt, device=0)
+    dt = args.icon_dt
+    nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
+    models = {"fp64": ICONShallowWater(dm, divergence_damping=nu, **earth_parameters())}
+    grid = models["fp64"].grid
+    models["fp32_bulk"] = ICONShallowWater(grid, divergence_damping=nu, backend=HIPBackend(0, "fp32_bulk"),
+                                           **earth_parameters())
+    y0 = seeded_state(grid)
+    n_or = args.oracle_steps
+    yo = IconOracle(models["fp64"]).rk4_steps(y0.copy(), dt, n_or)
+    yg = models["fp64"].upload_state(y0)
+    rk4_steps_(yg, dt, n_or, models["fp64"])
+    report = {"fp64_bitwise_equal_to_oracle": {"steps": n_or,
+                                               "equal": bool(np.array_equal(models["fp64"].download_state(yg), yo))}}
+    log(f"[icon accuracy] FP64 GPU vs oracle after {n_or} steps: {report['fp64_bitwise_equal_to_oracle']}")
+    states, mass0, energy0 = {}, {}, {}
+    for key, m in models.items():
+        m.use_graphs(True)
+        states[key] = m.upload_state(y0)
+        mass0[key], energy0[key], _ = diagnostics(m, states[key])
+    spd = int(round(86400.0 / dt))
+    t0 = time.perf_counter()
+    for day in range(1, args.icon_accuracy + 1):
+        for key, m in models.items():
+            rk4_steps_(states[key], dt, spd, m)
+        torch.cuda.synchronize()
+        log(f"[icon accuracy] day {day} done ({time.perf_counter() - t0:.0f} s)")
+        if day not in (1, args.icon_accuracy):
+            continue
+        ref = models["fp64"].download_state(states["fp64"])
+        entry = {}
+        for key, m in models.items():
+            mass, energy, _ = diagnostics(m, states[key])
+            e = {"mass_drift": (mass - mass0[key]) / mass0[key], "energy_drift": (energy - energy0[key]) / energy0[key]}
+            if key != "fp64":
+                e.update(field_errors(m, m.download_state(states[key]), ref))
+            entry[key] = e
+        report[f"day_{day}"] = entry
+    return {"accuracy": report, "level": args.icon_level, "cells": dm.nc, "dt": dt, "steps_per_day": spd,
+            "divergence_damping_nu": nu, "reference": "FP64 ICON (bitwise equal to the FP64 oracle)"}
+
+
 def run_accuracy(args, grid, y0, params):
     import torch
 
@@ -613,6 +667,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": run_adjoint(args)}), flush=True)
+        return
+    if args.icon_accuracy:
+        print(json.dumps({"icon_accuracy": run_icon_accuracy(args)}), flush=True)
         return
     if args.icon_only:
         print(json.dumps({"icon": run_icon(args)}), flush=True)

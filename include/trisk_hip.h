@@ -212,7 +212,9 @@ int trisk_info(const trisk_handle* h, int32_t* recomputed, const char** note);
  * trisk_tangential_velocity (the RBF tangential velocity, :219-224), trisk_set_timing,
  * trisk_kernel_times, trisk_kernel_bytes, trisk_wide_counts and trisk_info.  The TRiSK-only
  * calls (trisk_kinetic_energy, trisk_tendency_vjp, trisk_rk4_adjoint) return
- * TRISK_EINVAL on an ICON handle.  Only TRISK_FP64 is available for this model. */
+ * TRISK_EINVAL on an ICON handle.  TRISK_FP64 follows the reference's arithmetic bit for
+ * bit; TRISK_FP32_BULK runs FP32 stencils with the FP64 state, heights, pressure gradient,
+ * continuity (mass conserved exactly) and RK4 accumulation, and FP64 diagnostics. */
 typedef struct {
     int64_t nc, ne, nv;
     int64_t kv;                        /* vertex table width (6 on icosahedral meshes) */
