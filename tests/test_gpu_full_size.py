@@ -95,7 +95,8 @@ def test_level10_gassmann_vertex_ke_tendency():
 
 def test_icon_level9_bitwise_with_oracle():
     """ICONShallowWater on the SCVT triangle mesh of level 9 (5,242,880 cells), with ICON's
-    divergence damping: one tendency and two RK4 steps of 20 s bit for bit."""
+    divergence damping: one tendency and two RK4 steps of 20 s bit for bit in FP64, and the
+    two steps to FP32 rounding with exact mass in FP32-bulk."""
     from dmplex_amd import ICONShallowWater, create_spherical_mesh
     from oracle.oracle import IconOracle
 
@@ -111,7 +112,17 @@ def test_icon_level9_bitwise_with_oracle():
     assert np.array_equal(gather_state(m, dy), o.tendency(y))
     yd = m.upload_state(y)
     rk4_steps_(yd, dt, 2, m)
-    assert np.array_equal(gather_state(m, yd), o.rk4_steps(y.copy(), dt, 2))
+    ref2 = o.rk4_steps(y.copy(), dt, 2)
+    assert np.array_equal(gather_state(m, yd), ref2)
+    # FP32-bulk: two steps to FP32 rounding, mass exactly
+    m32 = ICONShallowWater(m.grid, divergence_damping=nu, backend=HIPBackend(0, "fp32_bulk"), **earth_parameters())
+    yd = m32.upload_state(y)
+    mass0 = diagnostics(m32, yd)[0]
+    rk4_steps_(yd, dt, 2, m32)
+    got = gather_state(m32, yd)
+    ne = m.grid.num_edges()
+    assert rel(got[:ne], ref2[:ne]) < 2e-6 and rel(got[ne:], ref2[ne:]) < 2e-7
+    assert abs(diagnostics(m32, yd)[0] - mass0) <= 1e-15 * mass0
 
 
 def test_mpas_hexagon7_30_levels_with_oracle():
