@@ -258,8 +258,8 @@ def run_icon(args):
                  mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0,
                  roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
                                frac=achieved * 1e9 / HBM_PEAK,
-                               traffic=load_traffic(dom["name"], "icon_damping" if damping else "icon")
-                               if args.icon_level == 9 and precision == "fp64" else None,
+                               traffic=load_traffic(dom["name"], ("icon" if precision == "fp64" else "icon_fp32_bulk") +
+                                                    ("_damping" if damping else "")) if args.icon_level == 9 else None,
                                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
                  kernels=stats)
         if damping:
