@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kB) void centroid_kernel(int64_t nv, int64_t nfixed
     }
     red[threadIdx.x] = r;
     __syncthreads();
-    for (int s = kB / 2; s > 0; s >>= 1) {
+    for (unsigned s = kB / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
