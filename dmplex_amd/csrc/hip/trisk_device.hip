@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
             }
             du = double(s) - grad;
         }
-        rk_store<U>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+        rk_store<U, true>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
     } else {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         const Index<W::cw> CE(p.ce16p, p.ce, p.nc, i, 3 * i);
         double s = 0.0;
         for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
-        rk_store<double>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double, true>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
 

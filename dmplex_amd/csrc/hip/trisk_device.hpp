@@ -261,26 +261,34 @@ __device__ __forceinline__ RkIn rk_load(const Rk& r, int64_t j) {
     return v;
 }
 
-template <class U>
+template <class U, bool NT = false>
 __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
+    // NT: nontemporal (streaming) stores; they help where the next reader is not the next
+    // kernel's gathers (measured: the table path's update kernel)
+    auto put = [](auto x, auto* p) {
+        if constexpr (NT)
+            __builtin_nontemporal_store(x, p);
+        else
+            *p = x;
+    };
     switch (r.mode) {
         case 0:
-            r.dy[j] = k;
+            put(k, r.dy + j);
             break;
         case 1:
-            r.acc[j] = k;
-            stage[0] = U(v.y + r.dt2 * k);
+            put(k, r.acc + j);
+            put(U(v.y + r.dt2 * k), stage);
             break;
         case 2:
-            r.acc[j] = v.acc + 2 * k;
-            stage[0] = U(v.y + r.dt2 * k);
+            put(v.acc + 2 * k, r.acc + j);
+            put(U(v.y + r.dt2 * k), stage);
             break;
         case 3:
-            r.acc[j] = v.acc + 2 * k;
-            stage[0] = U(v.y + r.dt * k);
+            put(v.acc + 2 * k, r.acc + j);
+            put(U(v.y + r.dt * k), stage);
             break;
         default:
-            r.y[j] = v.y + r.dt6 * (v.acc + k);
+            put(v.y + r.dt6 * (v.acc + k), r.y + j);
             break;
     }
 }
