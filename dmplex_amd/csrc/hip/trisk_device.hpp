@@ -263,8 +263,9 @@ __device__ __forceinline__ RkIn rk_load(const Rk& r, int64_t j) {
 
 template <class U, bool NT = false>
 __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
-    // NT: nontemporal (streaming) stores; they help where the next reader is not the next
-    // kernel's gathers (measured: the table path's update kernel)
+    // NT: nontemporal (streaming) stores.  Measured faster in the table path's update kernel
+    // and the ICON kernels; the cell-pair kernels keep plain stores (their stage vectors are
+    // gathered by the next kernel, which then ran slower)
     auto put = [](auto x, auto* p) {
         if constexpr (NT)
             __builtin_nontemporal_store(x, p);
