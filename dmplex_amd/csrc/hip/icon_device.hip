@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void icon_cell_vertex_kernel(IconParamsT<C>
             for (int k = 0; k < 3; ++k) d += wd[k] * C(u[E[k]]);
             p.D[i] = -d;
         }
-        rk_store<double, true>(r, p.ne + i, dh, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double, kStreamAll>(r, p.ne + i, dh, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     } else {
         const int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParamsT<C> p,
         const C G = (C(-1) * (p.D[rr] - p.D[l])) * invd;
         du = du + double(ld(p.nu + e) * G);
     }
-    rk_store<U, true>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    rk_store<U, kStreamAll>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------

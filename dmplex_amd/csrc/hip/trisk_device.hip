@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
             }
             du = double(s) - grad;
         }
-        rk_store<U, true>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+        rk_store<U, kStreamAll>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
     } else {
         int64_t i = t.first + threadIdx.x;
         if (i >= p.nc) return;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         const Index<W::cw> CE(p.ce16p, p.ce, p.nc, i, 3 * i);
         double s = 0.0;
         for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
-        rk_store<double, true>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+        rk_store<double, kStreamAll>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
 
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
             div += side ? -double(G[k]) : double(G[k]);
         }
     }
-    rk_store<double>(r, p.ne + i, -div / area, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
+    rk_store<double, kStreamAcc>(r, p.ne + i, -div / area, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
 }
 
 // W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
     ldrec(p.T + 2 * e, hs);
     const double du = pair_edge_tendency(C(hs[0]), C(hs[1]), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
-    rk_store<U>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
 // ---------------------------------------------------------------------------------

@@ -261,35 +261,45 @@ __device__ __forceinline__ RkIn rk_load(const Rk& r, int64_t j) {
     return v;
 }
 
-template <class U, bool NT = false>
+// Which of rk_store's writes are nontemporal (streaming) stores: none, the accumulator, state
+// and tendency (read next by the same entity's next stage), or also the stage vector.
+// Measured: kStreamAll is faster in the table path's update kernel and the ICON kernels;
+// the cell-pair kernels take kStreamAcc, since their stage vectors are gathered by the next
+// kernel, which ran slower when they were streamed.
+enum RkStores { kPlain = 0, kStreamAcc = 1, kStreamAll = 2 };
+
+template <class U, int STORES = kPlain>
 __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
-    // NT: nontemporal (streaming) stores.  Measured faster in the table path's update kernel
-    // and the ICON kernels; the cell-pair kernels keep plain stores (their stage vectors are
-    // gathered by the next kernel, which then ran slower)
-    auto put = [](auto x, auto* p) {
-        if constexpr (NT)
+    auto acc = [](auto x, auto* p) {
+        if constexpr (STORES != kPlain)
+            __builtin_nontemporal_store(x, p);
+        else
+            *p = x;
+    };
+    auto stg = [](auto x, auto* p) {
+        if constexpr (STORES == kStreamAll)
             __builtin_nontemporal_store(x, p);
         else
             *p = x;
     };
     switch (r.mode) {
         case 0:
-            put(k, r.dy + j);
+            acc(k, r.dy + j);
             break;
         case 1:
-            put(k, r.acc + j);
-            put(U(v.y + r.dt2 * k), stage);
+            acc(k, r.acc + j);
+            stg(U(v.y + r.dt2 * k), stage);
             break;
         case 2:
-            put(v.acc + 2 * k, r.acc + j);
-            put(U(v.y + r.dt2 * k), stage);
+            acc(v.acc + 2 * k, r.acc + j);
+            stg(U(v.y + r.dt2 * k), stage);
             break;
         case 3:
-            put(v.acc + 2 * k, r.acc + j);
-            put(U(v.y + r.dt * k), stage);
+            acc(v.acc + 2 * k, r.acc + j);
+            stg(U(v.y + r.dt * k), stage);
             break;
         default:
-            put(v.y + r.dt6 * (v.acc + k), r.y + j);
+            acc(v.y + r.dt6 * (v.acc + k), r.y + j);
             break;
     }
 }
