@@ -165,6 +165,12 @@ __device__ __forceinline__ Lanes<T, V> vload(const T* p) {
     return x;
 }
 template <class T, int V>
+__device__ __forceinline__ void vstore_nt(T* p, const Lanes<T, V>& x) {
+    __attribute__((address_space(1))) T* q = (__attribute__((address_space(1))) T*)p;
+#pragma unroll
+    for (int v = 0; v < V; ++v) __builtin_nontemporal_store(x.v[v], q + v);
+}
+template <class T, int V>
 __device__ __forceinline__ void vstore(T* p, const Lanes<T, V>& x) {
     __attribute__((address_space(1))) T* q = (__attribute__((address_space(1))) T*)p;
 #pragma unroll
@@ -172,7 +178,8 @@ __device__ __forceinline__ void vstore(T* p, const Lanes<T, V>& x) {
 }
 
 // rk_load / rk_store (trisk_device.hpp) for V consecutive elements, with the same arithmetic
-// per element: y and acc are loaded before the gathers
+// per element: y and acc are loaded before the gathers; the accumulator, state and tendency
+// are written with nontemporal stores, the stage vectors (gathered next) with plain ones
 template <int V>
 struct RkInLanes {
     Lanes<double, V> y, acc;
@@ -193,14 +200,14 @@ __device__ __forceinline__ void rk_store_lanes(const Rk& r, int64_t j, const Lan
                                                const RkInLanes<V>& x, U* stage) {
     Lanes<U, V> st;
     if (r.mode == 0) {
-        vstore(r.dy + j, k);
+        vstore_nt(r.dy + j, k);
     } else if (r.mode <= 3) {
         Lanes<double, V> acc = k;
         if (r.mode > 1) {
 #pragma unroll
             for (int v = 0; v < V; ++v) acc.v[v] = x.acc.v[v] + 2 * k.v[v];
         }
-        vstore(r.acc + j, acc);
+        vstore_nt(r.acc + j, acc);
         const double c = r.mode == 3 ? r.dt : r.dt2;
 #pragma unroll
         for (int v = 0; v < V; ++v) st.v[v] = U(x.y.v[v] + c * k.v[v]);
@@ -209,7 +216,7 @@ __device__ __forceinline__ void rk_store_lanes(const Rk& r, int64_t j, const Lan
         Lanes<double, V> y;
 #pragma unroll
         for (int v = 0; v < V; ++v) y.v[v] = x.y.v[v] + r.dt6 * (x.acc.v[v] + k.v[v]);
-        vstore(r.y + j, y);
+        vstore_nt(r.y + j, y);
     }
 }
 
