@@ -38,9 +38,10 @@ using trisk::tiles_per_xcd;
 constexpr double Rd = 287.0, Cpd = 1004.0, Cvd = Cpd - Rd, P0 = 1.0e5;
 // levels per thread of the cell and edge kernels (1, 2, 4, 16 and 32 measured slower)
 constexpr int kLevels = 8;
-// The vn tendency reuses its ten-entry TRiSK row over more levels (measured: 0.63 ms per call
-// with 32 levels per thread against 0.71 with 8 at hexagon 7 x 30 levels).
-constexpr int kLevelsVn = 32;
+// The vn tendency reuses its ten-entry TRiSK row over several levels (measured at hexagon 7 x
+// 30 levels, with each level's gathers issued together: 0.396 ms per call with 16 levels per
+// thread, 0.41 with 32 or 10).
+constexpr int kLevelsVn = 16;
 
 struct Params {
     int64_t nc, ne, nv;
