@@ -24,7 +24,8 @@ int set_err(int code, const std::string& msg);
             return set_err(TRISK_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-// workgroup size of the single-model kernels (4 waves of 64)
+// workgroup size of the single-model, ICON and MPAS kernels (4 waves of 64; 128 and 512
+// measured equal or slower on every path)
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;
 
