@@ -763,6 +763,7 @@ int trisk_destroy(trisk_handle* h) {
 
 int trisk_state_length(const trisk_handle* h, int64_t* n) {
     CHECK_HANDLE(h);
+    if (!n) return set_err(TRISK_EINVAL, "null output pointer");
     *n = h->L.ne + h->L.nc;
     return 0;
 }
@@ -925,6 +926,7 @@ int trisk_set_graphs(trisk_handle* h, int32_t on) {
 int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream) {
     CHECK_HANDLE(h);
     CHECK_TRISK(h);
+    if (!y_dev) return set_err(TRISK_EINVAL, "null state pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc = h->precision == TRISK_FP64
@@ -942,6 +944,7 @@ int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, do
 int trisk_vertex_pv(trisk_handle* h, const double* y_dev, double* qv_dev, double* zv_dev, double* hv_dev,
                     void* stream) {
     CHECK_HANDLE(h);
+    if (!y_dev) return set_err(TRISK_EINVAL, "null state pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (h->L.icon) return icon_vertex_pv(h, y_dev, qv_dev, zv_dev, hv_dev, s);
@@ -972,16 +975,15 @@ int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void*
 
 int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_dev, void* stream) {
     CHECK_HANDLE(h);
+    if (!y_dev || !zeta_dev) return set_err(TRISK_EINVAL, "null pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (h->L.icon) return icon_relative_vorticity(h, y_dev, zeta_dev, s);
     const Layout& L = h->L;
-    // FP64 curl table: the FP64 parameters exist only in FP64 mode, so gather from host copies
-    const int32_t* idx = h->precision == TRISK_FP64 ? h->p64.ve : h->p32.ve;
-    static_assert(sizeof(double) == 8, "");
+    // FP64: the curl of u from the FP64 curl table; FP32-bulk: the vertex pass's FP32 curl
     if (h->precision == TRISK_FP64) {
         hipLaunchKernelGGL(edge_sum_kernel, dim3(unsigned((L.nv + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, L.nv,
-                           L.kv, idx, h->p64.curl, y_dev, zeta_dev);
+                           L.kv, h->p64.ve, h->p64.curl, y_dev, zeta_dev);
         HIP_TRY(hipGetLastError());
         return 0;
     }
@@ -990,6 +992,7 @@ int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_
 
 int trisk_tangential_velocity(trisk_handle* h, const double* u_dev, double* vt_dev, void* stream) {
     CHECK_HANDLE(h);
+    if (!u_dev || !vt_dev) return set_err(TRISK_EINVAL, "null pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (h->L.icon) return icon_tangential_velocity(h, u_dev, vt_dev, s);
@@ -1036,6 +1039,7 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
 
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count) {
     CHECK_HANDLE(h);
+    if (!bytes && max > 0) return set_err(TRISK_EINVAL, "null bytes array");
     const Layout& L = h->L;
     // Algorithmic HBM bytes of one launch: every element of every array the launch must
     // read or write counted once (gathered arrays by their unique size), in a stage 2/3
@@ -1094,6 +1098,7 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
 
 int trisk_wide_counts(const trisk_handle* h, int64_t counts[3]) {
     CHECK_HANDLE(h);
+    if (!counts) return set_err(TRISK_EINVAL, "null counts array");
     counts[0] = h->L.wide_cells;
     counts[1] = h->L.wide_vertices;
     counts[2] = h->L.wide_edges;
