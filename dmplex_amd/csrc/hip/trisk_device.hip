@@ -327,12 +327,17 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
     }
     C H[6];
     pair_half_sums(n, W, G, Q, H);
+    // the half sums go out cell-major, 6 per cell in one contiguous run (slots past n are
+    // never read); the edge kernel gathers each edge's two from its cells
+    using I = typename P::I;
+    I* out = p.T + 6 * i;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = k < n ? I(H[k]) : I(0);
     double div = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         if (k < n) {
             const int side = (meta >> (4 + k)) & 1;
-            p.T[2 * int64_t(E[k]) + side] = typename P::I(H[k]);
             div += side ? -double(G[k]) : double(G[k]);
         }
     }
@@ -349,10 +354,11 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     if (e >= p.ne) return;
     const RkIn rin = rk_load(r, e);
     int32_t c[2];
-    I hs[2];
     load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
-    ldrec(p.T + 2 * e, hs);
-    const double du = pair_edge_tendency(C(hs[0]), C(hs[1]), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
+    const uint32_t sl = ld(p.eslot + e);
+    // the half sums of the two cells (their sum is the same in either order)
+    const I hs0 = p.T[6 * int64_t(c[0]) + (sl & 15)], hs1 = p.T[6 * int64_t(c[1]) + (sl >> 4)];
+    const double du = pair_edge_tendency(C(hs0), C(hs1), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
     rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }
 
@@ -462,6 +468,7 @@ static int setup_params(trisk_handle* H) {
             rc |= upload(o, L.rc_pair, &p.rc_pair);
         }
         rc |= upload(o, L.re_cells, &p.re_cells);
+        rc |= upload(o, L.eslot, &p.eslot);
         rc |= coef(L.re_lh, &p.re_lh);
         rc |= upload(o, L.rc_pce, &p.ce6);
         rc |= upload(o, L.ccv, &p.ccv);
@@ -477,7 +484,7 @@ static int setup_params(trisk_handle* H) {
     p.Fq = static_cast<Pair<I>*>(q);
     if ((rc = alloc(o, L.ne * sizeof(I), &q))) return rc;
     p.G = static_cast<I*>(q);
-    if ((rc = alloc(o, 2 * L.ne * sizeof(I), &q))) return rc;
+    if ((rc = alloc(o, 6 * L.nc * sizeof(I), &q))) return rc;
     p.T = static_cast<I*>(q);
     for (I** w : {&p.qv, &p.zv, &p.hv, &p.Kv}) {
         if ((rc = alloc(o, L.nv * sizeof(I), &q))) return rc;
@@ -1064,8 +1071,8 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
                     + ne * (ew + 2 * c + uw + c);                /* cells, l h weights, u, G */
         double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
                     + ne * c + nv * c                            /* G, q_v */
-                    + 2 * ne * c;                                /* half sums */
-        double w3 = ne * (2 * c + ew + c + rk_e) + nc * 8.0;    /* half sums, cells, d, RK4, B */
+                    + 6 * nc * c;                                /* half sums, 6 per cell */
+        double w3 = ne * (2 * c + 1.0 + ew + c + rk_e) + nc * 8.0;  /* half sums, slots, cells, d, RK4, B */
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
         // Gassmann: zeta_v, h_v written in P1 and gathered in P2 with the rhombus weights and

@@ -111,7 +111,8 @@ struct Params {
     const C* len;          // ne: l_e
     const C* dual;         // ne: d_e
     I* G;                  // ne: l_e F_e
-    I* T;                  // ne x 2: PV-flux half sums of the left / right cell
+    I* T;                  // 6 x nc, cell-major: PV-flux half sum of cell i at its edge k in T[6 i + k]
+    const uint8_t* eslot;  // ne: slot of e in its left cell | slot in its right cell << 4
     // entity records of the cell-pair path
     const int32_t* rc_state;  // nc x 6
     const C* rc_kin;          // nc x 6

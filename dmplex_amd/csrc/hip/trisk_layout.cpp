@@ -798,8 +798,10 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     // pair, stored at the neighbour
     L.adj_ecurl.assign(size_t(ne) * 2, 0.0);
     L.adj_ekin.assign(size_t(ne) * 2, 0.0);
+    L.eslot.assign(ne, 0);
 #pragma omp parallel for schedule(static) reduction(| : bad)
-    for (int64_t e = 0; e < ne; ++e)
+    for (int64_t e = 0; e < ne; ++e) {
+        bad |= L.ec[e] == L.ec[size_t(ne) + e];  // the half sums of e come from two cells
         for (int s2 = 0; s2 < 2; ++s2) {
             const int64_t v = L.ev[size_t(s2) * ne + e], c = L.ec[size_t(s2) * ne + e];
             int k = 0;
@@ -811,7 +813,9 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
             while (k < n && L.ce[size_t(k) * nc + c] != e) ++k;
             bad |= k == n;
             L.adj_ekin[2 * e + s2] = k < n ? L.kin[size_t(k) * nc + c] : 0.0;
+            L.eslot[e] |= uint8_t((k < n ? k : 0) << (4 * s2));
         }
+    }
     L.adj_cvh.assign(size_t(6) * nc, 0.0);
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int64_t i = 0; i < nc; ++i) {

@@ -64,6 +64,7 @@ struct Layout {
     std::vector<int32_t> re_cells;       // ne x 2: left, right cell
     std::vector<double> re_lh;           // ne x 2: l_e * edge_height
     std::vector<int32_t> rc_pce;         // 6 x nc: cell edges, pads the first edge
+    std::vector<uint8_t> eslot;          // ne: slot of e in its left cell | slot in its right cell << 4
     // 16-bit index deltas from a linear base (cell j: edges 3j, corners 2j; vertex v: edges
     // 3v/2, cells v/2; edge e: cells e/3); an entity with a delta outside +-32767 has the
     // sentinel -32768 in its first slot and is read from the int32 tables instead
