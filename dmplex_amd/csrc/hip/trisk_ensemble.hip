@@ -311,7 +311,7 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
 }
 
 // W2 of all levels: PV-flux half sums and continuity with its RK4 update; cf. pair_cell_kernel.
-// Half sums are stored side-major per edge: T[(2 e + side) M + m].
+// Half sums are stored cell-major: member m of cell i's slot k at T[(6 i + k) M + m].
 template <class P, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
     using C = typename P::C;
@@ -355,8 +355,7 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
             dh.v[v] = -div / area;
         }
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < n) vstore(p.T + (2 * int64_t(E[k]) + ((meta >> (4 + k)) & 1)) * M + it.m0, Hk[k]);
+        for (int k = 0; k < 6; ++k) vstore(p.T + (6 * it.ent + uint32_t(k)) * M + it.m0, Hk[k]);
         rk_store_lanes<double, V>(r, p.ne * M + it.o, dh, rin, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
     });
 }
@@ -378,8 +377,9 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         const RkInLanes<V> rin = rk_load_lanes<V>(r, it.o);
         int32_t c[2];
         load_edge_cells<NARROW>(p, it.ent, c);
-        const int64_t t = 2 * int64_t(it.ent) * M + it.m0;
-        const Lanes<I, V> t0 = vload<I, V>(p.T + t), t1 = vload<I, V>(p.T + t + M);
+        const uint32_t sl = ld(p.eslot + it.ent);
+        const Lanes<I, V> t0 = vload<I, V>(p.T + (6 * uint32_t(c[0]) + (sl & 15)) * M + it.m0);
+        const Lanes<I, V> t1 = vload<I, V>(p.T + (6 * uint32_t(c[1]) + (sl >> 4)) * M + it.m0);
         const Lanes<double, V> bl = vload<double, V>(p.B + uint32_t(c[0]) * M + it.m0);
         const Lanes<double, V> br = vload<double, V>(p.B + uint32_t(c[1]) * M + it.m0);
         const C d = C(ld(p.dual + it.ent));
@@ -605,7 +605,7 @@ static int setup_levels(trisk_ensemble* E) {
         if (rc) return rc;
         e.p.G = static_cast<I*>(get(L.ne * M * sizeof(I)));
         if (rc) return rc;
-        e.p.T = static_cast<I*>(get(2 * L.ne * M * sizeof(I)));
+        e.p.T = static_cast<I*>(get(6 * L.nc * M * sizeof(I)));
         if (rc) return rc;
         e.acc = static_cast<double*>(get((L.ne + L.nc) * M * sizeof(double)));
         if (rc) return rc;
@@ -812,7 +812,8 @@ int trisk_ensemble_create(const trisk_model_desc* const* levels, const int32_t* 
         }
         E->models.push_back(h);
         const int64_t items = (h->L.ne + h->L.nc) * int64_t(members[l]);
-        if (2 * h->L.ne * int64_t(members[l]) >= (int64_t(1) << 31) || items >= (int64_t(1) << 31)) {
+        if (2 * h->L.ne * int64_t(members[l]) >= (int64_t(1) << 31) ||
+            6 * h->L.nc * int64_t(members[l]) >= (int64_t(1) << 31) || items >= (int64_t(1) << 31)) {
             trisk_ensemble_destroy(E);
             return set_err(TRISK_EINVAL, "level " + std::to_string(l) + ": entities x members exceed 2^30");
         }
@@ -1022,8 +1023,8 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
         const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
         b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + ne * (ew + 2 * c) +
                 M * (nc * (8.0 + 8.0) + nv * c + ne * (uw + c));
-        b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 2 * ne * c);
-        b[2] += ne * (ew + c) + M * (ne * (2 * c + rk_e) + nc * 8.0);
+        b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 6 * nc * c);
+        b[2] += ne * (ew + c + 1.0) + M * (ne * (2 * c + rk_e) + nc * 8.0);
     }
     int k = 0;
     for (; k < kEnsKinds && k < max; ++k) {
