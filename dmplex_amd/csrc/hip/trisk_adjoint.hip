@@ -41,7 +41,7 @@ struct AdjParams {
     const C* cvh;          // 6 x nc
     const int32_t* vslot;  // nv
     double* Bbar;          // nc
-    I* Tbar;               // ne x 2  (edge, side) partials of Gbar
+    I* Tbar;               // 6 x nc, cell-major: cell i's partial of Gbar at its edge k in Tbar[6 i + k]
     I* QC;                 // nc x 6  (cell, corner) partials of qbar
     I* zbar;               // nv
     I* ebar;               // nv      cotangent of h_v
@@ -153,12 +153,11 @@ __global__ __launch_bounds__(kBlock) void adj_cell_kernel(Params<P> p, AdjParams
             Qb[nx] += qeb[k] / 2;
         }
     }
+    // six contiguous values per cell (slots past n hold 0 and are never read)
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        if (k < n) {
-            w.Tbar[2 * int64_t(E[k]) + ((meta >> (4 + k)) & 1)] = I(Gb[k]);
-            w.QC[6 * i + k] = I(Qb[k]);
-        }
+        w.Tbar[6 * i + k] = I(Gb[k]);
+        w.QC[6 * i + k] = I(Qb[k]);
     }
     w.Bbar[i] = bb;
 }
@@ -203,7 +202,8 @@ __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams
     ldrec(w.ecurl + 2 * e, cu);
     ldrec(w.ekin + 2 * e, kn);
     const int32_t v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
-    const C Gb = C(w.Tbar[2 * e]) + C(w.Tbar[2 * e + 1]);
+    const uint32_t sl = ld(p.eslot + e);
+    const C Gb = C(w.Tbar[6 * int64_t(c[0]) + (sl & 15)]) + C(w.Tbar[6 * int64_t(c[1]) + (sl >> 4)]);
     const C ue = C(u[e]);
     const C he = lh[0] * C(h[c[0]]) + lh[1] * C(h[c[1]]);
     const double s = double(Gb * he) + double(cu[0] * C(w.zbar[v0]) + cu[1] * C(w.zbar[v1])) +
@@ -321,7 +321,7 @@ static int adj_setup(trisk_handle* H) {
     void* q;
     if ((rc = alloc(st->owned, L.nc * sizeof(double), &q))) return rc;
     w.Bbar = static_cast<double*>(q);
-    if ((rc = alloc(st->owned, 2 * L.ne * sizeof(I), &q))) return rc;
+    if ((rc = alloc(st->owned, 6 * L.nc * sizeof(I), &q))) return rc;
     w.Tbar = static_cast<I*>(q);
     if ((rc = alloc(st->owned, 6 * L.nc * sizeof(I), &q))) return rc;
     w.QC = static_cast<I*>(q);
