@@ -344,8 +344,11 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
     rk_store<double, kStreamAcc>(r, p.ne + i, -div / area, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
 }
 
-// W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update
-template <class P>
+// W3: edges: du = (H_left + H_right) / (2 d_e) - (B_r - B_l) / d_e, then the RK4 update.
+// GNEXT (RK4 stages): also G_e = l_e h_e u_e of the state this stage writes, from the heights
+// the cell kernel has just written and the edge's own new value, so that the next stage's
+// state pass skips its edge range.  The operands and operations are those of the state pass.
+template <class P, bool GNEXT>
 __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
@@ -356,10 +359,20 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     int32_t c[2];
     load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
     const uint32_t sl = ld(p.eslot + e);
+    C lh[2] = {C(0), C(0)};
+    if constexpr (GNEXT) ldrec(p.re_lh + 2 * e, lh);
     // the half sums of the two cells (their sum is the same in either order)
     const I hs0 = p.T[6 * int64_t(c[0]) + (sl & 15)], hs1 = p.T[6 * int64_t(c[1]) + (sl >> 4)];
+    double hn0 = 0.0, hn1 = 0.0;
+    if constexpr (GNEXT) {
+        // the heights of the state this stage writes: the stage heights, or y after the last stage
+        const double* hn = r.mode == 4 ? r.y + p.ne : r.sh;
+        hn0 = hn[c[0]];
+        hn1 = hn[c[1]];
+    }
     const double du = pair_edge_tendency(C(hs0), C(hs1), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
     rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+    if constexpr (GNEXT) p.G[e] = I(pair_mass_flux(lh, C(hn0), C(hn1), C(rk_next(r, du, rin))));
 }
 
 // ---------------------------------------------------------------------------------
@@ -512,13 +525,15 @@ static int setup_params(trisk_handle* H) {
 
 // One tendency: P1, P2, P3 with the given stage input and update mode.
 template <class P, class UIn>
-static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
+static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s, bool state_edges,
+                    bool gnext) {
     const Params<P>& p = params<P>(H);
     const Layout& L = H->L;
     hipEvent_t stop;
     int rc;
     {
-        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = tiles_per_xcd(L.ne);
+        // state_edges false: G is already there, formed by the previous stage's edge kernel
+        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = state_edges ? tiles_per_xcd(L.ne) : 0;
         if ((rc = timed_begin(H, K_PAIR_STATE, s, &stop))) return rc;
         constexpr bool rec = P::id == TRISK_FP32_BULK;
         hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0,
@@ -536,7 +551,10 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     {
         if ((rc = timed_begin(H, K_PAIR_EDGE, s, &stop))) return rc;
         const int64_t t0 = tiles_per_xcd(L.ne);
-        hipLaunchKernelGGL((pair_edge_kernel<P>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
+        if (gnext)
+            hipLaunchKernelGGL((pair_edge_kernel<P, true>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
+        else
+            hipLaunchKernelGGL((pair_edge_kernel<P, false>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -607,11 +625,17 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
     return 0;
 }
 
+template <class P>
+static bool pair_run(trisk_handle* H) {
+    return H->L.pair && params<P>(H).qv_out == nullptr;
+}
+
+// state_edges, gnext: the cell-pair path's fusion of G into the previous stage's edge kernel
 template <class P, class UIn>
-static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
-    const Params<P>& p = params<P>(H);
+static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s,
+                        bool state_edges = true, bool gnext = false) {
     const Layout& L = H->L;
-    if (L.pair && p.qv_out == nullptr) return run_pair<P, UIn>(H, u, h, r, s);
+    if (pair_run<P>(H)) return run_pair<P, UIn>(H, u, h, r, s, state_edges, gnext);
     const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
     return hex ? run_widths<P, UIn, HexWidths>(H, u, h, r, s) : run_widths<P, UIn, AnyWidths>(H, u, h, r, s);
 }
@@ -629,12 +653,17 @@ static int rk4_steps(trisk_handle* H, double* y, double dt, int64_t nsteps, hipS
     r.dt2 = dt / 2;
     r.dt6 = dt / 6;
     int rc;
+    // cell-pair path: each stage's edge kernel also forms G of the state it writes, so only the
+    // first stage of the call runs the state pass over the edges
+    const bool fuse = pair_run<P>(H);
     for (int64_t step = 0; step < nsteps; ++step) {
         r.mode = 1;
-        if ((rc = run_tendency<P, double>(H, y, y + L.ne, r, s))) return rc;
+        if ((rc = run_tendency<P, double>(H, y, y + L.ne, r, s, !fuse || step == 0, fuse))) return rc;
         for (int mode = 2; mode <= 4; ++mode) {
             r.mode = mode;
-            if ((rc = run_tendency<P, U>(H, static_cast<const U*>(H->su), H->sh, r, s))) return rc;
+            const bool last = mode == 4 && step + 1 == nsteps;
+            if ((rc = run_tendency<P, U>(H, static_cast<const U*>(H->su), H->sh, r, s, !fuse, fuse && !last)))
+                return rc;
         }
     }
     return 0;
@@ -864,9 +893,9 @@ int pair_state_pass(trisk_handle* H, const void* u, bool u_double, const double*
 
 int pair_tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r,
                         hipStream_t s) {
-    if (H->precision == TRISK_FP64) return run_pair<Fp64, double>(H, static_cast<const double*>(u), h, r, s);
-    if (u_double) return run_pair<Fp32Bulk, double>(H, static_cast<const double*>(u), h, r, s);
-    return run_pair<Fp32Bulk, float>(H, static_cast<const float*>(u), h, r, s);
+    if (H->precision == TRISK_FP64) return run_pair<Fp64, double>(H, static_cast<const double*>(u), h, r, s, true, false);
+    if (u_double) return run_pair<Fp32Bulk, double>(H, static_cast<const double*>(u), h, r, s, true, false);
+    return run_pair<Fp32Bulk, float>(H, static_cast<const float*>(u), h, r, s, true, false);
 }
 
 int rk4_steps_direct(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s) {
@@ -1066,13 +1095,16 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const double rec8 = f64 ? 6 * iw : 16.0;                 /* FP32: 16-bit deltas, 8-wide records */
         const double ew = f64 ? 2 * iw : 4.0;                    /* edge cells */
         const double cw = f64 ? 12 * iw : 24.0;                  /* cell edges and corners */
+        // rk4_steps forms G in the previous stage's edge kernel: the state pass of stages after
+        // the first covers cells and vertices, and the edge kernel reads l h and the new heights
         double w1 = nc * (rec8 + 6 * c + 8.0 + 8.0)             /* edges, KE weights, h, B */
-                    + nv * (rec8 + vc + c)                       /* indices, coefficients, q_v */
-                    + ne * (ew + 2 * c + uw + c);                /* cells, l h weights, u, G */
+                    + nv * (rec8 + vc + c);                      /* indices, coefficients, q_v */
         double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
                     + ne * c + nv * c                            /* G, q_v */
                     + 6 * nc * c;                                /* half sums, 6 per cell */
-        double w3 = ne * (2 * c + 1.0 + ew + c + rk_e) + nc * 8.0;  /* half sums, slots, cells, d, RK4, B */
+        double w3 = ne * (2 * c + 1.0 + ew + c + rk_e)          /* half sums, slots, cells, d, RK4 */
+                    + nc * 8.0                                   /* B */
+                    + ne * (2 * c + c) + nc * 8.0;               /* l h weights, G; the new heights */
         v = {{kKernelNames[K_PAIR_STATE], w1}, {kKernelNames[K_PAIR_CELL], w2}, {kKernelNames[K_PAIR_EDGE], w3}};
     } else {
         // Gassmann: zeta_v, h_v written in P1 and gathered in P2 with the rhombus weights and

@@ -270,6 +270,20 @@ __device__ __forceinline__ RkIn rk_load(const Rk& r, int64_t j) {
 // kernel, which ran slower when they were streamed.
 enum RkStores { kPlain = 0, kStreamAcc = 1, kStreamAll = 2 };
 
+// The value rk_store leaves as the next stage's input (RK4 modes): the stage value, or y after
+// the last stage.  A stage vector of type U holds U(x), which reads back as the same C(x).
+__device__ __forceinline__ double rk_next(const Rk& r, double k, const RkIn& v) {
+    switch (r.mode) {
+        case 1:
+        case 2:
+            return v.y + r.dt2 * k;
+        case 3:
+            return v.y + r.dt * k;
+        default:
+            return v.y + r.dt6 * (v.acc + k);
+    }
+}
+
 template <class U, int STORES = kPlain>
 __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
     auto acc = [](auto x, auto* p) {
