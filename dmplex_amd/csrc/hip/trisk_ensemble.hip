@@ -24,6 +24,9 @@ namespace trisk {
 
 constexpr int kMaxLevels = 16;
 constexpr int kEnsKinds = 3;  // state, cell, edge kernels
+// tile lists: one per kernel, and the state kernel's without the edges (RK4 stages whose G the
+// previous stage's edge kernel formed)
+constexpr int kEnsLists = 4, kStateNoEdges = 3;
 
 // n / d for n < 2^31 with one 32x32->64-bit product: with l = ceil(log2 d) and
 // mul = ceil(2^(31+l) / d) (< 2^32), n * mul / 2^(31+l) = n/d + n e / (d 2^(31+l)) with
@@ -360,8 +363,9 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
     });
 }
 
-// W3 of all levels: du and its RK4 update; cf. pair_edge_kernel
-template <class P, int BLK, int V>
+// W3 of all levels: du and its RK4 update; cf. pair_edge_kernel.  GNEXT (RK4 stages): also G of
+// the state the stage writes, so that the next stage's state kernel skips the edges.
+template <class P, int BLK, int V, bool GNEXT>
 __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
@@ -378,15 +382,35 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         int32_t c[2];
         load_edge_cells<NARROW>(p, it.ent, c);
         const uint32_t sl = ld(p.eslot + it.ent);
+        C lh[2] = {C(0), C(0)};
+        if constexpr (GNEXT) ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
         const Lanes<I, V> t0 = vload<I, V>(p.T + (6 * uint32_t(c[0]) + (sl & 15)) * M + it.m0);
         const Lanes<I, V> t1 = vload<I, V>(p.T + (6 * uint32_t(c[1]) + (sl >> 4)) * M + it.m0);
         const Lanes<double, V> bl = vload<double, V>(p.B + uint32_t(c[0]) * M + it.m0);
         const Lanes<double, V> br = vload<double, V>(p.B + uint32_t(c[1]) * M + it.m0);
+        Lanes<double, V> hn0, hn1;
+        if constexpr (GNEXT) {
+            // the heights of the state this stage writes: the stage heights, or y after the last stage
+            const double* hn = r.mode == 4 ? r.y + p.ne * M : L.sh;
+            hn0 = vload<double, V>(hn + uint32_t(c[0]) * M + it.m0);
+            hn1 = vload<double, V>(hn + uint32_t(c[1]) * M + it.m0);
+        }
         const C d = C(ld(p.dual + it.ent));
         Lanes<double, V> du;
 #pragma unroll
         for (int v = 0; v < V; ++v) du.v[v] = pair_edge_tendency(C(t0.v[v]), C(t1.v[v]), d, bl.v[v], br.v[v]);
         rk_store_lanes<U, V>(r, it.o, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(L.su) + it.o : nullptr);
+        if constexpr (GNEXT) {
+            // the new values exactly as rk_store_lanes forms them
+            Lanes<I, V> G;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double un = r.mode <= 3 ? rin.y.v[v] + (r.mode == 3 ? r.dt : r.dt2) * du.v[v]
+                                              : rin.y.v[v] + r.dt6 * (rin.acc.v[v] + du.v[v]);
+                G.v[v] = I(pair_mass_flux(lh, C(hn0.v[v]), C(hn1.v[v]), C(un)));
+            }
+            vstore(p.G + it.o, G);
+        }
     });
 }
 
@@ -424,10 +448,10 @@ struct trisk_ensemble {
     std::vector<double> dts;
     // schedules of the three kernels
     trisk_schedule sched[kEnsKinds]{};
-    EnsSeg* seg_dev[kEnsKinds]{};
-    size_t seg_cap[kEnsKinds]{};
-    int32_t first[kEnsKinds][kXcd + 1]{};
-    int64_t grid[kEnsKinds]{};
+    EnsSeg* seg_dev[kEnsLists]{};
+    size_t seg_cap[kEnsLists]{};
+    int32_t first[kEnsLists][kXcd + 1]{};
+    int64_t grid[kEnsLists]{};
     double* scratch = nullptr;  // one member of the largest level
     // graph replay
     bool graphs = false;
@@ -465,8 +489,9 @@ static int check_alignment(const trisk_ensemble* E, const void* p) {
     return 0;
 }
 
-// The runs of kernel `kind` per XCD under its schedule.
-static int build_schedule(trisk_ensemble* E, int kind) {
+// The runs of tile list `list` per XCD under the schedule of its kernel.
+static int build_list(trisk_ensemble* E, int list) {
+    const int kind = list == kStateNoEdges ? TRISK_ENS_STATE : list;
     const trisk_schedule& sc = E->sched[kind];
     const int64_t blk = sc.block_size;
     const int64_t V = lanes_of(sc);
@@ -481,7 +506,7 @@ static int build_schedule(trisk_ensemble* E, int kind) {
         if (kind == TRISK_ENS_STATE) {
             ranges.push_back({l, 0, L.nc * M, 0});
             ranges.push_back({l, 1, L.nv * M, 0});
-            ranges.push_back({l, 2, L.ne * M, 0});
+            if (list != kStateNoEdges) ranges.push_back({l, 2, L.ne * M, 0});
         } else if (kind == TRISK_ENS_CELL) {
             ranges.push_back({l, 0, L.nc * M, 0});
         } else {
@@ -498,7 +523,8 @@ static int build_schedule(trisk_ensemble* E, int kind) {
         if (lo >= hi) return;
         per[x].push_back({r.level, r.kind, int32_t(lo), int32_t(hi - lo), 0, int32_t(r.items)});
     };
-    if (sc.split == TRISK_SPLIT_INTERLEAVED && kind == TRISK_ENS_STATE) {
+    // (the interleaved groups hold edge tiles: the list without edges takes the chunked split)
+    if (sc.split == TRISK_SPLIT_INTERLEAVED && list == TRISK_ENS_STATE) {
         // XCD x: the x-th contiguous chunk of every level's interleaved cells, vertices and
         // edges, in whole groups of 6 tiles (one cell tile with its vertex and edge tiles)
         ranges.clear();
@@ -533,7 +559,7 @@ static int build_schedule(trisk_ensemble* E, int kind) {
     std::vector<EnsSeg> segs;
     int64_t most = 0;
     for (int x = 0; x < kXcd; ++x) {
-        E->first[kind][x] = int32_t(segs.size());
+        E->first[list][x] = int32_t(segs.size());
         int64_t cum = 0;
         for (auto g : per[x]) {
             g.cum = int32_t(cum);
@@ -542,20 +568,27 @@ static int build_schedule(trisk_ensemble* E, int kind) {
         }
         most = std::max(most, cum);
     }
-    E->first[kind][kXcd] = int32_t(segs.size());
+    E->first[list][kXcd] = int32_t(segs.size());
     int64_t nbx = most;
     if (sc.blocks_per_cu > 0) nbx = std::min<int64_t>(most, int64_t(sc.blocks_per_cu) * ((E->cus + kXcd - 1) / kXcd));
-    E->grid[kind] = kXcd * std::max<int64_t>(nbx, 1);
+    E->grid[list] = kXcd * std::max<int64_t>(nbx, 1);
     if (segs.empty()) segs.push_back({0, 0, 0, 0, 0, 0});
-    if (segs.size() > E->seg_cap[kind]) {
+    if (segs.size() > E->seg_cap[list]) {
         void* q;
         int rc = alloc(E->owned, segs.size() * sizeof(EnsSeg), &q);
         if (rc) return rc;
-        E->seg_dev[kind] = static_cast<EnsSeg*>(q);
-        E->seg_cap[kind] = segs.size();
+        E->seg_dev[list] = static_cast<EnsSeg*>(q);
+        E->seg_cap[list] = segs.size();
     }
-    HIP_TRY(hipMemcpy(E->seg_dev[kind], segs.data(), segs.size() * sizeof(EnsSeg), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->seg_dev[list], segs.data(), segs.size() * sizeof(EnsSeg), hipMemcpyHostToDevice));
     return 0;
+}
+
+// The tile lists of kernel `kind` (the state kernel has two).
+static int build_schedule(trisk_ensemble* E, int kind) {
+    int rc = build_list(E, kind);
+    if (!rc && kind == TRISK_ENS_STATE) rc = build_list(E, kStateNoEdges);
+    return rc;
 }
 
 template <class P>
@@ -668,30 +701,35 @@ static int ens_timed_begin(trisk_ensemble* E, int kind, hipStream_t s, hipEvent_
 }
 
 template <class P, class UIn, int BLK, int V>
-static void launch_kind(int kind, dim3 grid, const EnsArgs& a, hipStream_t s) {
+static void launch_kind(int kind, bool gnext, dim3 grid, const EnsArgs& a, hipStream_t s) {
     if (kind == TRISK_ENS_STATE)
         hipLaunchKernelGGL((ens_state_kernel<P, UIn, BLK, V>), grid, dim3(BLK), 0, s, a);
     else if (kind == TRISK_ENS_CELL)
         hipLaunchKernelGGL((ens_cell_kernel<P, BLK, V>), grid, dim3(BLK), 0, s, a);
+    else if (gnext)
+        hipLaunchKernelGGL((ens_edge_kernel<P, BLK, V, true>), grid, dim3(BLK), 0, s, a);
     else
-        hipLaunchKernelGGL((ens_edge_kernel<P, BLK, V>), grid, dim3(BLK), 0, s, a);
+        hipLaunchKernelGGL((ens_edge_kernel<P, BLK, V, false>), grid, dim3(BLK), 0, s, a);
 }
 
 template <class P, class UIn, int BLK>
-static void launch_lanes(int kind, int V, dim3 grid, const EnsArgs& a, hipStream_t s) {
+static void launch_lanes(int kind, bool gnext, int V, dim3 grid, const EnsArgs& a, hipStream_t s) {
     if (V == 8)
-        launch_kind<P, UIn, BLK, 8>(kind, grid, a, s);
+        launch_kind<P, UIn, BLK, 8>(kind, gnext, grid, a, s);
     else if (V == 4)
-        launch_kind<P, UIn, BLK, 4>(kind, grid, a, s);
+        launch_kind<P, UIn, BLK, 4>(kind, gnext, grid, a, s);
     else if (V == 2)
-        launch_kind<P, UIn, BLK, 2>(kind, grid, a, s);
+        launch_kind<P, UIn, BLK, 2>(kind, gnext, grid, a, s);
     else
-        launch_kind<P, UIn, BLK, 1>(kind, grid, a, s);
+        launch_kind<P, UIn, BLK, 1>(kind, gnext, grid, a, s);
 }
 
 // One tendency of every member: the three kernels with the given stage input and RK4 mode.
+// state_edges false: G is there already, formed by the previous stage's edge kernel; gnext: the
+// edge kernel forms G of the state it writes.
 template <class P, class UIn>
-static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool from_y, hipStream_t s) {
+static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool from_y, hipStream_t s,
+                        bool state_edges = true, bool gnext = false) {
     EnsArgs a{};
     a.levels = E->levels_dev;
     a.y = y;
@@ -699,22 +737,23 @@ static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool
     a.mode = mode;
     a.from_y = from_y ? 1 : 0;
     for (int kind = 0; kind < kEnsKinds; ++kind) {
-        a.seg = E->seg_dev[kind];
-        std::memcpy(a.first, E->first[kind], sizeof(a.first));
+        const int list = kind == TRISK_ENS_STATE && !state_edges ? kStateNoEdges : kind;
+        a.seg = E->seg_dev[list];
+        std::memcpy(a.first, E->first[list], sizeof(a.first));
         hipEvent_t stop;
         int rc = ens_timed_begin(E, kind, s, &stop);
         if (rc) return rc;
-        const dim3 grid(unsigned(E->grid[kind]));
+        const dim3 grid(unsigned(E->grid[list]));
         const int V = lanes_of(E->sched[kind]);
         switch (E->sched[kind].block_size) {
             case 128:
-                launch_lanes<P, UIn, 128>(kind, V, grid, a, s);
+                launch_lanes<P, UIn, 128>(kind, gnext, V, grid, a, s);
                 break;
             case 512:
-                launch_lanes<P, UIn, 512>(kind, V, grid, a, s);
+                launch_lanes<P, UIn, 512>(kind, gnext, V, grid, a, s);
                 break;
             default:
-                launch_lanes<P, UIn, 256>(kind, V, grid, a, s);
+                launch_lanes<P, UIn, 256>(kind, gnext, V, grid, a, s);
                 break;
         }
         HIP_TRY(hipGetLastError());
@@ -727,10 +766,13 @@ template <class P>
 static int ens_steps(trisk_ensemble* E, double* y, int64_t nsteps, hipStream_t s) {
     using U = typename P::U;
     int rc;
+    // each stage's edge kernel forms G of the state it writes: only the first stage of the
+    // call runs the state kernel over the edges
     for (int64_t k = 0; k < nsteps; ++k) {
-        if ((rc = ens_tendency<P, double>(E, y, nullptr, 1, true, s))) return rc;
+        if ((rc = ens_tendency<P, double>(E, y, nullptr, 1, true, s, k == 0, true))) return rc;
         for (int mode = 2; mode <= 4; ++mode)
-            if ((rc = ens_tendency<P, U>(E, y, nullptr, mode, false, s))) return rc;
+            if ((rc = ens_tendency<P, U>(E, y, nullptr, mode, false, s, false, !(mode == 4 && k + 1 == nsteps))))
+                return rc;
     }
     return 0;
 }
@@ -1021,10 +1063,11 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
     for (int l = 0; l < e->nlevels; ++l) {
         const Layout& L = e->models[l]->L;
         const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
-        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + ne * (ew + 2 * c) +
-                M * (nc * (8.0 + 8.0) + nv * c + ne * (uw + c));
+        // RK4 stages after the first: G comes from the previous stage's edge kernel, which
+        // reads l h and the new heights for it
+        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + M * (nc * (8.0 + 8.0) + nv * c);
         b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 6 * nc * c);
-        b[2] += ne * (ew + c + 1.0) + M * (ne * (2 * c + rk_e) + nc * 8.0);
+        b[2] += ne * (ew + c + 1.0 + 2 * c) + M * (ne * (2 * c + rk_e + c) + nc * (8.0 + 8.0));
     }
     int k = 0;
     for (; k < kEnsKinds && k < max; ++k) {

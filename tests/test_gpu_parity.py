@@ -275,6 +275,24 @@ def test_pair_path_fp32_close_to_tables():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+def test_pair_rk4_call_boundaries_bitwise(precision):
+    """On the cell-pair path each RK4 stage's edge kernel forms the mass flux G of the state it
+    writes, and the first stage of a call forms it in the state pass instead.  Twelve one-step
+    calls (G always from the state pass at stage 1) and one twelve-step call (G from the
+    previous step's last stage) give the same bits."""
+    say(f"pair rk4 call boundaries {precision}")
+    dm = mesh(5)
+    m = TRiSKShallowWater(dm, centers=dm.generators, path="pair", backend=HIPBackend(precision=precision),
+                          **earth_parameters())
+    y0 = seeded_state(m.grid)
+    a, b = m.upload_state(y0), m.upload_state(y0)
+    for _ in range(12):
+        rk4_steps_(a, 600.0, 1, m)
+    rk4_steps_(b, 600.0, 12, m)
+    assert np.array_equal(gather_state(m, a), gather_state(m, b))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
 def test_graph_replay_equals_direct_launches(precision):
     """Graph replay (captured once, replayed with eager work between replays) gives the
     same bits as direct launches."""
