@@ -1098,7 +1098,8 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         // rk4_steps forms G in the previous stage's edge kernel: the state pass of stages after
         // the first covers cells and vertices, and the edge kernel reads l h and the new heights
         double w1 = nc * (rec8 + 6 * c + 8.0 + 8.0)             /* edges, KE weights, h, B */
-                    + nv * (rec8 + vc + c);                      /* indices, coefficients, q_v */
+                    + nv * (rec8 + vc + c)                       /* indices, coefficients, q_v */
+                    + ne * uw;                                   /* u, gathered */
         double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
                     + ne * c + nv * c                            /* G, q_v */
                     + 6 * nc * c;                                /* half sums, 6 per cell */

@@ -1065,7 +1065,7 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
         const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
         // RK4 stages after the first: G comes from the previous stage's edge kernel, which
         // reads l h and the new heights for it
-        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + M * (nc * (8.0 + 8.0) + nv * c);
+        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + M * (nc * (8.0 + 8.0) + nv * c + ne * uw);
         b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 6 * nc * c);
         b[2] += ne * (ew + c + 1.0 + 2 * c) + M * (ne * (2 * c + rk_e + c) + nc * (8.0 + 8.0));
     }
