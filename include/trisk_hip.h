@@ -186,7 +186,10 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
                        int32_t* count);
 
 /* Algorithmic HBM bytes of one launch of each hot-path kernel of this handle's path
- * (each array element the kernel must touch counted once), with the kernel names. */
+ * (each array element the kernel must touch counted once), with the kernel names, for an
+ * RK4 stage of trisk_rk4_steps after the first of the call.  On the cell-pair path the edge
+ * kernel of each such stage also forms the mass flux G of the state it writes, so the next
+ * stage's state kernel covers cells and vertices only. */
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count);
 
 /* Cells, vertices and edges whose indices do not fit the 16-bit deltas of the FP32-bulk
