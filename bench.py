@@ -185,6 +185,9 @@ def run_gpu(grid, y0, precision, args, params, path=None, trisk_cell=True, profi
                                traffic=load_traffic(dom["name"], "fp64_pair" if precision == "fp64" and
                                                     path == "pair" else precision) if profiled else None,
                                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"])
+        t = out["roofline"]["traffic"]
+        # the measured HBM bytes of a launch over its duration: the bandwidth the kernel draws
+        out["roofline"]["traffic_frac"] = None if t is None else t / (dom["avg_ms"] / 1e3) / HBM_PEAK
     frac = out["hbm_frac_model"] if trisk_cell else out["hbm_frac_kernel_model"]
     log(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/step, {value:.3e} cell-steps/s, "
         f"model HBM fraction {'n/a' if frac is None else f'{frac:.3f}'}")
