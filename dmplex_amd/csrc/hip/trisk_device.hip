@@ -355,22 +355,26 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     using U = typename P::U;
     const int64_t e = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;
     if (e >= p.ne) return;
+    // the edge's own streamed values first, so that they do not wait for the cell record
     const RkIn rin = rk_load(r, e);
-    int32_t c[2];
-    load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
     const uint32_t sl = ld(p.eslot + e);
+    const C d = C(ld(p.dual + e));
     C lh[2] = {C(0), C(0)};
     if constexpr (GNEXT) ldrec(p.re_lh + 2 * e, lh);
-    // the half sums of the two cells (their sum is the same in either order)
+    int32_t c[2];
+    load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
+    // then every gather from the two cells: the half sums (their sum is the same in either
+    // order), B and, with GNEXT, the heights of the state this stage writes (the stage
+    // heights, or y after the last stage)
     const I hs0 = p.T[6 * int64_t(c[0]) + (sl & 15)], hs1 = p.T[6 * int64_t(c[1]) + (sl >> 4)];
+    const double b0 = p.B[c[0]], b1 = p.B[c[1]];
     double hn0 = 0.0, hn1 = 0.0;
     if constexpr (GNEXT) {
-        // the heights of the state this stage writes: the stage heights, or y after the last stage
         const double* hn = r.mode == 4 ? r.y + p.ne : r.sh;
         hn0 = hn[c[0]];
         hn1 = hn[c[1]];
     }
-    const double du = pair_edge_tendency(C(hs0), C(hs1), C(ld(p.dual + e)), p.B[c[0]], p.B[c[1]]);
+    const double du = pair_edge_tendency(C(hs0), C(hs1), d, b0, b1);
     rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
     if constexpr (GNEXT) p.G[e] = I(pair_mass_flux(lh, C(hn0), C(hn1), C(rk_next(r, du, rin))));
 }

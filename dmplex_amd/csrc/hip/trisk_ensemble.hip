@@ -379,11 +379,13 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         const Item it = item_of<V>(L, j);
         const Rk r = level_rk(L, a);
         const RkInLanes<V> rin = rk_load_lanes<V>(r, it.o);
-        int32_t c[2];
-        load_edge_cells<NARROW>(p, it.ent, c);
+        // the edge's own streamed values first, so that they do not wait for the cell record
         const uint32_t sl = ld(p.eslot + it.ent);
+        const C d = C(ld(p.dual + it.ent));
         C lh[2] = {C(0), C(0)};
         if constexpr (GNEXT) ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
+        int32_t c[2];
+        load_edge_cells<NARROW>(p, it.ent, c);
         const Lanes<I, V> t0 = vload<I, V>(p.T + (6 * uint32_t(c[0]) + (sl & 15)) * M + it.m0);
         const Lanes<I, V> t1 = vload<I, V>(p.T + (6 * uint32_t(c[1]) + (sl >> 4)) * M + it.m0);
         const Lanes<double, V> bl = vload<double, V>(p.B + uint32_t(c[0]) * M + it.m0);
@@ -395,7 +397,6 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
             hn0 = vload<double, V>(hn + uint32_t(c[0]) * M + it.m0);
             hn1 = vload<double, V>(hn + uint32_t(c[1]) * M + it.m0);
         }
-        const C d = C(ld(p.dual + it.ent));
         Lanes<double, V> du;
 #pragma unroll
         for (int v = 0; v < V; ++v) du.v[v] = pair_edge_tendency(C(t0.v[v]), C(t1.v[v]), d, bl.v[v], br.v[v]);
