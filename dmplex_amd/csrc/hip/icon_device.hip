@@ -97,21 +97,30 @@ __global__ __launch_bounds__(kBlock) void icon_edge_kernel(IconParamsT<C> p, con
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
-    C s = 0;
+    C s = 0, F;
+    const C ue = C(u[e]);
     if constexpr (KR > 0) {
-        const Index<KR> RE(p.re16p, p.re, p.ne, e, e);
+        // both index records and the edge's streamed values first, then every gather
+        const IndexWords<KR> wre = index_words<KR>(p.re16p, p.ne, e);
+        const IndexWords<2> wec = index_words<2>(p.ec16p, p.ne, e);
         C w[KR];
 #pragma unroll
         for (int k = 0; k < KR; ++k) w[k] = ld(p.rw + k * p.ne + e);
+        const C eh0 = ld(p.eh + e), eh1 = ld(p.eh + p.ne + e);
+        const Index<KR> RE(wre, p.re, p.ne, e, e);
+        const Index<2> EC(wec, p.ec, p.ne, e, (2 * e) / 3);
+        C ur[KR];
 #pragma unroll
-        for (int k = 0; k < KR; ++k) s += w[k] * C(u[RE[k]]);
+        for (int k = 0; k < KR; ++k) ur[k] = C(u[RE[k]]);
+        const C hl = C(h[EC[0]]), hr = C(h[EC[1]]);
+#pragma unroll
+        for (int k = 0; k < KR; ++k) s += w[k] * ur[k];
+        F = (eh0 * hl + eh1 * hr) * ue;
     } else {
         for (int k = 0; k < p.kr; ++k) s += ld(p.rw + k * p.ne + e) * C(u[ld(p.re + k * p.ne + e)]);
+        const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
+        F = (ld(p.eh + e) * C(h[EC[0]]) + ld(p.eh + p.ne + e) * C(h[EC[1]])) * ue;
     }
-    const C ue = C(u[e]);
-    const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3);
-    const int32_t l = EC[0], r = EC[1];
-    const C F = (ld(p.eh + e) * C(h[l]) + ld(p.eh + p.ne + e) * C(h[r])) * ue;
     p.v[e] = s;
     p.KF[e] = KeFT<C>{(ue * ue + s * s) / 2, F};
 }
@@ -218,20 +227,31 @@ __global__ __launch_bounds__(kBlock) void icon_momentum_kernel(IconParamsT<C> p,
     const Tile t = tile_of(blockIdx.x, t0, 0);
     const int64_t e = t.first + threadIdx.x;
     if (e >= p.ne) return;
+    // the edge's own values and both index records first, then every gather
     const RkIn rin = rk_load(r, e);
-    const Index<2> EC(p.ec16p, p.ec, p.ne, e, (2 * e) / 3), EV(p.ev16p, p.ev, p.ne, e, e / 3);
-    const int32_t l = EC[0], rr = EC[1];
-    const int32_t v0 = EV[0], v1 = EV[1];
+    const IndexWords<2> wec = index_words<2>(p.ec16p, p.ne, e), wev = index_words<2>(p.ev16p, p.ne, e);
     const C g1 = ld(p.grad + e), g2 = ld(p.grad + p.ne + e);
     const C invd = ld(p.invd + e);
-    const C zeta = (p.zeta[v0] + p.zeta[v1]) / 2;
-    const C grad_K = p.KF[e].Ke * (g1 - g2) + g2 * p.K[rr] - g1 * p.K[l];
+    const C fe = ld(p.fe + e), ve = ld(p.v + e), Ke = p.KF[e].Ke;
+    const C nu = DAMP ? ld(p.nu + e) : C(0);
+    const Index<2> EC(wec, p.ec, p.ne, e, (2 * e) / 3), EV(wev, p.ev, p.ne, e, e / 3);
+    const int32_t l = EC[0], rr = EC[1];
+    const int32_t v0 = EV[0], v1 = EV[1];
+    const C z0 = p.zeta[v0], z1 = p.zeta[v1], Kr = p.K[rr], Kl = p.K[l];
+    const double hr = h[rr], hl = h[l];
+    C Dr = C(0), Dl = C(0);
+    if constexpr (DAMP) {
+        Dr = p.D[rr];
+        Dl = p.D[l];
+    }
+    const C zeta = (z0 + z1) / 2;
+    const C grad_K = Ke * (g1 - g2) + g2 * Kr - g1 * Kl;
     // the pressure gradient in FP64 in both precisions (for FP64 this is the reference's
     // left-to-right order of (f + zeta) v - grad K - g (h_r - h_l) / d)
-    double du = double((ld(p.fe + e) + zeta) * ld(p.v + e) - grad_K) - p.g * (h[rr] - h[l]) * double(invd);
+    double du = double((fe + zeta) * ve - grad_K) - p.g * (hr - hl) * double(invd);
     if constexpr (DAMP) {
-        const C G = (C(-1) * (p.D[rr] - p.D[l])) * invd;
-        du = du + double(ld(p.nu + e) * G);
+        const C G = (C(-1) * (Dr - Dl)) * invd;
+        du = du + double(nu * G);
     }
     rk_store<U, kStreamAll>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
 }

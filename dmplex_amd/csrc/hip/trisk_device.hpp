@@ -217,18 +217,34 @@ struct Rk {
 // decoded up front from the 16-bit deltas packed two per word (base + delta; the int32 row
 // when word 0 holds kWide), so all its loads issue together; with K = 0 each slot is read
 // from the int32 table when used.
+// The packed words of row j, loaded ahead of their decoding (index_words), let a kernel issue
+// the record loads of several tables together before any of them is decoded.
+template <int K>
+struct IndexWords {
+    int32_t w[K > 0 ? (K + 1) / 2 : 1];
+};
+template <int K>
+__device__ __forceinline__ IndexWords<K> index_words(const int32_t* p16, int64_t n, int64_t j) {
+    IndexWords<K> r{};
+    if constexpr (K > 0) {
+#pragma unroll
+        for (int q = 0; q < (K + 1) / 2; ++q) r.w[q] = ld(p16 + q * n + j);
+    }
+    return r;
+}
+
 template <int K>
 struct Index {
     int32_t v[K > 0 ? K : 1];
     const int32_t* t32;
     int64_t n, j;
     __device__ __forceinline__ Index(const int32_t* p16, const int32_t* t32_, int64_t n_, int64_t j_, int64_t base)
+        : Index(index_words<K>(p16, n_, j_), t32_, n_, j_, base) {}
+    __device__ __forceinline__ Index(const IndexWords<K>& words, const int32_t* t32_, int64_t n_, int64_t j_,
+                                     int64_t base)
         : t32(t32_), n(n_), j(j_) {
         if constexpr (K > 0) {
-            constexpr int W = (K + 1) / 2;
-            int32_t w[W];
-#pragma unroll
-            for (int q = 0; q < W; ++q) w[q] = ld(p16 + q * n + j);
+            const int32_t* w = words.w;
             if (int16_t(w[0] & 0xffff) == kWide) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) v[k] = ld(t32 + k * n + j);
