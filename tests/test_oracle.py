@@ -5,6 +5,7 @@ meshes), so that a dropped term, a wrong sign or a transposed index fails."""
 import numpy as np
 import pytest
 
+from dense_operators import operators
 from dmplex_amd import ModelTables, RK4Cache, TRiSKGrid, create_hexagon_mesh, create_spherical_mesh, rk4_step_
 from dmplex_amd.synthetic import earth_parameters, harmonic_coefficients, seeded_state, streamfunction, williamson2
 from oracle.oracle import Oracle
@@ -82,28 +83,12 @@ def test_rk4_known_answer():
 def dense_tendency(t, y):
     """The TRiSK tendency from dense operators assembled from the geometry alone."""
     g = t.grid
-    nc, ne, nv = g.num_cells(), g.num_edges(), g.num_vertices()
-    l, d, A, Av = g.primal_lengths, g.dual_lengths, g.cell_areas, g.vertex_areas
+    ne = g.num_edges()
+    l, d = g.primal_lengths, g.dual_lengths
     left, right = g.edge_cells
     a, b = g.edge_vertices
-    E = np.arange(ne)
-    D = np.zeros((nc, ne))
-    D[left, E] += l / A[left]
-    D[right, E] -= l / A[right]
-    Kmat = np.zeros((nc, ne))
-    Kmat[left, E] += l * g.cell_distances[0] / (2 * A[left])
-    Kmat[right, E] += l * g.cell_distances[1] / (2 * A[right])
-    Curl = np.zeros((nv, ne))
-    for v, s in ((a, 0), (b, 1)):
-        t_sign = np.sign(np.sum(g.edge_normals * np.cross(g.vertices[v], g.edge_points), axis=1))
-        Curl[v, E] += t_sign * d / Av[v]
-    Hv = np.zeros((nv, nc))
-    for k in range(g.cell_vertices.shape[0]):
-        real = g.cell_signs[k] != 0
-        np.add.at(Hv, (g.cell_vertices[k, real], np.arange(nc)[real]), g.cell_kites[k, real] / Av[g.cell_vertices[k, real]])
-    W = np.zeros((ne, ne))
-    for k in range(g.trisk_edges.shape[0]):
-        np.add.at(W, (E, g.trisk_edges[k]), g.trisk_weights[k])
+    ops = operators(g)
+    D, Kmat, Curl, Hv, W = (ops[k] for k in ("D", "K", "Curl", "Hv", "W"))
     u, h = y[:ne], y[ne:]
     K = Kmat @ (u * u)
     B = K + t.gravity * h
