@@ -1,6 +1,6 @@
 """The step at the benchmark's full sizes: SCVT level 9 (2,621,442 cells, TRiSK PV flux, cell
 kinetic energy, dt = 45 s) and level 10 (10,485,762 cells, Gassmann PV flux, vertex kinetic
-energy, without Lloyd iterations); the ICON model on triangle level 9, the MPAS-Gassmann stage
+energy); the ICON model on triangle level 9, the MPAS-Gassmann stage
 operators on hexagon 7 x 30 levels and the 64-member ensemble on levels 4-7, the sizes they are
 measured at.  The oracle finishes one tendency in about a second at these
 sizes, so the FP64 path is checked bit for bit against it; longer runs are checked through
@@ -78,18 +78,42 @@ def test_level9_paths_conservation_and_determinism(level9):
     assert rel(out["fp32_bulk"][:ne], out["fp64"][:ne]) < 1e-4
 
 
-def test_level10_gassmann_vertex_ke_tendency():
-    say("level 10 (no Lloyd): Gassmann PV flux and vertex kinetic energy, one FP64 tendency")
-    dm = create_hexagon_mesh(10, centroidal=False)
+def test_level10_gassmann_vertex_ke():
+    """Config 2's mesh and method: create_hexagon_mesh(10; centroidal=true) (10,485,762
+    cells; Lloyd on the GPU), Gassmann PV flux and vertex kinetic energy.  FP64: one tendency
+    and two RK4 steps of 22.5 s bit for bit with the oracle.  FP32-bulk: the same two steps to
+    FP32 rounding, with the mass kept exactly."""
+    say("level 10 SCVT mesh (Lloyd on the GPU)")
+    dm = create_hexagon_mesh(10, centroidal=True, device=0)
     grid = TRiSKGrid(dm, centers=dm.generators)
     del dm
     y = seeded_state(grid)
-    m = TRiSKShallowWater(grid, pv_flux="gassmann", kinetic_energy="vertex", **earth_parameters())
-    ref = Oracle(m).tendency(y)
+    say("level 10: Gassmann PV flux and vertex kinetic energy, FP64 tendency and 2 RK4 steps")
+    kw = dict(pv_flux="gassmann", kinetic_energy="vertex", **earth_parameters())
+    m = TRiSKShallowWater(grid, **kw)
+    o = Oracle(m)
     dy = m.zeros()
     tendency_(dy, m.upload_state(y), m)
-    assert np.array_equal(gather_state(m, dy), ref)
-    del m, dy, ref, grid
+    assert np.array_equal(gather_state(m, dy), o.tendency(y))
+    del dy
+    yd = m.upload_state(y)
+    rk4_steps_(yd, 22.5, 2, m)
+    ref = o.rk4_steps(y.copy(), 22.5, 2)
+    assert np.array_equal(gather_state(m, yd), ref)
+    del m, yd, o
+    gc.collect()
+    say("level 10: FP32-bulk, 2 RK4 steps")
+    m32 = TRiSKShallowWater(grid, backend=HIPBackend(0, "fp32_bulk"), **kw)
+    yd = m32.upload_state(y)
+    mass0 = diagnostics(m32, yd)[0]
+    rk4_steps_(yd, 22.5, 2, m32)
+    got = gather_state(m32, yd)
+    ne = grid.num_edges()
+    eu, eh = rel(got[:ne], ref[:ne]), rel(got[ne:], ref[ne:])
+    say(f"  FP32-bulk after 2 steps: u {eu:.2e}, h {eh:.2e}")
+    assert eu < 2e-6 and eh < 2e-7
+    assert abs(diagnostics(m32, yd)[0] - mass0) <= 1e-15 * mass0
+    del m32, yd, ref, grid
     gc.collect()
 
 

@@ -118,6 +118,41 @@ def test_gassmann_slot_and_table_forms(precision):
         assert np.array_equal(results[0][ne:], results[1][ne:])  # dh does not use the PV flux
 
 
+@pytest.mark.parametrize("pv_flux", ["gassmann", "gassmann_mean"])
+@pytest.mark.parametrize("kinetic_energy", ["mixed", "vertex"])
+def test_fp32_bulk_gassmann_vertex_ke(pv_flux, kinetic_energy):
+    """FP32-bulk stencils (FP64 state, heights, RK4 accumulation) with the Gassmann PV flux
+    and mixed or vertex kinetic energy (TRiSKShallowWater.jl:212-244,266-296) against the
+    FP64 oracle on the SCVT mesh of level 5: the tendency to FP32 rounding with
+    sum A dh = 0, then 12 RK4 steps of 600 s with the mass kept exactly."""
+    say(f"fp32-bulk {pv_flux} {kinetic_energy} level 5")
+    dm = mesh(5)
+    m = TRiSKShallowWater(dm, centers=dm.generators, pv_flux=pv_flux, kinetic_energy=kinetic_energy,
+                          backend=HIPBackend(0, "fp32_bulk"), **earth_parameters())
+    y = seeded_state(m.grid)
+    o = Oracle(m)
+    ref = o.tendency(y)
+    dy = m.zeros()
+    tendency_(dy, m.upload_state(y), m)
+    got = gather_state(m, dy)
+    ne = dm.ne
+    eu, eh = rel(got[:ne], ref[:ne]), rel(got[ne:], ref[ne:])
+    say(f"  tendency: u {eu:.2e}, h {eh:.2e}")
+    # FP32 stencils: du to a few 1e-7; dh, a divergence that cancels to ~1e-2 of its terms, to ~1e-5
+    assert eu < 2e-6 and eh < 1e-4
+    A = m.cell_areas
+    assert abs(np.dot(A, got[ne:])) < 1e-12 * np.dot(A, np.abs(got[ne:]))
+    yd = m.upload_state(y)
+    mass0 = diagnostics(m, yd)[0]
+    rk4_steps_(yd, 600.0, 12, m)
+    got = gather_state(m, yd)
+    ref = o.rk4_steps(y.copy(), 600.0, 12)
+    eu, eh = rel(got[:ne], ref[:ne]), rel(got[ne:], ref[ne:])
+    say(f"  12 steps: u {eu:.2e}, h {eh:.2e}")
+    assert eu < 1e-6 and eh < 1e-7
+    assert abs(diagnostics(m, yd)[0] - mass0) <= 1e-14 * mass0
+
+
 def test_level5_one_day_144_steps():
     """Level 5, TRiSK, cell KE, dt = 600 s, 144 steps (one day)."""
     say("level 5, 144 steps of 600 s")
