@@ -71,6 +71,8 @@ SIGNATURES = {
     "trisk_tendency": [vp, vp, vp, vp],
     "trisk_rk4_steps": [vp, vp, C.c_double, i64, vp],
     "trisk_diagnostics": [vp, vp, f64p, vp],
+    "trisk_check_state": [vp, vp, i64p, vp],
+    "trisk_set_finite_check": [vp, C.c_int32],
     "trisk_kinetic_energy": [vp, vp, vp, vp, vp],
     "trisk_vertex_pv": [vp, vp, vp, vp, vp, vp],
     "trisk_relative_vorticity": [vp, vp, vp, vp],
@@ -136,6 +138,8 @@ def lib() -> C.CDLL:
         dll = C.CDLL(str(path))
         dll.trisk_last_error.restype = C.c_char_p
         dll.trisk_last_error.argtypes = []
+        dll.trisk_handle_last_error.restype = C.c_char_p
+        dll.trisk_handle_last_error.argtypes = [vp]
         for name, args in SIGNATURES.items():
             fn = getattr(dll, name)
             fn.argtypes = args
@@ -144,9 +148,18 @@ def lib() -> C.CDLL:
     return _lib
 
 
+TRISK_ENONFINITE = -4
+
+
+class NonFiniteStateError(FloatingPointError):
+    """The state check of a model found NaN/Inf in the state or a vertex height <= 0."""
+
+
 def check(rc: int) -> None:
     if rc != 0:
         msg = lib().trisk_last_error().decode()
         if rc == -1:
             raise ValueError(msg)
+        if rc == TRISK_ENONFINITE:
+            raise NonFiniteStateError(msg)
         raise NativeError(f"libtrisk_hip call failed ({rc}): {msg}")

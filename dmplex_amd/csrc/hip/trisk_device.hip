@@ -32,11 +32,24 @@
 namespace trisk {
 
 static thread_local std::string g_error;
+// The message slot of the handle whose entry point is running on this thread (ErrorScope).
+static thread_local std::string* g_handle_error = nullptr;
 
 int set_err(int code, const std::string& msg) {
     g_error = msg;
+    if (g_handle_error) *g_handle_error = msg;
     return code;
 }
+
+// While an entry point on handle h runs, its errors are also kept in h (for
+// trisk_handle_last_error); nested calls on other handles restore the outer slot.
+struct ErrorScope {
+    std::string* prev;
+    explicit ErrorScope(const trisk_handle* h) : prev(g_handle_error) { g_handle_error = h ? &h->last_error : nullptr; }
+    ~ErrorScope() { g_handle_error = prev; }
+    ErrorScope(const ErrorScope&) = delete;
+    ErrorScope& operator=(const ErrorScope&) = delete;
+};
 
 int lloyd_set_err(int code, const std::string& msg) { return set_err(code, msg); }
 
@@ -717,9 +730,36 @@ static int diagnostic_pass(trisk_handle* H, const double* y, double* K_out, doub
 }
 
 
+// Non-finite entries of u and of h in the state y, and vertices whose height h_v (of the
+// last diagnostic pass, in diag_hv) is not > 0 or NaN: the divisor of q_v.  Synchronous.
+int state_counts(trisk_handle* H, const double* y, int64_t counts[3], hipStream_t s) {
+    const Layout& L = H->L;
+    double c[3];
+    int rc;
+    if ((rc = reduce(H, 3, L.ne, nullptr, y, nullptr, &c[0], s))) return rc;
+    if ((rc = reduce(H, 3, L.nc, nullptr, y + L.ne, nullptr, &c[1], s))) return rc;
+    if ((rc = reduce(H, 4, L.nv, nullptr, H->diag_hv, nullptr, &c[2], s))) return rc;
+    for (int k = 0; k < 3; ++k) counts[k] = int64_t(c[k]);
+    return 0;
+}
+
+// TRISK_ENONFINITE with the counts in the message when the state y (whose diagnostic pass
+// has just run) is not finite or has a vertex height that is not positive.
+int finite_status(trisk_handle* H, const double* y, hipStream_t s) {
+    int64_t c[3];
+    int rc = state_counts(H, y, c, s);
+    if (rc) return rc;
+    if (c[0] || c[1] || c[2])
+        return set_err(TRISK_ENONFINITE, "state check failed: " + std::to_string(c[0]) + " non-finite u, " +
+                                             std::to_string(c[1]) + " non-finite h, " + std::to_string(c[2]) +
+                                             " vertices with h_v <= 0 or NaN");
+    return 0;
+}
+
 }  // namespace trisk
 
-#define CHECK_HANDLE(h) \
+#define CHECK_HANDLE(h)             \
+    ErrorScope error_scope_((h)); \
     if (!(h)) return set_err(TRISK_EINVAL, "null handle")
 #define CHECK_TRISK(h) \
     if ((h)->L.icon) return set_err(TRISK_EINVAL, "not available for the ICON triangle model")
@@ -727,6 +767,8 @@ static int diagnostic_pass(trisk_handle* H, const double* y, double* K_out, doub
 extern "C" {
 
 const char* trisk_last_error(void) { return g_error.c_str(); }
+
+const char* trisk_handle_last_error(const trisk_handle* h) { return h ? h->last_error.c_str() : "null handle"; }
 
 static void drop_graph(trisk_handle* h);
 
@@ -1000,7 +1042,10 @@ int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void*
     if (!y_dev || !out) return set_err(TRISK_EINVAL, "null pointer");
     HIP_TRY(hipSetDevice(h->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->L.icon) return icon_diagnostics(h, y_dev, out, s);
+    if (h->L.icon) {
+        int rc = icon_diagnostics(h, y_dev, out, s);
+        return rc ? rc : h->finite_check ? finite_status(h, y_dev, s) : 0;
+    }
     const Layout& L = h->L;
     int rc = trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);  // also fills diag_K
     if (rc) return rc;
@@ -1010,7 +1055,25 @@ int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void*
     if ((rc = reduce(h, 1, L.nc, area_c, y_dev + L.ne, h->diag_K, &out[1], s))) return rc;
     if ((rc = reduce(h, 2, L.nv, area_v, h->diag_hv, h->diag_qv, &out[2], s))) return rc;
     out[2] /= 2;
+    return h->finite_check ? finite_status(h, y_dev, s) : 0;
+}
+
+int trisk_set_finite_check(trisk_handle* h, int32_t on) {
+    CHECK_HANDLE(h);
+    h->finite_check = on != 0;
     return 0;
+}
+
+int trisk_check_state(trisk_handle* h, const double* y_dev, int64_t counts[3], void* stream) {
+    CHECK_HANDLE(h);
+    if (!y_dev || !counts) return set_err(TRISK_EINVAL, "null pointer");
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the vertex heights h_v of the state (the diagnostic pass of either model)
+    int rc = h->L.icon ? icon_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, s)
+                       : trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);
+    if (rc) return rc;
+    return state_counts(h, y_dev, counts, s);
 }
 
 int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_dev, void* stream) {

@@ -572,7 +572,9 @@ __device__ __forceinline__ void pair_add(double& s, double& c, double s2, double
     c = (c + c2) + e;
 }
 
-// term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2
+// term 0: A_i h_i; 1: A_i h_i (K_i + g h_i / 2); 2: A_v h_v q_v^2; 3: 1 for every entry of a that
+// is NaN or +-Inf (an exponent of all ones), else 0; 4: 1 for every entry of a that is not > 0
+// (NaN included), else 0.  Terms 3 and 4 are exact counts (area and b unused).
 static __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t n, const double* __restrict__ area,
                                                         const double* __restrict__ a, const double* __restrict__ b,
                                                         double g, double* __restrict__ partial) {
@@ -587,8 +589,12 @@ static __global__ __launch_bounds__(kBlock) void reduce_kernel(int term, int64_t
             x = area[i] * a[i];
         else if (term == 1)
             x = area[i] * a[i] * (b[i] + g * a[i] / 2);
-        else
+        else if (term == 2)
             x = area[i] * a[i] * (b[i] * b[i]);
+        else if (term == 3)
+            x = (__double_as_longlong(a[i]) & 0x7ff0000000000000ll) == 0x7ff0000000000000ll ? 1.0 : 0.0;
+        else
+            x = a[i] > 0.0 ? 0.0 : 1.0;
         neumaier(s, c, x);
     }
 #pragma unroll
@@ -678,6 +684,8 @@ struct trisk_handle {
     double* diag_qv = nullptr;
     double* diag_zv = nullptr;
     double* diag_hv = nullptr;
+    bool finite_check = false;         // trisk_set_finite_check
+    mutable std::string last_error;    // trisk_handle_last_error
     std::vector<double> host_stage;
     // graph replay of trisk_rk4_steps: one captured graph of kGraphSteps steps per (y, dt)
     bool graphs = false;

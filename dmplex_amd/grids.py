@@ -6,6 +6,7 @@ Mirrors ``src/Grids/Icosahedral.jl``, ``src/Grids/Hexagonal.jl`` and
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 
@@ -29,6 +30,9 @@ def _lloyd_device(x: np.ndarray, tri: np.ndarray, device: int, fixed: int = 12, 
                              N.ptr(res, N.f64p))
     if rc < 0:
         D.check(rc)
+    if rc > 0:  # the reference's @warn (Icosahedral.jl:183)
+        warnings.warn(f"Lloyd's algorithm did not converge in {int(its[0])} iterations (residual "
+                      f"{float(res[0]):.3e} mean edge lengths, tolerance {tolerance:.1e})", RuntimeWarning, stacklevel=3)
     if info is not None:
         info.update(iterations=int(its[0]), residual=float(res[0]), converged=rc == 0)
     N.check(N.host().dmx_delaunay(N.ptr(x, N.f64p), nv, N.ptr(tri, N.i64p), nt))
@@ -45,7 +49,13 @@ def _icosahedral(subdivisions: int, centroidal: bool, device: int | None = None,
         its = np.zeros(max(subdivisions, 1), dtype=np.int64)
         N.check(N.host().dmx_icosahedral(subdivisions, int(centroidal), N.ptr(x, N.f64p), N.ptr(tri, N.i64p),
                                          N.ptr(its, N.i64p)))
-        return x, tri, its[:subdivisions]
+        its = its[:subdivisions]
+        if centroidal:
+            for level, n in enumerate(its, start=1):
+                if n >= 10_000:  # the reference's @warn (Icosahedral.jl:183)
+                    warnings.warn(f"Lloyd's algorithm did not converge in {n} iterations at level {level}",
+                                  RuntimeWarning, stacklevel=3)
+        return x, tri, its
     x = np.empty((12, 3))
     tri = np.empty((20, 3), dtype=np.int64)
     N.check(N.host().dmx_icosahedron(N.ptr(x, N.f64p), N.ptr(tri, N.i64p)))
@@ -115,8 +125,11 @@ def lloyd_(x: np.ndarray, triangles: np.ndarray, fixed: int = 0, tolerance: floa
     if device is not None:
         return _lloyd_device(x, tri, device, fixed, tolerance, history, max_iterations)
     its = np.zeros(1, dtype=np.int64)
-    N.check(N.host().dmx_lloyd(N.ptr(x, N.f64p), len(x), N.ptr(tri, N.i64p), len(tri), fixed, tolerance,
-                               history, max_iterations, N.ptr(its, N.i64p)), warn_ok=True)
+    rc = N.check(N.host().dmx_lloyd(N.ptr(x, N.f64p), len(x), N.ptr(tri, N.i64p), len(tri), fixed, tolerance,
+                                    history, max_iterations, N.ptr(its, N.i64p)), warn_ok=True)
+    if rc > 0:  # the reference's @warn (Icosahedral.jl:183)
+        warnings.warn(f"Lloyd's algorithm did not converge in {int(its[0])} iterations", RuntimeWarning,
+                      stacklevel=2)
     return int(its[0])
 
 

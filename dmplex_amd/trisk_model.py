@@ -179,6 +179,24 @@ class CGridShallowWater:
         D.check(D.lib().trisk_wide_counts(self._handle, ptr(n, i64p)))
         return tuple(int(x) for x in n)
 
+    def set_finite_check(self, on: bool = True) -> None:
+        """With the check on, ``diagnostics`` raises ``NonFiniteStateError`` for a state with
+        NaN/Inf entries or a vertex height h_v <= 0 (trisk_set_finite_check)."""
+        D.check(D.lib().trisk_set_finite_check(self._handle, int(on)))
+
+    def check_state(self, y) -> tuple[int, int, int]:
+        """(non-finite u entries, non-finite h entries, vertices with h_v <= 0 or NaN) of a
+        state (trisk_check_state)."""
+        torch = _torch()
+        yd, _ = self._device_state(y)
+        n = np.zeros(3, dtype=np.int64)
+        D.check(D.lib().trisk_check_state(self._handle, C.c_void_p(yd.data_ptr()), ptr(n, i64p), _stream(torch)))
+        return int(n[0]), int(n[1]), int(n[2])
+
+    def last_error(self) -> str:
+        """Message of the last failed call on this model's handle."""
+        return D.lib().trisk_handle_last_error(self._handle).decode()
+
     def use_graphs(self, on: bool = True) -> None:
         """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""
         D.check(D.lib().trisk_set_graphs(self._handle, int(on)))

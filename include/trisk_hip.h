@@ -47,6 +47,7 @@ enum {
     TRISK_EINVAL = -1,  /* bad argument or table (the reference's ArgumentError) */
     TRISK_EHIP = -2,    /* a HIP runtime error */
     TRISK_ENOMEM = -3,  /* device allocation failed */
+    TRISK_ENONFINITE = -4  /* the state check found NaN/Inf in y or a vertex height h_v <= 0 */
 };
 
 /* pv_flux keyword (TRiSKShallowWater.jl:94-98) */
@@ -125,6 +126,9 @@ typedef struct trisk_handle trisk_handle;
 
 /* Thread-local text of the last error. */
 const char* trisk_last_error(void);
+/* Text of the last error of a call on handle h (kept in the handle until the next error
+ * on it; "" if none). */
+const char* trisk_handle_last_error(const trisk_handle* h);
 
 /* Host-only validation of a descriptor (every index in range, table widths), and
  * whether TRISK_PATH_PAIR is available (*recomputed); the reason is in
@@ -167,6 +171,16 @@ int trisk_set_graphs(trisk_handle* h, int32_t on);
 /* out[0] = total_mass, out[1] = total_energy, out[2] = potential_enstrophy, summed with
  * Neumaier compensation in a fixed order (bitwise reproducible).  Synchronous. */
 int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void* stream);
+
+/* Failure detection (no counterpart in the reference, which has no NaN checks).
+ * trisk_check_state: counts[0] = entries of u that are NaN or Inf, counts[1] = the same for
+ * h, counts[2] = vertices whose height h_v (the divisor of q_v = (zeta_v + f_v) / h_v,
+ * TRiSKShallowWater.jl:258) is not > 0 or is NaN.  Synchronous.
+ * trisk_set_finite_check(h, 1): trisk_diagnostics also runs this check and returns
+ * TRISK_ENONFINITE (out[] still filled) when any count is nonzero; default 0, the
+ * reference's behaviour of returning NaN sums. */
+int trisk_check_state(trisk_handle* h, const double* y_dev, int64_t counts[3], void* stream);
+int trisk_set_finite_check(trisk_handle* h, int32_t on);
 
 /* kinetic_energy!(m, u, h): K and B at the cells (device, internal order). */
 int trisk_kinetic_energy(trisk_handle* h, const double* y_dev, double* K_dev, double* B_dev, void* stream);

@@ -353,6 +353,46 @@ def test_graph_replay_equals_direct_launches(precision):
     assert np.array_equal(gather_state(m, b), ref)
 
 
+@pytest.mark.parametrize("model", ["trisk", "icon"])
+def test_state_check_reports_nan_and_zero_vertex_height(model):
+    """Failure detection: a NaN planted in u, an Inf in h and a zero-height cell (whose
+    vertex heights h_v, the divisor of q_v at TRiSKShallowWater.jl:258, then vanish) are
+    counted by check_state; with the finite check on, diagnostics raises; a clean state
+    passes.  The conditions are set from the host; every kernel runs to its end."""
+    from dmplex_amd import ICONShallowWater, create_spherical_mesh
+    from dmplex_amd._device import NonFiniteStateError
+
+    say(f"state check {model}")
+    if model == "trisk":
+        dm = mesh(3)
+        m = TRiSKShallowWater(dm, centers=dm.generators, **earth_parameters())
+    else:
+        m = ICONShallowWater(create_spherical_mesh(3), **earth_parameters())
+    y = seeded_state(m.grid)
+    ne, nc = m.grid.num_edges(), m.grid.num_cells()
+    assert m.check_state(y) == (0, 0, 0)
+    m.set_finite_check(True)
+    diagnostics(m, m.upload_state(y))  # a finite state passes
+    bad = y.copy()
+    bad[5] = np.nan
+    bad[ne + 7] = np.inf
+    bad[ne + 8] = -np.inf
+    nu, nh, nhv = m.check_state(bad)
+    assert (nu, nh) == (1, 2) and nhv > 0  # h_v of the vertices around the infinite cells
+    with pytest.raises(NonFiniteStateError, match="1 non-finite u, 2 non-finite h"):
+        diagnostics(m, m.upload_state(bad))
+    assert "non-finite" in m.last_error()
+    dry = y.copy()
+    dry[ne:] = 0.0  # no fluid: every h_v is zero
+    assert m.check_state(dry) == (0, 0, m.grid.num_vertices())
+    with pytest.raises(NonFiniteStateError, match="vertices with h_v"):
+        diagnostics(m, m.upload_state(dry))
+    m.set_finite_check(False)
+    mass, _, _ = diagnostics(m, m.upload_state(bad))  # the reference's behaviour: NaN sums
+    assert not np.isfinite(mass)
+    assert nc > 0
+
+
 def test_invalid_tables_are_rejected():
     say("invalid tables")
     dm = mesh(2)

@@ -83,6 +83,19 @@ def test_anderson_matches_plain_lloyd():
     assert np.max(np.linalg.norm(y - z, axis=1)) < 1e-7
 
 
+def test_lloyd_warns_when_it_stops_short():
+    """The reference warns when lloyd! reaches max_iterations (Icosahedral.jl:183)."""
+    x, tri, _ = _icosahedral(3, False)
+    with pytest.warns(RuntimeWarning, match="did not converge in 5 iterations"):
+        assert lloyd_(x, tri, fixed=12, max_iterations=5) == 5
+    y, _, _ = _icosahedral(3, False)
+    import warnings
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert lloyd_(y, tri, fixed=12) < 10_000  # converged: no warning
+
+
 def test_scvt_improves_trisk_tangential_reconstruction():
     def tangential_error(centroidal):
         tri = create_spherical_mesh(4, centroidal=centroidal)
