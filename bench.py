@@ -38,6 +38,55 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def per_s(x, ms):
+    """x per second over ``ms`` milliseconds; None when the time is missing or not positive."""
+    return None if ms is None or not ms > 0 else x / (ms / 1e3)
+
+
+def frac_of_peak(x, ms):
+    r = per_s(x, ms)
+    return None if r is None else r / HBM_PEAK
+
+
+def roofline_of(stats, traffic=None, key=None):
+    """The roofline entry of the dominant kernel of ``stats`` (largest total time, or
+    ``key``): its algorithmic bytes per launch over its mean launch time, against the HBM
+    peak; ``traffic(name)`` gives the measured HBM bytes per launch, if known."""
+    dom = max(stats, key=key or (lambda k: k["avg_ms"] * k["launches"]))
+    bw = per_s(dom["bytes"], dom["avg_ms"])
+    t = traffic(dom["name"]) if traffic else None
+    return dict(bound="hbm", kernel=dom["name"], achieved=None if bw is None else bw / 1e9, peak=HBM_PEAK / 1e9,
+                unit="GB/s", frac=None if bw is None else bw / HBM_PEAK, traffic=t,
+                traffic_frac=None if t is None else frac_of_peak(t, dom["avg_ms"]),
+                avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"])
+
+
+def spherical_mesh(level, scvt, cache_dir="", log_fn=None):
+    """create_spherical_mesh(level; centroidal=scvt) with Lloyd on the GPU, kept in
+    ``cache_dir`` (an .npz of the vertices and triangles) so that later runs in the same
+    place (profiler passes) reuse the same mesh instead of repeating Lloyd."""
+    from dmplex_amd import create_spherical_mesh, plex_from_polygons
+
+    f = Path(cache_dir) / f"{'scvt' if scvt else 'bisection'}_level{level}.npz" if cache_dir else None
+    if f is not None and f.exists():
+        d = np.load(f)
+        x, tri = d["x"], d["tri"]
+        if len(x) == 10 * 4 ** level + 2 and len(tri) == 20 * 4 ** level:
+            return plex_from_polygons((np.arange(0, 3 * len(tri) + 1, 3, dtype=np.int64), tri.reshape(-1)), x)
+    dm = create_spherical_mesh(level, centroidal=scvt, device=0, log=log_fn)
+    if f is not None:
+        f.parent.mkdir(parents=True, exist_ok=True)
+        np.savez(f, x=dm.coordinates, tri=dm.cell_vertex_table())
+    return dm
+
+
+def hexagon_mesh(level, scvt, cache_dir="", log_fn=None):
+    """create_hexagon_mesh(level; centroidal=scvt): the Voronoi dual of spherical_mesh."""
+    from dmplex_amd import create_hexagon_mesh
+
+    return create_hexagon_mesh(level, primal=spherical_mesh(level, scvt, cache_dir, log_fn))
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -57,7 +106,9 @@ def parse():
     p.add_argument("--grid-dual", action="store_true",
                    help="with --grid-file: its dual mesh (the hexagons of an ICON file)")
     p.add_argument("--no-scvt", action="store_true",
-                   help="skip Lloyd (same topology and kernels; for short profiling runs only)")
+                   help="skip Lloyd (same topology and kernels; for short tests of the script only)")
+    p.add_argument("--mesh-cache", default="", metavar="DIR",
+                   help="keep the generated meshes in DIR and reuse them (profiler passes over the SCVT mesh)")
     p.add_argument("--accuracy", type=int, default=0, metavar="DAYS",
                    help="instead of timing: run FP64 (bit-exact), FP64 cell-pair and FP32-bulk for DAYS "
                         "simulated days from the same state and report their errors against FP64 after "
@@ -168,29 +219,33 @@ def run_gpu(grid, y0, precision, args, params, path=None, trisk_cell=True, profi
     ms, wall = timed(args.steps)
     mass, energy, _ = diagnostics(m, y)
     nc = grid.num_cells()
-    value = nc * args.steps / (ms / 1e3)
+    step_ms = ms / args.steps
+    value = per_s(nc * args.steps, ms)
+    # the kernels' own algorithmic bytes (trisk_kernel_bytes) per step, and the HBM bytes the
+    # counters measured for the same launches (profiles/traffic.json, level-9 workload only)
     kbytes = sum(k["bytes"] * k["launches"] for k in stats) / args.steps if stats else None
-    out = dict(value=value, ms_per_step=ms / args.steps, wall_s=wall, path=m.path_info()[1],
-               kernel_model_bytes_per_cell_step=None if kbytes is None else kbytes / nc,
-               hbm_frac_kernel_model=None if kbytes is None else kbytes / (ms / args.steps / 1e3) / HBM_PEAK,
+    tkey = "fp64_pair" if precision == "fp64" and path == "pair" else precision
+    traffic = (lambda name: load_traffic(name, tkey)) if profiled else None
+    mbytes = None
+    if stats and traffic and all(traffic(k["name"]) for k in stats):
+        mbytes = sum(traffic(k["name"]) * k["launches"] for k in stats) / args.steps
+    out = dict(value=value, ms_per_step=step_ms, wall_s=wall, path=m.path_info()[1],
+               kernel_bytes_per_cell_step=None if kbytes is None else kbytes / nc,
+               hbm_frac=None if kbytes is None else frac_of_peak(kbytes, step_ms),
+               measured_bytes_per_cell_step=None if mbytes is None else mbytes / nc,
+               hbm_frac_measured=None if mbytes is None else frac_of_peak(mbytes, step_ms),
+               reference_layout_bytes_per_cell_step=MODEL_BYTES[precision] if trisk_cell else None,
+               hbm_frac_reference_layout_effective=MODEL_BYTES[precision] * value / HBM_PEAK
+               if trisk_cell and value else None,
                graphs=not args.no_graphs,
                ms_per_step_direct_with_events=None if ms_direct is None else ms_direct / args.steps,
-               hbm_frac_model=MODEL_BYTES[precision] * value / HBM_PEAK if trisk_cell else None, path_name=path,
-               mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0, kernels=stats)
+               path_name=path, mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0,
+               kernels=stats)
     if stats:
-        dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
-        achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
-        out["roofline"] = dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                               frac=achieved * 1e9 / HBM_PEAK,
-                               traffic=load_traffic(dom["name"], "fp64_pair" if precision == "fp64" and
-                                                    path == "pair" else precision) if profiled else None,
-                               avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"])
-        t = out["roofline"]["traffic"]
-        # the measured HBM bytes of a launch over its duration: the bandwidth the kernel draws
-        out["roofline"]["traffic_frac"] = None if t is None else t / (dom["avg_ms"] / 1e3) / HBM_PEAK
-    frac = out["hbm_frac_model"] if trisk_cell else out["hbm_frac_kernel_model"]
-    log(f"[bench] {precision}: {out['ms_per_step']:.3f} ms/step, {value:.3e} cell-steps/s, "
-        f"model HBM fraction {'n/a' if frac is None else f'{frac:.3f}'}")
+        out["roofline"] = roofline_of(stats, traffic)
+    frac, mfrac = out["hbm_frac"], out["hbm_frac_measured"]
+    log(f"[bench] {precision}: {step_ms:.3f} ms/step, {value:.3e} cell-steps/s, HBM fraction: kernel bytes "
+        f"{'n/a' if frac is None else f'{frac:.3f}'}, measured {'n/a' if mfrac is None else f'{mfrac:.3f}'}")
     del m
     torch.cuda.empty_cache()
     return out
@@ -206,12 +261,12 @@ def run_icon(args):
     with ICON's fourth-order damping (divdamp_fac 0.0025, nu = 0.0025 mean-area^2 / dt)."""
     import torch
 
-    from dmplex_amd import HIPBackend, ICONShallowWater, create_spherical_mesh, diagnostics, rk4_steps_
+    from dmplex_amd import HIPBackend, ICONShallowWater, diagnostics, rk4_steps_
     from dmplex_amd import _device as D
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
     t = time.perf_counter()
-    dm = create_spherical_mesh(args.icon_level, centroidal=not args.no_scvt, device=0)
+    dm = spherical_mesh(args.icon_level, not args.no_scvt, args.mesh_cache)
     log(f"[bench] icon: SCVT triangle mesh level {args.icon_level} ({dm.nc} cells) in {time.perf_counter() - t:.1f} s")
     dt, steps = args.icon_dt, args.icon_steps
     nu = 0.0025 * (4 * np.pi / dm.nc) ** 2 / dt
@@ -251,25 +306,20 @@ def run_icon(args):
         ms = timed(steps)
         mass, energy, _ = diagnostics(m, y)
         kbytes = sum(k["bytes"] * k["launches"] for k in stats) / steps
-        dom = max(stats, key=lambda k: k["avg_ms"] * k["launches"])
-        achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
-        r = dict(precision=precision, value=dm.nc * steps / (ms / 1e3), unit="cell-steps/s", ms_per_step=ms / steps,
+        tkey = ("icon" if precision == "fp64" else "icon_fp32_bulk") + ("_damping" if damping else "")
+        traffic = (lambda name: load_traffic(name, tkey)) if args.icon_level == 9 and not args.no_scvt else None
+        step_ms = ms / steps
+        r = dict(precision=precision, value=per_s(dm.nc * steps, ms), unit="cell-steps/s", ms_per_step=step_ms,
                  ms_per_step_direct_with_events=ms_direct / steps,
-                 speedup_vs_reference_a100=ICON_REFERENCE_MS / (ms / steps),
-                 kernel_model_bytes_per_cell_step=kbytes / dm.nc,
-                 hbm_frac_kernel_model=kbytes / (ms / steps / 1e3) / HBM_PEAK,
+                 speedup_vs_reference_a100=ICON_REFERENCE_MS / step_ms if step_ms > 0 else None,
+                 kernel_bytes_per_cell_step=kbytes / dm.nc, hbm_frac=frac_of_peak(kbytes, step_ms),
                  mass_drift=(mass - mass0) / mass0, energy_drift=(energy - energy0) / energy0,
-                 roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                               frac=achieved * 1e9 / HBM_PEAK,
-                               traffic=load_traffic(dom["name"], ("icon" if precision == "fp64" else "icon_fp32_bulk") +
-                                                    ("_damping" if damping else "")) if args.icon_level == 9 else None,
-                               avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
-                 kernels=stats)
+                 roofline=roofline_of(stats, traffic), kernels=stats)
         if damping:
             r["divergence_damping_nu"] = nu
         out[key] = r
         log(f"[bench] icon {key}: {r['ms_per_step']:.3f} ms/step ({r['speedup_vs_reference_a100']:.2f}x the "
-            f"reference's A100), kernel-model HBM fraction {r['hbm_frac_kernel_model']:.3f}")
+            f"reference's A100), HBM fraction (kernel bytes) {r['hbm_frac']:.3f}")
         del m
         torch.cuda.empty_cache()
     return out
@@ -284,11 +334,11 @@ def run_mpas(args):
     reference publishes 4.5 ms of vn_tendency per step (two calls) on one A100 there."""
     import torch
 
-    from dmplex_amd import TRiSKGrid, create_hexagon_mesh
+    from dmplex_amd import TRiSKGrid
     from dmplex_amd.mpas_ops import MPASOperators, vertical_grid
 
     t = time.perf_counter()
-    dm = create_hexagon_mesh(args.mpas_level, centroidal=not args.no_scvt, device=0)
+    dm = hexagon_mesh(args.mpas_level, not args.no_scvt, args.mesh_cache)
     g = TRiSKGrid(dm, centers=dm.generators)
     v = vertical_grid(30, 30000.0)
     m = MPASOperators(g, v)
@@ -318,20 +368,15 @@ def run_mpas(args):
     ms = t0.elapsed_time(t1) / args.mpas_calls
     vn_ms = next(s["avg_ms"] for s in stats if s["name"] == "mpas_vn_tendency_kernel")
     kbytes = sum(s["bytes"] for s in stats)
-    dom = max(stats, key=lambda k: k["avg_ms"])
-    achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
+    traffic = (lambda name: load_traffic(name, "mpas")) if args.mpas_level == 7 and not args.no_scvt else None
     out = dict(workload=f"MPAS-Gassmann slow_tendencies! (ICON-IAP PV, mass-flux vertical coupling, five-band "
                         f"solver's K_w order), SCVT hexagon {args.mpas_level} ({nc} cells, {ne} edges) x 30 levels, "
                         f"FP64, flat ground", ms_per_call=ms, unit="ms per slow_tendencies! call",
                vn_tendency_ms_per_call=vn_ms, vn_tendency_ms_per_iap_step=2 * vn_ms,
                reference_a100_vn_tendency_ms_per_step=MPAS_REFERENCE_VN_MS,
-               vn_tendency_speedup_vs_reference_a100=MPAS_REFERENCE_VN_MS / (2 * vn_ms),
-               hbm_frac_kernel_model=kbytes / (ms / 1e3) / HBM_PEAK,
-               roofline=dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                             frac=achieved * 1e9 / HBM_PEAK,
-                             traffic=load_traffic(dom["name"], "mpas") if args.mpas_level == 7 else None,
-                             avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
-               kernels=stats)
+               vn_tendency_speedup_vs_reference_a100=MPAS_REFERENCE_VN_MS / (2 * vn_ms) if vn_ms > 0 else None,
+               hbm_frac=frac_of_peak(kbytes, ms),
+               roofline=roofline_of(stats, traffic, key=lambda k: k["avg_ms"]), kernels=stats)
     log(f"[bench] mpas: {ms:.3f} ms per call, vn_tendency {vn_ms:.3f} ms ({out['vn_tendency_speedup_vs_reference_a100']:.1f}x "
         f"the reference's A100 per step)")
     del m
@@ -348,7 +393,7 @@ def run_ensemble(args):
     launch per kernel; beside it the same members stepped model by model."""
     import torch
 
-    from dmplex_amd import HIPBackend, ModelTables, TRiSKEnsemble, TRiSKShallowWater, create_hexagon_mesh, rk4_steps_
+    from dmplex_amd import HIPBackend, ModelTables, TRiSKEnsemble, TRiSKShallowWater, rk4_steps_
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
     params = earth_parameters()
@@ -356,7 +401,7 @@ def run_ensemble(args):
     t = time.perf_counter()
     tabs = []
     for n in ENS_LEVELS:
-        dm = create_hexagon_mesh(n, centroidal=True, device=0)
+        dm = hexagon_mesh(n, True, args.mesh_cache)
         tabs.append(ModelTables(dm, centers=dm.generators, **params))
     ens = TRiSKEnsemble(tabs, [M] * len(tabs), backend=HIPBackend(0, "fp32_bulk"))
     y = ens.zeros()
@@ -399,10 +444,8 @@ def run_ensemble(args):
     ms = timed(lambda: ens.rk4_steps_(y, ENS_DTS, K))
     d1 = ens.diagnostics(y)
     cells = ens.cells()
-    value = cells * K / (ms / 1e3)
+    value = per_s(cells * K, ms)
     step_bytes = 4 * sum(k["bytes"] for k in stats)
-    dom = max(stats, key=lambda k: k["avg_ms"])
-    achieved = dom["bytes"] / (dom["avg_ms"] / 1e3) / 1e9
     log(f"[ensemble] {ms / K:.3f} ms/step, {value:.3e} cell-member-steps/s")
     out = {
         "workload": f"{len(seeds)} members, {M} each on SCVT levels {list(ENS_LEVELS)}, dt {list(ENS_DTS)} s, "
@@ -410,9 +453,13 @@ def run_ensemble(args):
         "unit": "cell-member-steps/s", "value": value, "ms_per_step": ms / K, "steps": K,
         "ms_per_step_direct_with_events": ms_direct / K, "graphs": True,
         "cell_members": cells, "schedule": sched, "autotune_kernel_ms": tune_ms,
-        "model_bytes_per_step": step_bytes, "hbm_frac_model": step_bytes / (ms / K / 1e3) / HBM_PEAK,
-        "roofline": dict(bound="hbm", kernel=dom["name"], achieved=achieved, peak=HBM_PEAK / 1e9, unit="GB/s",
-                         frac=achieved * 1e9 / HBM_PEAK, avg_ms=dom["avg_ms"], algorithmic_bytes=dom["bytes"]),
+        # one launch per kernel for every member of every level; an RK stage is three such
+        # launches (state, cell, edge), each of which gathers what the one before wrote
+        # anywhere on the meshes, so a stage is not one persistent launch
+        "launches_per_rk_stage": 3,
+        "persistent_kernels": [k for k, sc in zip(("state", "cell", "edge"), sched) if sc.get("blocks_per_cu")],
+        "kernel_bytes_per_step": step_bytes, "hbm_frac": frac_of_peak(step_bytes, ms / K),
+        "roofline": roofline_of(stats, key=lambda k: k["avg_ms"]),
         "kernels": stats,
         "max_member_mass_drift": float(np.max(np.abs((d1[:, 0] - d0[:, 0]) / d0[:, 0]))),
     }
@@ -430,12 +477,12 @@ def run_ensemble(args):
         loop_ms += timed(lambda: [rk4_steps_(ys, ENS_DTS[li], K, m) for ys in states])
         del m, states
     torch.cuda.empty_cache()
-    loop_value = cells * K / (loop_ms / 1e3)
+    loop_value = per_s(cells * K, loop_ms)
     log(f"[ensemble] member-by-member loop: {loop_ms / K:.3f} ms/step, {loop_value:.3e} cell-member-steps/s")
     out["member_loop"] = {"value": loop_value, "ms_per_step": loop_ms / K,
                           "how": "one FP32-bulk cell-pair model per level, each member stepped K steps in turn "
                                  "(graph replay)"}
-    out["speedup_vs_member_loop"] = value / loop_value
+    out["speedup_vs_member_loop"] = value / loop_value if value and loop_value else None
     return out
 
 
@@ -445,11 +492,10 @@ def run_adjoint(args):
     plain forward run and four central-difference checks (eps = 1e-6)."""
     import torch
 
-    from dmplex_amd import (HIPBackend, TRiSKShallowWater, create_hexagon_mesh, height_misfit, rk4_adjoint,
-                            rk4_steps, rk4_steps_)
+    from dmplex_amd import HIPBackend, TRiSKShallowWater, height_misfit, rk4_adjoint, rk4_steps, rk4_steps_
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
-    dm = create_hexagon_mesh(7, centroidal=True, device=0)
+    dm = hexagon_mesh(7, True, args.mesh_cache)
     params = earth_parameters()
     dt, N, ck = 150.0, 40, 8
     out = {"workload": f"SCVT level 7 ({dm.nc} cells), J = sum A (h_T - h*)^2 after {N} RK4 steps of {dt} s, "
@@ -654,7 +700,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from dmplex_amd import TRiSKGrid, create_hexagon_mesh
+    from dmplex_amd import TRiSKGrid
     from dmplex_amd.synthetic import earth_parameters, seeded_state
 
     if args.calibrate_copy:
@@ -690,8 +736,7 @@ def main():
         workload = f"grid file {os.path.basename(args.grid_file)}{' (dual)' if args.grid_dual else ''}"
     else:
         log(f"[bench] generating SCVT hexagon mesh level {args.level}")
-        dm = create_hexagon_mesh(args.level, centroidal=not args.no_scvt, device=0,
-                                 log=lambda m: log(f"[bench] {m}"))
+        dm = hexagon_mesh(args.level, not args.no_scvt, args.mesh_cache, log_fn=lambda m: log(f"[bench] {m}"))
         grid = TRiSKGrid(dm, centers=dm.generators)
         workload = f"SCVT hexagon level {args.level}"
     params = dict(earth_parameters(), pv_flux=args.pv_flux, kinetic_energy=args.kinetic_energy)
@@ -726,9 +771,15 @@ def main():
         "config": {"workload": f"{workload}, {args.pv_flux} PV flux, {args.kinetic_energy} KE, RK4 dt={args.dt} s",
                    "cells": grid.num_cells(), "edges": grid.num_edges(), "vertices": grid.num_vertices(),
                    "precision": head_p,
-                   "model_bytes_per_cell_step": MODEL_BYTES[head_p] if trisk_cell
-                   else head["kernel_model_bytes_per_cell_step"]},
-        "hbm_frac_model": head["hbm_frac_model"] if trisk_cell else head["hbm_frac_kernel_model"],
+                   "kernel_bytes_per_cell_step": head["kernel_bytes_per_cell_step"],
+                   "reference_layout_bytes_per_cell_step": head["reference_layout_bytes_per_cell_step"]},
+        # % of HBM peak: the kernels' own algorithmic bytes (each element a launch must touch,
+        # counted once) per step over the step time; beside it the HBM bytes the counters
+        # measured for the same launches, and the effective rate of the reference's table
+        # layout (bytes the reference's kernels would move for the same cell-steps)
+        "hbm_frac": head["hbm_frac"],
+        "hbm_frac_measured": head["hbm_frac_measured"],
+        "hbm_frac_reference_layout_effective": head["hbm_frac_reference_layout_effective"],
         "roofline": head.get("roofline"),
         "cpu_baseline": cpu,
     }
