@@ -7,6 +7,7 @@ no fallback implementation.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -58,7 +59,8 @@ def host() -> C.CDLL:
     """The host setup library (loaded once)."""
     global _host
     if _host is None:
-        path = LIB_DIR / "libdmplex_host.so"
+        # DMPLEX_HOST_LIBRARY selects another build of the same library (the sanitizer build)
+        path = Path(os.environ.get("DMPLEX_HOST_LIBRARY") or LIB_DIR / "libdmplex_host.so")
         if not path.exists():
             raise NativeError(f"{path} is missing: run `python -m dmplex_amd.build`")
         lib = C.CDLL(str(path))

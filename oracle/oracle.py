@@ -8,6 +8,7 @@ as the checker; the product (``dmplex_amd``) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -55,7 +56,8 @@ def build() -> Path:
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        path = HERE / "liboracle.so"
+        # DMPLEX_ORACLE_LIBRARY selects another build of the oracle (the sanitizer build)
+        path = Path(os.environ.get("DMPLEX_ORACLE_LIBRARY") or HERE / "liboracle.so")
         if not path.exists():
             build()
         d = C.CDLL(str(path))

@@ -174,8 +174,9 @@ def test_reader_errors(tmp_path):
 @pytest.mark.parametrize("fmt", _formats())
 def test_reader_survives_mutated_files(tmp_path, fmt):
     """Random byte changes and truncations of a valid file: the reader either reads every
-    variable or raises ValueError/OSError; it never crashes or reads outside the file.  (The
-    host library also ran this over 2700 mutated files under AddressSanitizer, clean.)"""
+    variable or raises ValueError/OSError; it never crashes or reads outside the file.
+    ``make -C oracle sanitize-test`` runs this test (with the other host-library tests)
+    against AddressSanitizer/UBSan builds of the host library and the oracle."""
     import random
 
     rng = random.Random(7)
