@@ -72,6 +72,7 @@ SIGNATURES = {
     "trisk_rk4_steps": [vp, vp, C.c_double, i64, vp],
     "trisk_diagnostics": [vp, vp, f64p, vp],
     "trisk_check_state": [vp, vp, i64p, vp],
+    "trisk_diagnostics_noalloc": [vp, vp, vp, vp],
     "trisk_set_finite_check": [vp, C.c_int32],
     "trisk_kinetic_energy": [vp, vp, vp, vp, vp],
     "trisk_vertex_pv": [vp, vp, vp, vp, vp, vp],

@@ -413,6 +413,14 @@ int icon_vertex_pv(trisk_handle* H, const double* y, double* qv, double* zv, dou
     return 0;
 }
 
+int icon_diagnostic_pass(trisk_handle* H, const double* y, const double** K, hipStream_t s) {
+    int rc;
+    if ((rc = icon_kinetic_pass(H, y, s))) return rc;
+    if ((rc = icon_vertex_pv(H, y, nullptr, nullptr, nullptr, s))) return rc;
+    *K = iparams(H).K;
+    return 0;
+}
+
 int icon_diagnostics(trisk_handle* H, const double* y, double out[3], hipStream_t s) {
     const Layout& L = H->L;
     IconParams& p = iparams(H);
@@ -648,6 +656,7 @@ int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_han
         delete H;
         return set_err(TRISK_EHIP, "no such HIP device");
     }
+    noalloc_setup(o.device);
     int rc = icon_setup(H);
     if (rc) {
         for (auto& b : H->owned) (void)hipFree(b.p);

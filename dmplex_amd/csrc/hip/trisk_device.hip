@@ -730,6 +730,64 @@ static int diagnostic_pass(trisk_handle* H, const double* y, double* K_out, doub
 }
 
 
+// ---------------------------------------------------------------------------------
+// Static scratch of trisk_diagnostics_noalloc: the block partials (sum, compensation) of
+// the three diagnostic sums.  Every GPU the library runs on holds one copy until the
+// process ends; calls that overlap on one GPU share it and race.
+
+constexpr int kNoallocTerms = 3;
+constexpr int kNoallocMaxBlocks = 1024;
+constexpr int kNoallocDoubles = kNoallocTerms * 2 * kNoallocMaxBlocks;  // 48 KiB
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static bool g_scratch_done[kMaxDevices];
+static double* g_scratch_ptr[kMaxDevices];     // the device address of `scratch` on each GPU
+static hipError_t g_scratch_err[kMaxDevices];  // the result of its lookup
+
+// Looks up the address of `scratch` on `device`, which must be current: once per device,
+// at handle creation.  A failed lookup is kept and reported by trisk_diagnostics_noalloc.
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_done[device]) return;
+    void* ptr = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? static_cast<double*>(ptr) : nullptr;
+    g_scratch_done[device] = true;
+}
+
+static double* noalloc_scratch(int device, hipError_t* err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kMaxDevices || !g_scratch_done[device]) {
+        *err = hipErrorNotInitialized;
+        return nullptr;
+    }
+    *err = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
+
+// The host combination of reduce() on the device: a fixed-order Neumaier sum of the block
+// partials of each term, bitwise the same as trisk_diagnostics' (FP contraction is off).
+static __global__ __launch_bounds__(64) void noalloc_combine_kernel(int blocks, const double* __restrict__ part,
+                                                                  double* __restrict__ out) {
+    const int term = threadIdx.x;
+    if (term >= kNoallocTerms) return;
+    const double* p = part + size_t(term) * 2 * blocks;
+    double S = 0.0, Cc = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        double x = p[2 * b], t = S + x;
+        if (fabs(S) >= fabs(x))
+            Cc += (S - t) + x;
+        else
+            Cc += (x - t) + S;
+        S = t;
+        Cc += p[2 * b + 1];
+    }
+    out[term] = term == 2 ? (S + Cc) / 2 : S + Cc;
+}
+
 // Non-finite entries of u and of h in the state y, and vertices whose height h_v (of the
 // last diagnostic pass, in diag_hv) is not > 0 or NaN: the divisor of q_v.  Synchronous.
 int state_counts(trisk_handle* H, const double* y, int64_t counts[3], hipStream_t s) {
@@ -821,6 +879,7 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
         delete H;
         return set_err(TRISK_EHIP, "no such HIP device");
     }
+    noalloc_setup(o.device);
     int rc = o.precision == TRISK_FP64 ? setup_params<Fp64>(H) : setup_params<Fp32Bulk>(H);
     if (rc) {
         for (auto& b : H->owned) (void)hipFree(b.p);
@@ -1074,6 +1133,40 @@ int trisk_check_state(trisk_handle* h, const double* y_dev, int64_t counts[3], v
                        : trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);
     if (rc) return rc;
     return state_counts(h, y_dev, counts, s);
+}
+
+int trisk_diagnostics_noalloc(trisk_handle* h, const double* y_dev, double* out_dev, void* stream) {
+    CHECK_HANDLE(h);
+    if (!y_dev || !out_dev) return set_err(TRISK_EINVAL, "null pointer");
+    const int blocks = h->reduce_blocks;
+    if (blocks < 1 || blocks > kNoallocMaxBlocks)
+        return set_err(TRISK_EINVAL, "trisk_diagnostics_noalloc: " + std::to_string(kNoallocTerms * 2 * blocks) +
+                                         " doubles of partial sums exceed the static scratch of " +
+                                         std::to_string(kNoallocDoubles));
+    hipError_t serr;
+    double* sp = noalloc_scratch(h->device, &serr);
+    if (!sp)
+        return set_err(TRISK_EHIP, std::string("trisk_diagnostics_noalloc: no scratch address on device ") +
+                                       std::to_string(h->device) + ": " + hipGetErrorString(serr));
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Layout& L = h->L;
+    // the diagnostic pass (K, q_v, h_v into the handle's work vectors), as trisk_diagnostics
+    const double* K = h->diag_K;
+    int rc = h->L.icon ? icon_diagnostic_pass(h, y_dev, &K, s)
+                       : trisk_vertex_pv(h, y_dev, nullptr, nullptr, nullptr, stream);
+    if (rc) return rc;
+    const double* area_c = h->precision == TRISK_FP64 || L.icon ? h->p64.area_c : h->p32.area_c;
+    const double* area_v = h->precision == TRISK_FP64 || L.icon ? h->p64.area_v : h->p32.area_v;
+    const dim3 grid{unsigned(blocks)}, block{unsigned(kBlock)};
+    hipLaunchKernelGGL(reduce_kernel, grid, block, 0, s, 0, L.nc, area_c, y_dev + L.ne, nullptr, L.gravity, sp);
+    hipLaunchKernelGGL(reduce_kernel, grid, block, 0, s, 1, L.nc, area_c, y_dev + L.ne, K, L.gravity,
+                       sp + 2 * blocks);
+    hipLaunchKernelGGL(reduce_kernel, grid, block, 0, s, 2, L.nv, area_v, h->diag_hv, h->diag_qv, L.gravity,
+                       sp + 4 * blocks);
+    hipLaunchKernelGGL(noalloc_combine_kernel, dim3(1), dim3(64), 0, s, blocks, sp, out_dev);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 int trisk_relative_vorticity(trisk_handle* h, const double* y_dev, double* zeta_dev, void* stream) {

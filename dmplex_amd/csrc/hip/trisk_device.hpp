@@ -771,6 +771,11 @@ inline int reduce(trisk_handle* H, int term, int64_t n, const double* area, cons
 int icon_tendency(trisk_handle* H, const double* y, double* dy, hipStream_t s);
 int icon_run_steps(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s);
 int icon_diagnostics(trisk_handle* H, const double* y, double out[3], hipStream_t s);
+// K, q_v and h_v of the state y (the diagnostic pass); *K receives the device cell K.
+int icon_diagnostic_pass(trisk_handle* H, const double* y, const double** K, hipStream_t s);
+// Per-device address of the library's static scratch (trisk_diagnostics_noalloc), looked
+// up once per device with that device current (trisk_device.hip).
+void noalloc_setup(int device);
 int icon_vertex_pv(trisk_handle* H, const double* y, double* qv, double* zv, double* hv, hipStream_t s);
 int icon_relative_vorticity(trisk_handle* H, const double* y, double* zeta, hipStream_t s);
 int icon_tangential_velocity(trisk_handle* H, const double* u, double* vt, hipStream_t s);

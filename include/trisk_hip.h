@@ -172,6 +172,17 @@ int trisk_set_graphs(trisk_handle* h, int32_t on);
  * Neumaier compensation in a fixed order (bitwise reproducible).  Synchronous. */
 int trisk_diagnostics(trisk_handle* h, const double* y_dev, double out[3], void* stream);
 
+/* trisk_diagnostics without allocation or synchronisation.  No counterpart in the
+ * reference: it is for callers that cannot allocate or wait (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  out_dev (device memory, 3
+ * doubles) receives the values trisk_diagnostics returns, bitwise, stream-ordered on
+ * `stream`.  The block partials live in a static device array of the library (48 KiB per
+ * GPU, held until the process ends): the handle's reduction may use at most 1024 blocks
+ * (TRISK_EINVAL otherwise).  All calls on one GPU share that array, so calls that overlap
+ * on one GPU (other streams, other threads) race.  No hipMalloc, hipFree or host
+ * allocation. */
+int trisk_diagnostics_noalloc(trisk_handle* h, const double* y_dev, double* out_dev, void* stream);
+
 /* Failure detection (no counterpart in the reference, which has no NaN checks).
  * trisk_check_state: counts[0] = entries of u that are NaN or Inf, counts[1] = the same for
  * h, counts[2] = vertices whose height h_v (the divisor of q_v = (zeta_v + f_v) / h_v,
