@@ -99,3 +99,41 @@ def test_options_and_schedule_structs_match_the_header():
     assert _struct_fields(text, "trisk_options") == [f[0] for f in D.TriskOptions._fields_]
     assert _struct_fields(text, "trisk_schedule") == [f[0] for f in TriskSchedule._fields_]
     assert C.sizeof(D.TriskOptions) == 32 and C.sizeof(TriskSchedule) == 16
+
+
+def _field_sizes(desc):
+    nc, ne, nv, kc, kv, kt, kg = desc.nc, desc.ne, desc.nv, desc.kc, desc.kv, desc.kt, desc.kg
+    return dict(cell_edges=kc * nc, divergence=kc * nc, kinetic=kc * nc, edge_cells=2 * ne, edge_vertices=2 * ne,
+                edge_height=2 * ne, inverse_dual=ne, vertex_edges=kv * nv, curl=kv * nv, vertex_cells=kv * nv,
+                vertex_height=kv * nv, coriolis=nv, trisk_edges=kt * ne, trisk=kt * ne, cell_areas=nc,
+                vertex_areas=nv, cell_vertices=kc * nc, vk_kites=kc * nc, vk_weights=kv * nv, vk_height=2 * ne,
+                g_edges=kg * ne, g_pv_edges=kg * ne, g_weights=kg * ne, g_rhombus=2 * ne, g_coriolis=ne,
+                cell_points=3 * nc, primal_lengths=ne, dual_lengths=ne, cell_kites=kc * nc)
+
+
+@pytest.mark.parametrize("pv_flux,kinetic_energy", [("trisk", "cell"), ("gassmann", "vertex"),
+                                                    ("gassmann_mean", "mixed")])
+def test_julia_extension_descriptor(pv_flux, kinetic_energy):
+    """The descriptor the Julia extension builds from the reference struct (1-based tables,
+    0-based copies, every Gassmann and vertex-kinetic-energy table) passes the library's
+    host validation and carries the same data, field by field, as the descriptor of the
+    Python binding; the cell-pair path is offered exactly for the TRiSK flux with cell KE."""
+    from julia_descriptor import descriptor, julia_struct
+
+    dm = create_hexagon_mesh(3, centroidal=True)
+    t = ModelTables(dm, centers=dm.generators, pv_flux=pv_flux, kinetic_energy=kinetic_energy, **earth_parameters())
+    desc, keep = descriptor(julia_struct(t))
+    flag = C.c_int32(-1)
+    D.check(D.lib().trisk_check_tables(C.byref(desc), C.byref(flag)))
+    assert flag.value == (1 if (pv_flux, kinetic_energy) == ("trisk", "cell") else 0)
+    ref = t.descriptor()
+    for name, ctype in D.TriskModelDesc._fields_:
+        a, b = getattr(desc, name), getattr(ref, name)
+        if not hasattr(ctype, "contents"):
+            assert a == b, name
+            continue
+        n = _field_sizes(desc)[name]
+        assert bool(a) == bool(b), name
+        if a:
+            assert np.array_equal(np.ctypeslib.as_array(a, (n,)), np.ctypeslib.as_array(b, (n,))), name
+    assert desc.kg == (12 if pv_flux != "trisk" else 0) and desc.pv_flux == D.PV_FLUX[pv_flux]

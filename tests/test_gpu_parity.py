@@ -393,6 +393,40 @@ def test_state_check_reports_nan_and_zero_vertex_height(model):
     assert nc > 0
 
 
+@pytest.mark.parametrize("pv_flux,kinetic_energy", [("trisk", "cell"), ("gassmann", "vertex")])
+def test_julia_extension_descriptor_tendency(pv_flux, kinetic_energy):
+    """A handle created from the descriptor the Julia extension builds (1-based struct
+    tables, 0-based copies, Gassmann and vertex-KE tables passed) gives the oracle's FP64
+    tendency bit for bit."""
+    import ctypes as C
+
+    import torch
+
+    from dmplex_amd import _device as D
+    from julia_descriptor import descriptor, julia_struct
+
+    say(f"julia descriptor {pv_flux} {kinetic_energy}")
+    dm = mesh(4)
+    t = ModelTables(dm, centers=dm.generators, pv_flux=pv_flux, kinetic_energy=kinetic_energy, **earth_parameters())
+    desc, keep = descriptor(julia_struct(t))
+    h = C.c_void_p()
+    D.check(D.lib().trisk_create(C.byref(desc), C.byref(D.TriskOptions(D.TRISK_FP64, 0, 0, 0, 0)), C.byref(h)))
+    try:
+        y = seeded_state(t.grid)
+        n = len(y)
+        yd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        dy = torch.zeros_like(yd)
+        s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        f64p = C.POINTER(C.c_double)
+        D.check(D.lib().trisk_upload_state(h, y.ctypes.data_as(f64p), C.c_void_p(yd.data_ptr()), s))
+        D.check(D.lib().trisk_tendency(h, C.c_void_p(yd.data_ptr()), C.c_void_p(dy.data_ptr()), s))
+        got = np.empty(n)
+        D.check(D.lib().trisk_download_state(h, C.c_void_p(dy.data_ptr()), got.ctypes.data_as(f64p), s))
+    finally:
+        D.check(D.lib().trisk_destroy(h))
+    assert np.array_equal(got, Oracle(t).tendency(y))
+
+
 def test_invalid_tables_are_rejected():
     say("invalid tables")
     dm = mesh(2)
