@@ -45,7 +45,8 @@ class MpasOpsDesc(C.Structure):
                 ("dz", f64p), ("dzh", f64p), ("wlow", f64p), ("edge_dz", f64p), ("edge_dzh", f64p),
                 ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p),
                 ("iap_pv", C.c_int32), ("vertical", C.c_int32), ("five_band", C.c_int32), ("reserved", C.c_int32),
-                ("cell_points", f64p)]
+                ("cell_points", f64p), ("orography", C.c_int32), ("reserved2", C.c_int32), ("z_full", f64p),
+                ("pi_ref", f64p), ("ground_q", f64p), ("ddxn", f64p), ("zlev", C.POINTER(C.c_int32)), ("zwt", f64p)]
 
 
 class TriskOptions(C.Structure):

@@ -75,6 +75,14 @@ struct Params {
     const double* vw;
     const double* fv;
     const double* vkw;
+    // orography: 0 flat, 1 pressure gradient along the levels, 2 at constant height
+    int oro;
+    const double* zf;     // cells x nlev
+    const double* piref;  // cells x nlev
+    const double* gq;     // cells x 3
+    const double* ddxn;   // edges x nlev
+    const int32_t* zlev;  // 2 x ne x nlev
+    const double* zwt;    // 2 x ne x nlev
     // work fields
     double* f[MPAS_FIELDS];
 };
@@ -318,7 +326,91 @@ __global__ __launch_bounds__(kBlock) void mpas_w_kinetic_kernel(Params p, const 
 }
 
 // ---------------------------------------------------------------------------------
-// D: vn_tendency_kernel! with edge_w (Gassmann/Tendencies.jl:4-67), flat levels (oro = 0)
+// Orography: the contravariant correction and the Exner deviation (Stepping.jl:58-76)
+
+// concorr_cell_kernel! (Operators.jl:135-144): w_c = sum_n inner[n, i] vn[e, k] ddxn[e, k]
+template <int KC>
+__global__ __launch_bounds__(kBlock) void mpas_concorr_cell_kernel(Params p, const double* __restrict__ vn,
+                                                                   int64_t t0) {
+    const Tile t = tile_of(blockIdx.x, t0, 0);
+    const int64_t i = t.first + threadIdx.x;
+    if (i >= p.nc) return;
+    const int k0 = int(blockIdx.y) * kLevels, k1 = min(k0 + kLevels, p.nlev);
+    const int kc = KC > 0 ? KC : p.kc;
+    constexpr int N = KC > 0 ? KC : 16;
+    int32_t E[N];
+    double In[N];
+    for (int n = 0; n < kc; ++n) {
+        E[n] = ld(p.ce + n * p.nc + i);
+        In[n] = ld(p.inner + n * p.nc + i);
+    }
+    for (int k = k0; k < k1; ++k) {
+        double s = 0.0;
+        for (int n = 0; n < kc; ++n) {
+            const int64_t a = E[n] + p.ne * k;
+            s += In[n] * vn[a] * p.ddxn[a];
+        }
+        p.f[MPAS_WCF][i + p.nc * k] = s;
+    }
+}
+
+// concorr_interface_kernel! (CGridOperators.jl:202-208): w_c at the interfaces, linear inside,
+// quadratic to the ground, zero at the lid; and contravariant_w_kernel! (:210-214), w - w_c
+__global__ __launch_bounds__(kBlock) void mpas_concorr_interface_kernel(Params p, const double* __restrict__ w,
+                                                                        int64_t t0) {
+    const Tile t = tile_of(blockIdx.x, t0, 0);
+    const int64_t i = t.first + threadIdx.x;
+    if (i >= p.nc) return;
+    const int k0 = int(blockIdx.y) * kLevels, k1 = min(k0 + kLevels, p.nlev);
+    const int j1 = k1 == p.nlev ? p.nlev + 1 : k1;  // the last chunk also has the top interface
+    const double* wcf = p.f[MPAS_WCF];
+    for (int j = k0; j < j1; ++j) {
+        const int64_t a = i + p.nc * j;
+        double wc;
+        if (j >= 1 && j <= p.nlev - 1)
+            wc = p.wlow[a] * wcf[a - p.nc] + (1 - p.wlow[a]) * wcf[a];
+        else if (j == 0)
+            wc = p.nlev >= 3 ? ld(p.gq + i) * wcf[i] + ld(p.gq + p.nc + i) * wcf[i + p.nc] +
+                                   ld(p.gq + 2 * p.nc + i) * wcf[i + 2 * p.nc]
+                             : wcf[i];
+        else
+            wc = 0.0;
+        p.f[MPAS_WCH][a] = wc;
+        p.f[MPAS_WHAT][a] = w[a] - wc;
+    }
+}
+
+// exner_deviation_centred_kernel! (CGridOperators.jl:93-100): pi' = pi - pi_ref and its
+// centred (one-sided at the ends) vertical derivative
+__global__ __launch_bounds__(kBlock) void mpas_exner_deviation_kernel(Params p, int64_t t0) {
+    const Tile t = tile_of(blockIdx.x, t0, 0);
+    const int64_t i = t.first + threadIdx.x;
+    if (i >= p.nc) return;
+    const int k0 = int(blockIdx.y) * kLevels, k1 = min(k0 + kLevels, p.nlev);
+    const double* pi = p.f[MPAS_EXNER];
+    for (int k = k0; k < k1; ++k) {
+        const int kl = k > 0 ? k - 1 : 0, ku = k < p.nlev - 1 ? k + 1 : p.nlev - 1;
+        const int64_t a = i + p.nc * k, al = i + p.nc * kl, au = i + p.nc * ku;
+        p.f[MPAS_PIP][a] = pi[a] - p.piref[a];
+        p.f[MPAS_DPIDZ][a] = ((pi[au] - p.piref[au]) - (pi[al] - p.piref[al])) / (p.zf[au] - p.zf[al]);
+    }
+}
+
+// w_concorr_tendency_kernel! (Operators.jl:146-152): tw += w_c dw/dz at the interior interfaces
+__global__ __launch_bounds__(kBlock) void mpas_w_concorr_kernel(Params p, const double* __restrict__ w, int64_t t0) {
+    const Tile t = tile_of(blockIdx.x, t0, 0);
+    const int64_t i = t.first + threadIdx.x;
+    if (i >= p.nc) return;
+    const int k0 = int(blockIdx.y) * kLevels, k1 = min(k0 + kLevels, p.nlev);
+    for (int j = k0 > 1 ? k0 : 1; j < k1; ++j) {
+        const int64_t a = i + p.nc * j;
+        p.f[MPAS_TW][a] = p.f[MPAS_TW][a] + p.f[MPAS_WCH][a] * (w[a + p.nc] - w[a - p.nc]) / (p.dzh[a] + p.dzh[a + p.nc]);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// D: vn_tendency_kernel! with edge_w (Gassmann/Tendencies.jl:4-67); over orography w is w - w_c,
+// pi the deviation pi' and the pressure gradient that of pressure_gradient (:13-30)
 
 // w~ = M_e / rho_e at interface j (edge_w, Tendencies.jl:4-11)
 __device__ __forceinline__ double edge_w(const Params& p, const double* __restrict__ w, const double* __restrict__ rho,
@@ -361,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     const double* Fs = p.f[MPAS_FS];
     const double* qe = p.f[MPAS_QE];
     const double* th = p.f[MPAS_THETA];
-    const double* pe = p.f[MPAS_EXNER];
+    const double* pe = p.oro ? p.f[MPAS_PIP] : p.f[MPAS_EXNER];
     const double* K = p.f[MPAS_K];
     const double* eta = p.f[MPAS_ETA];
     const int vert = VERT >= 0 ? VERT : p.vertical;
@@ -399,7 +491,20 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         const double te = ew1 * th[l + p.nc * k] + ew2 * th[r + p.nc * k];
         double tt = s - invd * (K[r + p.nc * k] - K[l + p.nc * k]);
         if (press) {
-            const double dpi = (pe[r + p.nc * k] - pe[l + p.nc * k]) * invd;
+            double dpi;
+            if (p.oro == 2) {
+                // height_difference (CGridOperators.jl:193-200): pi' of both cells at the edge height
+                const int64_t z = 2 * (e + p.ne * k);
+                const int jl = ld(p.zlev + z), jr = ld(p.zlev + z + 1);
+                const double tl = ld(p.zwt + z), tr = ld(p.zwt + z + 1);
+                const double pr = pe[r + p.nc * jr], pl = pe[l + p.nc * jl];
+                dpi = (pr + tr * (pe[r + p.nc * (jr + 1)] - pr) - pl - tl * (pe[l + p.nc * (jl + 1)] - pl)) * invd;
+            } else {
+                dpi = (pe[r + p.nc * k] - pe[l + p.nc * k]) * invd;
+                if (p.oro == 1)
+                    dpi -= p.ddxn[e + p.ne * k] *
+                           (ew1 * p.f[MPAS_DPIDZ][l + p.nc * k] + ew2 * p.f[MPAS_DPIDZ][r + p.nc * k]);
+            }
             tt -= pressure * (Cpd * te * dpi);
         }
         double out;
@@ -422,9 +527,11 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
 
 // ---------------------------------------------------------------------------------
 
-enum { K_A = 0, K_A2, K_B, K_C, K_E, K_D, K_N };
+enum { K_A = 0, K_A2, K_B, K_C, K_E, K_D, K_CC, K_CI, K_EX, K_WC, K_N };
 static const char* kNames[K_N] = {"mpas_cell_vertex_kernel", "mpas_cell_from_vertex_kernel", "mpas_edge_kernel",
-                                  "mpas_w_tendency_kernel",  "mpas_w_kinetic_kernel",        "mpas_vn_tendency_kernel"};
+                                  "mpas_w_tendency_kernel",  "mpas_w_kinetic_kernel",        "mpas_vn_tendency_kernel",
+                                  "mpas_concorr_cell_kernel", "mpas_concorr_interface_kernel",
+                                  "mpas_exner_deviation_kernel", "mpas_w_concorr_kernel"};
 
 }  // namespace mpas
 
@@ -491,6 +598,27 @@ static int slow_tendencies(mpas_ops* H, const double* vn, const double* w, const
                            p.vk ? 0 : kw_before);
     HIP_TRY(hipGetLastError());
     if ((rc = timed_end(stop, s))) return rc;
+    const dim3 gc(unsigned(kXcd * tc), chunks);
+    if (p.oro) {
+        // the contravariant correction of vn, w - w_c, and pi' with its vertical derivative
+        if ((rc = timed_begin(H, K_CC, s, &stop))) return rc;
+        if (p.kc == 6)
+            hipLaunchKernelGGL((mpas_concorr_cell_kernel<6>), gc, dim3(kBlock), 0, s, p, vn, tc);
+        else
+            hipLaunchKernelGGL((mpas_concorr_cell_kernel<0>), gc, dim3(kBlock), 0, s, p, vn, tc);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+        if ((rc = timed_begin(H, K_CI, s, &stop))) return rc;
+        hipLaunchKernelGGL(mpas_concorr_interface_kernel, gc, dim3(kBlock), 0, s, p, w, tc);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+        if (pressure > 0) {
+            if ((rc = timed_begin(H, K_EX, s, &stop))) return rc;
+            hipLaunchKernelGGL(mpas_exner_deviation_kernel, gc, dim3(kBlock), 0, s, p, tc);
+            HIP_TRY(hipGetLastError());
+            if ((rc = timed_end(stop, s))) return rc;
+        }
+    }
     if (p.vk) {
         if ((rc = timed_begin(H, K_A2, s, &stop))) return rc;
         hipLaunchKernelGGL(mpas_cell_from_vertex_kernel, dim3(unsigned(kXcd * tc), chunks), dim3(kBlock), 0, s, p, w,
@@ -516,6 +644,13 @@ static int slow_tendencies(mpas_ops* H, const double* vn, const double* w, const
         hipLaunchKernelGGL(mpas_w_kinetic_kernel, dim3(unsigned(kXcd * tc), chunks), dim3(kBlock), 0, s, p, w, tc);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
+    }
+    if (p.oro) {
+        if ((rc = timed_begin(H, K_WC, s, &stop))) return rc;
+        hipLaunchKernelGGL(mpas_w_concorr_kernel, gc, dim3(kBlock), 0, s, p, w, tc);
+        HIP_TRY(hipGetLastError());
+        if ((rc = timed_end(stop, s))) return rc;
+        w = p.f[MPAS_WHAT];  // the vn tendency couples vertically with w - w_c
     }
     if ((rc = timed_begin(H, K_D, s, &stop))) return rc;
     const unsigned vchunks = unsigned((p.nlev + kLevelsVn - 1) / kLevelsVn);
@@ -614,14 +749,28 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
         rc |= trisk::upload(o, L.vkk, &p.vkk);
         rc |= trisk::upload(o, L.vkw, &p.vkw);
     }
+    p.oro = L.oro;
+    if (L.oro) {
+        rc |= trisk::upload(o, L.zf, &p.zf);
+        rc |= trisk::upload(o, L.piref, &p.piref);
+        rc |= trisk::upload(o, L.gq, &p.gq);
+        rc |= trisk::upload(o, L.ddxn, &p.ddxn);
+    }
+    if (L.oro == 2) {
+        rc |= trisk::upload(o, L.zlev, &p.zlev);
+        rc |= trisk::upload(o, L.zwt, &p.zwt);
+    }
     // work fields: points x levels
-    const int64_t pts[MPAS_FIELDS] = {L.nc, L.nc, L.nc, L.nv, L.nv, L.nv, L.ne, L.ne, L.ne, L.ne, L.nc, L.ne, L.nv};
-    const int lev[MPAS_FIELDS] = {L.nlev, L.nlev, L.nlev, L.nlev, L.nlev, L.nlev, L.nlev,
-                                  L.nlev, L.nlev, L.nlev + 1, L.nlev + 1, L.nlev, L.nlev};
+    const int64_t pts[MPAS_FIELDS] = {L.nc, L.nc, L.nc, L.nv, L.nv, L.nv, L.ne, L.ne, L.ne,
+                                      L.ne, L.nc, L.ne, L.nv, L.nc, L.nc, L.nc, L.nc, L.nc};
+    const int lev[MPAS_FIELDS] = {L.nlev, L.nlev,     L.nlev,     L.nlev, L.nlev, L.nlev,
+                                  L.nlev, L.nlev,     L.nlev,     L.nlev + 1, L.nlev + 1, L.nlev,
+                                  L.nlev, L.nlev,     L.nlev + 1, L.nlev + 1, L.nlev, L.nlev};
     for (int f = 0; f < MPAS_FIELDS && !rc; ++f) {
         H->field_points[f] = pts[f];
         H->field_levels[f] = lev[f];
         if (f == MPAS_KV && !L.vk) continue;
+        if (f >= MPAS_WCF && !L.oro) continue;  // the orography fields
         void* q = nullptr;
         rc = trisk::alloc(o, size_t(pts[f]) * size_t(lev[f]) * sizeof(double), &q);
         p.f[f] = static_cast<double*>(q);
@@ -755,7 +904,16 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
     v[K_E] = vi && h->five_band ? nc * (L1 * c * 2 + L * c * 3) : 0.0;
     // D: F, q_e, theta, pi, K, [eta, w, rho, wlow, edge dz, dzh | w, vn, edge dz] -> tvn; tables
     v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
-             (vi ? ne * L1 * c * 2 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c);
+             (vi ? ne * L1 * c * 2 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
+             (p.oro == 1 ? ne * L * c + nc * L * c : p.oro == 2 ? 2 * ne * L * (ix + c) : 0.0);
+    if (p.oro) {
+        // CC: vn, ddxn, the cell rows -> w_c; CI: w_c, wlow, q, w -> w_c and w - w_c at the
+        // interfaces; EX: pi, pi_ref, z_full -> pi', dpi'/dz; WC: tw (read, written), w, w_c, dzh
+        v[K_CC] = 2 * ne * L * c + nc * KC * (ix + c) + nc * L * c;
+        v[K_CI] = nc * L * c + nc * L1 * c * 2 + nc * 3 * c + 2 * nc * L1 * c;
+        v[K_EX] = 5 * nc * L * c;
+        v[K_WC] = 5 * nc * L1 * c;
+    }
     int k2 = 0;
     for (int k = 0; k < K_N && k2 < max; ++k) {
         if (v[k] == 0.0) continue;

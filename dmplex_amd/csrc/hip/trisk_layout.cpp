@@ -418,6 +418,25 @@ bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) 
         err = "vertical must be 0 (advective), 1 (vector invariant) or 2 (ICON-IAP)";
         return false;
     }
+    if (d->orography < 0 || d->orography > 2) {
+        err = "orography must be 0 (flat), 1 (pressure gradient along the levels) or 2 (at constant height)";
+        return false;
+    }
+    if (d->orography && (!d->z_full || !d->pi_ref || !d->ground_q || !d->ddxn)) {
+        err = "orography needs z_full, pi_ref, ground_q and ddxn";
+        return false;
+    }
+    if (d->orography == 2) {
+        if (!d->zlev || !d->zwt || d->nlev < 2) {
+            err = "the pressure gradient at constant height needs zlev, zwt and at least two levels";
+            return false;
+        }
+        for (int64_t t = 0; t < 2 * d->ne * d->nlev; ++t)
+            if (d->zlev[t] < 0 || d->zlev[t] > d->nlev - 2) {
+                err = "level out of range in zlev";
+                return false;
+            }
+    }
     const bool vk = d->cell_vertices != nullptr;
     if (vk && (!d->vk_kites || !d->vk_weights)) {
         err = "vertex kinetic energy needs cell_vertices, vk_kites and vk_weights";
@@ -546,6 +565,26 @@ bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) 
     points(d->wlow, nc, NL + 1, L.cperm, L.wlow);
     points(d->edge_dz, ne, NL, L.eperm, L.edz);
     points(d->edge_dzh, ne, NL + 1, L.eperm, L.edzh);
+    L.oro = d->orography;
+    if (L.oro) {
+        points(d->z_full, nc, NL, L.cperm, L.zf);
+        points(d->pi_ref, nc, NL, L.cperm, L.piref);
+        points(d->ground_q, nc, 3, L.cperm, L.gq);
+        points(d->ddxn, ne, NL, L.eperm, L.ddxn);
+    }
+    if (L.oro == 2) {
+        // entry (s, e, k) at s + 2 (e + ne k): the edge index is renumbered, s and k stay
+        L.zlev.resize(size_t(2 * ne * NL));
+        L.zwt.resize(size_t(2 * ne * NL));
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < NL; ++k)
+            for (int64_t e = 0; e < ne; ++e)
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    const size_t to = size_t(s2 + 2 * (e + ne * k)), from = size_t(s2 + 2 * (L.eperm[e] + ne * k));
+                    L.zlev[to] = d->zlev[from];
+                    L.zwt[to] = d->zwt[from];
+                }
+    }
     return true;
 }
 

@@ -125,6 +125,10 @@ struct MpasLayout {
     std::vector<double> kin, inner, vkk, dz, dzh, wlow;
     std::vector<double> ew, invd, trw, rh, fe, edz, edzh;
     std::vector<double> curl, vw, fv, vkw;
+    // orography: 0 flat, 1 pressure gradient along the levels, 2 at constant height
+    int oro = 0;
+    std::vector<double> zf, piref, gq, ddxn, zwt;  // points x levels (gq: cells x 3; zwt: 2 x ne x nlev)
+    std::vector<int32_t> zlev;                      // 2 x ne x nlev, 0-based levels
 };
 bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err);
 

@@ -277,17 +277,21 @@ int icon_check_tables(const icon_model_desc* desc);
 int icon_create(const icon_model_desc* desc, const trisk_options* opt, trisk_handle** out);
 
 /* --- MPAS-Gassmann TRiSK operators with levels ---------------------------------------
- * The explicit tendencies of one stage of MPASGassmannNonhydrostatic on flat ground,
+ * The explicit tendencies of one stage of MPASGassmannNonhydrostatic,
  *   slow_tendencies!(m, ys, pressure)  src/Nonhydrostatic/MPAS/Gassmann/Stepping.jl:22-56
- * with the TRiSK operators of src/Nonhydrostatic/MPAS/Operators.jl:6-134 (diagnose, vertex
+ * with the TRiSK operators of src/Nonhydrostatic/MPAS/Operators.jl:6-154 (diagnose, vertex
  * and cell-from-vertex kinetic energy, w kinetic energy, eta, edge density and mass flux,
  * vertex and edge PV, the w tendencies) and vn_tendency_kernel! with edge_w
- * (Gassmann/Tendencies.jl:4-67, the pressure gradient along flat levels).  Fields are (points x levels) with the point index
- * fastest, as the reference's (n, nlev) arrays: vn (ne x nlev), w (nc x nlev+1), rho and
- * rhotheta (nc x nlev).  The library renumbers the points (cells along the Hilbert curve,
- * edges and vertices by first appearance); mpas_ops_upload_field / _download_field convert.
- * Terrain-following levels (orography), the contravariant correction and the pressure
- * gradient at constant height are not covered. */
+ * (Gassmann/Tendencies.jl:4-67).  Over orography (terrain-following levels) also the
+ * contravariant correction w_c (concorr_cell_kernel!, concorr_interface_kernel!,
+ * contravariant_w_kernel!, w_concorr_tendency_kernel!; Stepping.jl:58-76), the deviation of
+ * pi from the reference atmosphere (exner_deviation_centred_kernel!) and the pressure
+ * gradient along the levels or at constant height (pressure_gradient, Tendencies.jl:13-30;
+ * height_difference, Common/CGridOperators.jl:193-200).  Fields are (points x levels) with the
+ * point index fastest, as the reference's (n, nlev) arrays: vn (ne x nlev), w (nc x nlev+1),
+ * rho and rhotheta (nc x nlev).  The library renumbers the points (cells along the Hilbert
+ * curve, edges and vertices by first appearance); mpas_ops_upload_field / _download_field
+ * convert. */
 typedef struct mpas_ops mpas_ops;
 
 typedef struct {
@@ -326,6 +330,18 @@ typedef struct {
     int32_t five_band;              /* vertical_solver = :five_band */
     int32_t reserved;
     const double* cell_points;      /* nc x 3: Hilbert order of the cells, or NULL */
+    /* orography (MPASGassmannNonhydrostatic.jl:220-240): 0 flat ground (the fields below
+     * unused), 1 terrain-following levels with the pressure gradient along the levels,
+     * 2 with the pressure gradient at constant height (pressure_gradient = :height) */
+    int32_t orography;
+    int32_t reserved2;
+    const double* z_full;           /* nc x nlev: full-level heights of the cells (metric.z_full) */
+    const double* pi_ref;           /* nc x nlev: reference-atmosphere Exner pressure there (metric.πref) */
+    const double* ground_q;         /* nc x 3: quadratic extrapolation weights to the ground (metric.q) */
+    const double* ddxn;             /* ne x nlev: slope of the levels along the edge normal (metric.ddxn) */
+    const int32_t* zlev;            /* 2 x ne x nlev: 0-based level below the edge height in each edge cell
+                                       (metric.zlev - 1); orography 2 only, nlev >= 2 */
+    const double* zwt;              /* 2 x ne x nlev: the interpolation weights (metric.zwt); orography 2 only */
 } mpas_ops_desc;
 
 /* Work fields of a slow-tendency call (mpas_ops_field): kind, points x levels. */
@@ -343,7 +359,12 @@ enum {
     MPAS_TW = 10,    /* w tendency, cells x nlev+1 */
     MPAS_TVN = 11,   /* vn tendency, edges x nlev */
     MPAS_KV = 12,    /* vertex kinetic energy, vertices x nlev */
-    MPAS_FIELDS = 13
+    MPAS_WCF = 13,   /* contravariant correction w_c at the full levels, cells x nlev (orography) */
+    MPAS_WCH = 14,   /* w_c at the interfaces, cells x nlev+1 (orography) */
+    MPAS_WHAT = 15,  /* w - w_c, cells x nlev+1 (orography) */
+    MPAS_PIP = 16,   /* pi - pi_ref, cells x nlev (orography, pressure > 0) */
+    MPAS_DPIDZ = 17, /* d(pi - pi_ref)/dz, centred, cells x nlev (orography, pressure > 0) */
+    MPAS_FIELDS = 18
 };
 
 /* Host-only validation of a descriptor (index ranges, widths).  Touches no device. */

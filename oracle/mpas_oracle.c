@@ -1,14 +1,18 @@
 /*
  * mpas_oracle.c -- TEST INFRASTRUCTURE ONLY (see mpas_oracle.h).
  *
- * Restates, kernel for kernel, the explicit tendencies of one MPAS-Gassmann stage on flat
- * ground (slow_tendencies!, src/Nonhydrostatic/MPAS/Gassmann/Stepping.jl:22-56):
+ * Restates, kernel for kernel, the explicit tendencies of one MPAS-Gassmann stage
+ * (slow_tendencies!, src/Nonhydrostatic/MPAS/Gassmann/Stepping.jl:22-56):
  *   diagnose_kernel!          Operators.jl:6-15      vertex_kinetic_kernel!   :18-25
  *   cell_from_vertex_kernel!  :28-35                 w_kinetic_kernel!        :38-42
  *   eta_kernel!               :44-52                 edge_density_kernel!     :55-62
  *   w_tendency_vi_kernel!     :69-85                 vertex_pv_kernel!        :88-101
  *   edge_pv_kernel!           :104-115               w_tendency_kernel!       :118-133
- *   vn_tendency_kernel!, edge_w, pressure_gradient (oro = 0)   Gassmann/Tendencies.jl:4-67
+ *   vn_tendency_kernel!, edge_w, pressure_gradient             Gassmann/Tendencies.jl:4-67
+ * and over orography (Stepping.jl:26-29,51-54,58-76)
+ *   concorr_cell_kernel!      Operators.jl:135-144   w_concorr_tendency_kernel!  :146-152
+ *   exner_deviation_centred_kernel!, height_difference, concorr_interface_kernel!,
+ *   contravariant_w_kernel!   Common/CGridOperators.jl:93-100,193-214
  * One loop per kernel over (points, levels) with the reference's per-item arithmetic, in the
  * order slow_tendencies! launches them.  Indices are 0-based: layer k of the reference is k-1
  * here, interface j is j-1.
@@ -174,6 +178,73 @@ static void w_tendency(const mpas_oracle_model* m, const double* vn, const doubl
         }
 }
 
+/* concorr_cell_kernel! (Operators.jl:135-144) */
+static void concorr_cell(const mpas_oracle_model* m, const double* vn) {
+    const int64_t nc = m->nc, ne = m->ne, KC = m->kc;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t k = 0; k < m->nlev; ++k)
+        for (int64_t i = 0; i < nc; ++i) {
+            double s = 0.0;
+            for (int64_t n = 0; n < KC; ++n) {
+                const int64_t e = AT(m->cell_edges, KC, n, i);
+                s += AT(m->inner, KC, n, i) * F2(vn, ne, e, k) * F2(m->ddxn, ne, e, k);
+            }
+            F2(m->wcf, nc, i, k) = s;
+        }
+}
+
+/* concorr_interface_kernel! (CGridOperators.jl:203-208) */
+static void concorr_interface(const mpas_oracle_model* m) {
+    const int64_t nc = m->nc, nlev = m->nlev;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t j = 0; j <= nlev; ++j)
+        for (int64_t i = 0; i < nc; ++i) {
+            double wc;
+            if (j >= 1 && j <= nlev - 1)
+                wc = F2(m->wlow, nc, i, j) * F2(m->wcf, nc, i, j - 1) + (1 - F2(m->wlow, nc, i, j)) * F2(m->wcf, nc, i, j);
+            else if (j == 0)
+                wc = nlev >= 3 ? F2(m->ground_q, nc, i, 0) * F2(m->wcf, nc, i, 0) +
+                                     F2(m->ground_q, nc, i, 1) * F2(m->wcf, nc, i, 1) +
+                                     F2(m->ground_q, nc, i, 2) * F2(m->wcf, nc, i, 2)
+                               : F2(m->wcf, nc, i, 0);
+            else
+                wc = 0.0;
+            F2(m->wch, nc, i, j) = wc;
+        }
+}
+
+/* contravariant_w_kernel! (CGridOperators.jl:211-214) */
+static void contravariant_w(const mpas_oracle_model* m, const double* w) {
+    const int64_t nc = m->nc;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t j = 0; j <= m->nlev; ++j)
+        for (int64_t i = 0; i < nc; ++i) F2(m->what, nc, i, j) = F2(w, nc, i, j) - F2(m->wch, nc, i, j);
+}
+
+/* exner_deviation_centred_kernel! (CGridOperators.jl:93-100) */
+static void exner_deviation(const mpas_oracle_model* m) {
+    const int64_t nc = m->nc, nlev = m->nlev;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t k = 0; k < nlev; ++k)
+        for (int64_t i = 0; i < nc; ++i) {
+            const int64_t kl = k > 0 ? k - 1 : 0, ku = k < nlev - 1 ? k + 1 : nlev - 1;
+            F2(m->pip, nc, i, k) = F2(m->exner, nc, i, k) - F2(m->pi_ref, nc, i, k);
+            F2(m->dpidz, nc, i, k) = ((F2(m->exner, nc, i, ku) - F2(m->pi_ref, nc, i, ku)) -
+                                      (F2(m->exner, nc, i, kl) - F2(m->pi_ref, nc, i, kl))) /
+                                     (F2(m->z_full, nc, i, ku) - F2(m->z_full, nc, i, kl));
+        }
+}
+
+/* w_concorr_tendency_kernel! (Operators.jl:147-152) */
+static void w_concorr_tendency(const mpas_oracle_model* m, const double* w) {
+    const int64_t nc = m->nc, nlev = m->nlev;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t j = 1; j <= nlev - 1; ++j)
+        for (int64_t i = 0; i < nc; ++i)
+            F2(m->tw, nc, i, j) += F2(m->wch, nc, i, j) * (F2(w, nc, i, j + 1) - F2(w, nc, i, j - 1)) /
+                                   (F2(m->dzh, nc, i, j) + F2(m->dzh, nc, i, j + 1));
+}
+
 /* edge_w (Tendencies.jl:4-11) at interface j */
 static double edge_w(const mpas_oracle_model* m, const double* w, const double* rho, int64_t e, int64_t l, int64_t r,
                      int64_t j, int massweighted) {
@@ -208,7 +279,22 @@ static void vn_tendency(const mpas_oracle_model* m, const double* vn, const doub
             const double te = w1 * F2(m->theta, nc, l, k) + w2 * F2(m->theta, nc, r, k);
             double t = s - m->inverse_dual[e] * (F2(m->K, nc, r, k) - F2(m->K, nc, l, k));
             if (pressure > 0) {
-                const double dpi = (F2(m->exner, nc, r, k) - F2(m->exner, nc, l, k)) * m->inverse_dual[e];
+                /* pressure_gradient (Tendencies.jl:19-30): pi' over orography */
+                const double* pe = m->orography ? m->pip : m->exner;
+                double dpi;
+                if (m->orography == 2) {
+                    /* height_difference (CGridOperators.jl:194-200) */
+                    const int64_t z = 2 * (e + ne * k);
+                    const int64_t jl = m->zlev[z], jr = m->zlev[z + 1];
+                    const double tl = m->zwt[z], tr = m->zwt[z + 1];
+                    dpi = (F2(pe, nc, r, jr) + tr * (F2(pe, nc, r, jr + 1) - F2(pe, nc, r, jr)) - F2(pe, nc, l, jl) -
+                           tl * (F2(pe, nc, l, jl + 1) - F2(pe, nc, l, jl))) *
+                          m->inverse_dual[e];
+                } else {
+                    dpi = (F2(pe, nc, r, k) - F2(pe, nc, l, k)) * m->inverse_dual[e];
+                    if (m->orography == 1)
+                        dpi -= F2(m->ddxn, ne, e, k) * (w1 * F2(m->dpidz, nc, l, k) + w2 * F2(m->dpidz, nc, r, k));
+                }
                 t -= pressure * (Cpd * te * dpi);
             }
             if (m->vertical > 0) {
@@ -229,6 +315,12 @@ static void vn_tendency(const mpas_oracle_model* m, const double* vn, const doub
 void mpas_oracle_slow_tendencies(const mpas_oracle_model* m, const double* vn, const double* w, const double* rho,
                                  const double* rhotheta, double pressure) {
     diagnose(m, vn, rho, rhotheta);
+    if (m->orography) {
+        concorr_cell(m, vn);
+        concorr_interface(m);
+        contravariant_w(m, w);
+        if (pressure > 0) exner_deviation(m);
+    }
     if (m->cell_vertices) {
         vertex_kinetic(m, vn);
         cell_from_vertex(m);
@@ -244,7 +336,8 @@ void mpas_oracle_slow_tendencies(const mpas_oracle_model* m, const double* vn, c
     } else {
         w_tendency(m, vn, w);
     }
-    vn_tendency(m, vn, w, rho, pressure);
+    if (m->orography) w_concorr_tendency(m, w);
+    vn_tendency(m, vn, m->orography ? m->what : w, rho, pressure);
 }
 
 int mpas_oracle_kernel(const mpas_oracle_model* m, int id, const double* vn, const double* w, const double* rho,
@@ -261,6 +354,11 @@ int mpas_oracle_kernel(const mpas_oracle_model* m, int id, const double* vn, con
         case 8: w_tendency_vi(m, vn, m->vertical == 2); break;
         case 9: w_tendency(m, vn, w); break;
         case 10: vn_tendency(m, vn, w, rho, pressure); break;
+        case 11: concorr_cell(m, vn); break;
+        case 12: concorr_interface(m); break;
+        case 13: contravariant_w(m, w); break;
+        case 14: exner_deviation(m); break;
+        case 15: w_concorr_tendency(m, w); break;
         default: return -1;
     }
     return 0;

@@ -280,7 +280,10 @@ class MpasOracleModel(C.Structure):
                 ("cell_vertices", i64p), ("vk_kites", f64p), ("vk_weights", f64p),
                 ("iap_pv", C.c_int32), ("vertical", C.c_int32), ("five_band", C.c_int32), ("reserved", C.c_int32)] + \
                [(n, f64p) for n in ("theta", "exner", "K", "qv", "zeta", "rhov", "qe", "rhoe", "Fs", "eta", "tw",
-                                    "tvn", "Kv")]
+                                    "tvn", "Kv")] + \
+               [("orography", C.c_int32), ("reserved2", C.c_int32), ("z_full", f64p), ("pi_ref", f64p),
+                ("ground_q", f64p), ("ddxn", f64p), ("zlev", C.POINTER(C.c_int32)), ("zwt", f64p)] + \
+               [(n, f64p) for n in ("wcf", "wch", "what", "pip", "dpidz")]
 
 
 class MpasOracle:
@@ -289,7 +292,7 @@ class MpasOracle:
     (levels, points) arrays in the reference numbering."""
 
     SHAPES = dict(theta="c", exner="c", K="c", qv="v", zeta="v", rhov="v", qe="e", rhoe="e", Fs="e", eta="e+",
-                  tw="c+", tvn="e", Kv="v")
+                  tw="c+", tvn="e", Kv="v", wcf="c", wch="c+", what="c+", pip="c", dpidz="c")
 
     def __init__(self, m):
         self.m = m
@@ -311,7 +314,12 @@ class MpasOracle:
             _p(vk["weights"], f64p) if vk else None,
             int(m.iap_pv), m.vertical, int(m.five_band), 0,
             *(_p(w[k], f64p) for k in ("theta", "exner", "K", "qv", "zeta", "rhov", "qe", "rhoe", "Fs", "eta", "tw",
-                                       "tvn", "Kv")))
+                                       "tvn", "Kv")),
+            m.orography_mode(), 0,
+            *((_p(m.z_full, f64p), _p(m.pi_ref, f64p), _p(m.ground_q, f64p), _p(m.ddxn, f64p))
+              if m.orography else (None,) * 4),
+            *((_p(m.zlev, C.POINTER(C.c_int32)), _p(m.zwt, f64p)) if m.orography_mode() == 2 else (None, None)),
+            *(_p(w[k], f64p) for k in ("wcf", "wch", "what", "pip", "dpidz")))
 
     def slow_tendencies(self, vn, w, rho, rhotheta, pressure: float = 1.0) -> dict:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (vn, w, rho, rhotheta)]
@@ -320,7 +328,8 @@ class MpasOracle:
         return {k: v.copy() for k, v in self.work.items()}
 
     KERNELS = dict(diagnose=0, vertex_kinetic=1, cell_from_vertex=2, w_kinetic=3, vertex_pv=4, edge_pv=5,
-                   edge_density=6, eta=7, w_tendency_vi=8, w_tendency=9, vn_tendency=10)
+                   edge_density=6, eta=7, w_tendency_vi=8, w_tendency=9, vn_tendency=10, concorr_cell=11,
+                   concorr_interface=12, contravariant_w=13, exner_deviation=14, w_concorr_tendency=15)
 
     def kernel(self, name: str, vn=None, w=None, rho=None, rhotheta=None, pressure: float = 1.0) -> None:
         """One reference kernel on the current work fields (``self.work``, edited in place)."""

@@ -12,8 +12,11 @@ import numpy as np
 import pytest
 
 from dmplex_amd import TRiSKGrid, create_hexagon_mesh
-from dmplex_amd.mpas_ops import (Cvd, P0, Rd, MPASOperatorTables, iap_trisk_weights, trisk_pv_edges,
-                                 vertical_grid, vertical_grid_from_half)
+from dmplex_amd.mpas_ops import (ICON_CPD, ICON_GRAV, ICON_RD, Cpd, Cvd, P0, Rd, MPASOperatorTables,
+                                 column_metrics, dcmip_mountain_height, dcmip_rest, ground_extrapolation,
+                                 height_difference, height_interpolation, iap_trisk_weights, lonlat,
+                                 reference_atmosphere, terrain_following, trisk_pv_edges, vertical_grid,
+                                 vertical_grid_from_half)
 from oracle.oracle import MpasOracle
 
 
@@ -157,3 +160,129 @@ def test_table_errors(grid):
     m.trisk_index = bad
     with pytest.raises(ValueError, match="trisk_index"):
         m.check()
+
+
+# --- orography: terrain-following levels (test/extruded.jl:440-517) ---------------------
+
+def test_terrain_following_and_column_metrics():
+    """test/extruded.jl:441-449: the levels follow the ground, flat above flat_height, the basic
+    coordinate, crossing levels rejected; the column thicknesses add up to the column height."""
+    v = vertical_grid(10, 12000.0)
+    h = np.array([0.0, 1000.0, -200.0])
+    z = terrain_following(v, h, flat_height=8000.0)
+    assert np.array_equal(z[0], v.z_half) and z[1, 0] == 1000.0 and z[2, 0] == -200.0
+    above = v.z_half >= 8000
+    assert np.array_equal(z[1, above], v.z_half[above])
+    np.testing.assert_allclose(terrain_following(v, h, coordinate="basic")[1], v.z_half + 1000 * (1 - v.z_half / 12000))
+    with pytest.raises(ValueError, match="cross"):
+        terrain_following(v, [20000.0])
+    c = column_metrics(z)
+    np.testing.assert_allclose(c.dz.sum(axis=1), 12000 - h)
+    np.testing.assert_allclose(c.z_full[1], (z[1, :-1] + z[1, 1:]) / 2)
+
+
+def test_height_interpolation_known_values():
+    """test/extruded.jl:481-487: the layers and weights of the edge heights, and height_difference."""
+    edge_cells = np.array([[0], [1]])
+    zc = np.array([[0.0, 100.0, 300.0], [500.0, 700.0, 900.0]])
+    lev, wt = height_interpolation(edge_cells, zc, np.array([[600.0, 200.0, 1000.0]]))
+    assert np.array_equal(lev[:, 0, :], [[1, 1, 1], [0, 0, 1]])
+    np.testing.assert_allclose(wt[:, 0, 0], [2.5, 0.5])
+    np.testing.assert_allclose(wt[:, 0, 1], [0.5, -1.5])
+    pip = np.array([[1.0, 2.0, 4.0], [3.0, 5.0, 7.0]])
+    assert height_difference(pip, edge_cells, lev, wt, 0, 0) == pytest.approx((3 + 0.5 * 2) - (2 + 2.5 * 2))
+
+
+def test_reference_atmosphere_is_hydrostatic():
+    """ICON's reference atmosphere: T(0) = 288.15 K, p(0) = 101325 Pa and c_p dpi/dz = -g pi/T
+    (hydrostatic balance), by central differences."""
+    T0, pi0 = reference_atmosphere(0.0)
+    assert T0 == pytest.approx(288.15) and pi0 == pytest.approx((101325.0 / P0) ** (ICON_RD / ICON_CPD), rel=1e-15)
+    for z in (0.0, 3000.0, 11000.0, 25000.0):
+        dz = 0.5
+        dpi = (reference_atmosphere(z + dz)[1] - reference_atmosphere(z - dz)[1]) / (2 * dz)
+        T, pi = reference_atmosphere(z)
+        assert abs(dpi + ICON_GRAV * pi / (ICON_CPD * T)) < 1e-8 * abs(dpi)
+
+
+def test_ground_extrapolation_exact_for_quadratics():
+    v = vertical_grid(8, 12000.0, stretch=3.0)
+    z = terrain_following(v, np.array([0.0, 700.0, 1800.0]), coordinate="basic")
+    c = column_metrics(z)
+    q = ground_extrapolation(c)
+    for i in range(3):
+        f = lambda x: 2.0 - 3e-3 * x + 4e-7 * x * x  # noqa: E731
+        assert np.dot(q[i], f(c.z_full[i, :3])) == pytest.approx(f(z[i, 0]), rel=1e-12)
+
+
+def _mountain(grid, levels=10, top=12000.0, **kw):
+    v = vertical_grid(levels, top)
+    return MPASOperatorTables(grid, v, surface_height=dcmip_mountain_height, coordinate="basic", flat_height=top, **kw)
+
+
+def _rest_state(m):
+    """The DCMIP 2012 test 2-0 atmosphere at rest (T = T0 - Gamma z) at the levels of every column."""
+    g = m.grid
+    T, p = dcmip_rest(m.z_full)
+    rho = p / (Rd * T)
+    theta = T * (P0 / p) ** (Rd / Cpd)
+    nl, ne, nc = m.nlev, g.num_edges(), g.num_cells()
+    return np.zeros((nl, ne)), np.zeros((nl + 1, nc)), rho, rho * theta
+
+
+@pytest.mark.parametrize("pressure_gradient", ["levels", "height"])
+def test_pressure_gradient_at_rest_over_the_mountain(grid, pressure_gradient):
+    """At rest over the DCMIP mountain the horizontal pressure gradient at constant height
+    vanishes; along the sloping levels, with the metric correction (or interpolated to the edge
+    height), the discrete one is a truncation error, a small fraction of the uncorrected
+    gradient along the levels.  Flat ground gives exactly zero."""
+    m = _mountain(grid, pressure_gradient=pressure_gradient)
+    assert m.orography and np.max(m.cell_metrics.z_half[:, 0]) > 1500
+    state = _rest_state(m)
+    tvn = MpasOracle(m).slow_tendencies(*state, 1.0)["tvn"]
+    planted = _mountain(grid, pressure_gradient="levels")
+    planted.ddxn = np.zeros_like(planted.ddxn)   # no slope term: the gradient along the levels
+    raw = MpasOracle(planted).slow_tendencies(*state, 1.0)["tvn"]
+    # 7.5e-4 and 2.5e-5 m/s^2 at hexagon 3 x 10 levels; the reference's 24 steps of 300 s over this
+    # mountain stay below 0.5 m/s (test/extruded.jl:517), 0.18 m/s at this rate
+    assert np.max(np.abs(raw)) > 5e-4
+    assert np.max(np.abs(tvn)) < 0.06 * np.max(np.abs(raw))
+    flat = MPASOperatorTables(grid, vertical_grid(10, 12000.0))
+    assert not flat.orography
+    out = MpasOracle(flat).slow_tendencies(*_rest_state(flat), 1.0)
+    assert not np.any(out["tvn"])
+
+
+def _zonal_wind_correction(n):
+    """(wcf, exact, w, out) of the zonal wind a omega cos(lat) over the mountain at hexagon level n."""
+    g = TRiSKGrid(create_hexagon_mesh(n))
+    m = _mountain(g, levels=4)
+    a, omega = m.radius, 1e-5
+    vel = a * omega * np.stack([-g.edge_points[:, 1], g.edge_points[:, 0], np.zeros(g.num_edges())], axis=1)
+    vn = np.repeat(np.sum(vel * g.edge_normals, axis=1)[None, :], m.nlev, axis=0)
+    rng = np.random.default_rng(4)
+    w = rng.standard_normal((m.nlev + 1, g.num_cells()))
+    rho = np.ones((m.nlev, g.num_cells()))
+    out = MpasOracle(m).slow_tendencies(vn, w, rho, 300 * rho, 0.0)
+    lon, lat = lonlat(g.centers)
+    dl = 1e-6
+    dhdlon = (dcmip_mountain_height(lon + dl, lat) - dcmip_mountain_height(lon - dl, lat)) / (2 * dl)
+    zeta = np.asarray(m.vgrid.z_half)
+    b = 1 - zeta / zeta[-1]
+    exact = omega * dhdlon[None, :] * ((b[:-1] + b[1:]) / 2)[:, None]
+    return out["wcf"], exact, w, out
+
+
+def test_contravariant_correction_of_a_zonal_wind():
+    """w_c = u . grad z of the full levels: for the zonal wind u = a omega cos(lat) (solid-body
+    rotation about the axis) it is omega dz/dlon, which the cell inner product of vn dz/dn
+    approximates to second order (relative L2 error 0.175 at hexagon 5, 0.046 at hexagon 6; with
+    the sign of the slopes flipped it would be about 2); w_c vanishes at the lid and the vertical
+    coupling takes w - w_c."""
+    errs = []
+    for n in (5, 6):
+        wcf, exact, w, out = _zonal_wind_correction(n)
+        errs.append(np.linalg.norm(wcf - exact) / np.linalg.norm(exact))
+    assert errs[1] < 0.08 and errs[1] < errs[0] / 3, errs
+    assert not np.any(out["wch"][-1])
+    np.testing.assert_array_equal(out["what"], w - out["wch"])
