@@ -1,6 +1,7 @@
 // The explicit stage tendencies of MPAS-Gassmann on gfx950 (slow_tendencies!, src/Nonhydrostatic/
 // MPAS/Gassmann/Stepping.jl:22-56): the TRiSK operators of src/Nonhydrostatic/MPAS/Operators.jl
-// and vn_tendency_kernel! of Gassmann/Tendencies.jl with a level dimension, on flat ground.
+// and vn_tendency_kernel! of Gassmann/Tendencies.jl with a level dimension, on flat ground or over
+// terrain-following levels (the contravariant correction and pi' of Stepping.jl:58-76).
 //
 // Fields are points x levels with the point index fastest (the reference's (n, nlev) arrays),
 // in the device numbering (cells along the Hilbert curve, edges and vertices by first

@@ -1,7 +1,7 @@
 """The TRiSK operators of the MPAS-Gassmann nonhydrostatic dycore with a level dimension.
 
 Mirrors the parts of ``src/Nonhydrostatic`` that produce the explicit tendencies of one
-stage on flat ground, ``slow_tendencies!(m, ys, pressure)`` (``MPAS/Gassmann/Stepping.jl:
+stage on flat ground or over orography, ``slow_tendencies!(m, ys, pressure)`` (``MPAS/Gassmann/Stepping.jl:
 22-56``): the vertical grid (``Common/VerticalGrid.jl:16-42``), the ICON-IAP TRiSK tables
 (``trisk_pv_edges``, ``iap_trisk_weights``, ``FiniteVolumes/TRiSKGrid.jl:318-403``), the
 horizontal tables of ``MPASGassmannNonhydrostatic`` (``MPASGassmannNonhydrostatic.jl:

@@ -314,7 +314,7 @@ typedef struct {
     const double* trisk;            /* ne x kt, edge first: w l' / 2d */
     const double* rhombus;          /* 2 x ne */
     const double* edge_coriolis;    /* ne */
-    /* vertical metrics (flat ground): cells dz (nc x nlev), dzh and wlow (nc x nlev+1);
+    /* column metrics: cells dz (nc x nlev), dzh and wlow (nc x nlev+1);
      * edges dz (ne x nlev), dzh (ne x nlev+1) (metric.dz, metric.dzh) */
     const double* dz;
     const double* dzh;

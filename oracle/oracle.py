@@ -287,7 +287,7 @@ class MpasOracleModel(C.Structure):
 
 
 class MpasOracle:
-    """The reference's explicit MPAS-Gassmann stage tendencies on flat ground
+    """The reference's explicit MPAS-Gassmann stage tendencies on flat ground or over orography
     (slow_tendencies!) on the host tables of an ``MPASOperatorTables``.  Fields are
     (levels, points) arrays in the reference numbering."""
 
