@@ -77,7 +77,8 @@ using HexWidths = Widths<6, 3, 10, 12>;
 // ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
 
-template <class P, class UIn, class HIn, bool VK, bool ZH, class W = AnyWidths>
+// QV false: q_v is not stored (the Gassmann flux forms the edge PV from zeta_v and h_v)
+template <class P, class UIn, class HIn, bool VK, bool ZH, class W = AnyWidths, bool QV = true>
 __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const UIn* __restrict__ u,
                                                              const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         double hi = double(h[i]);
         double K = double(s);
         if (VK) {
-            p.K[i] = K;  // completed with the vertex part in P2
+            // completed with the vertex part in P2; a tendency reads only B, the diagnostics K
+            if (p.K_out) p.K[i] = K;
             p.B[i] = K + p.g * hi;
         } else {
             p.B[i] = K + p.g * hi;
@@ -108,16 +110,18 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         C z = 0, eta = 0;
         for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[VE[k]]);
         for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[VC[k]]);
-        C q = (z + ld(p.fv + v)) / eta;
-        p.qv[v] = I(q);
+        if (QV || p.qv_out) {
+            C q = (z + ld(p.fv + v)) / eta;
+            if (QV) p.qv[v] = I(q);
+            if (p.qv_out) {
+                p.qv_out[v] = double(q);
+                p.zv_out[v] = double(z);
+                p.hv_out[v] = double(eta);
+            }
+        }
         if (ZH) {
             p.zv[v] = I(z);
             p.hv[v] = I(eta);
-        }
-        if (p.qv_out) {
-            p.qv_out[v] = double(q);
-            p.zv_out[v] = double(z);
-            p.hv_out[v] = double(eta);
         }
         if (VK) {
             C s = 0;
@@ -164,10 +168,12 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
         const Index<W::cw> CV(p.cv16p, p.cv, p.nc, i, 2 * i);
         C s = 0;
         for (int k = 0; k < W::kc(p); ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[CV[k]]);
-        double K = p.K[i] + double(s);
         p.B[i] = p.B[i] + double(s);
-        p.K[i] = K;
-        if (p.K_out) p.K_out[i] = K;
+        if (p.K_out) {
+            double K = p.K[i] + double(s);
+            p.K[i] = K;
+            p.K_out[i] = K;
+        }
     }
 }
 
@@ -177,7 +183,11 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
 // GSLOT (12 Gassmann slots in the order of gassmann_coefficients): the PV of slot k is the q
 // of the pair already gathered for slot gassmann_pv_slot(k), so neither the PV-edge table nor
 // a second gather is needed; the products and their order are those of the table form.
-template <class P, bool GM, bool GMEAN, class W = AnyWidths, bool GSLOT = false>
+// GF (flux form, Layout::gflux): the pairs hold G = l F and the continuity is
+// -(sum_k s_k G_k)/A_i, with the signs from csign.  GREC (FP32-bulk slot form): each edge
+// reads its slot edges, cells, 1/d_e and weights as one record (Params::rg) with five 16-byte
+// loads instead of 20 slot-major loads; the products and their order are unchanged.
+template <class P, bool GM, bool GMEAN, class W = AnyWidths, bool GSLOT = false, bool GF = false, bool GREC = false>
 __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
@@ -185,6 +195,43 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
     Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
         int64_t e = t.first + threadIdx.x;
+        if constexpr (GREC) {
+            static_assert(GSLOT && P::id == TRISK_FP32_BULK, "records: FP32-bulk slot form");
+            // the block's records staged through LDS with coalesced 16-byte loads; at a stride
+            // of 80 bytes the 16-byte LDS reads of 16 lanes fall on distinct banks
+            __shared__ __attribute__((aligned(16))) int32_t srec[kBlock * 20];
+            if (t.first >= p.ne) return;  // the whole block
+            const int64_t n = p.ne - t.first < kBlock ? p.ne - t.first : kBlock;
+            const f32x4* src = reinterpret_cast<const f32x4*>(p.rg + 20 * t.first);
+            for (int w = threadIdx.x; w < int(5 * n); w += kBlock)
+                reinterpret_cast<f32x4*>(srec)[w] = __builtin_nontemporal_load(src + w);
+            __syncthreads();
+            if (e >= p.ne) return;
+            const RkIn rin = rk_load(r, e);
+            double du;
+            int32_t rec[20];
+            __builtin_memcpy(rec, &srec[20 * threadIdx.x], 80);
+            IndexWords<12> gw16;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) gw16.w[q] = rec[q];
+            IndexWords<2> ec16;
+            ec16.w[0] = rec[6];
+            const Index<2> EC(ec16, p.ec, p.ne, e, e / 3);
+            const Index<12> GE(gw16, p.ge, p.ne, e, e);
+            float invd, wk[12];
+            __builtin_memcpy(&invd, &rec[7], 4);
+            __builtin_memcpy(wk, &rec[8], 48);
+            const double grad = (p.B[EC[1]] - p.B[EC[0]]) * double(invd);
+            Pair<I> f[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) f[k] = p.Fq[GE[k]];
+            C s = 0;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) s += wk[k] * C(f[k].F) * C(f[gassmann_pv_slot(k)].q);
+            du = double(s) - grad;
+            rk_store<U, kStreamAll>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+            return;
+        }
         if (e >= p.ne) return;
         const RkIn rin = rk_load(r, e);
         double du;
@@ -228,7 +275,18 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
         const RkIn rin = rk_load(r, p.ne + i);
         const Index<W::cw> CE(p.ce16p, p.ce, p.nc, i, 3 * i);
         double s = 0.0;
-        for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
+        if constexpr (GF) {
+            const uint32_t m = ld(p.csign + i);
+            const double area = ld(p.area_c + i);
+            for (int k = 0; k < W::kc(p); ++k) {
+                const double G = double(p.Fq[CE[k]].F);
+                const uint32_t b = (m >> (2 * k)) & 3u;
+                s += b == 1u ? G : b == 2u ? -G : 0.0;
+            }
+            s = s / area;
+        } else {
+            for (int k = 0; k < W::kc(p); ++k) s += ld(p.div + k * p.nc + i) * double(p.Fq[CE[k]].F);
+        }
         rk_store<double, kStreamAll>(r, p.ne + i, -s, rin, r.mode >= 1 && r.mode <= 3 ? r.sh + i : nullptr);
     }
 }
@@ -426,6 +484,13 @@ Params<Fp32Bulk>& params<Fp32Bulk>(trisk_handle* h) {
     return h->p32;
 }
 
+// The FP32-bulk table kernels run in flux form when the layout allows it; FP64 keeps the
+// reference's divergence weights (bit-exact).
+template <class P>
+static bool flux_form(const trisk_handle* H) {
+    return P::id == TRISK_FP32_BULK && H->L.gflux;
+}
+
 template <class P>
 static int setup_params(trisk_handle* H) {
     using C = typename P::C;
@@ -448,23 +513,60 @@ static int setup_params(trisk_handle* H) {
             return upload(o, to_float(v), reinterpret_cast<const float**>(out));
         }
     };
+    // flux form (FP32-bulk): the edge heights times l_e, so that the edge kernel forms G = l F,
+    // and the momentum weights over l of the edge they multiply
+    const bool gf = flux_form<P>(H);
+    auto flux = [&](const std::vector<double>& v, const std::vector<int32_t>* idx, const C** out) -> int {
+        if constexpr (sizeof(C) == sizeof(float)) {
+            std::vector<float> w(v.size());
+            const int64_t ne = L.ne;
+#pragma omp parallel for schedule(static)
+            for (int64_t j = 0; j < int64_t(v.size()); ++j)
+                w[j] = float(idx ? v[j] / L.len[(*idx)[j]] : v[j] * L.len[j % ne]);
+            return upload(o, w, reinterpret_cast<const float**>(out));
+        } else {
+            return coef(v, out);
+        }
+    };
     int rc = 0;
     rc |= upload(o, L.ce, &p.ce);
-    rc |= upload(o, L.div, &p.div);
+    if (gf)
+        rc |= upload(o, L.csign, &p.csign);
+    else
+        rc |= upload(o, L.div, &p.div);
     rc |= coef(L.kin, &p.kin);
     rc |= upload(o, L.cv, &p.cv);
     rc |= coef(L.vkk, &p.vkk);
     rc |= upload(o, L.area_c, &p.area_c);
     rc |= upload(o, L.ec, &p.ec);
     rc |= upload(o, L.ev, &p.ev);
-    rc |= coef(L.eh, &p.eh);
+    rc |= gf ? flux(L.eh, nullptr, &p.eh) : coef(L.eh, &p.eh);
     rc |= coef(L.invd, &p.invd);
     rc |= upload(o, L.te, &p.te);
-    rc |= coef(L.tw, &p.tw);
-    rc |= coef(L.vkh, &p.vkh);
+    rc |= gf ? flux(L.tw, &L.te, &p.tw) : coef(L.tw, &p.tw);
+    rc |= gf ? flux(L.vkh, nullptr, &p.vkh) : coef(L.vkh, &p.vkh);
     rc |= upload(o, L.ge, &p.ge);
     rc |= upload(o, L.gp, &p.gp);
-    rc |= coef(L.gw, &p.gw);
+    rc |= gf ? flux(L.gw, &L.ge, &p.gw) : coef(L.gw, &p.gw);
+    p.rg = nullptr;
+    if (P::id == TRISK_FP32_BULK && L.gassmann_slots && L.kc == 6 && L.kv == 3 && L.kt == 10 && L.kg == 12) {
+        // the Gassmann edge records of the slot form (Params::rg)
+        const int64_t ne = L.ne;
+        std::vector<int32_t> rec(size_t(20) * ne);
+        for (int64_t e = 0; e < ne; ++e) {
+            int32_t* r = &rec[size_t(20) * e];
+            for (int q = 0; q < 6; ++q) r[q] = L.ge16p[size_t(q) * ne + e];
+            r[6] = L.ec16p[e];
+            const float invd = float(L.invd[e]);
+            std::memcpy(&r[7], &invd, 4);
+            for (int k = 0; k < 12; ++k) {
+                const size_t j = size_t(k) * ne + e;
+                const float w = float(gf ? L.gw[j] / L.len[L.ge[j]] : L.gw[j]);
+                std::memcpy(&r[8 + k], &w, 4);
+            }
+        }
+        rc |= upload(o, rec, &p.rg);
+    }
     rc |= coef(L.grh, &p.grh);
     rc |= coef(L.gf, &p.gf);
     rc |= upload(o, L.ve, &p.ve);
@@ -578,6 +680,26 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     return 0;
 }
 
+template <class P, class W, bool GF>
+static void launch_update(trisk_handle* H, dim3 grid, hipStream_t s, const Rk& r, int64_t t0, int64_t t1) {
+    const Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    if (!L.gassmann)
+        hipLaunchKernelGGL((update_kernel<P, false, false, W, false, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+    else if (L.gassmann_mean)
+        hipLaunchKernelGGL((update_kernel<P, true, true, W, false, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+    else if constexpr (W::gw == 12) {
+        if (p.rg)
+            hipLaunchKernelGGL((update_kernel<P, true, false, W, true, GF, P::id == TRISK_FP32_BULK>), grid, dim3(kBlock),
+                               0, s, p, r, t0, t1);
+        else if (L.gassmann_slots)
+            hipLaunchKernelGGL((update_kernel<P, true, false, W, true, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        else
+            hipLaunchKernelGGL((update_kernel<P, true, false, W, false, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+    } else
+        hipLaunchKernelGGL((update_kernel<P, true, false, W, false, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+}
+
 template <class P, class UIn, class W>
 static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
     const Params<P>& p = params<P>(H);
@@ -591,9 +713,15 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
         int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_CELL_VERTEX, s, &stop))) return rc;
-        if (vk)
+        if (vk && gm)
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W, false>), grid, dim3(kBlock), 0, s, p, u,
+                               h, t0, t1);
+        else if (vk)
             hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0,
                                t1);
+        else if (gm)
+            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W, false>), grid, dim3(kBlock), 0, s, p,
+                               u, h, t0, t1);
         else if (zh)
             hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W>), grid, dim3(kBlock), 0, s, p, u, h,
                                t0, t1);
@@ -625,17 +753,10 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
         int64_t t0 = tiles_per_xcd(L.ne), t1 = tiles_per_xcd(L.nc);
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_UPDATE, s, &stop))) return rc;
-        if (!gm)
-            hipLaunchKernelGGL((update_kernel<P, false, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
-        else if (L.gassmann_mean)
-            hipLaunchKernelGGL((update_kernel<P, true, true, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
-        else if constexpr (W::gw == 12) {
-            if (L.gassmann_slots)
-                hipLaunchKernelGGL((update_kernel<P, true, false, W, true>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
-            else
-                hipLaunchKernelGGL((update_kernel<P, true, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
-        } else
-            hipLaunchKernelGGL((update_kernel<P, true, false, W>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
+        if (flux_form<P>(H))
+            launch_update<P, W, true>(H, grid, s, r, t0, t1);
+        else
+            launch_update<P, W, false>(H, grid, s, r, t0, t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -873,6 +994,7 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
         H->L.pair_reason = o.precision == TRISK_FP64 ? "FP64: coefficient tables, bit-exact with the reference"
                                                      : "coefficient tables requested";
     }
+    if (o.precision == TRISK_FP32_BULK && !H->L.pair) build_flux_form(desc, H->L);
     int ndev = 0;
     hipError_t he = hipGetDeviceCount(&ndev);
     if (he != hipSuccess || o.device < 0 || o.device >= ndev || hipSetDevice(o.device) != hipSuccess) {
@@ -1278,13 +1400,18 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const double KG = L.kg;
         const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
         const double ix = hex ? 2.0 : iw;
-        double p1 = nc * (KC * (ix + c) + 8.0 + 8.0 + (vk ? 8.0 : 0.0)) + ne * uw +
-                    nv * (KV * 2 * (ix + c) + c + c + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
+        // q_v is stored only for the TRiSK flux (the Gassmann edge PV comes from zeta_v, h_v)
+        double p1 = nc * (KC * (ix + c) + 8.0 + 8.0) + ne * uw +
+                    nv * (KV * 2 * (ix + c) + c + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
+        // with vertex kinetic energy P1 writes B and P2 adds the vertex part (K only for the
+        // diagnostics)
         double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
-                    nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 32.0) + nv * c : 0.0);
+                    nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 16.0) + nv * c : 0.0);
         const bool gp_table = !L.gassmann_mean && !(hex && L.gassmann_slots);
         const double pv = gm ? KG * (ix + c + (gp_table ? ix : 0.0)) : KT * (ix + c);
-        double p3 = ne * (pv + 2 * ix + c + 2 * c + rk_e) + nc * (KC * (ix + 8.0) + 8.0 + rk_c);
+        // flux form (FP32-bulk): the continuity reads the 2-bit signs and A_i, not the divergence
+        const double cont = flux_form<Fp32Bulk>(h) && !f64 ? KC * ix + 4.0 + 8.0 : KC * (ix + 8.0);
+        double p3 = ne * (pv + 2 * ix + c + 2 * c + rk_e) + nc * (cont + rk_c);
         v = {{kKernelNames[K_CELL_VERTEX], p1}, {kKernelNames[K_EDGE], p2}, {kKernelNames[K_UPDATE], p3}};
     }
     int k = 0;

@@ -88,6 +88,7 @@ struct Params {
     const C* gw;
     const C* grh;
     const C* gf;
+    const uint32_t* csign;  // nc: flux form (Layout::gflux), 2 sign bits per cell slot
     // vertices
     const int32_t* ve;
     const C* curl;
@@ -129,6 +130,10 @@ struct Params {
     const int32_t* re_cells16;  // ne
     const int32_t* pc_ev16;     // 6 x nc
     const int32_t* rc_pair;     // nc x 13 (FP32-bulk cell kernel records)
+    // FP32-bulk Gassmann slot form: one 80-byte record per edge, the 12 slot edges as 16-bit
+    // deltas (6 words, kWide: the int32 table ge), the two cells (one word, kWide: ec), the
+    // FP32 bits of 1/d_e and of the 12 weights
+    const int32_t* rg;          // ne x 20
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;

@@ -588,6 +588,37 @@ bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) 
     return true;
 }
 
+void build_flux_form(const trisk_model_desc* d, Layout& L) {
+    L.gflux = false;
+    L.csign.clear();
+    if (!d->primal_lengths || L.kc > 16) return;
+    const int64_t nc = L.nc, ne = L.ne;
+    const int KC = L.kc;
+    const double* l = d->primal_lengths;
+    std::vector<uint32_t> sign(nc, 0);
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t j = 0; j < nc; ++j) {
+        const int64_t i = L.cperm[j];
+        uint32_t m = 0;
+        for (int k = 0; k < KC; ++k) {
+            const double div = d->divergence[k + KC * i];
+            if (div == 0.0) continue;  // padded slot
+            const double sg = div > 0 ? 1.0 : -1.0;
+            const double ref = (sg * l[d->cell_edges[k + KC * i]]) / d->cell_areas[i];
+            bad |= !(std::fabs(ref - div) <= 1e-14 * std::fabs(div));
+            m |= uint32_t(sg > 0 ? 1 : 2) << (2 * k);
+        }
+        sign[j] = m;
+    }
+    for (int64_t e = 0; e < ne; ++e) bad |= !(l[e] > 0.0);
+    if (bad) return;
+    L.len.resize(ne);
+    for (int64_t j = 0; j < ne; ++j) L.len[j] = l[L.eperm[j]];
+    L.csign = std::move(sign);
+    L.gflux = true;
+}
+
 void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
     L.pair = false;
     if (!d->primal_lengths || !d->dual_lengths || !d->cell_kites) {

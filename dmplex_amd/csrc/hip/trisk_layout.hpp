@@ -47,6 +47,13 @@ struct Layout {
     std::vector<int32_t> ce16p, cv16p, ec16p, ev16p, te16p, ge16p, gp16p, ve16p, vc16p;
     std::vector<int32_t> re16p;  // ICON: RBF edges (base e)
     int64_t wide_table_entities = 0;
+    // flux form of the FP32-bulk table kernels: the edge kernel stores G_e = l_e F_e (edge
+    // heights scaled by l_e on upload, momentum weights by 1/l of their edge) and the
+    // continuity is dh_i = -(sum_k s_k G_k)/A_i, which reads 2 bits per slot (csign: 1 for
+    // s = +1, 2 for s = -1, 0 for a padded slot) instead of the FP64 divergence weights.
+    // Needs the edge lengths and the divergence equal to (s l_e)/A_i; len holds l_e.
+    bool gflux = false;
+    std::vector<uint32_t> csign;  // nc
     // cell-pair path (TRiSK PV flux and cell kinetic energy only): per cell the 15
     // antisymmetric TRiSK weights w_{k,k'} (k < k') of its edge pairs
     bool pair = false;
@@ -102,6 +109,10 @@ inline int pair_index(int k, int k2) {  // k < k2 < 6
 // Validates `d` (every index in range, widths positive) and fills `L`.  Returns false
 // with a message on a bad table.
 bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err);
+
+// Sets L.gflux, L.csign and L.len when d gives the edge lengths and every divergence weight
+// is (s l_e)/A_i to 1e-14 (cells of up to 16 edges); otherwise L.gflux stays false.
+void build_flux_form(const trisk_model_desc* d, Layout& L);
 
 // Validates an ICON descriptor (triangles, every index in range) and fills the generic
 // fields of `L` (sizes, permutations, ce, div, ec, ev, eh, invd, ve, curl, vc, vh, fv,
