@@ -77,21 +77,42 @@ using HexWidths = Widths<6, 3, 10, 12>;
 // ---------------------------------------------------------------------------------
 // P1: cells (K, B) | vertices (q_v [, zeta_v, h_v] [, K_v])
 
-// QV false: q_v is not stored (the Gassmann flux forms the edge PV from zeta_v and h_v)
-template <class P, class UIn, class HIn, bool VK, bool ZH, class W = AnyWidths, bool QV = true>
+// QV false: q_v is not stored (the Gassmann flux forms the edge PV from zeta_v and h_v).
+// REC (FP32-bulk, hexagon widths): the rows come from the entity records r1c / r1v staged
+// through LDS instead of the slot-major arrays; the values and the order of the sums are
+// the same.
+template <class P, class UIn, class HIn, bool VK, bool ZH, class W = AnyWidths, bool QV = true, bool REC = false>
 __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const UIn* __restrict__ u,
                                                              const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
+    static_assert(!REC || (P::id == TRISK_FP32_BULK && W::cw == 6 && W::vw == 3), "records: FP32-bulk hexagons");
+    constexpr int RV = VK ? 14 : 11;
+    __shared__ __attribute__((aligned(16))) int32_t lds[REC ? kBlock * RV : 1];
     Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
         int64_t i = t.first + threadIdx.x;
-        if (i >= p.nc) return;
-        const Index<W::cw> E(p.ce16p, p.ce, p.nc, i, 3 * i);
+        C kin[W::cw ? W::cw : 1];
+        IndexWords<W::cw> ew;
+        if constexpr (REC) {
+            if (!stage_records<9>(p.r1c, t.first, p.nc, lds) || i >= p.nc) return;
+            const int32_t* r = lds + 9 * threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) ew.w[q] = r[q];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) kin[k] = fbits(r[3 + k]);
+        } else {
+            if (i >= p.nc) return;
+            ew = index_words<W::cw>(p.ce16p, p.nc, i);
+        }
+        const Index<W::cw> E(ew, p.ce, p.nc, i, 3 * i);
         C s = 0;
         for (int k = 0; k < W::kc(p); ++k) {
             C ue = C(u[E[k]]);
-            s += ld(p.kin + k * p.nc + i) * (ue * ue);
+            if constexpr (REC)
+                s += kin[k] * (ue * ue);
+            else
+                s += ld(p.kin + k * p.nc + i) * (ue * ue);
         }
         double hi = double(h[i]);
         double K = double(s);
@@ -105,13 +126,39 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
         }
     } else {
         int64_t v = t.first + threadIdx.x;
-        if (v >= p.nv) return;
-        const Index<W::vw> VE(p.ve16p, p.ve, p.nv, v, (3 * v) / 2), VC(p.vc16p, p.vc, p.nv, v, v / 2);
+        IndexWords<W::vw> vew, vcw;
+        C cu[W::vw ? W::vw : 1], ch[W::vw ? W::vw : 1], kw[W::vw ? W::vw : 1], f = 0;
+        if constexpr (REC) {
+            if (!stage_records<RV>(p.r1v, t.first, p.nv, lds) || v >= p.nv) return;
+            const int32_t* r = lds + RV * threadIdx.x;
+            vew.w[0] = r[0];
+            vew.w[1] = r[1];
+            vcw.w[0] = r[2];
+            vcw.w[1] = r[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                cu[k] = fbits(r[4 + k]);
+                ch[k] = fbits(r[7 + k]);
+                if (VK) kw[k] = fbits(r[11 + k]);
+            }
+            f = fbits(r[10]);
+        } else {
+            if (v >= p.nv) return;
+            vew = index_words<W::vw>(p.ve16p, p.nv, v);
+            vcw = index_words<W::vw>(p.vc16p, p.nv, v);
+        }
+        const Index<W::vw> VE(vew, p.ve, p.nv, v, (3 * v) / 2), VC(vcw, p.vc, p.nv, v, v / 2);
         C z = 0, eta = 0;
-        for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[VE[k]]);
-        for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[VC[k]]);
+        if constexpr (REC) {
+            for (int k = 0; k < 3; ++k) z += cu[k] * C(u[VE[k]]);
+            for (int k = 0; k < 3; ++k) eta += ch[k] * C(h[VC[k]]);
+        } else {
+            for (int k = 0; k < W::kv(p); ++k) z += ld(p.curl + k * p.nv + v) * C(u[VE[k]]);
+            for (int k = 0; k < W::kv(p); ++k) eta += ld(p.vh + k * p.nv + v) * C(h[VC[k]]);
+        }
         if (QV || p.qv_out) {
-            C q = (z + ld(p.fv + v)) / eta;
+            if constexpr (!REC) f = ld(p.fv + v);
+            C q = (z + f) / eta;
             if (QV) p.qv[v] = I(q);
             if (p.qv_out) {
                 p.qv_out[v] = double(q);
@@ -127,7 +174,10 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
             C s = 0;
             for (int k = 0; k < W::kv(p); ++k) {
                 C ue = C(u[VE[k]]);
-                s += ld(p.vkw + k * p.nv + v) * (ue * ue);
+                if constexpr (REC)
+                    s += kw[k] * (ue * ue);
+                else
+                    s += ld(p.vkw + k * p.nv + v) * (ue * ue);
             }
             p.Kv[v] = I(s);
         }
@@ -137,26 +187,62 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
 // ---------------------------------------------------------------------------------
 // P2: edges (F, q_e) | cells (vertex part of K, B; only with vertex kinetic energy)
 
-template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths>
+// REC (FP32-bulk, hexagon widths): the rows from the records r2e / r2c staged through LDS.
+template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths, bool REC = false>
 __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __restrict__ u,
                                                       const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
+    static_assert(!REC || (P::id == TRISK_FP32_BULK && W::cw == 6), "records: FP32-bulk hexagons");
+    constexpr int RE = 4 + (VK ? 2 : 0) + (GM ? 3 : 0);
+    __shared__ __attribute__((aligned(16))) int32_t lds[REC ? kBlock * 9 : 1];
     Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
         int64_t e = t.first + threadIdx.x;
-        if (e >= p.ne) return;
-        const Index<W::cw ? 2 : 0> EC(p.ec16p, p.ec, p.ne, e, e / 3), EV(p.ev16p, p.ev, p.ne, e, (2 * e) / 3);
+        IndexWords<W::cw ? 2 : 0> ecw, evw;
+        C eh0, eh1, vk0 = 0, vk1 = 0, r0 = 0, r1 = 0, fe = 0;
+        if constexpr (REC) {
+            if (!stage_records<RE>(p.r2e, t.first, p.ne, lds) || e >= p.ne) return;
+            const int32_t* r = lds + RE * threadIdx.x;
+            ecw.w[0] = r[0];
+            evw.w[0] = r[1];
+            eh0 = fbits(r[2]);
+            eh1 = fbits(r[3]);
+            if (VK) {
+                vk0 = fbits(r[4]);
+                vk1 = fbits(r[5]);
+            }
+            if (GM) {
+                r0 = fbits(r[RE - 3]);
+                r1 = fbits(r[RE - 2]);
+                fe = fbits(r[RE - 1]);
+            }
+        } else {
+            if (e >= p.ne) return;
+            ecw = index_words<W::cw ? 2 : 0>(p.ec16p, p.ne, e);
+            evw = index_words<W::cw ? 2 : 0>(p.ev16p, p.ne, e);
+            eh0 = ld(p.eh + e);
+            eh1 = ld(p.eh + p.ne + e);
+            if (VK) {
+                vk0 = ld(p.vkh + e);
+                vk1 = ld(p.vkh + p.ne + e);
+            }
+            if (GM) {
+                r0 = ld(p.grh + e);
+                r1 = ld(p.grh + p.ne + e);
+                fe = ld(p.gf + e);
+            }
+        }
+        const Index<W::cw ? 2 : 0> EC(ecw, p.ec, p.ne, e, e / 3), EV(evw, p.ev, p.ne, e, (2 * e) / 3);
         const int32_t c0 = EC[0], c1 = EC[1], v0 = EV[0], v1 = EV[1];
-        C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
-        if (VK) he = he + ld(p.vkh + e) * C(p.hv[v0]) + ld(p.vkh + p.ne + e) * C(p.hv[v1]);
+        C he = eh0 * C(h[c0]) + eh1 * C(h[c1]);
+        if (VK) he = he + vk0 * C(p.hv[v0]) + vk1 * C(p.hv[v1]);
         C F = he * C(u[e]);
         C q;
         if (GM) {
-            C r0 = ld(p.grh + e), r1 = ld(p.grh + p.ne + e);
             C z = r0 * C(p.zv[v0]) + r1 * C(p.zv[v1]);
             C eta = r0 * C(p.hv[v0]) + r1 * C(p.hv[v1]);
-            q = (z + ld(p.gf + e)) / eta;
+            q = (z + fe) / eta;
         } else {
             q = (C(p.qv[v0]) + C(p.qv[v1])) / 2;
         }
@@ -164,10 +250,27 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
     } else {
         if (!VK) return;
         int64_t i = t.first + threadIdx.x;
-        if (i >= p.nc) return;
-        const Index<W::cw> CV(p.cv16p, p.cv, p.nc, i, 2 * i);
+        IndexWords<W::cw> cvw;
+        C kk[W::cw ? W::cw : 1];
+        if constexpr (REC) {
+            if (!stage_records<9>(p.r2c, t.first, p.nc, lds) || i >= p.nc) return;
+            const int32_t* r = lds + 9 * threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) cvw.w[q] = r[q];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) kk[k] = fbits(r[3 + k]);
+        } else {
+            if (i >= p.nc) return;
+            cvw = index_words<W::cw>(p.cv16p, p.nc, i);
+        }
+        const Index<W::cw> CV(cvw, p.cv, p.nc, i, 2 * i);
         C s = 0;
-        for (int k = 0; k < W::kc(p); ++k) s += ld(p.vkk + k * p.nc + i) * C(p.Kv[CV[k]]);
+        for (int k = 0; k < W::kc(p); ++k) {
+            if constexpr (REC)
+                s += kk[k] * C(p.Kv[CV[k]]);
+            else
+                s += ld(p.vkk + k * p.nc + i) * C(p.Kv[CV[k]]);
+        }
         p.B[i] = p.B[i] + double(s);
         if (p.K_out) {
             double K = p.K[i] + double(s);
@@ -548,6 +651,64 @@ static int setup_params(trisk_handle* H) {
     rc |= upload(o, L.ge, &p.ge);
     rc |= upload(o, L.gp, &p.gp);
     rc |= gf ? flux(L.gw, &L.ge, &p.gw) : coef(L.gw, &p.gw);
+    p.r1c = p.r1v = p.r2e = p.r2c = nullptr;
+    if (P::id == TRISK_FP32_BULK && !L.pair && L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12)) {
+        // the P1 and P2 records of the FP32-bulk table kernels (Params::r1c, r1v, r2e, r2c)
+        const int64_t nc = L.nc, ne = L.ne, nv = L.nv;
+        const bool vk = L.vertex_ke, gm = L.gassmann;
+        auto bits = [](double x) {
+            const float f = float(x);
+            int32_t w;
+            std::memcpy(&w, &f, 4);
+            return w;
+        };
+        std::vector<int32_t> rc1(size_t(9) * nc), rc2(vk ? size_t(9) * nc : 0);
+        for (int64_t i = 0; i < nc; ++i) {
+            for (int q = 0; q < 3; ++q) rc1[9 * i + q] = L.ce16p[size_t(q) * nc + i];
+            for (int k = 0; k < 6; ++k) rc1[9 * i + 3 + k] = bits(L.kin[size_t(k) * nc + i]);
+            if (!vk) continue;
+            for (int q = 0; q < 3; ++q) rc2[9 * i + q] = L.cv16p[size_t(q) * nc + i];
+            for (int k = 0; k < 6; ++k) rc2[9 * i + 3 + k] = bits(L.vkk[size_t(k) * nc + i]);
+        }
+        const int RV = vk ? 14 : 11;
+        std::vector<int32_t> rv(size_t(RV) * nv);
+        for (int64_t v = 0; v < nv; ++v) {
+            int32_t* r = &rv[size_t(RV) * v];
+            for (int q = 0; q < 2; ++q) {
+                r[q] = L.ve16p[size_t(q) * nv + v];
+                r[2 + q] = L.vc16p[size_t(q) * nv + v];
+            }
+            for (int k = 0; k < 3; ++k) {
+                r[4 + k] = bits(L.curl[size_t(k) * nv + v]);
+                r[7 + k] = bits(L.vh[size_t(k) * nv + v]);
+                if (vk) r[11 + k] = bits(L.vkw[size_t(k) * nv + v]);
+            }
+            r[10] = bits(L.fv[v]);
+        }
+        const int RE = 4 + (vk ? 2 : 0) + (gm ? 3 : 0);
+        std::vector<int32_t> re(size_t(RE) * ne);
+        for (int64_t e = 0; e < ne; ++e) {
+            int32_t* r = &re[size_t(RE) * e];
+            const double l = gf ? L.len[e] : 1.0;  // flux form: G = l F
+            r[0] = L.ec16p[e];
+            r[1] = L.ev16p[e];
+            r[2] = bits(L.eh[e] * l);
+            r[3] = bits(L.eh[ne + e] * l);
+            if (vk) {
+                r[4] = bits(L.vkh[e] * l);
+                r[5] = bits(L.vkh[ne + e] * l);
+            }
+            if (gm) {
+                r[RE - 3] = bits(L.grh[e]);
+                r[RE - 2] = bits(L.grh[ne + e]);
+                r[RE - 1] = bits(L.gf[e]);
+            }
+        }
+        rc |= upload(o, rc1, &p.r1c);
+        rc |= upload(o, rv, &p.r1v);
+        rc |= upload(o, re, &p.r2e);
+        if (vk) rc |= upload(o, rc2, &p.r2c);
+    }
     p.rg = nullptr;
     if (P::id == TRISK_FP32_BULK && L.gassmann_slots && L.kc == 6 && L.kv == 3 && L.kt == 10 && L.kg == 12) {
         // the Gassmann edge records of the slot form (Params::rg)
@@ -700,6 +861,40 @@ static void launch_update(trisk_handle* H, dim3 grid, hipStream_t s, const Rk& r
         hipLaunchKernelGGL((update_kernel<P, true, false, W, false, GF>), grid, dim3(kBlock), 0, s, p, r, t0, t1);
 }
 
+template <class P, class UIn, class W, bool REC>
+static void launch_p1(const Params<P>& p, dim3 grid, hipStream_t s, const UIn* u, const double* h, int64_t t0,
+                      int64_t t1, bool vk, bool gm, bool zh) {
+    if (vk && gm)
+        hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W, false, REC>), grid, dim3(kBlock), 0, s, p,
+                           u, h, t0, t1);
+    else if (vk)
+        hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W, true, REC>), grid, dim3(kBlock), 0, s, p, u,
+                           h, t0, t1);
+    else if (gm)
+        hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W, false, REC>), grid, dim3(kBlock), 0, s, p,
+                           u, h, t0, t1);
+    else if (zh)
+        hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W, true, REC>), grid, dim3(kBlock), 0, s, p,
+                           u, h, t0, t1);
+    else
+        hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, false, W, true, REC>), grid, dim3(kBlock), 0, s, p,
+                           u, h, t0, t1);
+}
+
+template <class P, class UIn, class W, bool REC>
+static void launch_p2(const Params<P>& p, dim3 grid, hipStream_t s, const UIn* u, const double* h, int64_t t0,
+                      int64_t t1, bool vk, bool gm) {
+    if (vk && gm)
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+    else if (vk)
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+    else if (gm)
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, true, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+    else
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, false, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+                           t1);
+}
+
 template <class P, class UIn, class W>
 static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s) {
     const Params<P>& p = params<P>(H);
@@ -708,26 +903,18 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
     const bool zh = gm || vk || p.qv_out != nullptr;
     hipEvent_t stop;
     int rc;
+    // records (FP32-bulk, hexagon widths) when the handle holds them
+    constexpr bool can_rec = P::id == TRISK_FP32_BULK && W::cw == 6 && W::vw == 3;
+    const bool rec = can_rec && p.r1c != nullptr;
     // P1
     {
         int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_CELL_VERTEX, s, &stop))) return rc;
-        if (vk && gm)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W, false>), grid, dim3(kBlock), 0, s, p, u,
-                               h, t0, t1);
-        else if (vk)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, true, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0,
-                               t1);
-        else if (gm)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W, false>), grid, dim3(kBlock), 0, s, p,
-                               u, h, t0, t1);
-        else if (zh)
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, true, W>), grid, dim3(kBlock), 0, s, p, u, h,
-                               t0, t1);
+        if (rec)
+            launch_p1<P, UIn, W, can_rec>(p, grid, s, u, h, t0, t1, vk, gm, zh);
         else
-            hipLaunchKernelGGL((cell_vertex_kernel<P, UIn, double, false, false, W>), grid, dim3(kBlock), 0, s, p, u, h,
-                               t0, t1);
+            launch_p1<P, UIn, W, false>(p, grid, s, u, h, t0, t1, vk, gm, zh);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -736,15 +923,10 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
         int64_t t0 = tiles_per_xcd(L.ne), t1 = vk ? tiles_per_xcd(L.nc) : 0;
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_EDGE, s, &stop))) return rc;
-        if (vk && gm)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
-        else if (vk)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
-        else if (gm)
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, true, W>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+        if (rec)
+            launch_p2<P, UIn, W, can_rec>(p, grid, s, u, h, t0, t1, vk, gm);
         else
-            hipLaunchKernelGGL((edge_kernel<P, UIn, double, false, false, W>), grid, dim3(kBlock), 0, s, p, u, h, t0,
-                               t1);
+            launch_p2<P, UIn, W, false>(p, grid, s, u, h, t0, t1, vk, gm);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }

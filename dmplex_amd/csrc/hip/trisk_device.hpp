@@ -134,6 +134,12 @@ struct Params {
     // deltas (6 words, kWide: the int32 table ge), the two cells (one word, kWide: ec), the
     // FP32 bits of 1/d_e and of the 12 weights
     const int32_t* rg;          // ne x 20
+    // FP32-bulk table kernels on hexagon meshes: the slot-major tables of P1 and P2 as entity
+    // records (16-bit index words first, then the FP32 bits of the coefficients), staged
+    // through LDS: P1 cells [ce 3 words, kin 6], vertices [ve 2, vc 2, curl 3, vh 3, f, vkw 3
+    // with vertex KE]; P2 edges [ec, ev, eh 2, vkh 2 with vertex KE, rhombus 2 and f_e with
+    // the Gassmann flux], cells [cv 3, vkk 6] (vertex KE)
+    const int32_t *r1c, *r1v, *r2e, *r2c;
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
     double* qv_out;
@@ -204,6 +210,30 @@ __device__ __forceinline__ void ldrec(const T* __restrict__ p, T (&out)[N]) {
     }
 }
 
+
+// Copies the records of entities [first, min(first + kBlock, n)), R int32 words each, into
+// LDS with coalesced 16-byte nontemporal loads (first is a multiple of kBlock, so the span is
+// 16-byte aligned) and waits for the block; false (for the whole block) past the range's end.
+template <int R>
+__device__ __forceinline__ bool stage_records(const int32_t* __restrict__ rec, int64_t first, int64_t n,
+                                              int32_t* lds) {
+    if (first >= n) return false;
+    const int64_t cnt = n - first < kBlock ? n - first : kBlock;
+    const int nw = int(cnt * R), nw4 = nw / 4;
+    const int32_t* src = rec + first * R;
+    for (int w = threadIdx.x; w < nw4; w += kBlock)
+        reinterpret_cast<f32x4*>(lds)[w] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + w);
+    for (int w = 4 * nw4 + threadIdx.x; w < nw; w += kBlock) lds[w] = ld(src + w);
+    __syncthreads();
+    return true;
+}
+
+// The FP32 value whose bits are word w
+__device__ __forceinline__ float fbits(int32_t w) {
+    float f;
+    __builtin_memcpy(&f, &w, 4);
+    return f;
+}
 
 // ---------------------------------------------------------------------------------
 // RK4 stage update (rk4_step!, RungeKutta.jl:18-28)
