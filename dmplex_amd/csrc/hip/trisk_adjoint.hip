@@ -1,5 +1,6 @@
-// Adjoint of the TRiSK shallow-water tendency and of the RK4 step on the cell-pair path
-// (trisk_tendency_vjp, trisk_rk4_adjoint in include/trisk_hip.h).
+// Adjoint of the TRiSK shallow-water tendency and of the RK4 step (trisk_tendency_vjp,
+// trisk_rk4_adjoint in include/trisk_hip.h), on the cell-pair path (below) and on the table
+// path (every PV flux and kinetic-energy form; "Table path" further down).
 //
 // The reference has no adjoint; these kernels transpose the linearisation of the forward
 // cell-pair tendency (trisk_device.hip) of tendency! (TRiSKShallowWater.jl:379-394, TRiSK
@@ -238,6 +239,216 @@ __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjPara
     adj_store(r, p.ne + i, s, rin);
 }
 
+// ---------------------------------------------------------------------------------
+// Table path: the adjoint of the table kernels (trisk_device.hip P1-P3) for every PV flux
+// (TRiSK, Gassmann, Gassmann mean; TRiSKShallowWater.jl:261-296) and kinetic-energy form
+// (cell, mixed, vertex; :202-244).  With the forward pieces
+//   B_i = sum_k kin u_k^2 [+ sum_k kites K_v] + g h_i,  K_v = sum_k vkw u_k^2
+//   zeta_v = sum curl u,  h_v = sum vh h,  q_v = (zeta_v + f_v) / h_v
+//   F_e = (eh_0 h_c0 + eh_1 h_c1 [+ vkh_0 h_v0 + vkh_1 h_v1]) u_e
+//   q_e = (q_v0 + q_v1) / 2  or  (r_0 zeta_v0 + r_1 zeta_v1 + f_e) / (r_0 h_v0 + r_1 h_v1)
+//   du_e = sum_k w_k F_fk q_k - (B_c1 - B_c0) / d_e,  dh_i = -sum_k div_k F_k
+// (q_k the mean (q_e + q_fk)/2, or the PV of the slot's PV edge for the Gassmann flux) one
+// vector-Jacobian product is P1 and P2 of the stage and four gather-only launches:
+//   T1 edges: per slot the cotangents cF = lu w q_k of F_fk and cQ of the slot's PV
+//   T2 edges: Fbar_f from the slots that hold f (the stencils are symmetric: f is in the
+//      stencil of e exactly when e is in that of f; the host lists the (e, slot) pairs of
+//      every f, dropping zero weights, which contribute nothing) and from the continuity;
+//      qbar_f likewise, then the cotangents of the rhombus numerator and denominator
+//      | cells: Bbar_i = sum_k s_k lu_k / d_k
+//   T3 vertices: zbar_v, hbar_v (edge PV, mixed mass flux) and Kbar_v (vertex KE)
+//   T4 edges: ubar_e | cells: hbar_i, each ending in the RK4 adjoint epilogue
+// With the flux form of the FP32-bulk kernels F is G = l F and the continuity weights are
+// s / A_i; the adjoint reads the same (scaled) weights as the forward.
+
+template <class P>
+struct TabAdj {
+    using C = typename P::C;
+    using I = typename P::I;
+    int K = 0;                 // slots of the PV flux (kt or kg)
+    int TW = 0, QW = 0;        // widths of the transposed lists tf, tq
+    int CW = 0;                // corners per cell
+    const int32_t* nb = nullptr;    // K x ne: the stencil (trisk_edges or g_edges)
+    const C* w = nullptr;           // K x ne: its weights, as the forward reads them
+    const int32_t* pv = nullptr;    // K x ne: the PV edge of each slot (Gassmann)
+    const int32_t* tf_e = nullptr;  // TW x ne: the (edge, slot) pairs whose stencil holds f
+    const uint8_t* tf_k = nullptr;  //          (slot 255: none)
+    const int32_t* tq_e = nullptr;  // QW x ne: the (edge, slot) pairs whose PV edge is f (Gassmann)
+    const uint8_t* tq_k = nullptr;
+    const uint32_t* cbits = nullptr;  // nc: bit k: i is the right cell of its edge k; bit 16 + k: slot k is an edge
+    const uint32_t* vbits = nullptr;  // nv: bit k: v is the second vertex of its edge k; bits 8 + 4k: slot of v
+                                      //     among the cell_vertices of its cell k
+    const C* ecurl = nullptr;  // 2 x ne: curl coefficient of e in its vertex s
+    const C* ekin = nullptr;   // 2 x ne: kinetic-energy weight of e in its cell s
+    const C* evkw = nullptr;   // 2 x ne: vertex kinetic-energy weight of e in its vertex s
+    const C* edc = nullptr;    // 2 x ne: continuity weight of e in its cell s
+    const int32_t* corner = nullptr;  // CW x nc: the vertices whose vertex_cells hold i (pads: 0)
+    const C* cvh = nullptr;           // CW x nc: the vertex_height weight of i there (pads: 0)
+    I *cF = nullptr, *cQ = nullptr;   // K x ne
+    I *qown = nullptr, *Fb = nullptr, *Zb = nullptr, *Eb = nullptr;  // ne
+    double* Bb = nullptr;                                            // nc
+    I *zb = nullptr, *hvb = nullptr, *Kvb = nullptr;                 // nv
+};
+
+// T1: edges.  GM: the Gassmann form (each slot's PV from its PV edge); else the mean form.
+template <class P, bool GM>
+__global__ __launch_bounds__(kBlock) void tadj_slot_kernel(Params<P> p, TabAdj<P> a, const double* __restrict__ cot,
+                                                           double c) {
+    using C = typename P::C;
+    using I = typename P::I;
+    const int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (e >= p.ne) return;
+    const C lu = C(c * cot[e]);
+    const C qe = C(p.Fq[e].q);
+    C own = 0;
+    for (int k = 0; k < a.K; ++k) {
+        const int64_t j = k * p.ne + e;
+        const Pair<I> f = p.Fq[ld(a.nb + j)];
+        const C wl = lu * ld(a.w + j);
+        C cf, cq;
+        if constexpr (GM) {
+            cf = wl * C(p.Fq[ld(a.pv + j)].q);
+            cq = wl * C(f.F);
+        } else {
+            cf = wl * (qe + C(f.q)) / 2;
+            cq = wl * C(f.F) / 2;
+            own += cq;
+        }
+        a.cF[j] = I(cf);
+        a.cQ[j] = I(cq);
+    }
+    a.qown[e] = I(own);
+}
+
+// T2: edges (Fbar, and the cotangents of the edge PV's parts) | cells (Bbar).  RH: the
+// rhombus edge PV (Gassmann fluxes), else the mean of the vertex PV.
+template <class P, bool GM, bool RH>
+__global__ __launch_bounds__(kBlock) void tadj_flux_kernel(Params<P> p, TabAdj<P> a, const double* __restrict__ cot,
+                                                           double c, int64_t t0, int64_t t1) {
+    using C = typename P::C;
+    using I = typename P::I;
+    const Tile t = tile_of(blockIdx.x, t0, t1);
+    if (t.range == 0) {
+        const int64_t f = t.first + threadIdx.x;
+        if (f >= p.ne) return;
+        C Fb = 0, qb = GM ? C(0) : C(a.qown[f]);
+        for (int j = 0; j < a.TW; ++j) {
+            const int k = ld(a.tf_k + j * p.ne + f);
+            if (k == 255) break;
+            const int64_t q = k * p.ne + ld(a.tf_e + j * p.ne + f);
+            Fb += C(a.cF[q]);
+            if constexpr (!GM) qb += C(a.cQ[q]);
+        }
+        if constexpr (GM) {
+            for (int j = 0; j < a.QW; ++j) {
+                const int k = ld(a.tq_k + j * p.ne + f);
+                if (k == 255) break;
+                qb += C(a.cQ[k * p.ne + ld(a.tq_e + j * p.ne + f)]);
+            }
+        }
+        const int32_t c0 = ld(p.ec + f), c1 = ld(p.ec + p.ne + f);
+        Fb -= ld(a.edc + f) * C(c * cot[p.ne + c0]) + ld(a.edc + p.ne + f) * C(c * cot[p.ne + c1]);
+        a.Fb[f] = I(Fb);
+        if constexpr (RH) {
+            const int32_t v0 = ld(p.ev + f), v1 = ld(p.ev + p.ne + f);
+            const C eta = ld(p.grh + f) * C(p.hv[v0]) + ld(p.grh + p.ne + f) * C(p.hv[v1]);
+            a.Zb[f] = I(qb / eta);
+            a.Eb[f] = I(-qb * C(p.Fq[f].q) / eta);
+        } else {
+            a.Zb[f] = I(qb / 2);  // each vertex's share of the mean
+        }
+    } else {
+        const int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        const uint32_t bits = ld(a.cbits + i);
+        double bb = 0.0;
+        for (int k = 0; k < p.kc; ++k) {
+            if (!((bits >> (16 + k)) & 1)) continue;
+            const int32_t e = ld(p.ce + k * p.nc + i);
+            const double x = c * cot[e] * double(ld(p.invd + e));
+            bb += (bits >> k) & 1 ? -x : x;
+        }
+        a.Bb[i] = bb;
+    }
+}
+
+// T3: vertices (three edges and three cells each)
+template <class P, class UIn, bool RH, bool VK>
+__global__ __launch_bounds__(kBlock) void tadj_vertex_kernel(Params<P> p, TabAdj<P> a, const UIn* __restrict__ u) {
+    using C = typename P::C;
+    using I = typename P::I;
+    const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (v >= p.nv) return;
+    const uint32_t bits = ld(a.vbits + v);
+    C zb = 0, hb = 0;
+    if constexpr (RH) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int32_t e = ld(p.ve + k * p.nv + v);
+            const C r = ld(p.grh + ((bits >> k) & 1) * p.ne + e);
+            zb += r * C(a.Zb[e]);
+            hb += r * C(a.Eb[e]);
+        }
+    } else {
+        C qb = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) qb += C(a.Zb[ld(p.ve + k * p.nv + v)]);
+        const C hv = C(p.hv[v]);
+        zb = qb / hv;
+        hb = -qb * C(p.qv[v]) / hv;
+    }
+    if constexpr (VK) {
+        C kb = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int32_t e = ld(p.ve + k * p.nv + v);
+            hb += C(a.Fb[e]) * ld(p.vkh + ((bits >> k) & 1) * p.ne + e) * C(u[e]);
+            const int32_t i = ld(p.vc + k * p.nv + v);
+            kb += ld(p.vkk + ((bits >> (8 + 4 * k)) & 15) * p.nc + i) * C(a.Bb[i]);
+        }
+        a.Kvb[v] = I(kb);
+    }
+    a.zb[v] = I(zb);
+    a.hvb[v] = I(hb);
+}
+
+// T4: edges (ubar) | cells (hbar), each with the RK4 adjoint epilogue
+template <class P, class UIn, bool VK>
+__global__ __launch_bounds__(kBlock) void tadj_final_kernel(Params<P> p, TabAdj<P> a, const UIn* __restrict__ u,
+                                                            const double* __restrict__ h, AdjRk r, int64_t t0,
+                                                            int64_t t1) {
+    using C = typename P::C;
+    const Tile t = tile_of(blockIdx.x, t0, t1);
+    if (t.range == 0) {
+        const int64_t e = t.first + threadIdx.x;
+        if (e >= p.ne) return;
+        const AdjIn rin = adj_load(r, e);
+        const int32_t c0 = ld(p.ec + e), c1 = ld(p.ec + p.ne + e), v0 = ld(p.ev + e), v1 = ld(p.ev + p.ne + e);
+        const C ue = C(u[e]);
+        C he = ld(p.eh + e) * C(h[c0]) + ld(p.eh + p.ne + e) * C(h[c1]);
+        if constexpr (VK) he = he + ld(p.vkh + e) * C(p.hv[v0]) + ld(p.vkh + p.ne + e) * C(p.hv[v1]);
+        double s = double(C(a.Fb[e]) * he) +
+                   double(ld(a.ecurl + e) * C(a.zb[v0]) + ld(a.ecurl + p.ne + e) * C(a.zb[v1])) +
+                   2.0 * double(ue) * (double(ld(a.ekin + e)) * a.Bb[c0] + double(ld(a.ekin + p.ne + e)) * a.Bb[c1]);
+        if constexpr (VK)
+            s += 2.0 * double(ue) * double(ld(a.evkw + e) * C(a.Kvb[v0]) + ld(a.evkw + p.ne + e) * C(a.Kvb[v1]));
+        adj_store(r, e, s, rin);
+    } else {
+        const int64_t i = t.first + threadIdx.x;
+        if (i >= p.nc) return;
+        const AdjIn rin = adj_load(r, p.ne + i);
+        const uint32_t bits = ld(a.cbits + i);
+        double s = p.g * a.Bb[i];
+        for (int k = 0; k < p.kc; ++k) {
+            if (!((bits >> (16 + k)) & 1)) continue;
+            const int32_t e = ld(p.ce + k * p.nc + i);
+            s += double(C(a.Fb[e]) * ld(p.eh + ((bits >> k) & 1) * p.ne + e) * C(u[e]));
+        }
+        for (int j = 0; j < a.CW; ++j) s += double(ld(a.cvh + j * p.nc + i) * C(a.hvb[ld(a.corner + j * p.nc + i)]));
+        adj_store(r, p.ne + i, s, rin);
+    }
+}
+
 // the arguments a captured adjoint sweep bakes in
 struct SweepKey {
     const double* y0 = nullptr;
@@ -258,6 +469,8 @@ struct AdjState {
     std::vector<DevBuf> owned;
     AdjParams<Fp64> a64{};
     AdjParams<Fp32Bulk> a32{};
+    TabAdj<Fp64> t64{};
+    TabAdj<Fp32Bulk> t32{};
     // work of trisk_rk4_adjoint, kept (and grown) across calls so a captured sweep stays valid
     std::vector<DevBuf> work;
     int64_t work_ck_states = 0, work_seg_states = 0;
@@ -297,6 +510,165 @@ static AdjParams<P>& adj_params(AdjState* st) {
 }
 
 template <class P>
+static TabAdj<P>& tab_params(AdjState* st) {
+    if constexpr (P::id == TRISK_FP64)
+        return st->t64;
+    else
+        return st->t32;
+}
+
+template <class P>
+static const Params<P>& fwd_params(const trisk_handle* H) {
+    if constexpr (P::id == TRISK_FP64)
+        return H->p64;
+    else
+        return H->p32;
+}
+
+// The tables of the table-path adjoint (TabAdj), from the layout in the device numbering.
+template <class P>
+static int table_adj_setup(trisk_handle* H, AdjState* st) {
+    using C = typename P::C;
+    using I = typename P::I;
+    const Layout& L = H->L;
+    const Params<P>& p = fwd_params<P>(H);
+    TabAdj<P>& a = tab_params<P>(st);
+    const int64_t nc = L.nc, ne = L.ne, nv = L.nv;
+    const int KC = L.kc;
+    if (L.kv != 3) return set_err(TRISK_EINVAL, "the adjoint of the table kernels needs three edges per vertex");
+    const bool gm = L.gassmann, vk = L.vertex_ke, pvform = L.gassmann && !L.gassmann_mean;
+    const int K = gm ? L.kg : L.kt;
+    const std::vector<int32_t>& NB = gm ? L.ge : L.te;
+    const std::vector<double>& Wt = gm ? L.gw : L.tw;
+    auto slot_of = [](const std::vector<int32_t>& t, int k_max, int64_t n, int64_t j, int64_t x) {
+        for (int k = 0; k < k_max; ++k)
+            if (t[size_t(k) * n + j] == x) return k;
+        return -1;
+    };
+    // the PV edges of the Gassmann form (the slot order's PV slots when no table was kept)
+    std::vector<int32_t> PV;
+    if (pvform) {
+        PV.resize(size_t(K) * ne);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < K; ++k)
+                PV[size_t(k) * ne + e] =
+                    L.gp.empty() ? L.ge[size_t(gassmann_pv_slot(k)) * ne + e] : L.gp[size_t(k) * ne + e];
+    }
+    // (e, k) pairs with a nonzero weight whose target to[k, e] is f, by e then k
+    auto transpose = [&](const std::vector<int32_t>& to, std::vector<int32_t>& te, std::vector<uint8_t>& tk) {
+        std::vector<int32_t> cnt(ne, 0);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < K; ++k)
+                if (Wt[size_t(k) * ne + e] != 0.0) ++cnt[to[size_t(k) * ne + e]];
+        const int width = ne ? *std::max_element(cnt.begin(), cnt.end()) : 0;
+        te.assign(size_t(std::max(width, 1)) * ne, 0);
+        tk.assign(size_t(std::max(width, 1)) * ne, uint8_t(255));
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < K; ++k)
+                if (Wt[size_t(k) * ne + e] != 0.0) {
+                    const int64_t f = to[size_t(k) * ne + e];
+                    const int j = cnt[f]++;
+                    te[size_t(j) * ne + f] = int32_t(e);
+                    tk[size_t(j) * ne + f] = uint8_t(k);
+                }
+        return width;
+    };
+    std::vector<int32_t> tfe, tqe;
+    std::vector<uint8_t> tfk, tqk;
+    a.K = K;
+    a.TW = transpose(NB, tfe, tfk);
+    a.QW = pvform ? transpose(PV, tqe, tqk) : 0;
+    // cells: side and slot bits; the continuity and kinetic-energy weights of every edge side
+    const bool flux = P::id == TRISK_FP32_BULK && L.gflux;
+    std::vector<uint32_t> cbits(nc, 0);
+    for (int64_t i = 0; i < nc; ++i)
+        for (int k = 0; k < KC; ++k) {
+            if (L.div[size_t(k) * nc + i] == 0.0) continue;
+            const int64_t e = L.ce[size_t(k) * nc + i];
+            if (L.ec[e] != i && L.ec[ne + e] != i) return set_err(TRISK_EINVAL, "cell_edges and edge_cells disagree");
+            cbits[i] |= (1u << (16 + k)) | (L.ec[ne + e] == i ? 1u << k : 0u);
+        }
+    std::vector<double> ecurl(2 * ne), ekin(2 * ne), evkw(vk ? 2 * ne : 0), edc(2 * ne);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int sd = 0; sd < 2; ++sd) {
+            const int64_t c = L.ec[sd * ne + e], v = L.ev[sd * ne + e];
+            const int kc = slot_of(L.ce, KC, nc, c, e), kv = slot_of(L.ve, 3, nv, v, e);
+            if (kc < 0 || kv < 0) return set_err(TRISK_EINVAL, "edge tables and cell or vertex tables disagree");
+            const double dv = L.div[size_t(kc) * nc + c];
+            ekin[sd * ne + e] = L.kin[size_t(kc) * nc + c];
+            edc[sd * ne + e] = flux ? (dv > 0 ? 1.0 : -1.0) / L.area_c[c] : dv;
+            ecurl[sd * ne + e] = L.curl[size_t(kv) * nv + v];
+            if (vk) evkw[sd * ne + e] = L.vkw[size_t(kv) * nv + v];
+        }
+    // vertices: the side of v in each edge, its slot among the corners of each cell; the
+    // corners of every cell with their vertex_height weights
+    std::vector<uint32_t> vbits(nv, 0);
+    std::vector<int32_t> ncorner(nc, 0);
+    for (int64_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t e = L.ve[size_t(k) * nv + v], i = L.vc[size_t(k) * nv + v];
+            if (L.ev[e] != v && L.ev[ne + e] != v) return set_err(TRISK_EINVAL, "vertex_edges and edge_vertices disagree");
+            if (L.ev[ne + e] == v) vbits[v] |= 1u << k;
+            if (vk) {
+                const int sl = slot_of(L.cv, KC, nc, i, v);
+                if (sl < 0 || sl > 15) return set_err(TRISK_EINVAL, "vertex_cells and cell_vertices disagree");
+                vbits[v] |= uint32_t(sl) << (8 + 4 * k);
+            }
+            ++ncorner[i];
+        }
+    a.CW = nc ? *std::max_element(ncorner.begin(), ncorner.end()) : 0;
+    std::vector<int32_t> corner(size_t(std::max(a.CW, 1)) * nc, 0);
+    std::vector<double> cvh(size_t(std::max(a.CW, 1)) * nc, 0.0);
+    std::fill(ncorner.begin(), ncorner.end(), 0);
+    for (int64_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t i = L.vc[size_t(k) * nv + v];
+            const int j = ncorner[i]++;
+            corner[size_t(j) * nc + i] = int32_t(v);
+            cvh[size_t(j) * nc + i] = L.vh[size_t(k) * nv + v];
+        }
+    auto coef = [&](const std::vector<double>& v, const C** out) -> int {
+        if constexpr (sizeof(C) == sizeof(double))
+            return upload(st->owned, v, reinterpret_cast<const double**>(out));
+        else
+            return upload(st->owned, to_float(v), reinterpret_cast<const float**>(out));
+    };
+    int rc = 0;
+    a.nb = gm ? p.ge : p.te;
+    a.w = gm ? p.gw : p.tw;
+    if (pvform) rc |= upload(st->owned, PV, &a.pv);
+    rc |= upload(st->owned, tfe, &a.tf_e);
+    rc |= upload(st->owned, tfk, &a.tf_k);
+    if (pvform) {
+        rc |= upload(st->owned, tqe, &a.tq_e);
+        rc |= upload(st->owned, tqk, &a.tq_k);
+    }
+    rc |= upload(st->owned, cbits, &a.cbits);
+    rc |= upload(st->owned, vbits, &a.vbits);
+    rc |= coef(ecurl, &a.ecurl);
+    rc |= coef(ekin, &a.ekin);
+    if (vk) rc |= coef(evkw, &a.evkw);
+    rc |= coef(edc, &a.edc);
+    rc |= upload(st->owned, corner, &a.corner);
+    rc |= coef(cvh, &a.cvh);
+    if (rc) return rc;
+    void* q;
+    auto get = [&](size_t bytes) -> void* {
+        rc = alloc(st->owned, bytes, &q);
+        return rc ? nullptr : q;
+    };
+    if (!(a.cF = static_cast<I*>(get(size_t(K) * ne * sizeof(I)))) ||
+        !(a.cQ = static_cast<I*>(get(size_t(K) * ne * sizeof(I)))) || !(a.qown = static_cast<I*>(get(ne * sizeof(I)))) ||
+        !(a.Fb = static_cast<I*>(get(ne * sizeof(I)))) || !(a.Zb = static_cast<I*>(get(ne * sizeof(I)))) ||
+        !(a.Eb = static_cast<I*>(get(ne * sizeof(I)))) || !(a.Bb = static_cast<double*>(get(nc * sizeof(double)))) ||
+        !(a.zb = static_cast<I*>(get(nv * sizeof(I)))) || !(a.hvb = static_cast<I*>(get(nv * sizeof(I)))) ||
+        !(a.Kvb = static_cast<I*>(get(nv * sizeof(I)))))
+        return rc;
+    return 0;
+}
+
+template <class P>
 static int adj_setup(trisk_handle* H) {
     using C = typename P::C;
     using I = typename P::I;
@@ -304,6 +676,11 @@ static int adj_setup(trisk_handle* H) {
     auto* st = new AdjState;
     H->adj = st;
     H->adj_release = adj_release;
+    if (!H->L.pair) {
+        const int rc = table_adj_setup<P>(H, st);
+        if (rc) adj_release(H);  // a later call builds the tables again and reports the same error
+        return rc;
+    }
     const Layout& L = H->L;
     AdjParams<P>& w = adj_params<P>(st);
     auto coef = [&](const std::vector<double>& v, const C** out) -> int {
@@ -334,12 +711,43 @@ static int adj_setup(trisk_handle* H) {
     return 0;
 }
 
-template <class P>
-static const Params<P>& fwd_params(const trisk_handle* H) {
-    if constexpr (P::id == TRISK_FP64)
-        return H->p64;
+// One vector-Jacobian product of the table kernels (T1-T4 after P1 and P2 of the stage).
+template <class P, class UIn>
+static int table_vjp(trisk_handle* H, const UIn* u, const double* h, const double* cot, double c, const AdjRk& r,
+                     hipStream_t s) {
+    const Layout& L = H->L;
+    const Params<P>& p = fwd_params<P>(H);
+    const TabAdj<P>& a = tab_params<P>(static_cast<AdjState*>(H->adj));
+    int rc = table_state_pass(H, u, std::is_same<UIn, double>::value, h, s);
+    if (rc) return rc;
+    const bool gm = L.gassmann && !L.gassmann_mean, rh = L.gassmann, vk = L.vertex_ke;
+    const dim3 ge(unsigned((L.ne + kBlock - 1) / kBlock)), gv(unsigned((L.nv + kBlock - 1) / kBlock)), b(kBlock);
+    const int64_t t0 = tiles_per_xcd(L.ne), t1 = tiles_per_xcd(L.nc);
+    const dim3 g2(unsigned(kXcd * (t0 + t1)));
+    if (gm) {
+        hipLaunchKernelGGL((tadj_slot_kernel<P, true>), ge, b, 0, s, p, a, cot, c);
+        hipLaunchKernelGGL((tadj_flux_kernel<P, true, true>), g2, b, 0, s, p, a, cot, c, t0, t1);
+    } else {
+        hipLaunchKernelGGL((tadj_slot_kernel<P, false>), ge, b, 0, s, p, a, cot, c);
+        if (rh)
+            hipLaunchKernelGGL((tadj_flux_kernel<P, false, true>), g2, b, 0, s, p, a, cot, c, t0, t1);
+        else
+            hipLaunchKernelGGL((tadj_flux_kernel<P, false, false>), g2, b, 0, s, p, a, cot, c, t0, t1);
+    }
+    if (rh && vk)
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, true>), gv, b, 0, s, p, a, u);
+    else if (rh)
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, false>), gv, b, 0, s, p, a, u);
+    else if (vk)
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, true>), gv, b, 0, s, p, a, u);
     else
-        return H->p32;
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, false>), gv, b, 0, s, p, a, u);
+    if (vk)
+        hipLaunchKernelGGL((tadj_final_kernel<P, UIn, true>), g2, b, 0, s, p, a, u, h, r, t0, t1);
+    else
+        hipLaunchKernelGGL((tadj_final_kernel<P, UIn, false>), g2, b, 0, s, p, a, u, h, r, t0, t1);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 // One vector-Jacobian product at the stage (u, h) for the cotangent c * cot, ending in the
@@ -347,6 +755,7 @@ static const Params<P>& fwd_params(const trisk_handle* H) {
 template <class P, class UIn>
 static int vjp(trisk_handle* H, const UIn* u, const double* h, const double* cot, double c, const AdjRk& r,
                hipStream_t s) {
+    if (!H->L.pair) return table_vjp<P, UIn>(H, u, h, cot, c, r, s);
     const Layout& L = H->L;
     const Params<P>& p = fwd_params<P>(H);
     const AdjParams<P>& w = adj_params<P>(static_cast<AdjState*>(H->adj));
@@ -453,7 +862,7 @@ static int enqueue_sweep(trisk_handle* H, AdjState* st, const double* y0, double
                 r.sh = st->sh[i];
                 const void* uin = i == 0 ? static_cast<const void*>(yn) : st->su[i - 1];
                 const double* hin = i == 0 ? yn + L.ne : st->sh[i - 1];
-                if ((rc = pair_tendency_stage(H, uin, i == 0, hin, r, s))) return rc;
+                if ((rc = tendency_stage(H, uin, i == 0, hin, r, s))) return rc;
             }
             a.mode = 4;
             if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[2]), st->sh[2], la, dt / 6, a, s))) return rc;
@@ -516,10 +925,6 @@ static int rk4_adjoint(trisk_handle* H, const double* y0, double dt, int64_t N, 
 
 static int adj_check(trisk_handle* h) {
     if (!h) return set_err(TRISK_EINVAL, "null handle");
-    if (!h->L.pair)
-        return set_err(TRISK_EINVAL,
-                       "the adjoint runs on the cell-pair path (TRiSK PV flux, cell kinetic energy; create the "
-                       "model with TRISK_PATH_PAIR): " + h->L.pair_reason);
     return 0;
 }
 

@@ -1300,11 +1300,39 @@ int pair_state_pass(trisk_handle* H, const void* u, bool u_double, const double*
     return state_pass<Fp32Bulk, float>(H, static_cast<const float*>(u), h, s);
 }
 
-int pair_tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r,
-                        hipStream_t s) {
-    if (H->precision == TRISK_FP64) return run_pair<Fp64, double>(H, static_cast<const double*>(u), h, r, s, true, false);
-    if (u_double) return run_pair<Fp32Bulk, double>(H, static_cast<const double*>(u), h, r, s, true, false);
-    return run_pair<Fp32Bulk, float>(H, static_cast<const float*>(u), h, r, s, true, false);
+int tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r, hipStream_t s) {
+    if (H->precision == TRISK_FP64)
+        return run_tendency<Fp64, double>(H, static_cast<const double*>(u), h, r, s, true, false);
+    if (u_double) return run_tendency<Fp32Bulk, double>(H, static_cast<const double*>(u), h, r, s, true, false);
+    return run_tendency<Fp32Bulk, float>(H, static_cast<const float*>(u), h, r, s, true, false);
+}
+
+// P1 and P2 of the table kernels at (u, h): B, the vertex quantities (zeta_v, h_v, q_v, K_v
+// as the options need them) and the (F, q_e) pairs
+template <class P, class UIn>
+static int table_state(trisk_handle* H, const UIn* u, const double* h, hipStream_t s) {
+    const Params<P>& p = params<P>(H);
+    const Layout& L = H->L;
+    const bool vk = L.vertex_ke, gm = L.gassmann, zh = true;
+    const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
+    const int64_t c0 = tiles_per_xcd(L.nc), v1 = tiles_per_xcd(L.nv), e0 = tiles_per_xcd(L.ne),
+                  c1 = vk ? tiles_per_xcd(L.nc) : 0;
+    const dim3 g1(unsigned(kXcd * (c0 + v1))), g2(unsigned(kXcd * (e0 + c1)));
+    if (hex) {
+        launch_p1<P, UIn, HexWidths, false>(p, g1, s, u, h, c0, v1, vk, gm, zh);
+        launch_p2<P, UIn, HexWidths, false>(p, g2, s, u, h, e0, c1, vk, gm);
+    } else {
+        launch_p1<P, UIn, AnyWidths, false>(p, g1, s, u, h, c0, v1, vk, gm, zh);
+        launch_p2<P, UIn, AnyWidths, false>(p, g2, s, u, h, e0, c1, vk, gm);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int table_state_pass(trisk_handle* H, const void* u, bool u_double, const double* h, hipStream_t s) {
+    if (H->precision == TRISK_FP64) return table_state<Fp64, double>(H, static_cast<const double*>(u), h, s);
+    if (u_double) return table_state<Fp32Bulk, double>(H, static_cast<const double*>(u), h, s);
+    return table_state<Fp32Bulk, float>(H, static_cast<const float*>(u), h, s);
 }
 
 int rk4_steps_direct(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s) {

@@ -743,13 +743,14 @@ struct trisk_handle {
 
 namespace trisk {
 
-// Forward passes of the cell-pair path for the adjoint (trisk_device.hip).  The stage input
-// u is double (the state y) when u_double, else the handle's stage type (P::U).
-// W1 only: B, q_v and G of the stage (u, h).
+// Forward passes for the adjoint (trisk_device.hip).  The stage input u is double (the state
+// y) when u_double, else the handle's stage type (P::U).
+// Cell-pair path, W1 only: B, q_v and G of the stage (u, h).
 int pair_state_pass(trisk_handle* H, const void* u, bool u_double, const double* h, hipStream_t s);
-// One tendency (W1, W2, W3) with the RK4 stage epilogue r.
-int pair_tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r,
-                        hipStream_t s);
+// Table kernels, P1 and P2 only: B, zeta_v, h_v, q_v, K_v and the (F, q_e) pairs of the stage.
+int table_state_pass(trisk_handle* H, const void* u, bool u_double, const double* h, hipStream_t s);
+// One tendency of the handle's path with the RK4 stage epilogue r.
+int tendency_stage(trisk_handle* H, const void* u, bool u_double, const double* h, const Rk& r, hipStream_t s);
 // n fused RK4 steps of y with direct launches.
 int rk4_steps_direct(trisk_handle* H, double* y, double dt, int64_t n, hipStream_t s);
 

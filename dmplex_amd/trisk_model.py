@@ -490,7 +490,7 @@ def _check_state(m: TRiSKShallowWater, *ys) -> None:
 
 def tendency_vjp(m: TRiSKShallowWater, y, dybar):
     """(d tendency / dy)^T dybar at the device state ``y`` (the pullback of ``tendency!``,
-    TRiSKShallowWater.jl:379-394; cell-pair path: TRiSK PV flux, cell kinetic energy)."""
+    TRiSKShallowWater.jl:379-394, for every PV flux and kinetic-energy form)."""
     torch = _torch()
     _check_state(m, y, dybar)
     out = m.zeros()

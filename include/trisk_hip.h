@@ -397,9 +397,10 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
  * No counterpart in the reference (it has no adjoint).  These are the pullbacks of
  * tendency! (src/FiniteVolumes/TRiSKShallowWater.jl:379-394) and of nsteps rk4_step!
  * (src/TimeSteppers/RungeKutta.jl:18-28) that a ChainRulesCore rrule calls: the transposed
- * linearisation of the handle's own forward arithmetic.  They need the cell-pair path
- * (TRiSK PV flux, cell kinetic energy: a handle created with TRISK_PATH_PAIR, or any
- * FP32-bulk handle of such a model); other handles return TRISK_EINVAL. */
+ * linearisation of the handle's own forward arithmetic, on either path: the cell-pair path
+ * (TRiSK PV flux, cell kinetic energy) or the table kernels (every PV flux and
+ * kinetic-energy form; meshes with three edges per vertex).  Gather-only kernels, so the
+ * result does not depend on scheduling.  ICON handles return TRISK_EINVAL. */
 
 /* ybar = (d tendency / dy)^T dybar at the state y (device vectors, internal order; ybar
  * must not alias y or dybar). */

@@ -146,12 +146,12 @@ class Oracle:
         return y
 
     def tendency_vjp(self, y: np.ndarray, lam: np.ndarray) -> np.ndarray:
-        """(d tendency / dy)^T lam at y (TRiSK PV flux, cell kinetic energy)."""
+        """(d tendency / dy)^T lam at y (every PV flux and kinetic-energy form)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         out = np.empty_like(y)
         if lib().oracle_tendency_vjp(C.byref(self.s), self._ptr(y), self._ptr(lam), self._ptr(out)) != 0:
-            raise ValueError("the oracle adjoint covers the TRiSK PV flux with cell kinetic energy only")
+            raise ValueError("oracle_tendency_vjp failed")
         return out
 
     def rk4_adjoint(self, y0: np.ndarray, dt: float, nsteps: int, ybar: np.ndarray) -> np.ndarray:
@@ -159,7 +159,7 @@ class Oracle:
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         out = np.array(ybar, dtype=np.float64)
         if lib().oracle_rk4_adjoint(C.byref(self.s), self._ptr(y0), dt, nsteps, self._ptr(out)) != 0:
-            raise ValueError("the oracle adjoint covers the TRiSK PV flux with cell kinetic energy only")
+            raise ValueError("oracle_rk4_adjoint failed")
         return out
 
     def total_mass(self, y):

@@ -230,40 +230,67 @@ void oracle_edge_sum(int64_t n, int64_t k, const int64_t* edges, const double* w
 }
 
 /* ---------------------------------------------------------------------------------
- * Adjoint (vector-Jacobian product) of the tendency with TRiSK PV flux and cell kinetic
- * energy, and of rk4_step!.  The reference has no adjoint; this is the transpose of the
- * linearisation of oracle_tendency above, written as plain scatter-adds over the
- * reference's tables in one thread (an implementation independent of the device's
- * gather-only kernels).  It is pinned by finite differences in the tests. */
+ * Adjoint (vector-Jacobian product) of the tendency and of rk4_step!, for every PV flux
+ * (TRiSK, Gassmann, Gassmann mean) and kinetic-energy form (cell, mixed, vertex).  The
+ * reference has no adjoint; this is the transpose of the linearisation of oracle_tendency
+ * above (TRiSKShallowWater.jl:202-305), written as plain scatter-adds over the reference's
+ * tables in one thread (an implementation independent of the device's gather-only
+ * kernels).  It is pinned by finite differences in the tests. */
 
 int oracle_tendency_vjp(const oracle_model* m, const double* y, const double* lam, double* ybar) {
-    if (m->cell_vertices || m->g_edges) return -1;
-    const int64_t ne = m->ne, nc = m->nc, nv = m->nv, KC = m->kc, KV = m->kv, KT = m->kt;
+    const int64_t ne = m->ne, nc = m->nc, nv = m->nv, KC = m->kc, KV = m->kv, KT = m->kt, KG = m->kg;
     const double* u = y;
     const double* h = y + ne;
     const double* lu = lam;
     const double* lh = lam + ne;
     double* ub = ybar;
     double* hb = ybar + ne;
-    /* forward intermediates */
+    const int vk = m->cell_vertices != NULL, gm = m->g_edges != NULL;
+    /* forward intermediates: K_v, zeta_v, h_v, q_v, F and the edge PV */
     oracle_kinetic_energy(m, u, h);
     oracle_vertex_pv(m, u, h);
     oracle_mass_flux(m, u, h);
-    for (int64_t e = 0; e < ne; ++e)
-        m->qe[e] = (m->qv[AT(m->edge_vertices, 2, 0, e)] + m->qv[AT(m->edge_vertices, 2, 1, e)]) / 2;
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t v1 = AT(m->edge_vertices, 2, 0, e), v2 = AT(m->edge_vertices, 2, 1, e);
+        if (gm) {
+            const double r1 = AT(m->g_rhombus, 2, 0, e), r2 = AT(m->g_rhombus, 2, 1, e);
+            m->qe[e] = (r1 * m->zv[v1] + r2 * m->zv[v2] + m->g_coriolis[e]) / (r1 * m->hv[v1] + r2 * m->hv[v2]);
+        } else {
+            m->qe[e] = (m->qv[v1] + m->qv[v2]) / 2;
+        }
+    }
     double* Fb = (double*)calloc((size_t)ne, sizeof(double));
     double* qeb = (double*)calloc((size_t)ne, sizeof(double));
     double* Bb = (double*)calloc((size_t)nc, sizeof(double));
-    double* qb = (double*)calloc((size_t)nv, sizeof(double));
+    double* zb = (double*)calloc((size_t)nv, sizeof(double));
+    double* hvb = (double*)calloc((size_t)nv, sizeof(double));
+    double* Kvb = (double*)calloc((size_t)nv, sizeof(double));
     for (int64_t j = 0; j < ne + nc; ++j) ybar[j] = 0.0;
-    /* du_e = 1/2 sum_k w F_f (qe_e + qe_f) - (B_c1 - B_c0) / d_e */
+    /* du_e = sum_k w F_f q_k - (B_c2 - B_c1) / d_e, q_k the mean (qe_e + qe_f)/2 (TRiSK, Gassmann
+     * mean) or the PV of the pv edge (Gassmann) */
     for (int64_t e = 0; e < ne; ++e) {
-        for (int64_t k = 0; k < KT; ++k) {
-            int64_t f = AT(m->trisk_edges, KT, k, e);
-            double w = AT(m->trisk, KT, k, e);
-            Fb[f] += 0.5 * lu[e] * w * (m->qe[e] + m->qe[f]);
-            qeb[e] += 0.5 * lu[e] * w * m->F[f];
-            qeb[f] += 0.5 * lu[e] * w * m->F[f];
+        if (!gm) {
+            for (int64_t k = 0; k < KT; ++k) {
+                const int64_t f = AT(m->trisk_edges, KT, k, e);
+                const double w = AT(m->trisk, KT, k, e);
+                Fb[f] += 0.5 * lu[e] * w * (m->qe[e] + m->qe[f]);
+                qeb[e] += 0.5 * lu[e] * w * m->F[f];
+                qeb[f] += 0.5 * lu[e] * w * m->F[f];
+            }
+        } else {
+            for (int64_t k = 0; k < KG; ++k) {
+                const int64_t f = AT(m->g_edges, KG, k, e);
+                const double w = AT(m->g_weights, KG, k, e);
+                if (m->g_mean) {
+                    Fb[f] += lu[e] * w * (m->qe[e] + m->qe[f]) / 2;
+                    qeb[e] += lu[e] * w * m->F[f] / 2;
+                    qeb[f] += lu[e] * w * m->F[f] / 2;
+                } else {
+                    const int64_t pe = AT(m->g_pv_edges, KG, k, e);
+                    Fb[f] += lu[e] * w * m->qe[pe];
+                    qeb[pe] += lu[e] * w * m->F[f];
+                }
+            }
         }
         Bb[AT(m->edge_cells, 2, 1, e)] -= lu[e] * m->inverse_dual[e];
         Bb[AT(m->edge_cells, 2, 0, e)] += lu[e] * m->inverse_dual[e];
@@ -271,46 +298,71 @@ int oracle_tendency_vjp(const oracle_model* m, const double* y, const double* la
     /* dh_i = -sum_k div F */
     for (int64_t i = 0; i < nc; ++i)
         for (int64_t k = 0; k < KC; ++k) Fb[AT(m->cell_edges, KC, k, i)] -= AT(m->divergence, KC, k, i) * lh[i];
-    /* qe_e = (q_v0 + q_v1) / 2 */
+    /* the edge PV: the mean of q_v = (zeta_v + f_v) / h_v, or the rhombus PV
+     * (r1 zeta_1 + r2 zeta_2 + f_e) / (r1 h_1 + r2 h_2) */
     for (int64_t e = 0; e < ne; ++e) {
-        qb[AT(m->edge_vertices, 2, 0, e)] += qeb[e] / 2;
-        qb[AT(m->edge_vertices, 2, 1, e)] += qeb[e] / 2;
-    }
-    /* q_v = (zeta_v + f_v) / h_v */
-    for (int64_t v = 0; v < nv; ++v) {
-        double zb = qb[v] / m->hv[v], eb = -qb[v] * m->qv[v] / m->hv[v];
-        for (int64_t k = 0; k < KV; ++k) {
-            ub[AT(m->vertex_edges, KV, k, v)] += AT(m->curl, KV, k, v) * zb;
-            hb[AT(m->vertex_cells, KV, k, v)] += AT(m->vertex_height, KV, k, v) * eb;
+        const int64_t v1 = AT(m->edge_vertices, 2, 0, e), v2 = AT(m->edge_vertices, 2, 1, e);
+        if (gm) {
+            const double r1 = AT(m->g_rhombus, 2, 0, e), r2 = AT(m->g_rhombus, 2, 1, e);
+            const double eta = r1 * m->hv[v1] + r2 * m->hv[v2];
+            const double zbe = qeb[e] / eta, ebe = -qeb[e] * m->qe[e] / eta;
+            zb[v1] += r1 * zbe;
+            zb[v2] += r2 * zbe;
+            hvb[v1] += r1 * ebe;
+            hvb[v2] += r2 * ebe;
+        } else {
+            const double qb = qeb[e] / 2;
+            zb[v1] += qb / m->hv[v1];
+            zb[v2] += qb / m->hv[v2];
+            hvb[v1] -= qb * m->qv[v1] / m->hv[v1];
+            hvb[v2] -= qb * m->qv[v2] / m->hv[v2];
         }
     }
-    /* F_e = (eh_0 h_c0 + eh_1 h_c1) u_e */
+    /* F_e = (eh_1 h_c1 + eh_2 h_c2 [+ vkh_1 h_v1 + vkh_2 h_v2]) u_e */
     for (int64_t e = 0; e < ne; ++e) {
-        int64_t c0 = AT(m->edge_cells, 2, 0, e), c1 = AT(m->edge_cells, 2, 1, e);
-        double e0 = AT(m->edge_height, 2, 0, e), e1 = AT(m->edge_height, 2, 1, e);
-        ub[e] += Fb[e] * (e0 * h[c0] + e1 * h[c1]);
+        const int64_t c0 = AT(m->edge_cells, 2, 0, e), c1 = AT(m->edge_cells, 2, 1, e);
+        const double e0 = AT(m->edge_height, 2, 0, e), e1 = AT(m->edge_height, 2, 1, e);
+        double he = e0 * h[c0] + e1 * h[c1];
         hb[c0] += Fb[e] * e0 * u[e];
         hb[c1] += Fb[e] * e1 * u[e];
+        if (vk) {
+            const int64_t v0 = AT(m->edge_vertices, 2, 0, e), v1 = AT(m->edge_vertices, 2, 1, e);
+            const double k0 = AT(m->vk_height, 2, 0, e), k1 = AT(m->vk_height, 2, 1, e);
+            he = he + k0 * m->hv[v0] + k1 * m->hv[v1];
+            hvb[v0] += Fb[e] * k0 * u[e];
+            hvb[v1] += Fb[e] * k1 * u[e];
+        }
+        ub[e] += Fb[e] * he;
     }
-    /* B_i = sum_k kin u^2 + g h_i */
+    /* B_i = sum_k kin u^2 [+ sum_k kites K_v] + g h_i, K_v = sum_k vkw u^2 */
     for (int64_t i = 0; i < nc; ++i) {
         hb[i] += m->gravity * Bb[i];
         for (int64_t k = 0; k < KC; ++k) {
-            int64_t e = AT(m->cell_edges, KC, k, i);
+            const int64_t e = AT(m->cell_edges, KC, k, i);
             ub[e] += 2 * AT(m->kinetic, KC, k, i) * u[e] * Bb[i];
+            if (vk) Kvb[AT(m->cell_vertices, KC, k, i)] += AT(m->vk_kites, KC, k, i) * Bb[i];
         }
     }
+    /* zeta_v = sum curl u, h_v = sum vh h [, K_v] */
+    for (int64_t v = 0; v < nv; ++v)
+        for (int64_t k = 0; k < KV; ++k) {
+            const int64_t e = AT(m->vertex_edges, KV, k, v);
+            ub[e] += AT(m->curl, KV, k, v) * zb[v];
+            if (vk) ub[e] += 2 * AT(m->vk_weights, KV, k, v) * u[e] * Kvb[v];
+            hb[AT(m->vertex_cells, KV, k, v)] += AT(m->vertex_height, KV, k, v) * hvb[v];
+        }
     free(Fb);
     free(qeb);
     free(Bb);
-    free(qb);
+    free(zb);
+    free(hvb);
+    free(Kvb);
     return 0;
 }
 
 /* ybar (in: the cotangent of y after nsteps rk4_step! from y0; out: the cotangent of y0).
  * Every step's state is stored (small meshes only); each step's stages are recomputed. */
 int oracle_rk4_adjoint(const oracle_model* m, const double* y0, double dt, int64_t nsteps, double* ybar) {
-    if (m->cell_vertices || m->g_edges) return -1;
     const int64_t n = m->ne + m->nc;
     double* ys = (double*)malloc(sizeof(double) * (size_t)n * (size_t)(nsteps > 0 ? nsteps : 1));
     double* w = (double*)malloc(sizeof(double) * (size_t)n * 11);

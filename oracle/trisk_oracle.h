@@ -64,8 +64,8 @@ double oracle_total_energy(const oracle_model* m, const double* y);
 double oracle_potential_enstrophy(const oracle_model* m, const double* y);
 void oracle_edge_sum(int64_t n, int64_t k, const int64_t* edges, const double* weights, const double* x,
                      double* out);
-/* Adjoint, TRiSK PV flux and cell kinetic energy only (else -1): ybar = (d tendency/dy)^T lam
- * at y; and ybar <- (d y_T / d y0)^T ybar over nsteps rk4_step! from y0. */
+/* Adjoint, every PV flux and kinetic-energy form: ybar = (d tendency/dy)^T lam at y; and
+ * ybar <- (d y_T / d y0)^T ybar over nsteps rk4_step! from y0.  Both return 0. */
 int oracle_tendency_vjp(const oracle_model* m, const double* y, const double* lam, double* ybar);
 int oracle_rk4_adjoint(const oracle_model* m, const double* y0, double dt, int64_t nsteps, double* ybar);
 int oracle_threads(void);

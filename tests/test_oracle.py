@@ -186,18 +186,26 @@ def test_spherical_harmonics_orthonormal():
 
 # --- the oracle's adjoint (no reference counterpart: pinned by finite differences) -------------
 
-def _adjoint_case(n=3):
+ADJOINT_OPTIONS = [dict(), dict(kinetic_energy="mixed"), dict(kinetic_energy="vertex"),
+                   dict(pv_flux="gassmann"), dict(pv_flux="gassmann", kinetic_energy="vertex"),
+                   dict(pv_flux="gassmann_mean", kinetic_energy="mixed")]
+ADJOINT_IDS = ["trisk-cell", "trisk-mixed", "trisk-vertex", "gassmann-cell", "gassmann-vertex", "gassmann_mean-mixed"]
+
+
+def _adjoint_case(n=3, **opts):
     dm = create_hexagon_mesh(n, centroidal=True)
-    t = ModelTables(dm, centers=dm.generators, **earth_parameters())
+    t = ModelTables(dm, centers=dm.generators, **opts, **earth_parameters())
     y = seeded_state(t.grid)
     ne = dm.ne
     scale = np.concatenate([np.full(ne, np.std(y[:ne])), np.full(dm.nc, np.std(y[ne:]))])
     return t, y, ne, scale
 
 
-def test_oracle_tendency_vjp_dot_product():
-    """<w, J v> by central differences of the tendency equals <J^T w, v>."""
-    t, y, ne, scale = _adjoint_case()
+@pytest.mark.parametrize("opts", ADJOINT_OPTIONS, ids=ADJOINT_IDS)
+def test_oracle_tendency_vjp_dot_product(opts):
+    """<w, J v> by central differences of the tendency equals <J^T w, v>, for every PV flux
+    and kinetic-energy form."""
+    t, y, ne, scale = _adjoint_case(**opts)
     o = Oracle(t)
     rng = np.random.default_rng(11)
     for _ in range(4):
@@ -209,9 +217,10 @@ def test_oracle_tendency_vjp_dot_product():
         assert abs(a - b) < 1e-8 * abs(b)
 
 
-def test_oracle_rk4_adjoint_matches_finite_differences():
+@pytest.mark.parametrize("opts", [ADJOINT_OPTIONS[0], ADJOINT_OPTIONS[4]], ids=[ADJOINT_IDS[0], ADJOINT_IDS[4]])
+def test_oracle_rk4_adjoint_matches_finite_differences(opts):
     """Gradient of J = sum A (h_T - h*)^2 after 6 RK4 steps against central differences."""
-    t, y, ne, scale = _adjoint_case()
+    t, y, ne, scale = _adjoint_case(**opts)
     o = Oracle(t)
     dt, N = 1200.0, 6
     hstar = seeded_state(t.grid, seed_coef=3, seed_noise=4)[ne:]
@@ -242,6 +251,25 @@ def test_adjoint_check_is_sharp():
     bad.kinetic = np.asfortranarray(bad.kinetic * (1 + 1e-3))
     ob = Oracle(bad)
     rng = np.random.default_rng(13)
+    v = rng.standard_normal(len(y)) * scale
+    w = rng.standard_normal(len(y)) / scale
+    eps = 1e-4
+    a = w @ ((o.tendency(y + eps * v) - o.tendency(y - eps * v)) / (2 * eps))
+    assert abs(a - ob.tendency_vjp(y, w) @ v) > 1e-6 * abs(a)
+    assert abs(a - o.tendency_vjp(y, w) @ v) < 1e-8 * abs(a)
+
+
+@pytest.mark.parametrize("table", ["kites", "height", "rhombus", "weights"])
+def test_adjoint_check_is_sharp_for_gassmann_and_vertex_ke(table):
+    """The same check fails when one of the tables only the Gassmann flux or the vertex
+    kinetic energy uses is perturbed by 1e-3 in the adjoint's model alone."""
+    t, y, ne, scale = _adjoint_case(pv_flux="gassmann", kinetic_energy="vertex")
+    o = Oracle(t)
+    bad = ModelTables(t.grid, pv_flux="gassmann", kinetic_energy="vertex", **earth_parameters())
+    tables = bad.gassmann if table in ("rhombus", "weights") else bad.vertex_kinetic
+    tables[table] = np.asfortranarray(tables[table] * (1 + 1e-3))
+    ob = Oracle(bad)
+    rng = np.random.default_rng(14)
     v = rng.standard_normal(len(y)) * scale
     w = rng.standard_normal(len(y)) / scale
     eps = 1e-4
