@@ -84,8 +84,14 @@ struct Params {
     const double* ddxn;   // edges x nlev
     const int32_t* zlev;  // 2 x ne x nlev
     const double* zwt;    // 2 x ne x nlev
+    // the slot pattern of every TRiSK row (trisk::mpas_pv_slot), or NULL
+    const uint8_t* tpat;
     // work fields
     double* f[MPAS_FIELDS];
+    // (F, q_e) of every edge and level, interleaved: what the edge kernel stores and the vn
+    // tendency gathers (one 16-byte load per TRiSK slot); mpas_ops_field splits it into the
+    // MPAS_FS and MPAS_QE fields on request
+    double* fq;
 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
@@ -221,10 +227,9 @@ __global__ __launch_bounds__(kBlock) void mpas_edge_kernel(Params p, const doubl
             q = (p.f[MPAS_QV][v1 + p.nv * k] + p.f[MPAS_QV][v2 + p.nv * k]) / 2;
         }
         const int64_t a = e + p.ne * k;
-        p.f[MPAS_QE][a] = q;
         const double re = ew1 * rho[l + p.nc * k] + ew2 * rho[r + p.nc * k];
         p.f[MPAS_RHOE][a] = re;
-        p.f[MPAS_FS][a] = re * vn[a];
+        reinterpret_cast<double2*>(p.fq)[a] = double2{re * vn[a], q};
     }
     if (p.vertical > 0) {
         const int j1 = k1 == p.nlev ? p.nlev + 1 : k1;  // the last chunk also has the top interface
@@ -424,6 +429,19 @@ __device__ __forceinline__ double edge_w(const Params& p, const double* __restri
     return M / (ew1 * rhl + ew2 * rhr);
 }
 
+// The PV flux sum of a ten-entry row in slot pattern PT (trisk::mpas_pv_slot): the products
+// and their order are those of the row as stored, each PV read from the gathered pairs.
+template <int PT>
+__device__ __forceinline__ double pattern_sum(const double (&Tw)[10], const double2 (&P)[10], double qs) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+        const double qx = P[trisk::mpas_pv_slot(PT, j)].y;
+        s += Tw[j] * P[j].x * (((trisk::mpas_pv_mean(PT) >> j) & 1 ? qs : qx) + qx);
+    }
+    return s;
+}
+
 // VERT, PRESS: p.vertical and pressure > 0 fixed at compile time (-1: at run time)
 template <int KT, bool PAIRED, int VERT = -1, int PRESS = -1>
 __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, const double* __restrict__ vn,
@@ -451,8 +469,8 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     }
     const int32_t l = ld(p.ec + e), r = ld(p.ec + p.ne + e);
     const double ew1 = ld(p.ew + e), ew2 = ld(p.ew + p.ne + e), invd = ld(p.invd + e);
-    const double* Fs = p.f[MPAS_FS];
-    const double* qe = p.f[MPAS_QE];
+    const double2* FQ = reinterpret_cast<const double2*>(p.fq);
+    const int pat = PAIRED && KT == 10 && p.tpat ? ld(p.tpat + e) : -1;
     const double* th = p.f[MPAS_THETA];
     const double* pe = p.oro ? p.f[MPAS_PIP] : p.f[MPAS_EXNER];
     const double* K = p.f[MPAS_K];
@@ -469,25 +487,37 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         double s = 0.0;
         if constexpr (PAIRED && KT > 0) {
             // (q_b + q_c) with b = e: (q_e + q_x); with b = c: (q_x + q_x).  Every gather of
-            // the level is issued before the sum, so they are in flight together.
-            double Fv[KT], Qv[KT];
+            // the level is issued before the sum, so they are in flight together.  In a slot
+            // pattern every x is the flux edge of a slot of the row, so the (F, q) pairs of
+            // the ten flux edges hold every PV: ten 16-byte gathers instead of twenty.
+            double2 P[KT];
 #pragma unroll
-            for (int j = 0; j < KT; ++j) Fv[j] = Fs[I0[j] + p.ne * k];
+            for (int j = 0; j < KT; ++j) P[j] = FQ[I0[j] + p.ne * k];
+            const double qs = FQ[e + p.ne * k].y;
+            if (pat >= 0) {
+                if (pat == 0)
+                    s = pattern_sum<0>(Tw, P, qs);
+                else if (pat == 1)
+                    s = pattern_sum<1>(Tw, P, qs);
+                else
+                    s = pattern_sum<2>(Tw, P, qs);
+            } else {
+                double Qv[KT];
 #pragma unroll
-            for (int j = 0; j < KT; ++j) Qv[j] = qe[(I1[j] < 0 ? ~I1[j] : I1[j]) + p.ne * k];
-            const double qs = qe[e + p.ne * k];
+                for (int j = 0; j < KT; ++j) Qv[j] = FQ[(I1[j] < 0 ? ~I1[j] : I1[j]) + p.ne * k].y;
 #pragma unroll
-            for (int j = 0; j < KT; ++j) s += Tw[j] * Fv[j] * ((I1[j] < 0 ? Qv[j] : qs) + Qv[j]);
+                for (int j = 0; j < KT; ++j) s += Tw[j] * P[j].x * ((I1[j] < 0 ? Qv[j] : qs) + Qv[j]);
+            }
         } else if constexpr (PAIRED) {
-            const double qs = qe[e + p.ne * k];
+            const double qs = FQ[e + p.ne * k].y;
             for (int j = 0; j < kt; ++j) {
                 const bool twice = I1[j] < 0;
-                const double qx = qe[(twice ? ~I1[j] : I1[j]) + p.ne * k];
-                s += Tw[j] * Fs[I0[j] + p.ne * k] * ((twice ? qx : qs) + qx);
+                const double qx = FQ[(twice ? ~I1[j] : I1[j]) + p.ne * k].y;
+                s += Tw[j] * FQ[I0[j] + p.ne * k].x * ((twice ? qx : qs) + qx);
             }
         } else {
             for (int j = 0; j < kt; ++j)
-                s += Tw[j] * Fs[I0[j] + p.ne * k] * (qe[I1[j] + p.ne * k] + qe[I2[j] + p.ne * k]);
+                s += Tw[j] * FQ[I0[j] + p.ne * k].x * (FQ[I1[j] + p.ne * k].y + FQ[I2[j] + p.ne * k].y);
         }
         const double te = ew1 * th[l + p.nc * k] + ew2 * th[r + p.nc * k];
         double tt = s - invd * (K[r + p.nc * k] - K[l + p.nc * k]);
@@ -524,6 +554,16 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         }
         p.f[MPAS_TVN][e + p.ne * k] = out;
     }
+}
+
+// the pair plane split into F and q_e (mpas_ops_field)
+__global__ __launch_bounds__(kBlock) void mpas_split_pairs_kernel(const double* __restrict__ fq, double* __restrict__ F,
+                                                                  double* __restrict__ q, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = reinterpret_cast<const double2*>(fq)[i];
+    F[i] = v.x;
+    q[i] = v.y;
 }
 
 // ---------------------------------------------------------------------------------
@@ -733,6 +773,7 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
     rc |= trisk::upload(o, L.ev, &p.ev);
     rc |= trisk::upload(o, L.ti, &p.ti);
     if (L.paired) rc |= trisk::upload(o, L.ti2, &p.ti2);
+    if (L.pv_slots) rc |= trisk::upload(o, L.tpat, &p.tpat);
     rc |= trisk::upload(o, L.ew, &p.ew);
     rc |= trisk::upload(o, L.invd, &p.invd);
     rc |= trisk::upload(o, L.trw, &p.trw);
@@ -775,6 +816,11 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
         void* q = nullptr;
         rc = trisk::alloc(o, size_t(pts[f]) * size_t(lev[f]) * sizeof(double), &q);
         p.f[f] = static_cast<double*>(q);
+    }
+    if (!rc) {
+        void* q = nullptr;
+        rc = trisk::alloc(o, size_t(2) * size_t(L.ne) * size_t(L.nlev) * sizeof(double), &q);
+        p.fq = static_cast<double*>(q);
     }
     if (rc) {
         for (auto& b : o) (void)hipFree(b.p);
@@ -844,6 +890,16 @@ int mpas_slow_tendencies(mpas_ops* h, const double* vn, const double* w, const d
 int mpas_ops_field(mpas_ops* h, int32_t which, double** dev, int64_t* points, int64_t* levels) {
     if (!h) return set_err(TRISK_EINVAL, "null handle");
     if (which < 0 || which >= MPAS_FIELDS || !h->p.f[which]) return set_err(TRISK_EINVAL, "no such work field");
+    if (which == MPAS_FS || which == MPAS_QE) {
+        // F and q_e live interleaved in the pair plane: split it after the work in flight
+        HIP_TRY(hipSetDevice(h->device));
+        HIP_TRY(hipDeviceSynchronize());
+        const int64_t n = h->L.ne * int64_t(h->L.nlev);
+        hipLaunchKernelGGL(mpas_split_pairs_kernel, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, nullptr,
+                           h->p.fq, h->p.f[MPAS_FS], h->p.f[MPAS_QE], n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipDeviceSynchronize());
+    }
     if (dev) *dev = h->p.f[which];
     if (points) *points = h->field_points[which];
     if (levels) *levels = h->field_levels[which];

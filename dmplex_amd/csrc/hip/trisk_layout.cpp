@@ -560,6 +560,35 @@ bool build_mpas_layout(const mpas_ops_desc* d, MpasLayout& L, std::string& err) 
             }
         }
     if (!L.paired) L.ti2.clear();
+    // the slot pattern of every row (mpas_pv_slot), when all of them have one
+    L.pv_slots = L.paired && KT == 10;
+    if (L.pv_slots) {
+        L.tpat.assign(size_t(ne), 0);
+        int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+        for (int64_t e = 0; e < ne; ++e) {
+            int found = -1;
+            for (int pt = 0; pt < kMpasPvPatterns && found < 0; ++pt) {
+                bool ok = true;
+                for (int j = 0; j < 10 && ok; ++j) {
+                    const int32_t a = L.ti2[size_t(e + ne * j)], x = L.ti2[size_t(e + ne * (KT + j))];
+                    if ((mpas_pv_mean(pt) >> j) & 1)
+                        ok = x == a;  // (q_e + q_a)
+                    else
+                        ok = x < 0 && ~x == L.ti2[size_t(e + ne * mpas_pv_slot(pt, j))];  // (q_x + q_x)
+                }
+                if (ok) found = pt;
+            }
+            if (found < 0)
+                bad = 1;
+            else
+                L.tpat[size_t(e)] = uint8_t(found);
+        }
+        if (bad) {
+            L.pv_slots = false;
+            L.tpat.clear();
+        }
+    }
     points(d->dz, nc, NL, L.cperm, L.dz);
     points(d->dzh, nc, NL + 1, L.cperm, L.dzh);
     points(d->wlow, nc, NL + 1, L.cperm, L.wlow);

@@ -119,6 +119,21 @@ void build_flux_form(const trisk_model_desc* d, Layout& L);
 // areas) and the ICON ones, in the same numbering as build_layout.
 bool build_icon_layout(const icon_model_desc* d, Layout& L, std::string& err, int32_t narrow_limit = 32767);
 
+// Slot patterns of the ten-entry TRiSK rows of the MPAS-Gassmann operators on hexagon meshes
+// (the edge-first rows of trisk_index with the ICON-IAP PV edges, TRiSKGrid.jl:318-362): in
+// pattern p, slot j is a mean entry (q_e + q_a, with a the slot's own flux edge) when bit j of
+// mpas_pv_mean(p) is set, and otherwise takes (q_x + q_x) with x the flux edge of slot
+// mpas_pv_slot(p, j).  Pattern 0 holds every edge not next to a pentagon, 1 the edges next to
+// one, 2 the TRiSK mean flux (every slot a mean entry).
+constexpr int kMpasPvPatterns = 3;
+constexpr int mpas_pv_slot(int p, int j) {
+    constexpr int8_t pos[2][10] = {{9, 0, 2, 4, 5, 4, 5, 7, 9, 0}, {8, 0, 3, 4, 3, 4, 6, 8, 0, 9}};
+    return p == 2 ? j : pos[p][j];
+}
+constexpr uint32_t mpas_pv_mean(int p) {
+    return p == 0 ? (1u << 2) | (1u << 7) : p == 1 ? (1u << 6) | (1u << 9) : 0x3ffu;
+}
+
 // Tables of the MPAS-Gassmann operators (mpas_ops_create) in the device numbering: cells
 // along the Hilbert curve, edges and vertices by first appearance; entity-major tables
 // (slot k of entity j at k * n + j, int32 indices) and metric fields point-fastest per level.
@@ -133,6 +148,10 @@ struct MpasLayout {
     // (b == e: the PV pair is (q_e, q_x)) or ~x (b == c: (q_x, q_x)); at (c * kt + j) * ne + e
     bool paired = false;
     std::vector<int32_t> ti2;
+    // every row of ten paired entries in one of the fixed slot patterns (mpas_pv_slot): per
+    // edge its pattern; the vn tendency then takes every PV from the (F, q) pairs it gathers
+    bool pv_slots = false;
+    std::vector<uint8_t> tpat;
     std::vector<double> kin, inner, vkk, dz, dzh, wlow;
     std::vector<double> ew, invd, trw, rh, fe, edz, edzh;
     std::vector<double> curl, vw, fv, vkw;
