@@ -40,9 +40,9 @@ constexpr double Rd = 287.0, Cpd = 1004.0, Cvd = Cpd - Rd, P0 = 1.0e5;
 // levels per thread of the cell and edge kernels (1, 2, 4, 16 and 32 measured slower)
 constexpr int kLevels = 8;
 // The vn tendency reuses its ten-entry TRiSK row over several levels (measured at hexagon 7 x
-// 30 levels, with each level's gathers issued together: 0.396 ms per call with 16 levels per
-// thread, 0.41 with 32 or 10).
-constexpr int kLevelsVn = 16;
+// 30 levels with the (F, q_e) pair gathers: 0.325 ms per call with 32 levels per thread, 0.326
+// with 30, 0.329 with 16, 0.347 with 8).
+constexpr int kLevelsVn = 32;
 
 struct Params {
     int64_t nc, ne, nv;
