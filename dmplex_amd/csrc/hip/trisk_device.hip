@@ -188,13 +188,17 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
 // P2: edges (F, q_e) | cells (vertex part of K, B; only with vertex kinetic energy)
 
 // REC (FP32-bulk, hexagon widths): the rows from the records r2e / r2c staged through LDS.
+// With the records P1 stores A_v zeta_v and A_v h_v (its record coefficients carry A_v), so
+// the rhombus PV (r_0 zeta_0 + r_1 zeta_1 + f_e)/(r_0 h_0 + r_1 h_1), r_s = A_s/(A_0 + A_1),
+// is (A_0 zeta_0 + A_1 zeta_1 + (A_0 + A_1) f_e)/(A_0 h_0 + A_1 h_1): one coefficient per
+// edge, and the vertex part of the mass flux takes vkh_s / A_s.
 template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths, bool REC = false>
 __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __restrict__ u,
                                                       const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
     static_assert(!REC || (P::id == TRISK_FP32_BULK && W::cw == 6), "records: FP32-bulk hexagons");
-    constexpr int RE = 4 + (VK ? 2 : 0) + (GM ? 3 : 0);
+    constexpr int RE = 4 + (VK ? 2 : 0) + (GM ? 1 : 0);
     __shared__ __attribute__((aligned(16))) int32_t lds[REC ? kBlock * 9 : 1];
     Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
@@ -212,11 +216,7 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
                 vk0 = fbits(r[4]);
                 vk1 = fbits(r[5]);
             }
-            if (GM) {
-                r0 = fbits(r[RE - 3]);
-                r1 = fbits(r[RE - 2]);
-                fe = fbits(r[RE - 1]);
-            }
+            if (GM) fe = fbits(r[RE - 1]);  // (A_0 + A_1) f_e
         } else {
             if (e >= p.ne) return;
             ecw = index_words<W::cw ? 2 : 0>(p.ec16p, p.ne, e);
@@ -240,9 +240,13 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
         C F = he * C(u[e]);
         C q;
         if (GM) {
-            C z = r0 * C(p.zv[v0]) + r1 * C(p.zv[v1]);
-            C eta = r0 * C(p.hv[v0]) + r1 * C(p.hv[v1]);
-            q = (z + fe) / eta;
+            if constexpr (REC) {
+                q = (C(p.zv[v0]) + C(p.zv[v1]) + fe) / (C(p.hv[v0]) + C(p.hv[v1]));
+            } else {
+                C z = r0 * C(p.zv[v0]) + r1 * C(p.zv[v1]);
+                C eta = r0 * C(p.hv[v0]) + r1 * C(p.hv[v1]);
+                q = (z + fe) / eta;
+            }
         } else {
             q = (C(p.qv[v0]) + C(p.qv[v1])) / 2;
         }
@@ -678,31 +682,30 @@ static int setup_params(trisk_handle* H) {
                 r[q] = L.ve16p[size_t(q) * nv + v];
                 r[2 + q] = L.vc16p[size_t(q) * nv + v];
             }
+            // A_v zeta_v and A_v h_v (the edge kernel's records take the areas out again)
+            const double av = L.area_v[v];
             for (int k = 0; k < 3; ++k) {
-                r[4 + k] = bits(L.curl[size_t(k) * nv + v]);
-                r[7 + k] = bits(L.vh[size_t(k) * nv + v]);
+                r[4 + k] = bits(av * L.curl[size_t(k) * nv + v]);
+                r[7 + k] = bits(av * L.vh[size_t(k) * nv + v]);
                 if (vk) r[11 + k] = bits(L.vkw[size_t(k) * nv + v]);
             }
-            r[10] = bits(L.fv[v]);
+            r[10] = bits(av * L.fv[v]);
         }
-        const int RE = 4 + (vk ? 2 : 0) + (gm ? 3 : 0);
+        const int RE = 4 + (vk ? 2 : 0) + (gm ? 1 : 0);
         std::vector<int32_t> re(size_t(RE) * ne);
         for (int64_t e = 0; e < ne; ++e) {
             int32_t* r = &re[size_t(RE) * e];
             const double l = gf ? L.len[e] : 1.0;  // flux form: G = l F
+            const double a0 = L.area_v[L.ev[e]], a1 = L.area_v[L.ev[ne + e]];
             r[0] = L.ec16p[e];
             r[1] = L.ev16p[e];
             r[2] = bits(L.eh[e] * l);
             r[3] = bits(L.eh[ne + e] * l);
             if (vk) {
-                r[4] = bits(L.vkh[e] * l);
-                r[5] = bits(L.vkh[ne + e] * l);
+                r[4] = bits(L.vkh[e] * l / a0);
+                r[5] = bits(L.vkh[ne + e] * l / a1);
             }
-            if (gm) {
-                r[RE - 3] = bits(L.grh[e]);
-                r[RE - 2] = bits(L.grh[ne + e]);
-                r[RE - 1] = bits(L.gf[e]);
-            }
+            if (gm) r[RE - 1] = bits((a0 + a1) * L.gf[e]);
         }
         rc |= upload(o, rc1, &p.r1c);
         rc |= upload(o, rv, &p.r1v);
@@ -1615,7 +1618,9 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
                     nv * (KV * 2 * (ix + c) + c + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
         // with vertex kinetic energy P1 writes B and P2 adds the vertex part (K only for the
         // diagnostics)
-        double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? 3 * c : 0.0)) +
+        // the FP32-bulk records fold the rhombus weights into one coefficient per edge
+        const bool rec = !f64 && h->p32.r1c != nullptr;
+        double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? (rec ? 1 : 3) * c : 0.0)) +
                     nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 16.0) + nv * c : 0.0);
         const bool gp_table = !L.gassmann_mean && !(hex && L.gassmann_slots);
         const double pv = gm ? KG * (ix + c + (gp_table ? ix : 0.0)) : KT * (ix + c);

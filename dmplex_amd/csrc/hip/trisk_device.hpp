@@ -136,9 +136,9 @@ struct Params {
     const int32_t* rg;          // ne x 20
     // FP32-bulk table kernels on hexagon meshes: the slot-major tables of P1 and P2 as entity
     // records (16-bit index words first, then the FP32 bits of the coefficients), staged
-    // through LDS: P1 cells [ce 3 words, kin 6], vertices [ve 2, vc 2, curl 3, vh 3, f, vkw 3
-    // with vertex KE]; P2 edges [ec, ev, eh 2, vkh 2 with vertex KE, rhombus 2 and f_e with
-    // the Gassmann flux], cells [cv 3, vkk 6] (vertex KE)
+    // through LDS: P1 cells [ce 3 words, kin 6], vertices [ve 2, vc 2, A_v curl 3, A_v vh 3,
+    // A_v f, vkw 3 with vertex KE]; P2 edges [ec, ev, eh 2, vkh_s / A_s 2 with vertex KE,
+    // (A_0 + A_1) f_e with the Gassmann flux], cells [cv 3, vkk 6] (vertex KE)
     const int32_t *r1c, *r1v, *r2e, *r2c;
     // optional FP64 outputs of the diagnostics (NULL: not written)
     double* K_out;
