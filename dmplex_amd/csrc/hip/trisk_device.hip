@@ -87,7 +87,8 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
     using C = typename P::C;
     using I = typename P::I;
     static_assert(!REC || (P::id == TRISK_FP32_BULK && W::cw == 6 && W::vw == 3), "records: FP32-bulk hexagons");
-    constexpr int RV = VK ? 14 : 11;
+    // vertex record: indices 4 words, curl 3, vh 3, [f when q_v is formed], [vkw 3]
+    constexpr int RF = QV ? 1 : 0, RV = 10 + RF + (VK ? 3 : 0);
     __shared__ __attribute__((aligned(16))) int32_t lds[REC ? kBlock * RV : 1];
     Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
@@ -139,9 +140,9 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
             for (int k = 0; k < 3; ++k) {
                 cu[k] = fbits(r[4 + k]);
                 ch[k] = fbits(r[7 + k]);
-                if (VK) kw[k] = fbits(r[11 + k]);
+                if (VK) kw[k] = fbits(r[10 + RF + k]);
             }
-            f = fbits(r[10]);
+            if (QV) f = fbits(r[10]);
         } else {
             if (v >= p.nv) return;
             vew = index_words<W::vw>(p.ve16p, p.nv, v);
@@ -674,7 +675,8 @@ static int setup_params(trisk_handle* H) {
             for (int q = 0; q < 3; ++q) rc2[9 * i + q] = L.cv16p[size_t(q) * nc + i];
             for (int k = 0; k < 6; ++k) rc2[9 * i + 3 + k] = bits(L.vkk[size_t(k) * nc + i]);
         }
-        const int RV = vk ? 14 : 11;
+        // the vertex records of P1 (QV: with f, which the Gassmann flux does not read)
+        const int RF = gm ? 0 : 1, RV = 10 + RF + (vk ? 3 : 0);
         std::vector<int32_t> rv(size_t(RV) * nv);
         for (int64_t v = 0; v < nv; ++v) {
             int32_t* r = &rv[size_t(RV) * v];
@@ -687,9 +689,9 @@ static int setup_params(trisk_handle* H) {
             for (int k = 0; k < 3; ++k) {
                 r[4 + k] = bits(av * L.curl[size_t(k) * nv + v]);
                 r[7 + k] = bits(av * L.vh[size_t(k) * nv + v]);
-                if (vk) r[11 + k] = bits(L.vkw[size_t(k) * nv + v]);
+                if (vk) r[10 + RF + k] = bits(L.vkw[size_t(k) * nv + v]);
             }
-            r[10] = bits(av * L.fv[v]);
+            if (RF) r[10] = bits(av * L.fv[v]);
         }
         const int RE = 4 + (vk ? 2 : 0) + (gm ? 1 : 0);
         std::vector<int32_t> re(size_t(RE) * ne);
@@ -1613,13 +1615,16 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const double KG = L.kg;
         const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
         const double ix = hex ? 2.0 : iw;
-        // q_v is stored only for the TRiSK flux (the Gassmann edge PV comes from zeta_v, h_v)
+        // the FP32-bulk records (P1, P2 rows as entity records)
+        const bool rec = !f64 && h->p32.r1c != nullptr;
+        // q_v is stored only for the TRiSK flux (the Gassmann edge PV comes from zeta_v, h_v);
+        // the Gassmann records then leave out f_v too
         double p1 = nc * (KC * (ix + c) + 8.0 + 8.0) + ne * uw +
-                    nv * (KV * 2 * (ix + c) + c + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) + (vk ? KV * c + c : 0.0));
+                    nv * (KV * 2 * (ix + c) + (gm && rec ? 0.0 : c) + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) +
+                          (vk ? KV * c + c : 0.0));
         // with vertex kinetic energy P1 writes B and P2 adds the vertex part (K only for the
         // diagnostics)
         // the FP32-bulk records fold the rhombus weights into one coefficient per edge
-        const bool rec = !f64 && h->p32.r1c != nullptr;
         double p2 = ne * (2 * ix + 2 * c + 2 * ix + uw + 2 * c + (vk ? 2 * c : 0.0) + (gm ? (rec ? 1 : 3) * c : 0.0)) +
                     nc * 8.0 + nv * c * (gm || vk ? 2.0 : 1.0) + (vk ? nc * (KC * (c + ix) + 16.0) + nv * c : 0.0);
         const bool gp_table = !L.gassmann_mean && !(hex && L.gassmann_slots);

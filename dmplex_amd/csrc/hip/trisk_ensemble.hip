@@ -59,9 +59,13 @@ struct EnsLevel {
     double* sh;        // nc M         stage height
 };
 
-// A run of tiles of one range (cells, vertices or edges of one level) on one XCD.
+// A run of tiles of one range (cells, vertices or edges of one level) on one XCD.  The state
+// kernel takes a level's vertex run backward, so that it starts where the cell run ended and
+// finds that region's gathered member groups still in the XCD's L2.
+constexpr int32_t kEnsBackward = 8;
 struct EnsSeg {
-    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges, 3 all three interleaved
+    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges, 3 all three interleaved; | kEnsBackward
+                          // for a run taken last tile first
     int32_t tile0;        // first tile of the run within its range
     int32_t ntiles;
     int32_t cum;          // tiles before the run in its XCD's list
@@ -138,8 +142,9 @@ __device__ __forceinline__ void for_each_item(const EnsArgs& a, N&& count, F&& b
         g.ntiles = seg[s].ntiles;
         g.cum = seg[s].cum;
         g.nitems = seg[s].nitems;
-        int64_t tile = g.tile0 + (t - g.cum);
-        int kind = g.kind;
+        const bool back = (g.kind & kEnsBackward) != 0;
+        int64_t tile = g.tile0 + (back ? g.ntiles - 1 - (t - g.cum) : t - g.cum);
+        int kind = g.kind & ~kEnsBackward;
         int64_t nitems = g.nitems;
         if (kind == 3) {
             const int64_t sup = tile / 6, r = tile - 6 * sup;
@@ -539,11 +544,12 @@ static int build_list(trisk_ensemble* E, int list) {
                 add(x, r, 6 * std::min(groups, x * c), 6 * std::min(groups, (x + 1) * c));
             }
     } else if (sc.split != TRISK_SPLIT_BLOCKED) {
-        // XCD x: the x-th contiguous chunk of every range
+        // XCD x: the x-th contiguous chunk of every range (a level's vertices backward)
         for (int x = 0; x < kXcd; ++x)
             for (const auto& r : ranges) {
                 const int64_t c = (r.tiles + kXcd - 1) / kXcd;
                 add(x, r, std::min(r.tiles, x * c), std::min(r.tiles, (x + 1) * c));
+                if (r.kind == 1 && !per[x].empty() && per[x].back().kind == 1) per[x].back().kind |= kEnsBackward;
             }
     } else {
         // XCD x: the x-th contiguous chunk of the ranges laid end to end
