@@ -17,8 +17,11 @@
 //        per (cell, corner) partials
 //   A2   vertices: qbar_v from the corner partials of its three cells; zbar = qbar / h_v,
 //        hbar_v = -qbar q_v / h_v
-//   A3   edges: Gbar_e from its two sides; ubar_e = Gbar_e h_e + sum curl zbar + 2 u_e sum kin Bbar
-//   A4   cells: hbar_i = g Bbar_i + sum_k Gbar_k lh_k u_k + sum_k vh_k hbar_vk
+//   A3   edges: Gbar_e from its two sides; ubar_e = Gbar_e h_e + sum curl zbar + 2 u_e sum kin Bbar;
+//        and Gbar_e lh_s u_e for each side s
+//   A4   cells: hbar_i = g Bbar_i + sum_k Gbar_k lh_k u_k + sum_k vh_k hbar_vk, from the products
+//        A3 and A2 formed
+// W1 runs only where the sweep has not kept q_v and G of the stage (see enqueue_sweep).
 // A3 and A4 end in the RK4 adjoint epilogue, which combines the stage cotangents as the
 // transpose of y + dt/6 (k1 + 2 k2 + 2 k3 + k4).
 #include <hip/hip_runtime.h>
@@ -39,14 +42,14 @@ struct AdjParams {
     using I = typename P::I;
     const C* ecurl;        // ne x 2
     const C* ekin;         // ne x 2
-    const C* cvh;          // 6 x nc
+    const int32_t* cv3;    // 6 x nc: 3 v + the slot of cell i among the cells of its corner v
     const int32_t* vslot;  // nv
     double* Bbar;          // nc
     I* Tbar;               // 6 x nc, cell-major: cell i's partial of Gbar at its edge k in Tbar[6 i + k]
     I* QC;                 // nc x 6  (cell, corner) partials of qbar
     I* zbar;               // nv
-    I* ebar;               // nv      cotangent of h_v
-    I* Gbar;               // ne
+    I* VH;                 // 3 x nv: vh_j hbar_v, the term of vertex v in the hbar of its cell j
+    I* GL;                 // 2 x ne: Gbar_e lh_s u_e, the term of edge e in the hbar of its cell s
 };
 
 // Epilogue of A3/A4: the cotangent s of a stage state, combined as the RK4 adjoint.
@@ -163,7 +166,8 @@ __global__ __launch_bounds__(kBlock) void adj_cell_kernel(Params<P> p, AdjParams
     w.Bbar[i] = bb;
 }
 
-// A2: vertices (the stage state recomputes zeta_v, h_v and q_v as the forward does)
+// A2: vertices (the stage state recomputes zeta_v, h_v and q_v as the forward does); hbar_v
+// leaves as its three products with the vertex-height weights, one per cell
 template <class P, class UIn>
 __global__ __launch_bounds__(kBlock) void adj_vertex_kernel(Params<P> p, AdjParams<P> w, const UIn* __restrict__ u,
                                                             const double* __restrict__ h) {
@@ -184,7 +188,9 @@ __global__ __launch_bounds__(kBlock) void adj_vertex_kernel(Params<P> p, AdjPara
     for (int k = 0; k < 3; ++k) eta += ld(p.vh + k * p.nv + v) * C(h[ld(p.vc + k * p.nv + v)]);
     const C qv = (z + ld(p.fv + v)) / eta;
     w.zbar[v] = I(qb / eta);
-    w.ebar[v] = I(-qb * qv / eta);
+    const C eb = C(I(-qb * qv / eta));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w.VH[3 * v + k] = I(ld(p.vh + k * p.nv + v) * eb);
 }
 
 // A3: edges
@@ -209,7 +215,9 @@ __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams
     const C he = lh[0] * C(h[c[0]]) + lh[1] * C(h[c[1]]);
     const double s = double(Gb * he) + double(cu[0] * C(w.zbar[v0]) + cu[1] * C(w.zbar[v1])) +
                      2.0 * double(ue) * (double(kn[0]) * w.Bbar[c[0]] + double(kn[1]) * w.Bbar[c[1]]);
-    w.Gbar[e] = I(Gb);
+    const C gb = C(I(Gb));
+    w.GL[2 * e] = I(gb * lh[0] * ue);  // the terms of A4's sum over the edges of each cell
+    w.GL[2 * e + 1] = I(gb * lh[1] * ue);
     adj_store(r, e, s, rin);
 }
 
@@ -224,17 +232,17 @@ __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjPara
     const int32_t meta = ld(p.cmeta + i);
     const int n = meta & 15;
     double s = p.g * w.Bbar[i];
+    // the products were formed by A3 (Gbar lh u) and A2 (vh hbar_v) in the same order
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         if (k < n) {
             const int64_t e = ld(p.ce6 + k * p.nc + i);
-            const int side = (meta >> (4 + k)) & 1;
-            s += double(C(w.Gbar[e]) * ld(p.re_lh + 2 * e + side) * C(u[e]));
+            s += double(C(w.GL[2 * e + ((meta >> (4 + k)) & 1)]));
         }
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        if (k < n) s += double(ld(w.cvh + k * p.nc + i) * C(w.ebar[ld(p.ccv + k * p.nc + i)]));
+        if (k < n) s += double(C(w.VH[ld(w.cv3 + k * p.nc + i)]));
     }
     adj_store(r, p.ne + i, s, rin);
 }
@@ -475,8 +483,12 @@ struct AdjState {
     std::vector<DevBuf> work;
     int64_t work_ck_states = 0, work_seg_states = 0;
     double *ckp = nullptr, *seg = nullptr, *lam = nullptr, *acc = nullptr, *kbar = nullptr, *run = nullptr;
-    void* su[3] = {nullptr, nullptr, nullptr};
-    double* sh[3] = {nullptr, nullptr, nullptr};
+    // per step of a segment: the stage states s2, s3, s4 (su, sh: 3 per step) and, on the
+    // cell-pair path, q_v and G at the step's input and at s2, s3, s4 (sqv, sG: 4 per step), kept
+    // from the segment's recomputation for the VJPs at those states
+    std::vector<void*> su, sqv, sG;
+    std::vector<double*> sh;
+    double* scratch = nullptr;  // the discarded output of a segment's last step
     // the sweep of the last graph-replayed call
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -519,6 +531,14 @@ static TabAdj<P>& tab_params(AdjState* st) {
 
 template <class P>
 static const Params<P>& fwd_params(const trisk_handle* H) {
+    if constexpr (P::id == TRISK_FP64)
+        return H->p64;
+    else
+        return H->p32;
+}
+
+template <class P>
+static Params<P>& params_of(trisk_handle* H) {
     if constexpr (P::id == TRISK_FP64)
         return H->p64;
     else
@@ -692,7 +712,7 @@ static int adj_setup(trisk_handle* H) {
     int rc = 0;
     rc |= coef(L.adj_ecurl, &w.ecurl);
     rc |= coef(L.adj_ekin, &w.ekin);
-    rc |= coef(L.adj_cvh, &w.cvh);
+    rc |= upload(st->owned, L.adj_cv3, &w.cv3);
     rc |= upload(st->owned, L.adj_vslot, &w.vslot);
     if (rc) return rc;
     void* q;
@@ -704,10 +724,10 @@ static int adj_setup(trisk_handle* H) {
     w.QC = static_cast<I*>(q);
     if ((rc = alloc(st->owned, L.nv * sizeof(I), &q))) return rc;
     w.zbar = static_cast<I*>(q);
-    if ((rc = alloc(st->owned, L.nv * sizeof(I), &q))) return rc;
-    w.ebar = static_cast<I*>(q);
-    if ((rc = alloc(st->owned, L.ne * sizeof(I), &q))) return rc;
-    w.Gbar = static_cast<I*>(q);
+    if ((rc = alloc(st->owned, 3 * L.nv * sizeof(I), &q))) return rc;
+    w.VH = static_cast<I*>(q);
+    if ((rc = alloc(st->owned, 2 * L.ne * sizeof(I), &q))) return rc;
+    w.GL = static_cast<I*>(q);
     return 0;
 }
 
@@ -751,16 +771,23 @@ static int table_vjp(trisk_handle* H, const UIn* u, const double* h, const doubl
 }
 
 // One vector-Jacobian product at the stage (u, h) for the cotangent c * cot, ending in the
-// epilogue r.
+// epilogue r.  Cell-pair path: with kqv and kG given, the forward q_v and G of (u, h) (all the
+// VJP reads of W1) are those kept there by the segment's recomputation, and W1 is not run again.
 template <class P, class UIn>
 static int vjp(trisk_handle* H, const UIn* u, const double* h, const double* cot, double c, const AdjRk& r,
-               hipStream_t s) {
+               hipStream_t s, void* kqv = nullptr, void* kG = nullptr) {
     if (!H->L.pair) return table_vjp<P, UIn>(H, u, h, cot, c, r, s);
     const Layout& L = H->L;
-    const Params<P>& p = fwd_params<P>(H);
-    const AdjParams<P>& w = adj_params<P>(static_cast<AdjState*>(H->adj));
-    int rc = pair_state_pass(H, u, std::is_same<UIn, double>::value, h, s);
-    if (rc) return rc;
+    auto* st = static_cast<AdjState*>(H->adj);
+    Params<P> p = fwd_params<P>(H);
+    const AdjParams<P>& w = adj_params<P>(st);
+    if (kqv && kG) {
+        p.qv = static_cast<typename P::I*>(kqv);
+        p.G = static_cast<typename P::I*>(kG);
+    } else {
+        const int rc = pair_state_pass(H, u, std::is_same<UIn, double>::value, h, s);
+        if (rc) return rc;
+    }
     const dim3 gc(unsigned((L.nc + kBlock - 1) / kBlock)), gv(unsigned((L.nv + kBlock - 1) / kBlock)),
         ge(unsigned((L.ne + kBlock - 1) / kBlock)), b(kBlock);
     hipLaunchKernelGGL((adj_cell_kernel<P>), gc, b, 0, s, p, w, cot, c);
@@ -803,10 +830,18 @@ static int sweep_work(trisk_handle* H, AdjState* st, int64_t nck, int64_t seg) {
     };
     if (!(st->ckp = static_cast<double*>(get(nck * sb))) || !(st->seg = static_cast<double*>(get(seg * sb))) ||
         !(st->lam = static_cast<double*>(get(sb))) || !(st->acc = static_cast<double*>(get(sb))) ||
-        !(st->kbar = static_cast<double*>(get(sb))) || !(st->run = static_cast<double*>(get(sb))))
+        !(st->kbar = static_cast<double*>(get(sb))) || !(st->run = static_cast<double*>(get(sb))) ||
+        !(st->scratch = static_cast<double*>(get(sb))))
         return rc;
-    for (int k = 0; k < 3; ++k)
+    st->su.assign(size_t(3 * seg), nullptr);
+    st->sh.assign(size_t(3 * seg), nullptr);
+    for (int64_t k = 0; k < 3 * seg; ++k)
         if (!(st->su[k] = get(L.ne * sizeof(U))) || !(st->sh[k] = static_cast<double*>(get(L.nc * sizeof(double)))))
+            return rc;
+    st->sqv.assign(L.pair ? size_t(4 * seg) : 0, nullptr);
+    st->sG.assign(L.pair ? size_t(4 * seg) : 0, nullptr);
+    for (size_t k = 0; k < st->sqv.size(); ++k)
+        if (!(st->sqv[k] = get(L.nv * sizeof(typename P::I))) || !(st->sG[k] = get(L.ne * sizeof(typename P::I))))
             return rc;
     st->work_ck_states = nck;
     st->work_seg_states = seg;
@@ -845,33 +880,56 @@ static int enqueue_sweep(trisk_handle* H, AdjState* st, const double* y0, double
     r.dt = dt;
     r.dt2 = dt / 2;
     r.dt6 = dt / 6;
+    Params<P>& fp = params_of<P>(H);
+    const auto fwd_qv = fp.qv;
+    const auto fwd_G = fp.G;
     for (int64_t g = nck - 1; g >= 0; --g) {
         const int64_t first = g * ck, len = std::min(ck, N - first);
+        // the segment again from its checkpoint, stage by stage with the forward's own
+        // arithmetic: step k's stage states stay in su/sh[3k + i], and on the cell-pair path the
+        // q_v and G at its tendency inputs (y_k, s2, s3, s4) in sqv/sG[4k + i].  A step's last
+        // tendency updates a copy of y_k in place into y_{k+1}; the segment's last step leaves
+        // it out (its output is the next checkpoint) and its VJP at s4 runs W1 instead.
         HIP_TRY(hipMemcpyAsync(S0, C0 + g * n, sb, hipMemcpyDeviceToDevice, s));
-        for (int64_t k = 1; k < len; ++k) {
-            HIP_TRY(hipMemcpyAsync(S0 + k * n, S0 + (k - 1) * n, sb, hipMemcpyDeviceToDevice, s));
-            if ((rc = rk4_steps_direct(H, S0 + k * n, dt, 1, s))) return rc;
+        for (int64_t k = 0; k < len; ++k) {
+            double* yn = S0 + k * n;
+            double* ynext = k + 1 < len ? S0 + (k + 1) * n : st->scratch;
+            HIP_TRY(hipMemcpyAsync(ynext, yn, sb, hipMemcpyDeviceToDevice, s));
+            r.y = ynext;
+            const int stages = k + 1 < len ? 4 : 3;
+            for (int i = 0; i < stages; ++i) {
+                r.mode = i + 1;
+                if (i < 3) {
+                    r.su = st->su[3 * k + i];
+                    r.sh = st->sh[3 * k + i];
+                }
+                const void* uin = i == 0 ? static_cast<const void*>(yn) : st->su[3 * k + i - 1];
+                const double* hin = i == 0 ? yn + L.ne : st->sh[3 * k + i - 1];
+                if (L.pair) {
+                    fp.qv = static_cast<typename P::I*>(st->sqv[4 * k + i]);
+                    fp.G = static_cast<typename P::I*>(st->sG[4 * k + i]);
+                }
+                rc = tendency_stage(H, uin, i == 0, hin, r, s);
+                fp.qv = fwd_qv;
+                fp.G = fwd_G;
+                if (rc) return rc;
+            }
         }
         for (int64_t k = len - 1; k >= 0; --k) {
             double* yn = S0 + k * n;
-            // stage states s2, s3, s4 of the step from yn (the forward's own arithmetic)
-            r.y = yn;
-            for (int i = 0; i < 3; ++i) {
-                r.mode = i + 1;
-                r.su = st->su[i];
-                r.sh = st->sh[i];
-                const void* uin = i == 0 ? static_cast<const void*>(yn) : st->su[i - 1];
-                const double* hin = i == 0 ? yn + L.ne : st->sh[i - 1];
-                if ((rc = tendency_stage(H, uin, i == 0, hin, r, s))) return rc;
-            }
+            auto kqv = [&](int i) { return L.pair && (i < 3 || k + 1 < len) ? st->sqv[4 * k + i] : nullptr; };
+            auto kG = [&](int i) { return L.pair && (i < 3 || k + 1 < len) ? st->sG[4 * k + i] : nullptr; };
+            const U* su2 = static_cast<const U*>(st->su[3 * k + 2]);
+            const U* su1 = static_cast<const U*>(st->su[3 * k + 1]);
+            const U* su0 = static_cast<const U*>(st->su[3 * k]);
             a.mode = 4;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[2]), st->sh[2], la, dt / 6, a, s))) return rc;
+            if ((rc = vjp<P, U>(H, su2, st->sh[3 * k + 2], la, dt / 6, a, s, kqv(3), kG(3)))) return rc;
             a.mode = 3;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[1]), st->sh[1], a.kbar, 1.0, a, s))) return rc;
+            if ((rc = vjp<P, U>(H, su1, st->sh[3 * k + 1], a.kbar, 1.0, a, s, kqv(2), kG(2)))) return rc;
             a.mode = 2;
-            if ((rc = vjp<P, U>(H, static_cast<const U*>(st->su[0]), st->sh[0], a.kbar, 1.0, a, s))) return rc;
+            if ((rc = vjp<P, U>(H, su0, st->sh[3 * k], a.kbar, 1.0, a, s, kqv(1), kG(1)))) return rc;
             a.mode = 1;
-            if ((rc = vjp<P, double>(H, yn, yn + L.ne, a.kbar, 1.0, a, s))) return rc;
+            if ((rc = vjp<P, double>(H, yn, yn + L.ne, a.kbar, 1.0, a, s, kqv(0), kG(0)))) return rc;
         }
     }
     HIP_TRY(hipMemcpyAsync(ybar0, la, sb, hipMemcpyDeviceToDevice, s));

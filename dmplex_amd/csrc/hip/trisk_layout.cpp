@@ -915,7 +915,7 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
             L.eslot[e] |= uint8_t((k < n ? k : 0) << (4 * s2));
         }
     }
-    L.adj_cvh.assign(size_t(6) * nc, 0.0);
+    L.adj_cv3.assign(size_t(6) * nc, 0);
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int64_t i = 0; i < nc; ++i) {
         const int n = L.cmeta[i] & 15;
@@ -924,7 +924,7 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
             int j = 0;
             while (j < 3 && L.vc[size_t(j) * nv + v] != i) ++j;
             bad |= j == 3;
-            L.adj_cvh[size_t(k) * nc + i] = j < 3 ? L.vh[size_t(j) * nv + v] : 0.0;
+            L.adj_cv3[size_t(k) * nc + i] = int32_t(3 * v + (j < 3 ? j : 0));
         }
     }
     L.adj_vslot.assign(nv, 0);

@@ -84,7 +84,7 @@ struct Layout {
     // transposed coefficients for the adjoint of the cell-pair path
     std::vector<double> adj_ecurl;       // ne x 2: curl coefficient of edge e in vertex ev[s]
     std::vector<double> adj_ekin;        // ne x 2: kinetic-energy weight of edge e in cell ec[s]
-    std::vector<double> adj_cvh;         // 6 x nc: vertex-height weight of cell i in its corner k
+    std::vector<int32_t> adj_cv3;        // 6 x nc: 3 v + the slot of cell i among the cells of its corner v
     std::vector<int32_t> adj_vslot;      // nv: corner index of v in its cell j at bits 4j..4j+3
     // ICON triangle model (icon_create): kc = 3; ce, div, ec, ev, eh (c_lin_e), invd, ve,
     // curl, vc, vh, fv and the areas as above, plus

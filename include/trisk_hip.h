@@ -410,9 +410,11 @@ int trisk_tendency_vjp(trisk_handle* h, const double* y_dev, const double* dybar
 /* The cotangent ybar0 of y0 for the cotangent ybarT of y_T, y_T being y0 after nsteps RK4
  * steps of dt (as trisk_rk4_steps).  The forward run keeps the state every
  * checkpoint_every steps (0: 8) and each segment is recomputed from its checkpoint during
- * the backward sweep.  yT_dev (may be NULL) receives y_T.  ybar0 may alias ybarT.  The
- * handle keeps (ceil(nsteps / checkpoint_every) + checkpoint_every + 4) state vectors of
- * work memory (grown as needed) until trisk_destroy.  With trisk_set_graphs(h, 1) and a
+ * the backward sweep, keeping every step's stage states for its transposition.  yT_dev
+ * (may be NULL) receives y_T.  ybar0 may alias ybarT.  The handle keeps
+ * (ceil(nsteps / checkpoint_every) + checkpoint_every + 5) state vectors of work memory,
+ * plus per step of a segment its three stage states and, on the cell-pair path, four copies
+ * of q_v and G (grown as needed), until trisk_destroy.  With trisk_set_graphs(h, 1) and a
  * stream, the whole sweep is captured as one HIP graph and replayed while the pointers,
  * dt, nsteps and checkpoint_every stay those of the capture.  Synchronous. */
 int trisk_rk4_adjoint(trisk_handle* h, const double* y0_dev, double dt, int64_t nsteps, int64_t checkpoint_every,
