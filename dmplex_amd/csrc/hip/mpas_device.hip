@@ -88,10 +88,11 @@ struct Params {
     const uint8_t* tpat;
     // work fields
     double* f[MPAS_FIELDS];
-    // (F, q_e) of every edge and level, interleaved: what the edge kernel stores and the vn
-    // tendency gathers (one 16-byte load per TRiSK slot); mpas_ops_field splits it into the
-    // MPAS_FS and MPAS_QE fields on request
-    double* fq;
+    // Interleaved pairs of work fields that a later kernel gathers together (one 16-byte load
+    // each); mpas_ops_field splits them into their MPAS_* fields on request:
+    double* fq;  // edges x nlev: (F = rho_e vn, q_e), gathered by the vn tendency
+    double* zr;  // vertices x nlev: (zeta_v, rho_v), gathered by the edge kernel
+    double* re;  // edges x (nlev + 1): (rho_e of layer j (0 at the top), eta at interface j), the w tendency's
 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
@@ -177,8 +178,7 @@ __global__ __launch_bounds__(kBlock) void mpas_cell_vertex_kernel(Params p, cons
                     for (int j = 0; j < kv; ++j) kv_s += p.vkw[j * p.nv + v] * sq(vn[p.ve[j * p.nv + v] + p.ne * k]);
             }
             const int64_t a = v + p.nv * k;
-            p.f[MPAS_ZETA][a] = z;
-            p.f[MPAS_RHOV][a] = eta;
+            reinterpret_cast<double2*>(p.zr)[a] = double2{z, eta};
             p.f[MPAS_QV][a] = (z + f) / eta;
             if (p.vk) p.f[MPAS_KV][a] = kv_s;
         }
@@ -220,27 +220,25 @@ __global__ __launch_bounds__(kBlock) void mpas_edge_kernel(Params p, const doubl
     for (int k = k0; k < k1; ++k) {
         double q;
         if (p.iap_pv) {
-            const double z = rh1 * p.f[MPAS_ZETA][v1 + p.nv * k] + rh2 * p.f[MPAS_ZETA][v2 + p.nv * k];
-            const double eta = rh1 * p.f[MPAS_RHOV][v1 + p.nv * k] + rh2 * p.f[MPAS_RHOV][v2 + p.nv * k];
+            const double2 a1 = reinterpret_cast<const double2*>(p.zr)[v1 + p.nv * k];
+            const double2 a2 = reinterpret_cast<const double2*>(p.zr)[v2 + p.nv * k];
+            const double z = rh1 * a1.x + rh2 * a2.x;
+            const double eta = rh1 * a1.y + rh2 * a2.y;
             q = (z + fe) / eta;
         } else {
             q = (p.f[MPAS_QV][v1 + p.nv * k] + p.f[MPAS_QV][v2 + p.nv * k]) / 2;
         }
         const int64_t a = e + p.ne * k;
         const double re = ew1 * rho[l + p.nc * k] + ew2 * rho[r + p.nc * k];
-        p.f[MPAS_RHOE][a] = re;
         reinterpret_cast<double2*>(p.fq)[a] = double2{re * vn[a], q};
+        // (rho_e, eta) at layer and lower interface k; eta only with the vector-invariant coupling
+        double eta = 0.0;
+        if (p.vertical > 0 && k >= 1)
+            eta = (vn[e + p.ne * k] - vn[e + p.ne * (k - 1)]) / ld(p.edzh + e + p.ne * k) -
+                  (w[r + p.nc * k] - w[l + p.nc * k]) * invd;
+        reinterpret_cast<double2*>(p.re)[a] = double2{re, eta};
     }
-    if (p.vertical > 0) {
-        const int j1 = k1 == p.nlev ? p.nlev + 1 : k1;  // the last chunk also has the top interface
-        for (int j = k0; j < j1; ++j) {
-            double eta = 0.0;
-            if (j >= 1 && j <= p.nlev - 1)
-                eta = (vn[e + p.ne * j] - vn[e + p.ne * (j - 1)]) / ld(p.edzh + e + p.ne * j) -
-                      (w[r + p.nc * j] - w[l + p.nc * j]) * invd;
-            p.f[MPAS_ETA][e + p.ne * j] = eta;
-        }
-    }
+    if (k1 == p.nlev) reinterpret_cast<double2*>(p.re)[e + p.ne * p.nlev] = double2{0.0, 0.0};  // the top
 }
 
 // ---------------------------------------------------------------------------------
@@ -272,10 +270,11 @@ __global__ __launch_bounds__(kBlock) void mpas_w_tendency_kernel(Params p, const
     // vn, rho_e of the cell's edges and K of the layer below the interface, carried from one
     // interface to the next (the layer above interface j is the one below interface j + 1)
     double VP[N], RP[N], Kp = 0.0;
+    const double2* RE = reinterpret_cast<const double2*>(p.re);
     if (vi && k0 >= 1) {
         for (int n = 0; n < kc; ++n) {
             VP[n] = vn[E[n] + p.ne * (k0 - 1)];
-            RP[n] = mw ? p.f[MPAS_RHOE][E[n] + p.ne * (k0 - 1)] : 1.0;
+            RP[n] = mw ? RE[E[n] + p.ne * (k0 - 1)].x : 1.0;
         }
         Kp = p.f[MPAS_K][i + p.nc * (k0 - 1)];
     }
@@ -287,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void mpas_w_tendency_kernel(Params p, const
         if (vi && j == 0) {
             for (int n = 0; n < kc; ++n) {
                 VP[n] = vn[E[n]];
-                RP[n] = mw ? p.f[MPAS_RHOE][E[n]] : 1.0;
+                RP[n] = mw ? RE[E[n]].x : 1.0;
             }
             Kp = p.f[MPAS_K][i];
         }
@@ -298,8 +297,9 @@ __global__ __launch_bounds__(kBlock) void mpas_w_tendency_kernel(Params p, const
                 for (int n = 0; n < kc; ++n) {
                     const int64_t up = E[n] + p.ne * j;
                     const double rl = RP[n], vl = VP[n];
-                    const double ru = mw ? p.f[MPAS_RHOE][up] : 1.0, vu = vn[up];
-                    s += Wn[n] * p.f[MPAS_ETA][up] * (rl * vl + ru * vu) / (2 * (wl * rl + (1 - wl) * ru));
+                    const double2 rn = RE[up];
+                    const double ru = mw ? rn.x : 1.0, vu = vn[up];
+                    s += Wn[n] * rn.y * (rl * vl + ru * vu) / (2 * (wl * rl + (1 - wl) * ru));
                     VP[n] = vu;
                     RP[n] = ru;
                 }
@@ -474,12 +474,14 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     const double* th = p.f[MPAS_THETA];
     const double* pe = p.oro ? p.f[MPAS_PIP] : p.f[MPAS_EXNER];
     const double* K = p.f[MPAS_K];
-    const double* eta = p.f[MPAS_ETA];
+    const double2* RE = reinterpret_cast<const double2*>(p.re);  // (rho_e, eta): eta at .y
     const int vert = VERT >= 0 ? VERT : p.vertical;
     const bool press = PRESS >= 0 ? PRESS == 1 : pressure > 0;
     const bool mw = vert == 2;
     // w~ at the lower interface of the level; the upper one is the next level's lower
     double Wnext = vert > 0 && k0 > 0 ? edge_w(p, w, rho, l, r, k0, ew1, ew2, mw) : 0.0;
+    // eta at the lower interface of the level, carried like W~ (the upper one is the next level's lower)
+    double Enext = vert > 0 && k0 < p.nlev ? RE[e + p.ne * k0].y : 0.0;
 #pragma unroll
     for (int q = 0; q < kLevelsVn; ++q) {
         const int k = k0 + q;
@@ -543,8 +545,9 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
             const double Wl = Wnext;
             const double Wu = k < p.nlev - 1 ? edge_w(p, w, rho, l, r, k + 1, ew1, ew2, mw) : 0.0;
             Wnext = Wu;
-            out = tt - (ld(p.edzh + e + p.ne * k) * eta[e + p.ne * k] * Wl +
-                        ld(p.edzh + e + p.ne * (k + 1)) * eta[e + p.ne * (k + 1)] * Wu) /
+            const double El = Enext, Eu = RE[e + p.ne * (k + 1)].y;
+            Enext = Eu;
+            out = tt - (ld(p.edzh + e + p.ne * k) * El * Wl + ld(p.edzh + e + p.ne * (k + 1)) * Eu * Wu) /
                            (2 * ld(p.edz + e + p.ne * k));
         } else {
             const double we = (w[l + p.nc * k] + w[r + p.nc * k] + w[l + p.nc * (k + 1)] + w[r + p.nc * (k + 1)]) / 4;
@@ -556,14 +559,16 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     }
 }
 
-// the pair plane split into F and q_e (mpas_ops_field)
-__global__ __launch_bounds__(kBlock) void mpas_split_pairs_kernel(const double* __restrict__ fq, double* __restrict__ F,
-                                                                  double* __restrict__ q, int64_t n) {
+// a pair plane of n pairs split into its two fields (mpas_ops_field); the first field has
+// only the first n1 entries
+__global__ __launch_bounds__(kBlock) void mpas_split_pairs_kernel(const double* __restrict__ pairs,
+                                                                  double* __restrict__ a, double* __restrict__ b,
+                                                                  int64_t n, int64_t n1) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double2 v = reinterpret_cast<const double2*>(fq)[i];
-    F[i] = v.x;
-    q[i] = v.y;
+    const double2 v = reinterpret_cast<const double2*>(pairs)[i];
+    if (i < n1) a[i] = v.x;
+    b[i] = v.y;
 }
 
 // ---------------------------------------------------------------------------------
@@ -822,6 +827,16 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
         rc = trisk::alloc(o, size_t(2) * size_t(L.ne) * size_t(L.nlev) * sizeof(double), &q);
         p.fq = static_cast<double*>(q);
     }
+    if (!rc) {
+        void* q = nullptr;
+        rc = trisk::alloc(o, size_t(2) * size_t(L.nv) * size_t(L.nlev) * sizeof(double), &q);
+        p.zr = static_cast<double*>(q);
+    }
+    if (!rc) {
+        void* q = nullptr;
+        rc = trisk::alloc(o, size_t(2) * size_t(L.ne) * size_t(L.nlev + 1) * sizeof(double), &q);
+        p.re = static_cast<double*>(q);
+    }
     if (rc) {
         for (auto& b : o) (void)hipFree(b.p);
         delete H;
@@ -890,13 +905,26 @@ int mpas_slow_tendencies(mpas_ops* h, const double* vn, const double* w, const d
 int mpas_ops_field(mpas_ops* h, int32_t which, double** dev, int64_t* points, int64_t* levels) {
     if (!h) return set_err(TRISK_EINVAL, "null handle");
     if (which < 0 || which >= MPAS_FIELDS || !h->p.f[which]) return set_err(TRISK_EINVAL, "no such work field");
+    // fields that live interleaved in a pair plane: split it after the work in flight
+    const double* pairs = nullptr;
+    double *first = nullptr, *second = nullptr;
+    int64_t n = 0, n1 = 0;
     if (which == MPAS_FS || which == MPAS_QE) {
-        // F and q_e live interleaved in the pair plane: split it after the work in flight
+        pairs = h->p.fq, first = h->p.f[MPAS_FS], second = h->p.f[MPAS_QE];
+        n = n1 = h->L.ne * int64_t(h->L.nlev);
+    } else if (which == MPAS_ZETA || which == MPAS_RHOV) {
+        pairs = h->p.zr, first = h->p.f[MPAS_ZETA], second = h->p.f[MPAS_RHOV];
+        n = n1 = h->L.nv * int64_t(h->L.nlev);
+    } else if (which == MPAS_RHOE || which == MPAS_ETA) {
+        pairs = h->p.re, first = h->p.f[MPAS_RHOE], second = h->p.f[MPAS_ETA];
+        n1 = h->L.ne * int64_t(h->L.nlev);  // rho_e has no top row
+        n = n1 + h->L.ne;
+    }
+    if (pairs) {
         HIP_TRY(hipSetDevice(h->device));
         HIP_TRY(hipDeviceSynchronize());
-        const int64_t n = h->L.ne * int64_t(h->L.nlev);
         hipLaunchKernelGGL(mpas_split_pairs_kernel, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, nullptr,
-                           h->p.fq, h->p.f[MPAS_FS], h->p.f[MPAS_QE], n);
+                           pairs, first, second, n, n1);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipDeviceSynchronize());
     }
@@ -952,16 +980,16 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
              (3 + (p.vk ? 1 : 0)) * nv * L * c + nc * KC * (ix + c) + nv * (KV * 2 * (ix + c) + c + (p.vk ? KV * c : 0));
     v[K_A2] = p.vk ? nc * KC * (ix + c) + nv * L * c + 2 * nc * L * c + (kwb ? nc * (2 * L1 + L) * c : 0.0) : 0.0;
     // B: zeta, rho_v (or q_v), rho, vn [, w] read; q_e, rho_e, F [, eta] written; tables
-    v[K_B] = nv * L * c * (p.iap_pv ? 2 : 1) + nc * L * c + ne * L * c + 3 * ne * L * c +
-             (vi ? nc * L1 * c + ne * L1 * c * 2 : 0.0) + ne * (4 * ix + 7 * c);
-    // C: eta, vn [rho_e], K, dzh, wlow -> tw (vector invariant); vn, w, dz -> tw (advective)
-    v[K_C] = vi ? ne * L1 * c + ne * L * c * (p.vertical == 2 ? 2 : 1) + nc * L * c + 2 * nc * L1 * c + nc * L1 * c +
-                      nc * KC * (ix + c)
+    // (the pair planes: (F, q_e) per layer, (rho_e, eta) per interface)
+    v[K_B] = nv * L * c * (p.iap_pv ? 2 : 1) + nc * L * c + ne * L * c + 2 * ne * L * c + 2 * ne * L1 * c +
+             (vi ? nc * L1 * c + ne * L1 * c : 0.0) + ne * (4 * ix + 7 * c);
+    // C: (rho_e, eta), vn, K, dzh, wlow -> tw (vector invariant); vn, w, dz -> tw (advective)
+    v[K_C] = vi ? 2 * ne * L1 * c + ne * L * c + nc * L * c + 2 * nc * L1 * c + nc * L1 * c + nc * KC * (ix + c)
                 : ne * L * c + nc * L1 * c + nc * L * c + nc * L1 * c + nc * KC * (ix + c) + ne * (2 * ix + c);
     v[K_E] = vi && h->five_band ? nc * (L1 * c * 2 + L * c * 3) : 0.0;
     // D: F, q_e, theta, pi, K, [eta, w, rho, wlow, edge dz, dzh | w, vn, edge dz] -> tvn; tables
     v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
-             (vi ? ne * L1 * c * 2 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
+             (vi ? ne * L1 * c * 3 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
              (p.oro == 1 ? ne * L * c + nc * L * c : p.oro == 2 ? 2 * ne * L * (ix + c) : 0.0);
     if (p.oro) {
         // CC: vn, ddxn, the cell rows -> w_c; CI: w_c, wlow, q, w -> w_c and w - w_c at the
