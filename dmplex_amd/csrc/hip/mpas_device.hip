@@ -93,6 +93,7 @@ struct Params {
     double* fq;  // edges x nlev: (F = rho_e vn, q_e), gathered by the vn tendency
     double* zr;  // vertices x nlev: (zeta_v, rho_v), gathered by the edge kernel
     double* re;  // edges x (nlev + 1): (rho_e of layer j (0 at the top), eta at interface j), the w tendency's
+    double* tp;  // cells x nlev: (theta, pi), gathered by the vn tendency
 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
@@ -140,8 +141,7 @@ __global__ __launch_bounds__(kBlock) void mpas_cell_vertex_kernel(Params p, cons
             double K = s;
             if (add_kw) K = K + w_kinetic(p, w, i, k);
             p.f[MPAS_K][a] = K;
-            p.f[MPAS_THETA][a] = rt[a] / rho[a];
-            p.f[MPAS_EXNER][a] = pow(Rd * rt[a] / P0, Rd / Cvd);
+            reinterpret_cast<double2*>(p.tp)[a] = double2{rt[a] / rho[a], pow(Rd * rt[a] / P0, Rd / Cvd)};
         }
     } else {
         const int64_t v = t.first + threadIdx.x;
@@ -393,12 +393,12 @@ __global__ __launch_bounds__(kBlock) void mpas_exner_deviation_kernel(Params p, 
     const int64_t i = t.first + threadIdx.x;
     if (i >= p.nc) return;
     const int k0 = int(blockIdx.y) * kLevels, k1 = min(k0 + kLevels, p.nlev);
-    const double* pi = p.f[MPAS_EXNER];
+    const double2* TP = reinterpret_cast<const double2*>(p.tp);  // pi at .y
     for (int k = k0; k < k1; ++k) {
         const int kl = k > 0 ? k - 1 : 0, ku = k < p.nlev - 1 ? k + 1 : p.nlev - 1;
         const int64_t a = i + p.nc * k, al = i + p.nc * kl, au = i + p.nc * ku;
-        p.f[MPAS_PIP][a] = pi[a] - p.piref[a];
-        p.f[MPAS_DPIDZ][a] = ((pi[au] - p.piref[au]) - (pi[al] - p.piref[al])) / (p.zf[au] - p.zf[al]);
+        p.f[MPAS_PIP][a] = TP[a].y - p.piref[a];
+        p.f[MPAS_DPIDZ][a] = ((TP[au].y - p.piref[au]) - (TP[al].y - p.piref[al])) / (p.zf[au] - p.zf[al]);
     }
 }
 
@@ -418,13 +418,12 @@ __global__ __launch_bounds__(kBlock) void mpas_w_concorr_kernel(Params p, const 
 // D: vn_tendency_kernel! with edge_w (Gassmann/Tendencies.jl:4-67); over orography w is w - w_c,
 // pi the deviation pi' and the pressure gradient that of pressure_gradient (:13-30)
 
-// w~ = M_e / rho_e at interface j (edge_w, Tendencies.jl:4-11)
-__device__ __forceinline__ double edge_w(const Params& p, const double* __restrict__ w, const double* __restrict__ rho,
-                                         int32_t l, int32_t r, int j, double ew1, double ew2, bool mw) {
-    const double rhl = mw ? p.wlow[l + p.nc * j] * rho[l + p.nc * (j - 1)] + (1 - p.wlow[l + p.nc * j]) * rho[l + p.nc * j]
-                          : 1.0;
-    const double rhr = mw ? p.wlow[r + p.nc * j] * rho[r + p.nc * (j - 1)] + (1 - p.wlow[r + p.nc * j]) * rho[r + p.nc * j]
-                          : 1.0;
+// w~ = M_e / rho_e at interface j (edge_w, Tendencies.jl:4-11); with mass weighting (mw) the
+// densities of the layers below (bl, br) and above (al, ar) the interface at the two cells
+__device__ __forceinline__ double edge_w(const Params& p, const double* __restrict__ w, int32_t l, int32_t r, int j,
+                                         double ew1, double ew2, bool mw, double bl, double al, double br, double ar) {
+    const double rhl = mw ? p.wlow[l + p.nc * j] * bl + (1 - p.wlow[l + p.nc * j]) * al : 1.0;
+    const double rhr = mw ? p.wlow[r + p.nc * j] * br + (1 - p.wlow[r + p.nc * j]) * ar : 1.0;
     const double M = ew1 * rhl * w[l + p.nc * j] + ew2 * rhr * w[r + p.nc * j];
     return M / (ew1 * rhl + ew2 * rhr);
 }
@@ -471,15 +470,30 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
     const double ew1 = ld(p.ew + e), ew2 = ld(p.ew + p.ne + e), invd = ld(p.invd + e);
     const double2* FQ = reinterpret_cast<const double2*>(p.fq);
     const int pat = PAIRED && KT == 10 && p.tpat ? ld(p.tpat + e) : -1;
-    const double* th = p.f[MPAS_THETA];
-    const double* pe = p.oro ? p.f[MPAS_PIP] : p.f[MPAS_EXNER];
+    const double2* TP = reinterpret_cast<const double2*>(p.tp);  // (theta, pi)
+    const double* pip = p.f[MPAS_PIP];                           // pi' over orography
     const double* K = p.f[MPAS_K];
     const double2* RE = reinterpret_cast<const double2*>(p.re);  // (rho_e, eta): eta at .y
     const int vert = VERT >= 0 ? VERT : p.vertical;
     const bool press = PRESS >= 0 ? PRESS == 1 : pressure > 0;
     const bool mw = vert == 2;
     // w~ at the lower interface of the level; the upper one is the next level's lower
-    double Wnext = vert > 0 && k0 > 0 ? edge_w(p, w, rho, l, r, k0, ew1, ew2, mw) : 0.0;
+    // the layer densities at the two cells below the next interface, carried from level to level
+    double Rl = 0.0, Rr = 0.0;
+    if (mw && vert > 0 && k0 > 0) {
+        Rl = rho[l + p.nc * (k0 - 1)];
+        Rr = rho[r + p.nc * (k0 - 1)];
+    }
+    double Wnext = 0.0;
+    if (vert > 0 && k0 > 0) {
+        const double al = mw ? rho[l + p.nc * k0] : 0.0, ar = mw ? rho[r + p.nc * k0] : 0.0;
+        Wnext = edge_w(p, w, l, r, k0, ew1, ew2, mw, Rl, al, Rr, ar);
+        Rl = al;
+        Rr = ar;
+    } else if (mw && vert > 0) {
+        Rl = rho[l];
+        Rr = rho[r];
+    }
     // eta at the lower interface of the level, carried like W~ (the upper one is the next level's lower)
     double Enext = vert > 0 && k0 < p.nlev ? RE[e + p.ne * k0].y : 0.0;
 #pragma unroll
@@ -521,19 +535,20 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
             for (int j = 0; j < kt; ++j)
                 s += Tw[j] * FQ[I0[j] + p.ne * k].x * (FQ[I1[j] + p.ne * k].y + FQ[I2[j] + p.ne * k].y);
         }
-        const double te = ew1 * th[l + p.nc * k] + ew2 * th[r + p.nc * k];
         double tt = s - invd * (K[r + p.nc * k] - K[l + p.nc * k]);
         if (press) {
+            const double2 Tl = TP[l + p.nc * k], Tr = TP[r + p.nc * k];
+            const double te = ew1 * Tl.x + ew2 * Tr.x;
             double dpi;
             if (p.oro == 2) {
                 // height_difference (CGridOperators.jl:193-200): pi' of both cells at the edge height
                 const int64_t z = 2 * (e + p.ne * k);
                 const int jl = ld(p.zlev + z), jr = ld(p.zlev + z + 1);
                 const double tl = ld(p.zwt + z), tr = ld(p.zwt + z + 1);
-                const double pr = pe[r + p.nc * jr], pl = pe[l + p.nc * jl];
-                dpi = (pr + tr * (pe[r + p.nc * (jr + 1)] - pr) - pl - tl * (pe[l + p.nc * (jl + 1)] - pl)) * invd;
+                const double pr = pip[r + p.nc * jr], pl = pip[l + p.nc * jl];
+                dpi = (pr + tr * (pip[r + p.nc * (jr + 1)] - pr) - pl - tl * (pip[l + p.nc * (jl + 1)] - pl)) * invd;
             } else {
-                dpi = (pe[r + p.nc * k] - pe[l + p.nc * k]) * invd;
+                dpi = (p.oro ? pip[r + p.nc * k] - pip[l + p.nc * k] : Tr.y - Tl.y) * invd;
                 if (p.oro == 1)
                     dpi -= p.ddxn[e + p.ne * k] *
                            (ew1 * p.f[MPAS_DPIDZ][l + p.nc * k] + ew2 * p.f[MPAS_DPIDZ][r + p.nc * k]);
@@ -543,7 +558,13 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         double out;
         if (vert > 0) {
             const double Wl = Wnext;
-            const double Wu = k < p.nlev - 1 ? edge_w(p, w, rho, l, r, k + 1, ew1, ew2, mw) : 0.0;
+            double Wu = 0.0;
+            if (k < p.nlev - 1) {
+                const double al = mw ? rho[l + p.nc * (k + 1)] : 0.0, ar = mw ? rho[r + p.nc * (k + 1)] : 0.0;
+                Wu = edge_w(p, w, l, r, k + 1, ew1, ew2, mw, Rl, al, Rr, ar);
+                Rl = al;
+                Rr = ar;
+            }
             Wnext = Wu;
             const double El = Enext, Eu = RE[e + p.ne * (k + 1)].y;
             Enext = Eu;
@@ -834,6 +855,11 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
     }
     if (!rc) {
         void* q = nullptr;
+        rc = trisk::alloc(o, size_t(2) * size_t(L.nc) * size_t(L.nlev) * sizeof(double), &q);
+        p.tp = static_cast<double*>(q);
+    }
+    if (!rc) {
+        void* q = nullptr;
         rc = trisk::alloc(o, size_t(2) * size_t(L.ne) * size_t(L.nlev + 1) * sizeof(double), &q);
         p.re = static_cast<double*>(q);
     }
@@ -915,6 +941,9 @@ int mpas_ops_field(mpas_ops* h, int32_t which, double** dev, int64_t* points, in
     } else if (which == MPAS_ZETA || which == MPAS_RHOV) {
         pairs = h->p.zr, first = h->p.f[MPAS_ZETA], second = h->p.f[MPAS_RHOV];
         n = n1 = h->L.nv * int64_t(h->L.nlev);
+    } else if (which == MPAS_THETA || which == MPAS_EXNER) {
+        pairs = h->p.tp, first = h->p.f[MPAS_THETA], second = h->p.f[MPAS_EXNER];
+        n = n1 = h->L.nc * int64_t(h->L.nlev);
     } else if (which == MPAS_RHOE || which == MPAS_ETA) {
         pairs = h->p.re, first = h->p.f[MPAS_RHOE], second = h->p.f[MPAS_ETA];
         n1 = h->L.ne * int64_t(h->L.nlev);  // rho_e has no top row
@@ -988,7 +1017,9 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
                 : ne * L * c + nc * L1 * c + nc * L * c + nc * L1 * c + nc * KC * (ix + c) + ne * (2 * ix + c);
     v[K_E] = vi && h->five_band ? nc * (L1 * c * 2 + L * c * 3) : 0.0;
     // D: F, q_e, theta, pi, K, [eta, w, rho, wlow, edge dz, dzh | w, vn, edge dz] -> tvn; tables
-    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
+    // (theta, pi pairs; over orography also pi')
+    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + (p.oro ? nc * L * c : 0.0) + ne * L * c +
+             ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
              (vi ? ne * L1 * c * 3 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
              (p.oro == 1 ? ne * L * c + nc * L * c : p.oro == 2 ? 2 * ne * L * (ix + c) : 0.0);
     if (p.oro) {
@@ -996,7 +1027,7 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
         // interfaces; EX: pi, pi_ref, z_full -> pi', dpi'/dz; WC: tw (read, written), w, w_c, dzh
         v[K_CC] = 2 * ne * L * c + nc * KC * (ix + c) + nc * L * c;
         v[K_CI] = nc * L * c + nc * L1 * c * 2 + nc * 3 * c + 2 * nc * L1 * c;
-        v[K_EX] = 5 * nc * L * c;
+        v[K_EX] = 6 * nc * L * c;  // (theta, pi) pairs, pi_ref, z_full -> pi', dpi'/dz
         v[K_WC] = 5 * nc * L1 * c;
     }
     int k2 = 0;
