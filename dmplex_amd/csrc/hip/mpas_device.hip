@@ -1012,22 +1012,22 @@ int mpas_ops_kernel_bytes(mpas_ops* h, int32_t max, const char** names, double* 
     // (the pair planes: (F, q_e) per layer, (rho_e, eta) per interface)
     v[K_B] = nv * L * c * (p.iap_pv ? 2 : 1) + nc * L * c + ne * L * c + 2 * ne * L * c + 2 * ne * L1 * c +
              (vi ? nc * L1 * c + ne * L1 * c : 0.0) + ne * (4 * ix + 7 * c);
-    // C: (rho_e, eta), vn, K, dzh, wlow -> tw (vector invariant); vn, w, dz -> tw (advective)
-    v[K_C] = vi ? 2 * ne * L1 * c + ne * L * c + nc * L * c + 2 * nc * L1 * c + nc * L1 * c + nc * KC * (ix + c)
+    // C: eta, vn [rho_e], K, dzh, wlow -> tw (vector invariant); vn, w, dz -> tw (advective)
+    v[K_C] = vi ? ne * L1 * c + ne * L * c * (p.vertical == 2 ? 2 : 1) + nc * L * c + 2 * nc * L1 * c + nc * L1 * c +
+                      nc * KC * (ix + c)
                 : ne * L * c + nc * L1 * c + nc * L * c + nc * L1 * c + nc * KC * (ix + c) + ne * (2 * ix + c);
     v[K_E] = vi && h->five_band ? nc * (L1 * c * 2 + L * c * 3) : 0.0;
     // D: F, q_e, theta, pi, K, [eta, w, rho, wlow, edge dz, dzh | w, vn, edge dz] -> tvn; tables
-    // (theta, pi pairs; over orography also pi')
-    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + (p.oro ? nc * L * c : 0.0) + ne * L * c +
-             ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
-             (vi ? ne * L1 * c * 3 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
+    // (the values the kernel needs: theta, pi or pi', K; eta, not the rho_e beside it in its pairs)
+    v[K_D] = ne * L * c * 2 + 3 * nc * L * c + ne * L * c + ne * (KT * ((p.ti2 ? 2 : 3) * ix + c) + 2 * ix + 3 * c) +
+             (vi ? ne * L1 * c * 2 + nc * L1 * c * 2 + nc * L * c + ne * L * c : nc * L1 * c + 2 * ne * L * c) +
              (p.oro == 1 ? ne * L * c + nc * L * c : p.oro == 2 ? 2 * ne * L * (ix + c) : 0.0);
     if (p.oro) {
         // CC: vn, ddxn, the cell rows -> w_c; CI: w_c, wlow, q, w -> w_c and w - w_c at the
         // interfaces; EX: pi, pi_ref, z_full -> pi', dpi'/dz; WC: tw (read, written), w, w_c, dzh
         v[K_CC] = 2 * ne * L * c + nc * KC * (ix + c) + nc * L * c;
         v[K_CI] = nc * L * c + nc * L1 * c * 2 + nc * 3 * c + 2 * nc * L1 * c;
-        v[K_EX] = 6 * nc * L * c;  // (theta, pi) pairs, pi_ref, z_full -> pi', dpi'/dz
+        v[K_EX] = 5 * nc * L * c;  // pi, pi_ref, z_full -> pi', dpi'/dz
         v[K_WC] = 5 * nc * L1 * c;
     }
     int k2 = 0;
