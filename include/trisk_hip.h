@@ -384,7 +384,10 @@ int mpas_ops_download_field(mpas_ops* h, int32_t kind, int64_t levels, const dou
  * fields. */
 int mpas_slow_tendencies(mpas_ops* h, const double* vn, const double* w, const double* rho, const double* rhotheta,
                          double pressure, void* stream);
-/* Device pointer and shape (points, levels) of a work field (MPAS_*). */
+/* Device pointer and shape (points, levels) of a work field (MPAS_*).  The kernels keep the
+ * pairs MPAS_FS/MPAS_QE, MPAS_THETA/MPAS_EXNER, MPAS_ZETA/MPAS_RHOV and MPAS_RHOE/MPAS_ETA
+ * interleaved in one plane each (one 16-byte gather for both); for these fields the call waits
+ * for the device, writes the two fields of the pair from the plane and returns the asked one. */
 int mpas_ops_field(mpas_ops* h, int32_t which, double** dev, int64_t* points, int64_t* levels);
 /* HIP-event timing of the kernels of mpas_slow_tendencies (as trisk_set_timing and
  * trisk_kernel_times), and their algorithmic HBM bytes per launch. */
