@@ -408,27 +408,17 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 //   (1/2 d_e) sum over its two cells c of H_c(e),  H_c(e_k) = sum_k' w_{k,k'} G_k' (q_k + q_k'),
 // and the continuity is dh_i = -(sum_k s_k G_k)/A_i (TRiSKShallowWater.jl:275-305).
 
-// W1: cells (K, B) | vertices (q_v) | edges (G = l_e h_e u_e).  REC: the tables as entity
-// records read with vector loads (fewer instructions; faster for the FP32 tables), else
-// slot-major arrays (faster for the FP64 ones, whose records span several cache lines).
+// W1: vertices (q_v) | edges (G = l_e h_e u_e).  REC: the tables as entity records read with
+// vector loads (fewer instructions; faster for the FP32 tables), else slot-major arrays (faster
+// for the FP64 ones, whose records span several cache lines).  B is formed by the cell kernel,
+// which gathers u at the same edges as G.
 template <class P, class UIn, bool REC>
 __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const UIn* __restrict__ u,
-                                                            const double* __restrict__ h, int64_t t0, int64_t t1,
-                                                            int64_t t2) {
+                                                            const double* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
     using I = typename P::I;
-    Tile t = tile_of3(blockIdx.x, t0, t1, t2);
+    Tile t = tile_of(blockIdx.x, t0, t1);
     if (t.range == 0) {
-        int64_t i = t.first + threadIdx.x;
-        if (i >= p.nc) return;
-        int32_t E[6];
-        C kin[6];
-        load_cell_state<REC>(p, i, E, kin);
-        C ue[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) ue[k] = C(u[E[k]]);
-        p.B[i] = double(pair_kinetic(kin, ue)) + p.g * h[i];
-    } else if (t.range == 1) {
         int64_t v = t.first + threadIdx.x;
         if (v >= p.nv) return;
         int32_t idx[6];
@@ -452,34 +442,38 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
     }
 }
 
-// W2: cells: PV-flux half sums of their edges and the continuity with its RK4 update (XCD x
-// takes the x-th eighth of the cells, so the G and q_v its tiles share stay in its L2).  The
-// FP32-bulk kernel stages its block's cell records through LDS.
-template <class P>
-__global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, int64_t t0) {
+// W2: cells: K_i and B_i = K_i + g h_i at the stage (u, h), the PV-flux half sums of their
+// edges, and the continuity with its RK4 update (XCD x takes the x-th eighth of the cells, so
+// the G, u and q_v its tiles share stay in its L2).  The FP32-bulk kernel stages its block's
+// cell records through LDS.
+template <class P, class UIn>
+__global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, const UIn* __restrict__ u,
+                                                           const double* __restrict__ h, Rk r, int64_t t0) {
     using C = typename P::C;
     const int64_t nc = p.nc;
     const int64_t first = tile_of(blockIdx.x, t0, 0).first;
     const int64_t i = first + threadIdx.x;
-    // the streaming loads of the RK4 update first: they do not wait for the gathers
+    // the streaming loads first: they do not wait for the gathers
     const RkIn rin = i < nc ? rk_load(r, p.ne + i) : RkIn{0.0, 0.0};
     const double area = i < nc ? ld(p.area_c + i) : 1.0;
+    const double hi = i < nc ? ld(h + i) : 0.0;
     int32_t meta, E[6], V[6];
-    C W[15];
+    C W[15], kin[6];
     if constexpr (P::id == TRISK_FP32_BULK) {
-        // the block's cell records (52 B each), staged through LDS with 16-byte coalesced
+        // the block's cell records (76 B each), staged through LDS with 16-byte coalesced
         // loads: a quarter of the load instructions of the slot-major arrays
-        __shared__ __attribute__((aligned(16))) int32_t rec[kBlock * 13];
+        constexpr int RW = kPairRecord;
+        __shared__ __attribute__((aligned(16))) int32_t rec[kBlock * RW];
         if (first >= nc) return;  // the whole block
         const int64_t ncell = nc - first < kBlock ? nc - first : kBlock;
-        const int nw = int(ncell * 13), nw4 = nw / 4;
-        const int32_t* src = p.rc_pair + first * 13;
+        const int nw = int(ncell * RW), nw4 = nw / 4;
+        const int32_t* src = p.rc_pair + first * RW;
         for (int w = threadIdx.x; w < nw4; w += kBlock)
             reinterpret_cast<f32x4*>(rec)[w] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + w);
         for (int w = 4 * nw4 + threadIdx.x; w < nw; w += kBlock) rec[w] = ld(src + w);
         __syncthreads();
         if (i >= nc) return;
-        const int32_t* rr = rec + 13 * threadIdx.x;
+        const int32_t* rr = rec + RW * threadIdx.x;
         meta = rr[0];
         const bool wide = int16_t(rr[1] & 0xffff) == kWide;
         const int32_t be = int32_t(3 * i), bv = int32_t(2 * i);
@@ -491,19 +485,26 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, Rk r, in
         }
         C R[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) __builtin_memcpy(&R[k], &rr[7 + k], 4);
+        for (int k = 0; k < 6; ++k) {
+            __builtin_memcpy(&R[k], &rr[7 + k], 4);
+            __builtin_memcpy(&kin[k], &rr[13 + k], 4);
+        }
         pair_weights(meta, R, W);
     } else {
         if (i >= nc) return;
         load_cell_pairs<false>(p, i, meta, E, V, W);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) kin[k] = k < p.kc ? ld(p.kin + k * nc + i) : C(0);
     }
     const int n = meta & 15;
-    C G[6], Q[6];
+    C G[6], Q[6], ue[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         G[k] = C(p.G[E[k]]);
         Q[k] = C(p.qv[V[k]]);
+        ue[k] = C(u[E[k]]);
     }
+    p.B[i] = double(pair_kinetic(kin, ue)) + p.g * hi;
     C H[6];
     pair_half_sums(n, W, G, Q, H);
     // the half sums go out cell-major, 6 per cell in one contiguous run (slots past n are
@@ -818,18 +819,19 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
     int rc;
     {
         // state_edges false: G is already there, formed by the previous stage's edge kernel
-        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = state_edges ? tiles_per_xcd(L.ne) : 0;
+        int64_t t0 = tiles_per_xcd(L.nv), t1 = state_edges ? tiles_per_xcd(L.ne) : 0;
         if ((rc = timed_begin(H, K_PAIR_STATE, s, &stop))) return rc;
         constexpr bool rec = P::id == TRISK_FP32_BULK;
-        hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0,
-                           s, p, u, h, t0, t1, t2);
+        hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1))), dim3(kBlock), 0, s, p,
+                           u, h, t0, t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
     {
         if ((rc = timed_begin(H, K_PAIR_CELL, s, &stop))) return rc;
         const int64_t t0 = tiles_per_xcd(L.nc);
-        hipLaunchKernelGGL((pair_cell_kernel<P>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
+        hipLaunchKernelGGL((pair_cell_kernel<P, UIn>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, u, h, r,
+                           t0);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -1291,10 +1293,10 @@ namespace trisk {
 template <class P, class UIn>
 static int state_pass(trisk_handle* H, const UIn* u, const double* h, hipStream_t s) {
     const Layout& L = H->L;
-    const int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv), t2 = tiles_per_xcd(L.ne);
+    const int64_t t0 = tiles_per_xcd(L.nv), t1 = tiles_per_xcd(L.ne);
     constexpr bool rec = P::id == TRISK_FP32_BULK;
-    hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1 + t2))), dim3(kBlock), 0, s,
-                       params<P>(H), u, h, t0, t1, t2);
+    hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1))), dim3(kBlock), 0, s,
+                       params<P>(H), u, h, t0, t1);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1593,12 +1595,12 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const double ew = f64 ? 2 * iw : 4.0;                    /* edge cells */
         const double cw = f64 ? 12 * iw : 24.0;                  /* cell edges and corners */
         // rk4_steps forms G in the previous stage's edge kernel: the state pass of stages after
-        // the first covers cells and vertices, and the edge kernel reads l h and the new heights
-        double w1 = nc * (rec8 + 6 * c + 8.0 + 8.0)             /* edges, KE weights, h, B */
-                    + nv * (rec8 + vc + c)                       /* indices, coefficients, q_v */
-                    + ne * uw;                                   /* u, gathered */
+        // the first covers the vertices, and the edge kernel reads l h and the new heights
+        double w1 = nv * (rec8 + vc + c)                         /* indices, coefficients, q_v */
+                    + ne * uw + nc * 8.0;                        /* u, h, gathered */
         double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
-                    + ne * c + nv * c                            /* G, q_v */
+                    + nc * (6 * c + 8.0 + 8.0)                   /* KE weights, h, B */
+                    + ne * c + nv * c + ne * uw                  /* G, q_v, u */
                     + 6 * nc * c;                                /* half sums, 6 per cell */
         double w3 = ne * (2 * c + 1.0 + ew + c + rk_e)          /* half sums, slots, cells, d, RK4 */
                     + nc * 8.0                                   /* B */

@@ -129,7 +129,7 @@ struct Params {
     const int16_t* rv_idx16;    // nv x 8
     const int32_t* re_cells16;  // ne
     const int32_t* pc_ev16;     // 6 x nc
-    const int32_t* rc_pair;     // nc x 13 (FP32-bulk cell kernel records)
+    const int32_t* rc_pair;     // nc x kPairRecord (FP32-bulk cell kernel records)
     // FP32-bulk Gassmann slot form: one 80-byte record per edge, the 12 slot edges as 16-bit
     // deltas (6 words, kWide: the int32 table ge), the two cells (one word, kWide: ec), the
     // FP32 bits of 1/d_e and of the 12 weights

@@ -845,16 +845,18 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
             L.pc_ev16[size_t(k) * nc + j] = int32_t(uint32_t(uint16_t(de)) | (uint32_t(uint16_t(dv)) << 16));
         }
     }
-    // per-cell records of the cell kernel: meta, the packed deltas, the FP32 kite fractions
-    L.rc_pair.assign(size_t(nc) * 13, 0);
+    // per-cell records of the cell kernel: meta, the packed deltas, the FP32 kite fractions,
+    // the FP32 kinetic-energy weights
+    L.rc_pair.assign(size_t(nc) * kPairRecord, 0);
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < nc; ++j) {
-        int32_t* r = L.rc_pair.data() + 13 * j;
+        int32_t* r = L.rc_pair.data() + kPairRecord * j;
         r[0] = L.cmeta[j];
         for (int k = 0; k < 6; ++k) {
             r[1 + k] = L.pc_ev16[size_t(k) * nc + j];
-            const float R = float(L.pr[size_t(k) * nc + j]);
+            const float R = float(L.pr[size_t(k) * nc + j]), W = float(L.rc_kin[6 * j + k]);
             std::memcpy(&r[7 + k], &R, 4);
+            std::memcpy(&r[13 + k], &W, 4);
         }
     }
     L.rv_idx16.assign(size_t(nv) * 8, 0);

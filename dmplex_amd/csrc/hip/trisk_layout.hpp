@@ -79,7 +79,8 @@ struct Layout {
     std::vector<int16_t> rv_idx16;       // nv x 8: edges (3), cells (3), the FP32 bits of f (2)
     std::vector<int32_t> re_cells16;     // ne: left | right << 16
     std::vector<int32_t> pc_ev16;        // 6 x nc: edge | corner << 16
-    std::vector<int32_t> rc_pair;        // nc x 13: meta, 6 x (edge | corner << 16), 6 x FP32 bits of R
+    std::vector<int32_t> rc_pair;        // nc x kPairRecord: meta, 6 x (edge | corner << 16), 6 x FP32
+                                         // bits of R, 6 x FP32 bits of the kinetic-energy weights
     int64_t wide_cells = 0, wide_vertices = 0, wide_edges = 0;
     // transposed coefficients for the adjoint of the cell-pair path
     std::vector<double> adj_ecurl;       // ne x 2: curl coefficient of edge e in vertex ev[s]
@@ -102,6 +103,7 @@ constexpr int16_t kWide = -32768;
 
 constexpr int kPairSlots = 6;   // cells of up to six edges
 constexpr int kPairs = 15;      // pairs k < k' of six slots, in the order of pair_index()
+constexpr int kPairRecord = 19; // words of a cell record of the FP32-bulk cell kernel
 inline int pair_index(int k, int k2) {  // k < k2 < 6
     return k * (11 - k) / 2 + (k2 - k - 1);
 }
