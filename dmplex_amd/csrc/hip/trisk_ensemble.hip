@@ -60,11 +60,12 @@ struct EnsLevel {
 };
 
 // A run of tiles of one range (cells, vertices or edges of one level) on one XCD.  The state
-// kernel takes a level's vertex run backward, so that it starts where the cell run ended and
-// finds that region's gathered member groups still in the XCD's L2.
+// kernel takes a level's vertex run backward, so that it starts where the previous launch (the
+// edge kernel, forward through the same XCD's share) ended, and, in the first stage of a call,
+// the edge run that follows starts where the vertex run ended.
 constexpr int32_t kEnsBackward = 8;
 struct EnsSeg {
-    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges, 3 all three interleaved; | kEnsBackward
+    int32_t level, kind;  // kind: 0 cells, 1 vertices, 2 edges, 3 vertices and edges interleaved; | kEnsBackward
                           // for a run taken last tile first
     int32_t tile0;        // first tile of the run within its range
     int32_t ntiles;
@@ -100,8 +101,8 @@ __device__ __forceinline__ Rk level_rk(const EnsLevel<P>& L, const EnsArgs& a) {
 // body(level, kind, item) for every item of the tiles of this block's XCD (blockIdx mod 8:
 // the hardware deals workgroups to the XCDs round-robin); the XCD's blocks stride over its
 // tile list, so a grid smaller than the list is a persistent launch.  A run of kind 3 holds
-// the cells, vertices and edges of one level interleaved along the Hilbert curve: tile
-// 6k is cell tile k, tiles 6k+1, 6k+2 vertex tiles 2k, 2k+1 and tiles 6k+3..6k+5 edge
+// the vertices and edges of one level interleaved along the Hilbert curve, in groups of five
+// tiles per cell tile k: tiles 5k, 5k+1 are vertex tiles 2k, 2k+1 and tiles 5k+2..5k+4 edge
 // tiles 3k..3k+2 (a level has fewer than 2 vertices and 3 edges per cell), so the blocks
 // running together touch one stretch of the mesh; count(level, kind) gives the items of
 // the level's cells (0), vertices (1) or edges (2).
@@ -147,9 +148,9 @@ __device__ __forceinline__ void for_each_item(const EnsArgs& a, N&& count, F&& b
         int kind = g.kind & ~kEnsBackward;
         int64_t nitems = g.nitems;
         if (kind == 3) {
-            const int64_t sup = tile / 6, r = tile - 6 * sup;
-            kind = r == 0 ? 0 : r <= 2 ? 1 : 2;
-            tile = sup * (kind + 1) + (r - (kind == 0 ? 0 : kind == 1 ? 1 : 3));
+            const int64_t sup = tile / 5, r = tile - 5 * sup;
+            kind = r <= 1 ? 1 : 2;
+            tile = kind == 1 ? 2 * sup + r : 3 * sup + (r - 2);
             nitems = count(g.level, kind);
         }
         const int64_t j = tile * BLK + threadIdx.x;
@@ -245,7 +246,9 @@ __device__ __forceinline__ Item item_of(const EnsLevel<P>& L, uint32_t j) {
     return {ent, o - ent * uint32_t(L.M), o};
 }
 
-// W1 of all levels: cells (B), vertices (q_v), edges (G); cf. pair_state_kernel
+// W1 of all levels: vertices (q_v), edges (G, in the first stage of a call); cf.
+// pair_state_kernel.  B is formed by the cell kernel, which gathers u at the edges where it
+// gathers G.
 template <class P, class UIn, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
     using C = typename P::C;
@@ -262,24 +265,7 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
         const Item it = item_of<V>(L, j);
         const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
         const double* h = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
-        if (kind == 0) {
-            int32_t E[6];
-            C kin[6];
-            load_cell_state<NARROW>(p, it.ent, E, kin);
-            Lanes<UIn, V> uk[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) uk[k] = vload<UIn, V>(u + uint32_t(E[k]) * M + it.m0);
-            const Lanes<double, V> hh = vload<double, V>(h + it.o);
-            Lanes<double, V> B;
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                C ue[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) ue[k] = C(uk[k].v[v]);
-                B.v[v] = double(pair_kinetic(kin, ue)) + p.g * hh.v[v];
-            }
-            vstore(p.B + it.o, B);
-        } else if (kind == 1) {
+        if (kind == 1) {
             int32_t idx[6];
             C cf[8];
             load_vertex_record<NARROW>(p, it.ent, idx, cf);
@@ -318,9 +304,10 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
     });
 }
 
-// W2 of all levels: PV-flux half sums and continuity with its RK4 update; cf. pair_cell_kernel.
-// Half sums are stored cell-major: member m of cell i's slot k at T[(6 i + k) M + m].
-template <class P, int BLK, int V>
+// W2 of all levels: K and B = K + g h at the stage (u, h), the PV-flux half sums and the
+// continuity with its RK4 update; cf. pair_cell_kernel.  Half sums are stored cell-major:
+// member m of cell i's slot k at T[(6 i + k) M + m].
+template <class P, class UIn, int BLK, int V>
 __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
     using C = typename P::C;
     using I = typename P::I;
@@ -333,26 +320,41 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
         const Item it = item_of<V>(L, j);
         const Rk r = level_rk(L, a);
         const RkInLanes<V> rin = rk_load_lanes<V>(r, p.ne * M + it.o);
+        // the stage heights: those of the state itself when the input is y (already loaded by
+        // rk_load_lanes in RK4 modes; a tendency call, mode 0, loads them here)
+        const double* hs = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
+        const Lanes<double, V> hh = a.from_y && r.mode >= 1 ? rin.y : vload<double, V>(hs + it.o);
+        const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
         int32_t meta, E[6], V6[6];
-        C W[15];
+        C W[15], kin[6];
         load_cell_pairs<NARROW>(p, it.ent, meta, E, V6, W);
+        if constexpr (NARROW) {
+            ldrec(p.rc_kin + 6 * int64_t(it.ent), kin);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) kin[k] = k < p.kc ? ld(p.kin + k * p.nc + it.ent) : C(0);
+        }
         const int n = meta & 15;
         Lanes<I, V> Gk[6], Qk[6], Hk[6];
+        Lanes<UIn, V> uk[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             Gk[k] = vload<I, V>(p.G + uint32_t(E[k]) * M + it.m0);
             Qk[k] = vload<I, V>(p.qv + uint32_t(V6[k]) * M + it.m0);
+            uk[k] = vload<UIn, V>(u + uint32_t(E[k]) * M + it.m0);
         }
         const double area = ld(p.area_c + it.ent);
-        Lanes<double, V> dh;
+        Lanes<double, V> dh, B;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            C G[6], Q[6], H[6];
+            C G[6], Q[6], H[6], ue[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
                 G[k] = C(Gk[k].v[v]);
                 Q[k] = C(Qk[k].v[v]);
+                ue[k] = C(uk[k].v[v]);
             }
+            B.v[v] = double(pair_kinetic(kin, ue)) + p.g * hh.v[v];
             pair_half_sums(n, W, G, Q, H);
             double div = 0.0;
 #pragma unroll
@@ -362,6 +364,7 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
             }
             dh.v[v] = -div / area;
         }
+        vstore(p.B + it.o, B);
 #pragma unroll
         for (int k = 0; k < 6; ++k) vstore(p.T + (6 * it.ent + uint32_t(k)) * M + it.m0, Hk[k]);
         rk_store_lanes<double, V>(r, p.ne * M + it.o, dh, rin, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
@@ -510,7 +513,6 @@ static int build_list(trisk_ensemble* E, int list) {
         const Layout& L = E->models[l]->L;
         const int64_t M = E->M[l] / V;  // items per entity
         if (kind == TRISK_ENS_STATE) {
-            ranges.push_back({l, 0, L.nc * M, 0});
             ranges.push_back({l, 1, L.nv * M, 0});
             if (list != kStateNoEdges) ranges.push_back({l, 2, L.ne * M, 0});
         } else if (kind == TRISK_ENS_CELL) {
@@ -531,17 +533,17 @@ static int build_list(trisk_ensemble* E, int list) {
     };
     // (the interleaved groups hold edge tiles: the list without edges takes the chunked split)
     if (sc.split == TRISK_SPLIT_INTERLEAVED && list == TRISK_ENS_STATE) {
-        // XCD x: the x-th contiguous chunk of every level's interleaved cells, vertices and
-        // edges, in whole groups of 6 tiles (one cell tile with its vertex and edge tiles)
+        // XCD x: the x-th contiguous chunk of every level's interleaved vertices and edges, in
+        // whole groups of 5 tiles (the vertex and edge tiles of one stretch of cell tile size)
         ranges.clear();
         for (int l = 0; l < E->nlevels; ++l) {
             const int64_t items = E->models[l]->L.nc * (E->M[l] / V);
-            ranges.push_back({l, 3, items, 6 * ((items + blk - 1) / blk)});
+            ranges.push_back({l, 3, items, 5 * ((items + blk - 1) / blk)});
         }
         for (int x = 0; x < kXcd; ++x)
             for (const auto& r : ranges) {
-                const int64_t groups = r.tiles / 6, c = (groups + kXcd - 1) / kXcd;
-                add(x, r, 6 * std::min(groups, x * c), 6 * std::min(groups, (x + 1) * c));
+                const int64_t groups = r.tiles / 5, c = (groups + kXcd - 1) / kXcd;
+                add(x, r, 5 * std::min(groups, x * c), 5 * std::min(groups, (x + 1) * c));
             }
     } else if (sc.split != TRISK_SPLIT_BLOCKED) {
         // XCD x: the x-th contiguous chunk of every range (a level's vertices backward)
@@ -712,7 +714,7 @@ static void launch_kind(int kind, bool gnext, dim3 grid, const EnsArgs& a, hipSt
     if (kind == TRISK_ENS_STATE)
         hipLaunchKernelGGL((ens_state_kernel<P, UIn, BLK, V>), grid, dim3(BLK), 0, s, a);
     else if (kind == TRISK_ENS_CELL)
-        hipLaunchKernelGGL((ens_cell_kernel<P, BLK, V>), grid, dim3(BLK), 0, s, a);
+        hipLaunchKernelGGL((ens_cell_kernel<P, UIn, BLK, V>), grid, dim3(BLK), 0, s, a);
     else if (gnext)
         hipLaunchKernelGGL((ens_edge_kernel<P, BLK, V, true>), grid, dim3(BLK), 0, s, a);
     else
@@ -1072,8 +1074,10 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
         const double nc = double(L.nc), ne = double(L.ne), nv = double(L.nv), M = e->M[l];
         // RK4 stages after the first: G comes from the previous stage's edge kernel, which
         // reads l h and the new heights for it
-        b[0] += nc * (rec8 + 6 * c) + nv * (rec8 + vc) + M * (nc * (8.0 + 8.0) + nv * c + ne * uw);
-        b[1] += nc * (cw + 6 * c + iw + 8.0) + M * (nc * rk_c + ne * c + nv * c + 6 * nc * c);
+        b[0] += nv * (rec8 + vc) + M * (nv * c + ne * uw + nc * 8.0);
+        // the cell kernel forms B: KE weights, u gathered, the stage heights, B written
+        b[1] += nc * (cw + 6 * c + iw + 8.0 + 6 * c) + M * (nc * rk_c + ne * c + nv * c + 6 * nc * c + ne * uw +
+                                                          nc * (8.0 + 8.0));
         b[2] += ne * (ew + c + 1.0 + 2 * c) + M * (ne * (2 * c + rk_e + c) + nc * (8.0 + 8.0));
     }
     int k = 0;

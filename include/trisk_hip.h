@@ -443,8 +443,8 @@ typedef struct trisk_ensemble trisk_ensemble;
 /* How one kernel's tiles (block_size items each) are laid on the device. */
 enum { TRISK_SPLIT_CHUNKED = 0,     /* XCD x takes the x-th eighth of every level's range */
        TRISK_SPLIT_BLOCKED = 1,     /* XCD x takes the x-th eighth of all ranges laid end to end */
-       TRISK_SPLIT_INTERLEAVED = 2  /* as CHUNKED; the state kernel takes each stretch of cells
-                                       together with its vertices and edges */ };
+       TRISK_SPLIT_INTERLEAVED = 2  /* as CHUNKED; the state kernel's first-stage launch takes the
+                                       vertices and edges of each stretch of the curve together */ };
 enum { TRISK_ENS_STATE = 0, TRISK_ENS_CELL = 1, TRISK_ENS_EDGE = 2 };  /* the three kernels */
 
 typedef struct {
