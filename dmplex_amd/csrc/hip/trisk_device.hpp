@@ -118,7 +118,7 @@ struct Params {
     const int32_t* rc_state;  // nc x 6
     const C* rc_kin;          // nc x 6
     const int32_t* rv_idx;    // nv x 6
-    const C* rv_coef;         // nv x 6 (FP32-bulk)
+    const C* rv_coef;         // nv x 4, then nv x 2 (FP32-bulk)
     const int32_t* re_cells;  // ne x 2
     const C* re_lh;           // ne x 2
     const int32_t* ce6;       // 6 x nc cell edges (pads: the first edge)
@@ -423,7 +423,7 @@ __device__ __forceinline__ void load_cell_state(const Params<P>& p, int64_t i, i
 template <bool NARROW, class P>
 __device__ __forceinline__ void load_vertex_record(const Params<P>& p, int64_t v, int32_t (&idx)[6],
                                                    typename P::C (&cf)[8]) {
-    if constexpr (NARROW) {  // FP32 records: indices and f in 16 bytes, curl and vertex heights in 24
+    if constexpr (NARROW) {  // FP32 records: indices and f in 16 bytes, curl and vertex heights in 16 + 8
         int16_t d[8];
         ldrec(p.rv_idx16 + 8 * v, d);
         if (d[0] == kWide) {
@@ -436,10 +436,16 @@ __device__ __forceinline__ void load_vertex_record(const Params<P>& p, int64_t v
                 idx[3 + k] = b1 + d[3 + k];
             }
         }
-        typename P::C c6[6];
-        ldrec(p.rv_coef + 6 * v, c6);
+        // the coefficients as a 16-byte record (curl 3, the first vertex height) and an 8-byte
+        // one (the other two) in the array after it: every load of a wave is contiguous (one
+        // 24-byte record read as three 8-byte loads touched each cache line three times)
+        typename P::C c4[4], c2[2];
+        ldrec(p.rv_coef + 4 * v, c4);
+        ldrec(p.rv_coef + 4 * p.nv + 2 * v, c2);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) cf[k] = c6[k];
+        for (int k = 0; k < 4; ++k) cf[k] = c4[k];
+        cf[4] = c2[0];
+        cf[5] = c2[1];
         static_assert(sizeof(typename P::C) == 4, "the records hold FP32 coefficients");
         __builtin_memcpy(&cf[6], &d[6], 4);
         cf[7] = 0;

@@ -808,8 +808,9 @@ void build_pairs(const trisk_model_desc* d, Layout& L, int32_t narrow_limit) {
         for (int k = 0; k < 3; ++k) {
             L.rv_idx[6 * v + k] = L.ve[size_t(k) * nv + v];
             L.rv_idx[6 * v + 3 + k] = L.vc[size_t(k) * nv + v];
-            L.rv_coef[6 * v + k] = L.curl[size_t(k) * nv + v];
-            L.rv_coef[6 * v + 3 + k] = L.vh[size_t(k) * nv + v];
+            L.rv_coef[4 * v + k] = L.curl[size_t(k) * nv + v];
+            // vertex heights: the first in the 4-wide record, the others in the 2-wide one
+            L.rv_coef[k == 0 ? 4 * v + 3 : 4 * nv + 2 * v + (k - 1)] = L.vh[size_t(k) * nv + v];
         }
     }
     L.re_cells.assign(size_t(ne) * 2, 0);

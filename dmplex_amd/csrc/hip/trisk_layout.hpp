@@ -67,7 +67,7 @@ struct Layout {
     std::vector<int32_t> rc_state;       // nc x 6: cell edges
     std::vector<double> rc_kin;          // nc x 6: kinetic-energy weights
     std::vector<int32_t> rv_idx;         // nv x 6: vertex edges, vertex cells
-    std::vector<double> rv_coef;         // nv x 6: curl (3), vertex heights (3)
+    std::vector<double> rv_coef;         // nv x 4: curl (3), vertex height 0; then nv x 2: vertex heights 1, 2
     std::vector<int32_t> re_cells;       // ne x 2: left, right cell
     std::vector<double> re_lh;           // ne x 2: l_e * edge_height
     std::vector<int32_t> rc_pce;         // 6 x nc: cell edges, pads the first edge
