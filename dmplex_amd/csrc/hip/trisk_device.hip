@@ -436,8 +436,13 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         if (e >= p.ne) return;
         int32_t c[2];
         C lh[2];
-        load_edge_cells<REC>(p, e, c);
-        ldrec(p.re_lh + 2 * e, lh);
+        if constexpr (REC) {
+            float d;
+            load_edge_record(p, e, c, d, lh);
+        } else {
+            load_edge_cells<REC>(p, e, c);
+            ldrec(p.re_lh + 2 * e, lh);
+        }
         p.G[e] = I(pair_mass_flux(lh, C(h[c[0]]), C(h[c[1]]), C(u[e])));
     }
 }
@@ -538,11 +543,16 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
     // the edge's own streamed values first, so that they do not wait for the cell record
     const RkIn rin = rk_load(r, e);
     const uint32_t sl = ld(p.eslot + e);
-    const C d = C(ld(p.dual + e));
+    C d;
     C lh[2] = {C(0), C(0)};
-    if constexpr (GNEXT) ldrec(p.re_lh + 2 * e, lh);
     int32_t c[2];
-    load_edge_cells<P::id == TRISK_FP32_BULK>(p, e, c);
+    if constexpr (P::id == TRISK_FP32_BULK) {
+        load_edge_record(p, e, c, d, lh);  // one 16-byte load: cells, d, l h
+    } else {
+        d = ld(p.dual + e);
+        if constexpr (GNEXT) ldrec(p.re_lh + 2 * e, lh);
+        load_edge_cells<false>(p, e, c);
+    }
     // then every gather from the two cells: the half sums (their sum is the same in either
     // order), B and, with GNEXT, the heights of the state this stage writes (the stage
     // heights, or y after the last stage)
@@ -763,6 +773,15 @@ static int setup_params(trisk_handle* H) {
             rc |= upload(o, L.rc_state16, &p.rc_state16);
             rc |= upload(o, L.rv_idx16, &p.rv_idx16);
             rc |= upload(o, L.re_cells16, &p.re_cells16);
+            {
+                std::vector<int32_t> rec(size_t(L.ne) * 4);
+                for (int64_t e = 0; e < L.ne; ++e) {
+                    const float w[3] = {float(L.dual[e]), float(L.re_lh[2 * e]), float(L.re_lh[2 * e + 1])};
+                    rec[4 * e] = L.re_cells16[e];
+                    std::memcpy(&rec[4 * e + 1], w, sizeof(w));
+                }
+                rc |= upload(o, rec, &p.re_rec);
+            }
             rc |= upload(o, L.pc_ev16, &p.pc_ev16);
             rc |= upload(o, L.rc_pair, &p.rc_pair);
         }

@@ -128,6 +128,7 @@ struct Params {
     const int16_t* rc_state16;  // nc x 8
     const int16_t* rv_idx16;    // nv x 8
     const int32_t* re_cells16;  // ne
+    const int32_t* re_rec;      // ne x 4 (FP32-bulk): the cells (as re_cells16), the FP32 bits of d_e, l h_0, l h_1
     const int32_t* pc_ev16;     // 6 x nc
     const int32_t* rc_pair;     // nc x kPairRecord (FP32-bulk cell kernel records)
     // FP32-bulk Gassmann slot form: one 80-byte record per edge, the 12 slot edges as 16-bit
@@ -389,6 +390,26 @@ __device__ __forceinline__ void edge_cells_narrow(const Params<P>& p, int64_t e,
         c[0] = b + d0;
         c[1] = b + d1;
     }
+}
+
+// The FP32-bulk record of edge e (one 16-byte load): its two cells (16-bit deltas from e/3, or
+// the int32 pair for an edge flagged kWide), d_e and the two weights l_e h_e,s of the mass flux.
+template <class P>
+__device__ __forceinline__ void load_edge_record(const Params<P>& p, int64_t e, int32_t (&c)[2], float& d,
+                                                 float (&lh)[2]) {
+    int32_t w[4];
+    ldrec(p.re_rec + 4 * e, w);
+    const int16_t d0 = int16_t(w[0] & 0xffff), d1 = int16_t(uint32_t(w[0]) >> 16);
+    if (d0 == kWide) {
+        ldrec(p.re_cells + 2 * e, c);
+    } else {
+        const int32_t b = int32_t(e / 3);
+        c[0] = b + d0;
+        c[1] = b + d1;
+    }
+    __builtin_memcpy(&d, &w[1], 4);
+    __builtin_memcpy(&lh[0], &w[2], 4);
+    __builtin_memcpy(&lh[1], &w[3], 4);
 }
 
 // Table records of the cell-pair path.  NARROW: the FP32-bulk layout (16-bit index

@@ -291,8 +291,13 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
         } else {
             int32_t c[2];
             C lh[2];
-            load_edge_cells<NARROW>(p, it.ent, c);
-            ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
+            if constexpr (NARROW) {
+                float d;
+                load_edge_record(p, it.ent, c, d, lh);
+            } else {
+                load_edge_cells<NARROW>(p, it.ent, c);
+                ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
+            }
             const Lanes<double, V> h0 = vload<double, V>(h + uint32_t(c[0]) * M + it.m0);
             const Lanes<double, V> h1 = vload<double, V>(h + uint32_t(c[1]) * M + it.m0);
             const Lanes<UIn, V> ue = vload<UIn, V>(u + it.o);
@@ -389,11 +394,16 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
         const RkInLanes<V> rin = rk_load_lanes<V>(r, it.o);
         // the edge's own streamed values first, so that they do not wait for the cell record
         const uint32_t sl = ld(p.eslot + it.ent);
-        const C d = C(ld(p.dual + it.ent));
+        C d;
         C lh[2] = {C(0), C(0)};
-        if constexpr (GNEXT) ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
         int32_t c[2];
-        load_edge_cells<NARROW>(p, it.ent, c);
+        if constexpr (NARROW) {
+            load_edge_record(p, it.ent, c, d, lh);  // one 16-byte load: cells, d, l h
+        } else {
+            d = ld(p.dual + it.ent);
+            if constexpr (GNEXT) ldrec(p.re_lh + 2 * int64_t(it.ent), lh);
+            load_edge_cells<NARROW>(p, it.ent, c);
+        }
         const Lanes<I, V> t0 = vload<I, V>(p.T + (6 * uint32_t(c[0]) + (sl & 15)) * M + it.m0);
         const Lanes<I, V> t1 = vload<I, V>(p.T + (6 * uint32_t(c[1]) + (sl >> 4)) * M + it.m0);
         const Lanes<double, V> bl = vload<double, V>(p.B + uint32_t(c[0]) * M + it.m0);
