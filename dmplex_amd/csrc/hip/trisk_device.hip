@@ -189,11 +189,14 @@ __global__ __launch_bounds__(kBlock) void cell_vertex_kernel(Params<P> p, const 
 // P2: edges (F, q_e) | cells (vertex part of K, B; only with vertex kinetic energy)
 
 // REC (FP32-bulk, hexagon widths): the rows from the records r2e / r2c staged through LDS.
+// BH (vertex kinetic energy, every cell kinetic weight 0): P1 skipped the cells, and the cell
+// part forms B = g h + the vertex part itself; P1 would have stored 0 + g h, which is g h, so B
+// is bit for bit the same.
 // With the records P1 stores A_v zeta_v and A_v h_v (its record coefficients carry A_v), so
 // the rhombus PV (r_0 zeta_0 + r_1 zeta_1 + f_e)/(r_0 h_0 + r_1 h_1), r_s = A_s/(A_0 + A_1),
 // is (A_0 zeta_0 + A_1 zeta_1 + (A_0 + A_1) f_e)/(A_0 h_0 + A_1 h_1): one coefficient per
 // edge, and the vertex part of the mass flux takes vkh_s / A_s.
-template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths, bool REC = false>
+template <class P, class UIn, class HIn, bool VK, bool GM, class W = AnyWidths, bool REC = false, bool BH = false>
 __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __restrict__ u,
                                                       const HIn* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -276,7 +279,10 @@ __global__ __launch_bounds__(kBlock) void edge_kernel(Params<P> p, const UIn* __
             else
                 s += ld(p.vkk + k * p.nc + i) * C(p.Kv[CV[k]]);
         }
-        p.B[i] = p.B[i] + double(s);
+        if constexpr (BH)
+            p.B[i] = p.g * double(h[i]) + double(s);
+        else
+            p.B[i] = p.B[i] + double(s);
         if (p.K_out) {
             double K = p.K[i] + double(s);
             p.K[i] = K;
@@ -909,9 +915,15 @@ static void launch_p1(const Params<P>& p, dim3 grid, hipStream_t s, const UIn* u
 
 template <class P, class UIn, class W, bool REC>
 static void launch_p2(const Params<P>& p, dim3 grid, hipStream_t s, const UIn* u, const double* h, int64_t t0,
-                      int64_t t1, bool vk, bool gm) {
-    if (vk && gm)
+                      int64_t t1, bool vk, bool gm, bool bh = false) {
+    if (vk && gm && bh)
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true, W, REC, true>), grid, dim3(kBlock), 0, s, p, u, h,
+                           t0, t1);
+    else if (vk && gm)
         hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, true, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+    else if (vk && bh)
+        hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false, W, REC, true>), grid, dim3(kBlock), 0, s, p, u, h,
+                           t0, t1);
     else if (vk)
         hipLaunchKernelGGL((edge_kernel<P, UIn, double, true, false, W, REC>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
     else if (gm)
@@ -932,9 +944,12 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
     // records (FP32-bulk, hexagon widths) when the handle holds them
     constexpr bool can_rec = P::id == TRISK_FP32_BULK && W::cw == 6 && W::vw == 3;
     const bool rec = can_rec && p.r1c != nullptr;
+    // vertex kinetic energy with every cell weight 0: P1 covers the vertices only and P2 forms B
+    // (not when the diagnostics ask for K)
+    const bool bh = vk && L.cell_ke_zero && p.K_out == nullptr;
     // P1
     {
-        int64_t t0 = tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
+        int64_t t0 = bh ? 0 : tiles_per_xcd(L.nc), t1 = tiles_per_xcd(L.nv);
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_CELL_VERTEX, s, &stop))) return rc;
         if (rec)
@@ -950,9 +965,9 @@ static int run_widths(trisk_handle* H, const UIn* u, const double* h, const Rk& 
         dim3 grid(unsigned(kXcd * (t0 + t1)));
         if ((rc = timed_begin(H, K_EDGE, s, &stop))) return rc;
         if (rec)
-            launch_p2<P, UIn, W, can_rec>(p, grid, s, u, h, t0, t1, vk, gm);
+            launch_p2<P, UIn, W, can_rec>(p, grid, s, u, h, t0, t1, vk, gm, bh);
         else
-            launch_p2<P, UIn, W, false>(p, grid, s, u, h, t0, t1, vk, gm);
+            launch_p2<P, UIn, W, false>(p, grid, s, u, h, t0, t1, vk, gm, bh);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
@@ -1640,7 +1655,10 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const bool rec = !f64 && h->p32.r1c != nullptr;
         // q_v is stored only for the TRiSK flux (the Gassmann edge PV comes from zeta_v, h_v);
         // the Gassmann records then leave out f_v too
-        double p1 = nc * (KC * (ix + c) + 8.0 + 8.0) + ne * uw +
+        // with every cell kinetic weight 0 (vertex kinetic energy) P1 skips the cells and P2
+        // reads h instead of B
+        const bool bh = vk && L.cell_ke_zero;
+        double p1 = (bh ? 0.0 : nc * (KC * (ix + c) + 8.0 + 8.0)) + ne * uw +
                     nv * (KV * 2 * (ix + c) + (gm && rec ? 0.0 : c) + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) +
                           (vk ? KV * c + c : 0.0));
         // with vertex kinetic energy P1 writes B and P2 adds the vertex part (K only for the

@@ -236,6 +236,7 @@ bool build_layout(const trisk_model_desc* d, Layout& L, std::string& err) {
     remap_index(d->cell_edges, KC, nc, L.cperm, L.einv, L.ce);
     remap_value(d->divergence, KC, nc, L.cperm, L.div);
     remap_value(d->kinetic, KC, nc, L.cperm, L.kin);
+    L.cell_ke_zero = std::all_of(L.kin.begin(), L.kin.end(), [](double w) { return w == 0.0; });
     remap_value(d->cell_areas, 1, nc, L.cperm, L.area_c);
     remap_index(d->edge_cells, 2, ne, L.eperm, L.cinv, L.ec);
     remap_index(d->edge_vertices, 2, ne, L.eperm, L.vinv, L.ev);

@@ -23,6 +23,9 @@ struct Layout {
     int64_t nc = 0, ne = 0, nv = 0;
     int kc = 0, kv = 0, kt = 0, kg = 0;
     bool vertex_ke = false, gassmann = false, gassmann_mean = false;
+    // every cell kinetic-energy weight is 0 (vertex kinetic energy with alpha = 0): the cell part
+    // of K is 0, so the tendency need not gather u for it
+    bool cell_ke_zero = false;
     // Gassmann with 12 slots in the order of gassmann_coefficients (per vertex of e: inner f1,
     // the two outer edges of f1, inner f2, the two outer edges of f2): slot k takes the PV of
     // its slot gassmann_pv_slot(k), so the PV-edge table is not needed on the device
