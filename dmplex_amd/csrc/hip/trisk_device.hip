@@ -1655,10 +1655,10 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const bool rec = !f64 && h->p32.r1c != nullptr;
         // q_v is stored only for the TRiSK flux (the Gassmann edge PV comes from zeta_v, h_v);
         // the Gassmann records then leave out f_v too
-        // with every cell kinetic weight 0 (vertex kinetic energy) P1 skips the cells and P2
-        // reads h instead of B
+        // with every cell kinetic weight 0 (vertex kinetic energy) P1 skips the cells, whose h
+        // its vertices still gather, and P2 reads h instead of B
         const bool bh = vk && L.cell_ke_zero;
-        double p1 = (bh ? 0.0 : nc * (KC * (ix + c) + 8.0 + 8.0)) + ne * uw +
+        double p1 = (bh ? nc * 8.0 : nc * (KC * (ix + c) + 8.0 + 8.0)) + ne * uw +
                     nv * (KV * 2 * (ix + c) + (gm && rec ? 0.0 : c) + (gm ? 0.0 : c) + (gm || vk ? 2 * c : 0.0) +
                           (vk ? KV * c + c : 0.0));
         // with vertex kinetic energy P1 writes B and P2 adds the vertex part (K only for the
