@@ -315,13 +315,14 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
             // of 80 bytes the 16-byte LDS reads of 16 lanes fall on distinct banks
             __shared__ __attribute__((aligned(16))) int32_t srec[kBlock * 20];
             if (t.first >= p.ne) return;  // the whole block
+            // the RK4 loads first: they do not wait for the records
+            const RkIn rin = e < p.ne ? rk_load(r, e) : RkIn{0.0, 0.0};
             const int64_t n = p.ne - t.first < kBlock ? p.ne - t.first : kBlock;
             const f32x4* src = reinterpret_cast<const f32x4*>(p.rg + 20 * t.first);
             for (int w = threadIdx.x; w < int(5 * n); w += kBlock)
                 reinterpret_cast<f32x4*>(srec)[w] = __builtin_nontemporal_load(src + w);
             __syncthreads();
             if (e >= p.ne) return;
-            const RkIn rin = rk_load(r, e);
             double du;
             int32_t rec[20];
             __builtin_memcpy(rec, &srec[20 * threadIdx.x], 80);
