@@ -494,8 +494,10 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
         Rl = rho[l];
         Rr = rho[r];
     }
-    // eta at the lower interface of the level, carried like W~ (the upper one is the next level's lower)
+    // eta at the lower interface of the level, carried like W~ (the upper one is the next level's lower),
+    // and so is the layer thickness factor edzh of the interface
     double Enext = vert > 0 && k0 < p.nlev ? RE[e + p.ne * k0].y : 0.0;
+    double Dnext = vert > 0 && k0 < p.nlev ? ld(p.edzh + e + p.ne * k0) : 0.0;
 #pragma unroll
     for (int q = 0; q < kLevelsVn; ++q) {
         const int k = k0 + q;
@@ -568,8 +570,9 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
             Wnext = Wu;
             const double El = Enext, Eu = RE[e + p.ne * (k + 1)].y;
             Enext = Eu;
-            out = tt - (ld(p.edzh + e + p.ne * k) * El * Wl + ld(p.edzh + e + p.ne * (k + 1)) * Eu * Wu) /
-                           (2 * ld(p.edz + e + p.ne * k));
+            const double Dl = Dnext, Du = ld(p.edzh + e + p.ne * (k + 1));
+            Dnext = Du;
+            out = tt - (Dl * El * Wl + Du * Eu * Wu) / (2 * ld(p.edz + e + p.ne * k));
         } else {
             const double we = (w[l + p.nc * k] + w[r + p.nc * k] + w[l + p.nc * (k + 1)] + w[r + p.nc * (k + 1)]) / 4;
             const double vlow = k > 0 ? (vn[e + p.ne * (k - 1)] + vn[e + p.ne * k]) / 2 : vn[e + p.ne * k];
