@@ -69,6 +69,7 @@ struct Params {
     const double* fe;
     const double* edz;
     const double* edzh;
+    const double* edzp;  // edges x nlev: (edz of layer k, edzh of interface k + 1), the vn tendency's
     // vertices
     const int32_t* ve;
     const int32_t* vc;
@@ -570,9 +571,15 @@ __global__ __launch_bounds__(kBlock) void mpas_vn_tendency_kernel(Params p, cons
             Wnext = Wu;
             const double El = Enext, Eu = RE[e + p.ne * (k + 1)].y;
             Enext = Eu;
-            const double Dl = Dnext, Du = ld(p.edzh + e + p.ne * (k + 1));
+            // (edz of the layer, edzh of its upper interface) in one 16-byte load
+            double dz2[2];
+            {
+                const trisk::f32x4 v = __builtin_nontemporal_load((trisk::gptr<trisk::f32x4>)(p.edzp + 2 * (e + p.ne * k)));
+                __builtin_memcpy(dz2, &v, 16);
+            }
+            const double Dl = Dnext, Du = dz2[1];
             Dnext = Du;
-            out = tt - (Dl * El * Wl + Du * Eu * Wu) / (2 * ld(p.edz + e + p.ne * k));
+            out = tt - (Dl * El * Wl + Du * Eu * Wu) / (2 * dz2[0]);
         } else {
             const double we = (w[l + p.nc * k] + w[r + p.nc * k] + w[l + p.nc * (k + 1)] + w[r + p.nc * (k + 1)]) / 4;
             const double vlow = k > 0 ? (vn[e + p.ne * (k - 1)] + vn[e + p.ne * k]) / 2 : vn[e + p.ne * k];
@@ -810,6 +817,15 @@ int mpas_ops_create(const mpas_ops_desc* desc, int32_t device, mpas_ops** out) {
     rc |= trisk::upload(o, L.fe, &p.fe);
     rc |= trisk::upload(o, L.edz, &p.edz);
     rc |= trisk::upload(o, L.edzh, &p.edzh);
+    {
+        std::vector<double> pz(size_t(2) * L.ne * L.nlev);
+        for (int k = 0; k < L.nlev; ++k)
+            for (int64_t e = 0; e < L.ne; ++e) {
+                pz[2 * (e + L.ne * k)] = L.edz[e + L.ne * k];
+                pz[2 * (e + L.ne * k) + 1] = L.edzh[e + L.ne * (k + 1)];
+            }
+        rc |= trisk::upload(o, pz, &p.edzp);
+    }
     rc |= trisk::upload(o, L.ve, &p.ve);
     rc |= trisk::upload(o, L.vc, &p.vc);
     rc |= trisk::upload(o, L.curl, &p.curl);
