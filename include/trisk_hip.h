@@ -214,7 +214,9 @@ int trisk_kernel_times(trisk_handle* h, int32_t max, const char** names, double*
  * (each array element the kernel must touch counted once), with the kernel names, for an
  * RK4 stage of trisk_rk4_steps after the first of the call.  On the cell-pair path the edge
  * kernel of each such stage also forms the mass flux G of the state it writes, so the next
- * stage's state kernel covers cells and vertices only. */
+ * stage's state kernel covers the vertices only (the cell kernel forms K and B).  With vertex
+ * kinetic energy whose cell weights are all 0, the table path's first kernel covers the
+ * vertices only and the second forms B. */
 int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double* bytes, int32_t* count);
 
 /* Cells, vertices and edges whose indices do not fit the 16-bit deltas of the FP32-bulk
