@@ -106,10 +106,10 @@ __device__ __forceinline__ void adj_store(const AdjRk& r, int64_t j, double s, c
 // A1: cells.  cot = [lu; lh] scaled by c.
 template <class P>
 __global__ __launch_bounds__(kBlock) void adj_cell_kernel(Params<P> p, AdjParams<P> w, const double* __restrict__ cot,
-                                                          double c) {
+                                                          double c, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t i = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;  // XCD tiles
     if (i >= p.nc) return;
     int32_t meta, E[6], V[6];
     C W[15];
@@ -170,10 +170,10 @@ __global__ __launch_bounds__(kBlock) void adj_cell_kernel(Params<P> p, AdjParams
 // leaves as its three products with the vertex-height weights, one per cell
 template <class P, class UIn>
 __global__ __launch_bounds__(kBlock) void adj_vertex_kernel(Params<P> p, AdjParams<P> w, const UIn* __restrict__ u,
-                                                            const double* __restrict__ h) {
+                                                            const double* __restrict__ h, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
-    const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t v = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;  // XCD tiles
     if (v >= p.nv) return;
     const int32_t slots = ld(w.vslot + v);
     C qb = 0, z = 0, eta = 0;
@@ -196,10 +196,10 @@ __global__ __launch_bounds__(kBlock) void adj_vertex_kernel(Params<P> p, AdjPara
 // A3: edges
 template <class P, class UIn>
 __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams<P> w, const UIn* __restrict__ u,
-                                                          const double* __restrict__ h, AdjRk r) {
+                                                          const double* __restrict__ h, AdjRk r, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
-    const int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t e = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;  // XCD tiles
     if (e >= p.ne) return;
     const AdjIn rin = adj_load(r, e);
     int32_t c[2];
@@ -224,9 +224,9 @@ __global__ __launch_bounds__(kBlock) void adj_edge_kernel(Params<P> p, AdjParams
 // A4: cells
 template <class P, class UIn>
 __global__ __launch_bounds__(kBlock) void adj_height_kernel(Params<P> p, AdjParams<P> w, const UIn* __restrict__ u,
-                                                            AdjRk r) {
+                                                            AdjRk r, int64_t t0) {
     using C = typename P::C;
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t i = tile_of(blockIdx.x, t0, 0).first + threadIdx.x;  // XCD tiles
     if (i >= p.nc) return;
     const AdjIn rin = adj_load(r, p.ne + i);
     const int32_t meta = ld(p.cmeta + i);
@@ -301,10 +301,10 @@ struct TabAdj {
 // T1: edges.  GM: the Gassmann form (each slot's PV from its PV edge); else the mean form.
 template <class P, bool GM>
 __global__ __launch_bounds__(kBlock) void tadj_slot_kernel(Params<P> p, TabAdj<P> a, const double* __restrict__ cot,
-                                                           double c) {
+                                                           double c, int64_t te) {
     using C = typename P::C;
     using I = typename P::I;
-    const int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t e = tile_of(blockIdx.x, te, 0).first + threadIdx.x;  // XCD tiles
     if (e >= p.ne) return;
     const C lu = C(c * cot[e]);
     const C qe = C(p.Fq[e].q);
@@ -382,10 +382,11 @@ __global__ __launch_bounds__(kBlock) void tadj_flux_kernel(Params<P> p, TabAdj<P
 
 // T3: vertices (three edges and three cells each)
 template <class P, class UIn, bool RH, bool VK>
-__global__ __launch_bounds__(kBlock) void tadj_vertex_kernel(Params<P> p, TabAdj<P> a, const UIn* __restrict__ u) {
+__global__ __launch_bounds__(kBlock) void tadj_vertex_kernel(Params<P> p, TabAdj<P> a, const UIn* __restrict__ u,
+                                                             int64_t tv) {
     using C = typename P::C;
     using I = typename P::I;
-    const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t v = tile_of(blockIdx.x, tv, 0).first + threadIdx.x;  // XCD tiles
     if (v >= p.nv) return;
     const uint32_t bits = ld(a.vbits + v);
     C zb = 0, hb = 0;
@@ -741,27 +742,28 @@ static int table_vjp(trisk_handle* H, const UIn* u, const double* h, const doubl
     int rc = table_state_pass(H, u, std::is_same<UIn, double>::value, h, s);
     if (rc) return rc;
     const bool gm = L.gassmann && !L.gassmann_mean, rh = L.gassmann, vk = L.vertex_ke;
-    const dim3 ge(unsigned((L.ne + kBlock - 1) / kBlock)), gv(unsigned((L.nv + kBlock - 1) / kBlock)), b(kBlock);
+    const int64_t te = tiles_per_xcd(L.ne), tv = tiles_per_xcd(L.nv);
+    const dim3 ge(unsigned(kXcd * te)), gv(unsigned(kXcd * tv)), b(kBlock);
     const int64_t t0 = tiles_per_xcd(L.ne), t1 = tiles_per_xcd(L.nc);
     const dim3 g2(unsigned(kXcd * (t0 + t1)));
     if (gm) {
-        hipLaunchKernelGGL((tadj_slot_kernel<P, true>), ge, b, 0, s, p, a, cot, c);
+        hipLaunchKernelGGL((tadj_slot_kernel<P, true>), ge, b, 0, s, p, a, cot, c, te);
         hipLaunchKernelGGL((tadj_flux_kernel<P, true, true>), g2, b, 0, s, p, a, cot, c, t0, t1);
     } else {
-        hipLaunchKernelGGL((tadj_slot_kernel<P, false>), ge, b, 0, s, p, a, cot, c);
+        hipLaunchKernelGGL((tadj_slot_kernel<P, false>), ge, b, 0, s, p, a, cot, c, te);
         if (rh)
             hipLaunchKernelGGL((tadj_flux_kernel<P, false, true>), g2, b, 0, s, p, a, cot, c, t0, t1);
         else
             hipLaunchKernelGGL((tadj_flux_kernel<P, false, false>), g2, b, 0, s, p, a, cot, c, t0, t1);
     }
     if (rh && vk)
-        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, true>), gv, b, 0, s, p, a, u);
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, true>), gv, b, 0, s, p, a, u, tv);
     else if (rh)
-        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, false>), gv, b, 0, s, p, a, u);
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, true, false>), gv, b, 0, s, p, a, u, tv);
     else if (vk)
-        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, true>), gv, b, 0, s, p, a, u);
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, true>), gv, b, 0, s, p, a, u, tv);
     else
-        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, false>), gv, b, 0, s, p, a, u);
+        hipLaunchKernelGGL((tadj_vertex_kernel<P, UIn, false, false>), gv, b, 0, s, p, a, u, tv);
     if (vk)
         hipLaunchKernelGGL((tadj_final_kernel<P, UIn, true>), g2, b, 0, s, p, a, u, h, r, t0, t1);
     else
@@ -788,12 +790,14 @@ static int vjp(trisk_handle* H, const UIn* u, const double* h, const double* cot
         const int rc = pair_state_pass(H, u, std::is_same<UIn, double>::value, h, s);
         if (rc) return rc;
     }
-    const dim3 gc(unsigned((L.nc + kBlock - 1) / kBlock)), gv(unsigned((L.nv + kBlock - 1) / kBlock)),
-        ge(unsigned((L.ne + kBlock - 1) / kBlock)), b(kBlock);
-    hipLaunchKernelGGL((adj_cell_kernel<P>), gc, b, 0, s, p, w, cot, c);
-    hipLaunchKernelGGL((adj_vertex_kernel<P, UIn>), gv, b, 0, s, p, w, u, h);
-    hipLaunchKernelGGL((adj_edge_kernel<P, UIn>), ge, b, 0, s, p, w, u, h, r);
-    hipLaunchKernelGGL((adj_height_kernel<P, UIn>), gc, b, 0, s, p, w, u, r);
+    // XCD x takes the x-th eighth of each range (tile_of), so the gathers of its tiles stay in
+    // its L2
+    const int64_t tc = tiles_per_xcd(L.nc), tv = tiles_per_xcd(L.nv), te = tiles_per_xcd(L.ne);
+    const dim3 gc(unsigned(kXcd * tc)), gv(unsigned(kXcd * tv)), ge(unsigned(kXcd * te)), b(kBlock);
+    hipLaunchKernelGGL((adj_cell_kernel<P>), gc, b, 0, s, p, w, cot, c, tc);
+    hipLaunchKernelGGL((adj_vertex_kernel<P, UIn>), gv, b, 0, s, p, w, u, h, tv);
+    hipLaunchKernelGGL((adj_edge_kernel<P, UIn>), ge, b, 0, s, p, w, u, h, r, te);
+    hipLaunchKernelGGL((adj_height_kernel<P, UIn>), gc, b, 0, s, p, w, u, r, tc);
     HIP_TRY(hipGetLastError());
     return 0;
 }
