@@ -455,7 +455,8 @@ def run_ensemble(args):
         "cell_members": cells, "schedule": sched, "autotune_kernel_ms": tune_ms,
         # one launch per kernel for every member of every level; an RK stage is three such
         # launches (state, cell, edge), each of which gathers what the one before wrote
-        # anywhere on the meshes, so a stage is not one persistent launch
+        # anywhere on the meshes; "fused_stage" below times the one-launch stage (cooperative,
+        # grid-wide barriers), which is slower
         "launches_per_rk_stage": 3,
         "persistent_kernels": [k for k, sc in zip(("state", "cell", "edge"), sched) if sc.get("blocks_per_cu")],
         "kernel_bytes_per_step": step_bytes, "hbm_frac": frac_of_peak(step_bytes, ms / K),
@@ -463,6 +464,26 @@ def run_ensemble(args):
         "kernels": stats,
         "max_member_mass_drift": float(np.max(np.abs((d1[:, 0] - d0[:, 0]) / d0[:, 0]))),
     }
+    # one cooperative launch per RK stage (state, cell and edge phases with grid-wide barriers
+    # between them), direct launches; every member rounds as in the three-launch stage
+    try:
+        ens.use_fused(True)
+    except (ValueError, RuntimeError) as exc:
+        out["fused_stage"] = {"available": False, "why": str(exc)}
+    else:
+        ens.rk4_steps_(y, ENS_DTS, 4)
+        ens.set_timing(True)
+        msf_direct = timed(lambda: ens.rk4_steps_(y, ENS_DTS, K))
+        ens.set_timing(False)
+        fstats = ens.kernel_stats()
+        msf = timed(lambda: ens.rk4_steps_(y, ENS_DTS, K))
+        ens.use_fused(False)
+        fbytes = 4 * sum(k["bytes"] for k in fstats)
+        out["fused_stage"] = {"available": True, "launches_per_rk_stage": 1, "graphs": False,
+                              "value": per_s(cells * K, msf), "ms_per_step": msf / K,
+                              "ms_per_step_direct_with_events": msf_direct / K,
+                              "hbm_frac": frac_of_peak(fbytes, msf / K), "kernels": fstats}
+        log(f"[ensemble] fused stage (one cooperative launch per RK stage): {msf / K:.3f} ms/step")
     if args.no_member_loop:
         return out
     # the same members, one model per level, stepped member after member (graph replay)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "trisk_ensemble_tendency": [vp, vp, vp, vp],
     "trisk_ensemble_rk4_steps": [vp, vp, f64p, i64, vp],
     "trisk_ensemble_set_graphs": [vp, C.c_int32],
+    "trisk_ensemble_set_fused": [vp, C.c_int32],
     "trisk_ensemble_diagnostics": [vp, vp, f64p, vp],
     "trisk_ensemble_set_schedule": [vp, vp],
     "trisk_ensemble_get_schedule": [vp, vp],

@@ -11,6 +11,7 @@
 // the host (the schedule), optionally as a persistent grid.  The arithmetic is the
 // single-model kernels' (trisk_device.hpp), so every member rounds exactly as a model of
 // its own.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -250,7 +251,7 @@ __device__ __forceinline__ Item item_of(const EnsLevel<P>& L, uint32_t j) {
 // pair_state_kernel.  B is formed by the cell kernel, which gathers u at the edges where it
 // gathers G.
 template <class P, class UIn, int BLK, int V>
-__global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
+__device__ __forceinline__ void ens_state_body(const EnsArgs& a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
 // continuity with its RK4 update; cf. pair_cell_kernel.  Half sums are stored cell-major:
 // member m of cell i's slot k at T[(6 i + k) M + m].
 template <class P, class UIn, int BLK, int V>
-__global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
+__device__ __forceinline__ void ens_cell_body(const EnsArgs& a) {
     using C = typename P::C;
     using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
 // W3 of all levels: du and its RK4 update; cf. pair_edge_kernel.  GNEXT (RK4 stages): also G of
 // the state the stage writes, so that the next stage's state kernel skips the edges.
 template <class P, int BLK, int V, bool GNEXT>
-__global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
+__device__ __forceinline__ void ens_edge_body(const EnsArgs& a) {
     using C = typename P::C;
     using I = typename P::I;
     using U = typename P::U;
@@ -433,6 +434,43 @@ __global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
     });
 }
 
+template <class P, class UIn, int BLK, int V>
+__global__ __launch_bounds__(BLK) void ens_state_kernel(EnsArgs a) {
+    ens_state_body<P, UIn, BLK, V>(a);
+}
+template <class P, class UIn, int BLK, int V>
+__global__ __launch_bounds__(BLK) void ens_cell_kernel(EnsArgs a) {
+    ens_cell_body<P, UIn, BLK, V>(a);
+}
+template <class P, int BLK, int V, bool GNEXT>
+__global__ __launch_bounds__(BLK) void ens_edge_kernel(EnsArgs a) {
+    ens_edge_body<P, BLK, V, GNEXT>(a);
+}
+
+// One RK stage of every member in one cooperative launch: the state, cell and edge phases of
+// the kernels above, with a grid-wide barrier between them, since each phase gathers what the
+// previous one wrote anywhere on the mesh.  The launch is cooperative (hipLaunchCooperativeKernel),
+// so every workgroup of the grid is resident at once and the barrier is reached by all of
+// them; the grid (a multiple of the XCD count) is sized by the occupancy of this kernel.  The
+// phases walk their own tile lists as persistent loops (kFused: block size and members per
+// work-item of each phase).
+// The grid-wide barrier costs in proportion to the workgroups that reach it (about 31, 75 and
+// 144 us per barrier for 256, 1024 and 2048 workgroups of 256 on MI355X), so the stage is
+// slower than three launches replayed from a graph: 0.36 against 0.19 ms per stage for the
+// 64-member ensemble.  Four members per work-item in the state and cell phases hold the grid
+// at two workgroups per CU (the cell phase's registers), which measured faster than two
+// members per work-item at four per CU (0.47 ms).
+constexpr int kFusedBlock = 256, kFusedV0 = 4, kFusedV1 = 4, kFusedV2 = 2;
+template <class P, class UIn, bool GNEXT>
+__global__ __launch_bounds__(kFusedBlock) void ens_stage_kernel(EnsArgs as, EnsArgs ac, EnsArgs ae) {
+    cooperative_groups::grid_group g = cooperative_groups::this_grid();
+    ens_state_body<P, UIn, kFusedBlock, kFusedV0>(as);
+    g.sync();
+    ens_cell_body<P, UIn, kFusedBlock, kFusedV1>(ac);
+    g.sync();
+    ens_edge_body<P, kFusedBlock, kFusedV2, GNEXT>(ae);
+}
+
 // member m of a level block <-> a single-member state (dir 0: get, 1: set)
 __global__ __launch_bounds__(256) void ens_member_kernel(double* __restrict__ block, double* __restrict__ one,
                                                          int64_t ne, int64_t n, int32_t M, int32_t m, int32_t dir) {
@@ -445,7 +483,10 @@ __global__ __launch_bounds__(256) void ens_member_kernel(double* __restrict__ bl
         block[at] = one[k];
 }
 
-static const char* kEnsNames[kEnsKinds] = {"ens_state_kernel", "ens_cell_kernel", "ens_edge_kernel"};
+// timing kinds: the three kernels, and the fused stage
+constexpr int kEnsFused = kEnsKinds, kEnsTimed = kEnsKinds + 1;
+static const char* kEnsNames[kEnsTimed] = {"ens_state_kernel", "ens_cell_kernel", "ens_edge_kernel",
+                                           "ens_stage_kernel"};
 
 }  // namespace trisk
 
@@ -472,6 +513,13 @@ struct trisk_ensemble {
     int32_t first[kEnsLists][kXcd + 1]{};
     int64_t grid[kEnsLists]{};
     double* scratch = nullptr;  // one member of the largest level
+    // the fused stage (trisk_ensemble_set_fused): one cooperative launch per RK stage, with its
+    // own tile lists and grid
+    bool fused = false;
+    EnsSeg* fseg_dev[kEnsLists]{};
+    size_t fseg_cap[kEnsLists]{};
+    int32_t ffirst[kEnsLists][kXcd + 1]{};
+    int64_t fgrid = 0;
     // graph replay
     bool graphs = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -500,7 +548,7 @@ static bool valid_schedule(const trisk_ensemble* E, const trisk_schedule& s) {
 
 // State vectors must be aligned to the widest member group any kernel loads.
 static int check_alignment(const trisk_ensemble* E, const void* p) {
-    int V = 1;
+    int V = E->fused ? std::max(kFusedV0, std::max(kFusedV1, kFusedV2)) : 1;
     for (const auto& s : E->sched) V = std::max(V, lanes_of(s));
     if (reinterpret_cast<uintptr_t>(p) % (8 * V) != 0)
         return set_err(TRISK_EINVAL, "ensemble state vectors must be aligned to " + std::to_string(8 * V) +
@@ -508,10 +556,18 @@ static int check_alignment(const trisk_ensemble* E, const void* p) {
     return 0;
 }
 
-// The runs of tile list `list` per XCD under the schedule of its kernel.
-static int build_list(trisk_ensemble* E, int list) {
+// Where build_list_to writes a tile list: its device copy (grown as needed), the first run of
+// every XCD and the grid that walks it.
+struct ListOut {
+    EnsSeg** dev;
+    size_t* cap;
+    int32_t* first;  // kXcd + 1
+    int64_t* grid;
+};
+
+// The runs of tile list `list` per XCD under schedule `sc`.
+static int build_list_to(trisk_ensemble* E, int list, const trisk_schedule& sc, const ListOut& out) {
     const int kind = list == kStateNoEdges ? TRISK_ENS_STATE : list;
-    const trisk_schedule& sc = E->sched[kind];
     const int64_t blk = sc.block_size;
     const int64_t V = lanes_of(sc);
     struct Range {
@@ -578,7 +634,7 @@ static int build_list(trisk_ensemble* E, int list) {
     std::vector<EnsSeg> segs;
     int64_t most = 0;
     for (int x = 0; x < kXcd; ++x) {
-        E->first[list][x] = int32_t(segs.size());
+        out.first[x] = int32_t(segs.size());
         int64_t cum = 0;
         for (auto g : per[x]) {
             g.cum = int32_t(cum);
@@ -587,20 +643,26 @@ static int build_list(trisk_ensemble* E, int list) {
         }
         most = std::max(most, cum);
     }
-    E->first[list][kXcd] = int32_t(segs.size());
+    out.first[kXcd] = int32_t(segs.size());
     int64_t nbx = most;
     if (sc.blocks_per_cu > 0) nbx = std::min<int64_t>(most, int64_t(sc.blocks_per_cu) * ((E->cus + kXcd - 1) / kXcd));
-    E->grid[list] = kXcd * std::max<int64_t>(nbx, 1);
+    *out.grid = kXcd * std::max<int64_t>(nbx, 1);
     if (segs.empty()) segs.push_back({0, 0, 0, 0, 0, 0});
-    if (segs.size() > E->seg_cap[list]) {
+    if (segs.size() > *out.cap) {
         void* q;
         int rc = alloc(E->owned, segs.size() * sizeof(EnsSeg), &q);
         if (rc) return rc;
-        E->seg_dev[list] = static_cast<EnsSeg*>(q);
-        E->seg_cap[list] = segs.size();
+        *out.dev = static_cast<EnsSeg*>(q);
+        *out.cap = segs.size();
     }
-    HIP_TRY(hipMemcpy(E->seg_dev[list], segs.data(), segs.size() * sizeof(EnsSeg), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*out.dev, segs.data(), segs.size() * sizeof(EnsSeg), hipMemcpyHostToDevice));
     return 0;
+}
+
+static int build_list(trisk_ensemble* E, int list) {
+    const int kind = list == kStateNoEdges ? TRISK_ENS_STATE : list;
+    return build_list_to(E, list, E->sched[kind],
+                         ListOut{&E->seg_dev[list], &E->seg_cap[list], E->first[list], &E->grid[list]});
 }
 
 // The tile lists of kernel `kind` (the state kernel has two).
@@ -755,6 +817,25 @@ static int ens_tendency(trisk_ensemble* E, double* y, double* dy, int mode, bool
     a.dy = dy;
     a.mode = mode;
     a.from_y = from_y ? 1 : 0;
+    if (E->fused) {
+        EnsArgs as = a, ac = a, ae = a;
+        const int ls = state_edges ? int(TRISK_ENS_STATE) : kStateNoEdges;
+        as.seg = E->fseg_dev[ls];
+        std::memcpy(as.first, E->ffirst[ls], sizeof(as.first));
+        ac.seg = E->fseg_dev[TRISK_ENS_CELL];
+        std::memcpy(ac.first, E->ffirst[TRISK_ENS_CELL], sizeof(ac.first));
+        ae.seg = E->fseg_dev[TRISK_ENS_EDGE];
+        std::memcpy(ae.first, E->ffirst[TRISK_ENS_EDGE], sizeof(ae.first));
+        hipEvent_t stop;
+        int rc = ens_timed_begin(E, kEnsFused, s, &stop);
+        if (rc) return rc;
+        void* args[] = {&as, &ac, &ae};
+        const void* f = gnext ? reinterpret_cast<const void*>(&ens_stage_kernel<P, UIn, true>)
+                              : reinterpret_cast<const void*>(&ens_stage_kernel<P, UIn, false>);
+        HIP_TRY(hipLaunchCooperativeKernel(f, dim3(unsigned(E->fgrid)), dim3(kFusedBlock), args, 0, s));
+        if (stop) HIP_TRY(hipEventRecord(stop, s));
+        return 0;
+    }
     for (int kind = 0; kind < kEnsKinds; ++kind) {
         const int list = kind == TRISK_ENS_STATE && !state_edges ? kStateNoEdges : kind;
         a.seg = E->seg_dev[list];
@@ -967,7 +1048,8 @@ int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt,
         e->events_used = 0;
         return run_ens_steps(e, y_dev, nsteps, s);
     }
-    if (!e->graphs || s == nullptr || nsteps < kEnsGraphSteps) return run_ens_steps(e, y_dev, nsteps, s);
+    // the fused stage runs by direct cooperative launches (four per step)
+    if (!e->graphs || e->fused || s == nullptr || nsteps < kEnsGraphSteps) return run_ens_steps(e, y_dev, nsteps, s);
     // the graph bakes in y and the schedules; the per-level dt is read from device memory
     if (!e->graph_exec || e->graph_y != y_dev) {
         ens_drop_graph(e);
@@ -994,6 +1076,58 @@ int trisk_ensemble_set_graphs(trisk_ensemble* e, int32_t on) {
     HIP_TRY(hipSetDevice(e->device));
     e->graphs = on != 0;
     if (!e->graphs) ens_drop_graph(e);
+    return 0;
+}
+
+int trisk_ensemble_set_fused(trisk_ensemble* e, int32_t on) {
+    CHECK_ENS(e);
+    HIP_TRY(hipSetDevice(e->device));
+    ens_drop_graph(e);
+    if (!on) {
+        e->fused = false;
+        return 0;
+    }
+    int coop = 0;
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, e->device));
+    if (!coop) return set_err(TRISK_EINVAL, "the device does not support cooperative launches");
+    for (int l = 0; l < e->nlevels; ++l)
+        if (e->M[l] % kFusedV0 || e->M[l] % kFusedV1 || e->M[l] % kFusedV2)
+            return set_err(TRISK_EINVAL, "the fused stage needs a member count divisible by " +
+                                             std::to_string(std::max(kFusedV0, kFusedV1)) + " at every level");
+    // workgroups of the fused kernel that one CU holds at once (the least over its variants)
+    int per_cu = 1 << 30;
+    auto occupancy = [&](const void* f) -> int {
+        int n = 0;
+        hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, kFusedBlock, 0);
+        if (err != hipSuccess) return set_err(TRISK_EHIP, std::string("hipOccupancyMaxActiveBlocksPerMultiprocessor: ") +
+                                                              hipGetErrorString(err));
+        per_cu = std::min(per_cu, n);
+        return 0;
+    };
+    int rc;
+    if (e->precision == TRISK_FP64) {
+        if ((rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp64, double, true>))) ||
+            (rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp64, double, false>))))
+            return rc;
+    } else {
+        if ((rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp32Bulk, double, true>))) ||
+            (rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp32Bulk, double, false>))) ||
+            (rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp32Bulk, float, true>))) ||
+            (rc = occupancy(reinterpret_cast<const void*>(&ens_stage_kernel<Fp32Bulk, float, false>))))
+            return rc;
+    }
+    if (per_cu < 1) return set_err(TRISK_EINVAL, "the fused stage kernel does not fit a CU");
+    // a multiple of the XCD count, so that every XCD walks its share with the same stride
+    e->fgrid = int64_t(per_cu) * ((e->cus / kXcd) * kXcd);
+    if (e->fgrid < kXcd) return set_err(TRISK_EINVAL, "too few compute units for the fused stage");
+    const int lanes[kEnsLists] = {kFusedV0, kFusedV1, kFusedV2, kFusedV0};
+    for (int list = 0; list < kEnsLists; ++list) {
+        int64_t grid = 0;
+        const trisk_schedule sc{kFusedBlock, per_cu, TRISK_SPLIT_CHUNKED, lanes[list]};
+        if ((rc = build_list_to(e, list, sc, ListOut{&e->fseg_dev[list], &e->fseg_cap[list], e->ffirst[list], &grid})))
+            return rc;
+    }
+    e->fused = true;
     return 0;
 }
 
@@ -1047,8 +1181,8 @@ int trisk_ensemble_kernel_times(trisk_ensemble* e, int32_t max, const char** nam
                                 int64_t* launches, int32_t* count) {
     CHECK_ENS(e);
     HIP_TRY(hipSetDevice(e->device));
-    double ms[kEnsKinds] = {0};
-    int64_t n[kEnsKinds] = {0};
+    double ms[kEnsTimed] = {0};
+    int64_t n[kEnsTimed] = {0};
     for (size_t i = 0; i + 1 < e->events_used; i += 2) {
         HIP_TRY(hipEventSynchronize(e->events[i + 1]));
         float t = 0;
@@ -1057,7 +1191,7 @@ int trisk_ensemble_kernel_times(trisk_ensemble* e, int32_t max, const char** nam
         n[e->event_kernel[i / 2]] += 1;
     }
     int c = 0;
-    for (int k = 0; k < kEnsKinds && c < max; ++k) {
+    for (int k = 0; k < kEnsTimed && c < max; ++k) {
         if (n[k] == 0) continue;
         if (names) names[c] = kEnsNames[k];
         if (avg_ms) avg_ms[c] = ms[k] / double(n[k]);
@@ -1090,10 +1224,12 @@ int trisk_ensemble_kernel_bytes(trisk_ensemble* e, int32_t max, const char** nam
                                                           nc * (8.0 + 8.0));
         b[2] += ne * (ew + c + 1.0 + 2 * c) + M * (ne * (2 * c + rk_e + c) + nc * (8.0 + 8.0));
     }
+    // the fused stage moves what its three phases move
+    const double all[kEnsTimed] = {b[0], b[1], b[2], b[0] + b[1] + b[2]};
     int k = 0;
-    for (; k < kEnsKinds && k < max; ++k) {
+    for (; k < kEnsTimed && k < max; ++k) {
         if (names) names[k] = kEnsNames[k];
-        bytes[k] = b[k];
+        bytes[k] = all[k];
     }
     if (count) *count = k;
     return 0;
@@ -1117,7 +1253,8 @@ int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double
         ~Free() { (void)hipFree(p); }
     } guard{work};
     HIP_TRY(hipMemcpyAsync(work, y_dev, e->n_state * sizeof(double), hipMemcpyDeviceToDevice, s));
-    const bool timing = e->timing;
+    const bool timing = e->timing, fused = e->fused;
+    e->fused = false;  // the candidates are the three-launch schedules
     trisk_schedule keep[kEnsKinds];
     for (int k = 0; k < kEnsKinds; ++k) keep[k] = e->sched[k];
     trisk_schedule win[kEnsKinds] = {keep[0], keep[1], keep[2]};
@@ -1150,6 +1287,7 @@ int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double
                         }
                 }
     e->timing = timing;
+    e->fused = fused;
     e->events_used = 0;
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < kEnsKinds; ++k) {

@@ -163,6 +163,12 @@ class TRiSKEnsemble:
     def use_graphs(self, on: bool = True) -> None:
         D.check(D.lib().trisk_ensemble_set_graphs(self._handle, int(on)))
 
+    def use_fused(self, on: bool = True) -> None:
+        """One cooperative launch per RK stage for all members (state, cell and edge phases with
+        a grid-wide barrier between them) instead of three launches; every member rounds the
+        same either way.  Needs member counts divisible by 4; graphs are not used meanwhile."""
+        D.check(D.lib().trisk_ensemble_set_fused(self._handle, int(on)))
+
     def diagnostics(self, y) -> np.ndarray:
         """(members, 3): total mass, total energy, potential enstrophy of every member."""
         torch = _torch()
@@ -200,16 +206,17 @@ class TRiSKEnsemble:
 
     def kernel_stats(self) -> list[dict]:
         """Mean duration, launches and algorithmic bytes of each kernel over the last timed call."""
-        names = (C.c_char_p * 3)()
-        avg = np.zeros(3)
-        nl = np.zeros(3, dtype=np.int64)
+        # the three kernels, and the fused stage (use_fused)
+        names = (C.c_char_p * 4)()
+        avg = np.zeros(4)
+        nl = np.zeros(4, dtype=np.int64)
         cnt = C.c_int32()
-        D.check(D.lib().trisk_ensemble_kernel_times(self._handle, 3, names, ptr(avg, f64p), ptr(nl, i64p),
+        D.check(D.lib().trisk_ensemble_kernel_times(self._handle, 4, names, ptr(avg, f64p), ptr(nl, i64p),
                                                     C.byref(cnt)))
-        by = np.zeros(3)
-        bn = (C.c_char_p * 3)()
+        by = np.zeros(4)
+        bn = (C.c_char_p * 4)()
         cnt2 = C.c_int32()
-        D.check(D.lib().trisk_ensemble_kernel_bytes(self._handle, 3, bn, ptr(by, f64p), C.byref(cnt2)))
+        D.check(D.lib().trisk_ensemble_kernel_bytes(self._handle, 4, bn, ptr(by, f64p), C.byref(cnt2)))
         bytes_of = {bn[i].decode(): float(by[i]) for i in range(cnt2.value)}
         return [dict(name=names[i].decode(), avg_ms=float(avg[i]), launches=int(nl[i]),
                      bytes=bytes_of.get(names[i].decode(), 0.0)) for i in range(cnt.value)]

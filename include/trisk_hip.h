@@ -480,6 +480,16 @@ int trisk_ensemble_tendency(trisk_ensemble* e, const double* y_dev, double* dy_d
 int trisk_ensemble_rk4_steps(trisk_ensemble* e, double* y_dev, const double* dt, int64_t nsteps, void* stream);
 /* Replay trisk_ensemble_rk4_steps through a captured graph of 8 steps (as trisk_set_graphs). */
 int trisk_ensemble_set_graphs(trisk_ensemble* e, int32_t on);
+/* on != 0: every RK stage (and tendency) of the ensemble is one cooperative launch
+ * (hipLaunchCooperativeKernel) whose workgroups run the state, cell and edge phases with a
+ * grid-wide barrier between them, instead of three launches; its grid holds as many workgroups
+ * as the device keeps resident at once, and each phase walks its tiles XCD by XCD.  Results
+ * are bitwise those of the three launches.  Needs every level's member count divisible by 4
+ * and a device with cooperative launches (TRISK_EINVAL otherwise); graph replay is not used
+ * while it is on, and trisk_ensemble_autotune tunes the three-launch schedules.  The grid-wide
+ * barriers make it slower than the three launches of a replayed graph (the default).  No counterpart
+ * in the reference, which steps one model at a time (RungeKutta.jl:18-28). */
+int trisk_ensemble_set_fused(trisk_ensemble* e, int32_t on);
 /* out[3k .. 3k+2] = trisk_diagnostics of member k, members numbered level by level.  Synchronous. */
 int trisk_ensemble_diagnostics(trisk_ensemble* e, const double* y_dev, double* out, void* stream);
 /* The schedules of the three kernels (TRISK_ENS_*); the default is {256, 0, CHUNKED, 1}. */
@@ -491,7 +501,8 @@ int trisk_ensemble_get_schedule(const trisk_ensemble* e, trisk_schedule sched[3]
  * Synchronous. */
 int trisk_ensemble_autotune(trisk_ensemble* e, const double* y_dev, const double* dt, int32_t steps, void* stream,
                             trisk_schedule best[3], double best_ms[3]);
-/* Kernel timing and algorithmic bytes, as trisk_set_timing / trisk_kernel_times / trisk_kernel_bytes. */
+/* Kernel timing and algorithmic bytes, as trisk_set_timing / trisk_kernel_times / trisk_kernel_bytes
+ * (the three kernels, then the fused stage of trisk_ensemble_set_fused: four entries). */
 int trisk_ensemble_set_timing(trisk_ensemble* e, int32_t on);
 int trisk_ensemble_kernel_times(trisk_ensemble* e, int32_t max, const char** names, double* avg_ms,
                                 int64_t* launches, int32_t* count);

@@ -132,6 +132,45 @@ def test_every_schedule_gives_the_same_bits(precision, lanes, block_size, blocks
     assert np.array_equal(ens.get_member(y, 1, 7), s.download_state(yd))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+def test_fused_stage_gives_the_same_bits(precision):
+    """One cooperative launch per RK stage (the state, cell and edge phases with grid-wide
+    barriers between them) gives the three-launch states bitwise: RK4 steps in calls of
+    several lengths (each call's first stage also forms G) and a tendency."""
+    say(f"ensemble fused stage {precision}")
+    ens = TRiSKEnsemble([tables(n) for n in LEVELS], EVEN, backend=HIPBackend(0, precision))
+    y = ens.zeros()
+    for li, (n, M) in enumerate(zip(LEVELS, EVEN)):
+        for m in range(M):
+            ens.set_member(y, li, m, member_state(tables(n), li, m))
+    ref = y.clone()
+    ens.rk4_steps_(ref, DTS, 5)
+    dref = ens.zeros()
+    ens.tendency_(dref, y)
+    ens.use_fused(True)
+    for split in ((5,), (2, 3)):
+        got = y.clone()
+        for n in split:
+            ens.rk4_steps_(got, DTS, n)
+        assert bool((got == ref).all()), split
+    d = ens.zeros()
+    ens.tendency_(d, y)
+    assert bool((d == dref).all())
+    ens.set_timing(True)
+    got = y.clone()
+    ens.rk4_steps_(got, DTS, 5)
+    ens.set_timing(False)
+    assert bool((got == ref).all())
+    stats = ens.kernel_stats()
+    assert [k["name"] for k in stats] == ["ens_stage_kernel"] and stats[0]["launches"] == 20
+    ens.use_fused(False)
+    got = y.clone()
+    ens.rk4_steps_(got, DTS, 5)
+    assert bool((got == ref).all())
+    with pytest.raises(ValueError, match="divisible"):
+        TRiSKEnsemble([tables(n) for n in LEVELS], MEMBERS, backend=HIPBackend(0, precision)).use_fused(True)
+
+
 def test_fp64_members_match_oracle():
     say("ensemble fp64 member vs oracle, level 4, 12 steps")
     ens = TRiSKEnsemble([tables(4)], [4], backend=HIPBackend(0, "fp64"))
