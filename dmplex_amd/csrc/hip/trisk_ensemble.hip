@@ -313,68 +313,142 @@ __device__ __forceinline__ void ens_state_body(const EnsArgs& a) {
 // W2 of all levels: K and B = K + g h at the stage (u, h), the PV-flux half sums and the
 // continuity with its RK4 update; cf. pair_cell_kernel.  Half sums are stored cell-major:
 // member m of cell i's slot k at T[(6 i + k) M + m].
+// One item of the cell kernel, from its cell's decoded table row (meta, edges, corners, pair
+// weights, kinetic-energy weights).
+template <class P, class UIn, int V>
+__device__ __forceinline__ void ens_cell_item(const EnsArgs& a, const EnsLevel<P>& L, uint32_t j, int32_t meta,
+                                              const int32_t (&E)[6], const int32_t (&V6)[6],
+                                              const typename P::C (&W)[15], const typename P::C (&kin)[6],
+                                              const RkInLanes<V>& rin) {
+    using C = typename P::C;
+    using I = typename P::I;
+    const Params<P>& p = L.p;
+    const uint32_t M = uint32_t(L.M);
+    const Item it = item_of<V>(L, j);
+    const Rk r = level_rk(L, a);
+    // the stage heights: those of the state itself when the input is y (already loaded by
+    // rk_load_lanes in RK4 modes; a tendency call, mode 0, loads them here)
+    const double* hs = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
+    const Lanes<double, V> hh = a.from_y && r.mode >= 1 ? rin.y : vload<double, V>(hs + it.o);
+    const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
+    const int n = meta & 15;
+    Lanes<I, V> Gk[6], Qk[6], Hk[6];
+    Lanes<UIn, V> uk[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        Gk[k] = vload<I, V>(p.G + uint32_t(E[k]) * M + it.m0);
+        Qk[k] = vload<I, V>(p.qv + uint32_t(V6[k]) * M + it.m0);
+        uk[k] = vload<UIn, V>(u + uint32_t(E[k]) * M + it.m0);
+    }
+    const double area = ld(p.area_c + it.ent);
+    Lanes<double, V> dh, B;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        C G[6], Q[6], H[6], ue[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            G[k] = C(Gk[k].v[v]);
+            Q[k] = C(Qk[k].v[v]);
+            ue[k] = C(uk[k].v[v]);
+        }
+        B.v[v] = double(pair_kinetic(kin, ue)) + p.g * hh.v[v];
+        pair_half_sums(n, W, G, Q, H);
+        double div = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            Hk[k].v[v] = I(H[k]);
+            if (k < n) div += (meta >> (4 + k)) & 1 ? -double(G[k]) : double(G[k]);
+        }
+        dh.v[v] = -div / area;
+    }
+    vstore(p.B + it.o, B);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vstore(p.T + (6 * it.ent + uint32_t(k)) * M + it.m0, Hk[k]);
+    rk_store_lanes<double, V>(r, p.ne * M + it.o, dh, rin, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
+}
+
+// W2 of all levels: K and B = K + g h at the stage (u, h), the PV-flux half sums and the
+// continuity with its RK4 update; cf. pair_cell_kernel.  Half sums are stored cell-major:
+// member m of cell i's slot k at T[(6 i + k) M + m].  FP32-bulk: the tiles walk as in
+// for_each_item, and each stages the cell records of its items (76 B, the single model's
+// rc_pair) through LDS with 16-byte loads, from a start rounded down to a multiple of 4 cells
+// (16-byte aligned); its items then read their cell's record there.  FP64 reads the slot-major
+// tables.
 template <class P, class UIn, int BLK, int V>
 __device__ __forceinline__ void ens_cell_body(const EnsArgs& a) {
     using C = typename P::C;
-    using I = typename P::I;
     constexpr bool NARROW = P::id == TRISK_FP32_BULK;
-    auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
-    for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
-        const EnsLevel<P> L = load_level<P>(a.levels, level);
-        const Params<P>& p = L.p;
-        const uint32_t M = uint32_t(L.M);
-        const Item it = item_of<V>(L, j);
-        const Rk r = level_rk(L, a);
-        const RkInLanes<V> rin = rk_load_lanes<V>(r, p.ne * M + it.o);
-        // the stage heights: those of the state itself when the input is y (already loaded by
-        // rk_load_lanes in RK4 modes; a tendency call, mode 0, loads them here)
-        const double* hs = a.from_y ? a.y + L.yoff + p.ne * M : L.sh;
-        const Lanes<double, V> hh = a.from_y && r.mode >= 1 ? rin.y : vload<double, V>(hs + it.o);
-        const UIn* u = a.from_y ? reinterpret_cast<const UIn*>(a.y + L.yoff) : static_cast<const UIn*>(L.su);
-        int32_t meta, E[6], V6[6];
-        C W[15], kin[6];
-        load_cell_pairs<NARROW>(p, it.ent, meta, E, V6, W);
-        if constexpr (NARROW) {
-            ldrec(p.rc_kin + 6 * int64_t(it.ent), kin);
-        } else {
+    if constexpr (!NARROW) {
+        auto count = [](int, int) -> int64_t { return 0; };  // runs of one kind only
+        for_each_item<BLK>(a, count, [&](int level, int, uint32_t j) {
+            const EnsLevel<P> L = load_level<P>(a.levels, level);
+            const Params<P>& p = L.p;
+            const Item it = item_of<V>(L, j);
+            const RkInLanes<V> rin = rk_load_lanes<V>(level_rk(L, a), p.ne * uint32_t(L.M) + it.o);
+            int32_t meta, E[6], V6[6];
+            C W[15], kin[6];
+            load_cell_pairs<false>(p, it.ent, meta, E, V6, W);
 #pragma unroll
             for (int k = 0; k < 6; ++k) kin[k] = k < p.kc ? ld(p.kin + k * p.nc + it.ent) : C(0);
-        }
-        const int n = meta & 15;
-        Lanes<I, V> Gk[6], Qk[6], Hk[6];
-        Lanes<UIn, V> uk[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            Gk[k] = vload<I, V>(p.G + uint32_t(E[k]) * M + it.m0);
-            Qk[k] = vload<I, V>(p.qv + uint32_t(V6[k]) * M + it.m0);
-            uk[k] = vload<UIn, V>(u + uint32_t(E[k]) * M + it.m0);
-        }
-        const double area = ld(p.area_c + it.ent);
-        Lanes<double, V> dh, B;
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            C G[6], Q[6], H[6], ue[6];
+            ens_cell_item<P, UIn, V>(a, L, j, meta, E, V6, W, kin, rin);
+        });
+    } else {
+        constexpr int RW = kPairRecord;
+        __shared__ __attribute__((aligned(16))) int32_t srec[(BLK + 4) * RW];
+        const int x = int(blockIdx.x % kXcd);
+        const int nbx = int(gridDim.x / kXcd);
+        const int s0 = a.first[x], s1 = a.first[x + 1];
+        if (s1 <= s0) return;  // the whole block
+        const cptr<EnsSeg> seg = (cptr<EnsSeg>)a.seg;
+        const int total = seg[s1 - 1].cum + seg[s1 - 1].ntiles;
+        int s = s0;
+        // t is the same for every thread of the block, so the barriers are reached by all
+        for (int t = int(blockIdx.x / kXcd); t < total; t += nbx) {
+            while (s + 1 < s1 && t >= seg[s + 1].cum) ++s;  // t only grows
+            const int32_t kind = seg[s].kind;
+            const int32_t ntiles = seg[s].ntiles, cum = seg[s].cum;
+            const int64_t tile = seg[s].tile0 + ((kind & kEnsBackward) ? ntiles - 1 - (t - cum) : t - cum);
+            const int64_t nitems = seg[s].nitems;
+            const EnsLevel<P> L = load_level<P>(a.levels, seg[s].level);
+            const Params<P>& p = L.p;
+            const int64_t j0 = tile * BLK, jl = (j0 + BLK < nitems ? j0 + BLK : nitems) - 1;
+            const int64_t e0 = int64_t(L.md[lane_log<V>()](uint32_t(j0))) & ~int64_t(3);
+            const int64_t e1 = int64_t(L.md[lane_log<V>()](uint32_t(jl))) + 1;
+            const int64_t j = j0 + threadIdx.x;
+            const bool on = j <= jl;
+            // the RK4 loads first: they do not wait for the records
+            RkInLanes<V> rin{};
+            if (on) rin = rk_load_lanes<V>(level_rk(L, a), p.ne * uint32_t(L.M) + uint32_t(j) * V);
+            __syncthreads();  // the previous tile's records have been read
+            const int nw = int((e1 - e0) * RW), nw4 = nw / 4;
+            const int32_t* src = p.rc_pair + e0 * RW;
+            for (int w = threadIdx.x; w < nw4; w += BLK)
+                reinterpret_cast<f32x4*>(srec)[w] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(src) + w);
+            for (int w = 4 * nw4 + threadIdx.x; w < nw; w += BLK) srec[w] = ld(src + w);
+            __syncthreads();
+            if (!on) continue;
+            const Item it = item_of<V>(L, uint32_t(j));
+            const int32_t* rr = srec + RW * (int64_t(it.ent) - e0);
+            const int32_t meta = rr[0];
+            const bool wide = int16_t(rr[1] & 0xffff) == kWide;
+            const int32_t be = int32_t(3 * it.ent), bv = int32_t(2 * it.ent);
+            int32_t E[6], V6[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                G[k] = C(Gk[k].v[v]);
-                Q[k] = C(Qk[k].v[v]);
-                ue[k] = C(uk[k].v[v]);
+                const int32_t pk = rr[1 + k];
+                E[k] = wide ? ld(p.ce6 + k * p.nc + it.ent) : be + int16_t(pk & 0xffff);
+                V6[k] = wide ? ld(p.ccv + k * p.nc + it.ent) : bv + int16_t(uint32_t(pk) >> 16);
             }
-            B.v[v] = double(pair_kinetic(kin, ue)) + p.g * hh.v[v];
-            pair_half_sums(n, W, G, Q, H);
-            double div = 0.0;
+            C R[6], kin[6], W[15];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                Hk[k].v[v] = I(H[k]);
-                if (k < n) div += (meta >> (4 + k)) & 1 ? -double(G[k]) : double(G[k]);
+                __builtin_memcpy(&R[k], &rr[7 + k], 4);
+                __builtin_memcpy(&kin[k], &rr[13 + k], 4);
             }
-            dh.v[v] = -div / area;
+            pair_weights(meta, R, W);
+            ens_cell_item<P, UIn, V>(a, L, uint32_t(j), meta, E, V6, W, kin, rin);
         }
-        vstore(p.B + it.o, B);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) vstore(p.T + (6 * it.ent + uint32_t(k)) * M + it.m0, Hk[k]);
-        rk_store_lanes<double, V>(r, p.ne * M + it.o, dh, rin, r.mode >= 1 && r.mode <= 3 ? L.sh + it.o : nullptr);
-    });
+    }
 }
 
 // W3 of all levels: du and its RK4 update; cf. pair_edge_kernel.  GNEXT (RK4 stages): also G of
