@@ -81,6 +81,7 @@ SIGNATURES = {
     "trisk_tangential_velocity": [vp, vp, vp, vp],
     "trisk_set_timing": [vp, C.c_int32],
     "trisk_set_graphs": [vp, C.c_int32],
+    "trisk_set_pair_plane": [vp, C.c_int32],
     "trisk_kernel_times": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, i64p, C.POINTER(C.c_int32)],
     "trisk_kernel_bytes": [vp, C.c_int32, C.POINTER(C.c_char_p), f64p, C.POINTER(C.c_int32)],
     "trisk_info": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_char_p)],

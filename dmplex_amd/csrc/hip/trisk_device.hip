@@ -419,7 +419,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(Params<P> p, Rk r, int64
 // vector loads (fewer instructions; faster for the FP32 tables), else slot-major arrays (faster
 // for the FP64 ones, whose records span several cache lines).  B is formed by the cell kernel,
 // which gathers u at the same edges as G.
-template <class P, class UIn, bool REC>
+// PLANE (rk4_steps' (G, u) plane, one pair per edge): kPlaneNone keeps G and the stage u in
+// separate arrays; kPlaneWrite (the first stage of a call) reads u and writes the edges' (G,
+// C(u)) pairs; kPlaneRead (later stages, no edge range) gathers the vertices' u from the plane,
+// which holds C(u) of the stage input, the value the vertices would convert u to.
+enum PairPlane { kPlaneNone = 0, kPlaneWrite = 1, kPlaneRead = 2 };
+
+template <class P, class UIn, bool REC, int PLANE = kPlaneNone>
 __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const UIn* __restrict__ u,
                                                             const double* __restrict__ h, int64_t t0, int64_t t1) {
     using C = typename P::C;
@@ -434,7 +440,10 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
         C uz[3], hz[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            uz[k] = C(u[idx[k]]);
+            if constexpr (PLANE == kPlaneRead)
+                uz[k] = C(p.gu[idx[k]].q);
+            else
+                uz[k] = C(u[idx[k]]);
             hz[k] = C(h[idx[3 + k]]);
         }
         p.qv[v] = I(pair_vertex_pv(cf, uz, hz));
@@ -450,15 +459,20 @@ __global__ __launch_bounds__(kBlock) void pair_state_kernel(Params<P> p, const U
             load_edge_cells<REC>(p, e, c);
             ldrec(p.re_lh + 2 * e, lh);
         }
-        p.G[e] = I(pair_mass_flux(lh, C(h[c[0]]), C(h[c[1]]), C(u[e])));
+        const C ue = C(u[e]);
+        const I G = I(pair_mass_flux(lh, C(h[c[0]]), C(h[c[1]]), ue));
+        if constexpr (PLANE == kPlaneWrite)
+            p.gu[e] = Pair<I>{G, I(ue)};
+        else
+            p.G[e] = G;
     }
 }
 
 // W2: cells: K_i and B_i = K_i + g h_i at the stage (u, h), the PV-flux half sums of their
 // edges, and the continuity with its RK4 update (XCD x takes the x-th eighth of the cells, so
 // the G, u and q_v its tiles share stay in its L2).  The FP32-bulk kernel stages its block's
-// cell records through LDS.
-template <class P, class UIn>
+// cell records through LDS.  PLANE: G and u gathered as one (G, u) pair per edge.
+template <class P, class UIn, bool PLANE = false>
 __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, const UIn* __restrict__ u,
                                                            const double* __restrict__ h, Rk r, int64_t t0) {
     using C = typename P::C;
@@ -512,9 +526,15 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, const UI
     C G[6], Q[6], ue[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-        G[k] = C(p.G[E[k]]);
+        if constexpr (PLANE) {
+            const Pair<typename P::I> gk = p.gu[E[k]];
+            G[k] = C(gk.F);
+            ue[k] = C(gk.q);
+        } else {
+            G[k] = C(p.G[E[k]]);
+            ue[k] = C(u[E[k]]);
+        }
         Q[k] = C(p.qv[V[k]]);
-        ue[k] = C(u[E[k]]);
     }
     p.B[i] = double(pair_kinetic(kin, ue)) + p.g * hi;
     C H[6];
@@ -540,7 +560,9 @@ __global__ __launch_bounds__(kBlock) void pair_cell_kernel(Params<P> p, const UI
 // GNEXT (RK4 stages): also G_e = l_e h_e u_e of the state this stage writes, from the heights
 // the cell kernel has just written and the edge's own new value, so that the next stage's
 // state pass skips its edge range.  The operands and operations are those of the state pass.
-template <class P, bool GNEXT>
+// PLANE (with GNEXT): the new G and C(u) go to the (G, u) plane as one pair, in place of G and
+// the stage vector (modes 1-3; mode 4 still writes y).
+template <class P, bool GNEXT, bool PLANE = false>
 __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, int64_t t0) {
     using C = typename P::C;
     using I = typename P::I;
@@ -572,8 +594,15 @@ __global__ __launch_bounds__(kBlock) void pair_edge_kernel(Params<P> p, Rk r, in
         hn1 = hn[c[1]];
     }
     const double du = pair_edge_tendency(C(hs0), C(hs1), d, b0, b1);
-    rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
-    if constexpr (GNEXT) p.G[e] = I(pair_mass_flux(lh, C(hn0), C(hn1), C(rk_next(r, du, rin))));
+    if constexpr (PLANE) {
+        static_assert(GNEXT, "the (G, u) plane is formed with the next stage's G");
+        rk_store<U, kStreamAcc, false>(r, e, du, rin, static_cast<U*>(nullptr));
+        const C un = C(rk_next(r, du, rin));
+        p.gu[e] = Pair<I>{I(pair_mass_flux(lh, C(hn0), C(hn1), un)), I(un)};
+    } else {
+        rk_store<U, kStreamAcc>(r, e, du, rin, r.mode >= 1 && r.mode <= 3 ? static_cast<U*>(r.su) + e : nullptr);
+        if constexpr (GNEXT) p.G[e] = I(pair_mass_flux(lh, C(hn0), C(hn1), C(rk_next(r, du, rin))));
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -809,6 +838,11 @@ static int setup_params(trisk_handle* H) {
     p.Fq = static_cast<Pair<I>*>(q);
     if ((rc = alloc(o, L.ne * sizeof(I), &q))) return rc;
     p.G = static_cast<I*>(q);
+    p.gu = nullptr;
+    if (L.pair) {
+        if ((rc = alloc(o, L.ne * sizeof(Pair<I>), &q))) return rc;
+        p.gu = static_cast<Pair<I>*>(q);
+    }
     if ((rc = alloc(o, 6 * L.nc * sizeof(I), &q))) return rc;
     p.T = static_cast<I*>(q);
     for (I** w : {&p.qv, &p.zv, &p.hv, &p.Kv}) {
@@ -835,10 +869,12 @@ static int setup_params(trisk_handle* H) {
 }
 
 
-// One tendency: P1, P2, P3 with the given stage input and update mode.
+// One tendency: P1, P2, P3 with the given stage input and update mode.  plane (RK4 stages with
+// gnext, the stage input written by the previous edge kernel or, with state_edges, the state):
+// G and u travel in the (G, u) plane.
 template <class P, class UIn>
 static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s, bool state_edges,
-                    bool gnext) {
+                    bool gnext, bool plane) {
     const Params<P>& p = params<P>(H);
     const Layout& L = H->L;
     hipEvent_t stop;
@@ -848,23 +884,37 @@ static int run_pair(trisk_handle* H, const UIn* u, const double* h, const Rk& r,
         int64_t t0 = tiles_per_xcd(L.nv), t1 = state_edges ? tiles_per_xcd(L.ne) : 0;
         if ((rc = timed_begin(H, K_PAIR_STATE, s, &stop))) return rc;
         constexpr bool rec = P::id == TRISK_FP32_BULK;
-        hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), dim3(unsigned(kXcd * (t0 + t1))), dim3(kBlock), 0, s, p,
-                           u, h, t0, t1);
+        const dim3 grid(unsigned(kXcd * (t0 + t1)));
+        if (!plane)
+            hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec>), grid, dim3(kBlock), 0, s, p, u, h, t0, t1);
+        else if (state_edges)
+            hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec, kPlaneWrite>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+                               t1);
+        else
+            hipLaunchKernelGGL((pair_state_kernel<P, UIn, rec, kPlaneRead>), grid, dim3(kBlock), 0, s, p, u, h, t0,
+                               t1);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
     {
         if ((rc = timed_begin(H, K_PAIR_CELL, s, &stop))) return rc;
         const int64_t t0 = tiles_per_xcd(L.nc);
-        hipLaunchKernelGGL((pair_cell_kernel<P, UIn>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, u, h, r,
-                           t0);
+        if (plane)
+            hipLaunchKernelGGL((pair_cell_kernel<P, UIn, true>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, u, h,
+                               r, t0);
+        else
+            hipLaunchKernelGGL((pair_cell_kernel<P, UIn>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, u, h, r,
+                               t0);
         HIP_TRY(hipGetLastError());
         if ((rc = timed_end(stop, s))) return rc;
     }
     {
         if ((rc = timed_begin(H, K_PAIR_EDGE, s, &stop))) return rc;
         const int64_t t0 = tiles_per_xcd(L.ne);
-        if (gnext)
+        if (gnext && plane)
+            hipLaunchKernelGGL((pair_edge_kernel<P, true, true>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r,
+                               t0);
+        else if (gnext)
             hipLaunchKernelGGL((pair_edge_kernel<P, true>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
         else
             hipLaunchKernelGGL((pair_edge_kernel<P, false>), dim3(unsigned(kXcd * t0)), dim3(kBlock), 0, s, p, r, t0);
@@ -995,9 +1045,9 @@ static bool pair_run(trisk_handle* H) {
 // state_edges, gnext: the cell-pair path's fusion of G into the previous stage's edge kernel
 template <class P, class UIn>
 static int run_tendency(trisk_handle* H, const UIn* u, const double* h, const Rk& r, hipStream_t s,
-                        bool state_edges = true, bool gnext = false) {
+                        bool state_edges = true, bool gnext = false, bool plane = false) {
     const Layout& L = H->L;
-    if (pair_run<P>(H)) return run_pair<P, UIn>(H, u, h, r, s, state_edges, gnext);
+    if (pair_run<P>(H)) return run_pair<P, UIn>(H, u, h, r, s, state_edges, gnext, plane);
     const bool hex = L.kc == 6 && L.kv == 3 && L.kt == 10 && (L.kg == 0 || L.kg == 12);
     return hex ? run_widths<P, UIn, HexWidths>(H, u, h, r, s) : run_widths<P, UIn, AnyWidths>(H, u, h, r, s);
 }
@@ -1016,15 +1066,18 @@ static int rk4_steps(trisk_handle* H, double* y, double dt, int64_t nsteps, hipS
     r.dt6 = dt / 6;
     int rc;
     // cell-pair path: each stage's edge kernel also forms G of the state it writes, so only the
-    // first stage of the call runs the state pass over the edges
+    // first stage of the call runs the state pass over the edges; with the (G, u) plane that
+    // edge kernel writes G and the stage u as one pair (no stage vector), which the next
+    // stage's state and cell kernels gather
     const bool fuse = pair_run<P>(H);
+    const bool plane = fuse && H->pair_plane;
     for (int64_t step = 0; step < nsteps; ++step) {
         r.mode = 1;
-        if ((rc = run_tendency<P, double>(H, y, y + L.ne, r, s, !fuse || step == 0, fuse))) return rc;
+        if ((rc = run_tendency<P, double>(H, y, y + L.ne, r, s, !fuse || step == 0, fuse, plane))) return rc;
         for (int mode = 2; mode <= 4; ++mode) {
             r.mode = mode;
             const bool last = mode == 4 && step + 1 == nsteps;
-            if ((rc = run_tendency<P, U>(H, static_cast<const U*>(H->su), H->sh, r, s, !fuse, fuse && !last)))
+            if ((rc = run_tendency<P, U>(H, static_cast<const U*>(H->su), H->sh, r, s, !fuse, fuse && !last, plane)))
                 return rc;
         }
     }
@@ -1196,6 +1249,9 @@ int trisk_create(const trisk_model_desc* desc, const trisk_options* opt, trisk_h
         return set_err(TRISK_EINVAL, "block_size must be 0 or the library's workgroup size");
     auto* H = new trisk_handle;
     H->precision = o.precision;
+    // the (G, u) plane: 8-byte FP32 pairs measured faster, 16-byte FP64 pairs slower (they
+    // double the lines the vertex kernel gathers u from)
+    H->pair_plane = o.precision == TRISK_FP32_BULK;
     H->device = o.device;
     std::string err;
     if (!build_layout(desc, H->L, err)) {
@@ -1430,6 +1486,14 @@ int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, v
     return run_steps(h, y_dev, dt, nsteps - reps * kGraphSteps, s);
 }
 
+int trisk_set_pair_plane(trisk_handle* h, int32_t on) {
+    CHECK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    h->pair_plane = on != 0;
+    drop_graph(h);  // a captured graph holds the kernels of the other layout
+    return 0;
+}
+
 int trisk_set_graphs(trisk_handle* h, int32_t on) {
     CHECK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
@@ -1631,8 +1695,9 @@ int trisk_kernel_bytes(trisk_handle* h, int32_t max, const char** names, double*
         const double cw = f64 ? 12 * iw : 24.0;                  /* cell edges and corners */
         // rk4_steps forms G in the previous stage's edge kernel: the state pass of stages after
         // the first covers the vertices, and the edge kernel reads l h and the new heights
+        // with the (G, u) plane the vertices gather u from it (both halves of its lines)
         double w1 = nv * (rec8 + vc + c)                         /* indices, coefficients, q_v */
-                    + ne * uw + nc * 8.0;                        /* u, h, gathered */
+                    + ne * (h->pair_plane ? 2 * c : uw) + nc * 8.0;  /* u, h, gathered */
         double w2 = nc * (cw + 6 * c + iw + 8.0 + rk_c)         /* edges, corners, kite fractions, meta, A, RK4 */
                     + nc * (6 * c + 8.0 + 8.0)                   /* KE weights, h, B */
                     + ne * c + nv * c + ne * uw                  /* G, q_v, u */

@@ -112,6 +112,7 @@ struct Params {
     const C* len;          // ne: l_e
     const C* dual;         // ne: d_e
     I* G;                  // ne: l_e F_e
+    Pair<I>* gu;           // ne: (G_e, u_e) of the stage input, one pair per edge (the (G, u) plane)
     I* T;                  // 6 x nc, cell-major: PV-flux half sum of cell i at its edge k in T[6 i + k]
     const uint8_t* eslot;  // ne: slot of e in its left cell | slot in its right cell << 4
     // entity records of the cell-pair path
@@ -342,7 +343,8 @@ __device__ __forceinline__ double rk_next(const Rk& r, double k, const RkIn& v) 
     }
 }
 
-template <class U, int STORES = kPlain>
+// STAGE false: modes 1-3 leave the stage vector to the caller (the (G, u) plane holds it).
+template <class U, int STORES = kPlain, bool STAGE = true>
 __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const RkIn& v, U* stage) {
     auto acc = [](auto x, auto* p) {
         if constexpr (STORES != kPlain)
@@ -362,15 +364,15 @@ __device__ __forceinline__ void rk_store(const Rk& r, int64_t j, double k, const
             break;
         case 1:
             acc(k, r.acc + j);
-            stg(U(v.y + r.dt2 * k), stage);
+            if constexpr (STAGE) stg(U(v.y + r.dt2 * k), stage);
             break;
         case 2:
             acc(v.acc + 2 * k, r.acc + j);
-            stg(U(v.y + r.dt2 * k), stage);
+            if constexpr (STAGE) stg(U(v.y + r.dt2 * k), stage);
             break;
         case 3:
             acc(v.acc + 2 * k, r.acc + j);
-            stg(U(v.y + r.dt * k), stage);
+            if constexpr (STAGE) stg(U(v.y + r.dt * k), stage);
             break;
         default:
             acc(v.y + r.dt6 * (v.acc + k), r.y + j);
@@ -753,6 +755,7 @@ struct trisk_handle {
     double* diag_zv = nullptr;
     double* diag_hv = nullptr;
     bool finite_check = false;         // trisk_set_finite_check
+    bool pair_plane = false;           // trisk_set_pair_plane: rk4_steps keeps (G, u) in one plane
     mutable std::string last_error;    // trisk_handle_last_error
     std::vector<double> host_stage;
     // graph replay of trisk_rk4_steps: one captured graph of kGraphSteps steps per (y, dt)

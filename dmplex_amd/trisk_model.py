@@ -201,6 +201,12 @@ class CGridShallowWater:
         """Replay the RK4 steps through a captured HIP graph (trisk_set_graphs)."""
         D.check(D.lib().trisk_set_graphs(self._handle, int(on)))
 
+    def use_pair_plane(self, on: bool = True) -> None:
+        """Cell-pair path: keep each RK4 stage's G_e and u_e as one pair per edge (the
+        FP32-bulk default) or in two arrays (the FP64 default); the results are bitwise the
+        same (trisk_set_pair_plane)."""
+        D.check(D.lib().trisk_set_pair_plane(self._handle, int(on)))
+
     def permutation(self):
         """(cells, edges, vertices): reference index of every device entity."""
         g = self.grid

@@ -162,6 +162,15 @@ int trisk_tendency(trisk_handle* h, const double* y_dev, double* dy_dev, void* s
  * update fused into the tendency kernels (same arithmetic as the reference broadcasts). */
 int trisk_rk4_steps(trisk_handle* h, double* y_dev, double dt, int64_t nsteps, void* stream);
 
+/* Cell-pair path: trisk_rk4_steps keeps each stage's mass flux G_e and velocity u_e as one
+ * (G, u) pair per edge (on = 1, the default on FP32-bulk handles), which the next stage's
+ * vertex and cell kernels gather, or as two arrays (on = 0, the default on FP64 handles, whose
+ * 16-byte pairs would double the lines the vertex kernel gathers u from).  The same values
+ * either way (bitwise); the pairs halve the cell kernel's gathers at the edges.  No counterpart in the reference (its kernels read
+ * u and F from separate arrays, TRiSKShallowWater.jl:202-305): a layout switch for tests and
+ * measurement. */
+int trisk_set_pair_plane(trisk_handle* h, int32_t on);
+
 /* Replay trisk_rk4_steps through a captured HIP graph of 8 RK4 steps (on = 1; default 0).
  * The graph is captured on the call's stream (which must not be NULL) the first time and
  * again whenever y_dev or dt change; runs shorter than 8 steps and timed runs launch the

@@ -328,6 +328,29 @@ def test_pair_rk4_call_boundaries_bitwise(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+def test_pair_plane_equals_separate_arrays(precision):
+    """RK4 stages that keep G_e and the stage velocity u_e as one (G, u) pair per edge give the
+    bits of the same stages with G and the stage vector in two arrays, over one call of 13
+    steps and 13 calls of one step (each call's first stage forms the pairs from the state)."""
+    say(f"pair plane {precision}")
+    dm = mesh(5)
+    m = TRiSKShallowWater(dm, centers=dm.generators, path="pair", backend=HIPBackend(precision=precision),
+                          **earth_parameters())
+    y0 = seeded_state(m.grid)
+    out = {}
+    for on in (False, True):
+        m.use_pair_plane(on)
+        a, b = m.upload_state(y0), m.upload_state(y0)
+        rk4_steps_(a, 600.0, 13, m)
+        for _ in range(13):
+            rk4_steps_(b, 600.0, 1, m)
+        out[on] = gather_state(m, a), gather_state(m, b)
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][0])
+    assert np.array_equal(out[False][1], out[False][0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
 def test_graph_replay_equals_direct_launches(precision):
     """Graph replay (captured once, replayed with eager work between replays) gives the
     same bits as direct launches."""
